@@ -1,0 +1,165 @@
+"""GPU parity of the marginal-likelihood kernels (jk_lnl_f64 / jk_lnl_mixed,
+through the C ABI) against the CPU oracle of fast_likelihood.pyx:255-469."""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from tests._golden import lnl_stars, oracle_star, rel_err
+from thejoker_amd import device as D
+from thejoker_amd import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+STARS = lnl_stars()
+
+# Relative tolerance (|lnL| floored at 1) of the FP64 kernel against the
+# oracle's restatement of the reference algorithm.  The "long" star
+# (cond(B) ~ 1e10) is compared with the oracle's well-conditioned O(N L^2)
+# form at the same tolerance and with the LU form at its own accuracy.
+RTOL_F64 = 1e-10
+
+
+def _device_star(st, mode):
+    kw = {}
+    if st["kprior"] is not None:
+        kw = dict(k_prior_mode=1, sigma_K0=st["kprior"][0], P0=st["kprior"][1],
+                  max_K=st["kprior"][2])
+    return D.DeviceStar(st["t"], st["rv"], 1.0 / st["err"] ** 2, st["trend"], st["mu"],
+                        st["lam"], st["t_ref"], jitter_mode=mode, **kw)
+
+
+def _gpu_lnl(star, rows, precision="f64"):
+    cache = D.PriorCache.from_packed(rows)
+    return D.lnl(star, cache, precision=precision).cpu().numpy()
+
+
+@pytest.mark.parametrize("name", sorted(STARS))
+@pytest.mark.parametrize("mode", ["reference", "physical"])
+def test_f64_vs_oracle_golden_stars(gpu, name, mode):
+    st = STARS[name]
+    rows = st["rows"]
+    ost = oracle_star(st, 0 if mode == "reference" else 1)
+    ll_lu, capped = oracle.batch_lnl(ost, rows, method="reference", return_capped=True)
+    ll_ch = oracle.batch_lnl(ost, rows, method="chol")
+    got = _gpu_lnl(_device_star(st, mode), rows)
+    ok = ~capped
+    print(f"{name}/{mode}: capped rows {capped.sum()}, max rel vs LU "
+          f"{rel_err(got[ok], ll_lu[ok]).max():.2e}, vs chol {rel_err(got[ok], ll_ch[ok]).max():.2e}")
+    assert rel_err(got[ok], ll_ch[ok]).max() < RTOL_F64
+    tol_lu = 1e-6 if name == "long" else RTOL_F64
+    assert rel_err(got[ok], ll_lu[ok]).max() < tol_lu
+    # the golden dense vectors, at their own accuracy
+    want = st[f"ll_{mode}_mode"]
+    assert rel_err(got[ok], want[ok]).max() < (1e-6 if name == "long" else 1e-9)
+
+
+@pytest.mark.parametrize("jitter,mode", [(False, "physical"), (True, "physical"),
+                                         (True, "reference")])
+def test_f64_single_star_workload(gpu, jitter, mode):
+    """The 16-epoch star with prior rows of the plain-method workload (2^16 of
+    them; s sampled for the jitter variant)."""
+    star = syn.single_star()
+    cols = syn.prior_columns(1 << 16, seed=0, jitter=jitter)
+    rows = oracle.pack_rows(cols["P"], cols["e"], cols["omega"], cols["M0"], cols["s"])
+    ost = oracle.OracleStar(star.t, star.rv, star.ivar(), star.trend(), np.zeros(2),
+                            np.array([900.0, 1e4]), star.t_ref,
+                            jitter_mode=0 if mode == "reference" else 1)
+    want, capped = oracle.batch_lnl(ost, rows, n_threads=16, return_capped=True)
+    dstar = D.DeviceStar(star.t, star.rv, star.ivar(), star.trend(), np.zeros(2),
+                         np.array([900.0, 1e4]), star.t_ref, jitter_mode=mode)
+    got = _gpu_lnl(dstar, rows)
+    ok = ~capped
+    err = rel_err(got[ok], want[ok])
+    print(f"jitter={jitter} mode={mode}: {capped.sum()} capped rows of {len(rows)}; "
+          f"max rel err {err.max():.3e} (p99.99 {np.quantile(err, 0.9999):.3e})")
+    assert err.max() < RTOL_F64
+    # the capped rows still get a finite, converged value on the GPU
+    assert np.all(np.isfinite(got))
+
+
+def test_mixed_vs_f64(gpu):
+    """Mixed precision against FP64 on the same rows: the live set (lnL within
+    40 of the maximum) within 1e-3, every row within 0.5."""
+    star = syn.single_star()
+    cols = syn.prior_columns(1 << 18, seed=0)
+    rows = oracle.pack_rows(cols["P"], cols["e"], cols["omega"], cols["M0"], cols["s"])
+    dstar = D.DeviceStar(star.t, star.rv, star.ivar(), star.trend(), np.zeros(2),
+                         np.array([900.0, 1e4]), star.t_ref)
+    cache = D.PriorCache.from_packed(rows)
+    f64 = D.lnl(dstar, cache, precision="f64").cpu().numpy()
+    mix = D.lnl(dstar, cache, precision="mixed").cpu().numpy()
+    d = np.abs(mix - f64)
+    live = f64 >= f64.max() - 40
+    print(f"mixed vs f64: max |dlnL| all {d.max():.3e}, live ({live.sum()} rows) "
+          f"{d[live].max():.3e}, p99 all {np.quantile(d, 0.99):.3e}")
+    assert d[live].max() < 1e-3
+    assert d.max() < 0.5
+
+
+def test_rows_indirection_and_offsets(gpu):
+    import torch
+    st = STARS["cfg1"]
+    dstar = _device_star(st, "physical")
+    cache = D.PriorCache.from_packed(st["rows"])
+    full = D.lnl(dstar, cache).cpu().numpy()
+    perm = np.random.default_rng(3).permutation(len(st["rows"]))[:100]
+    r = torch.as_tensor(perm, device="cuda")
+    assert np.array_equal(D.lnl(dstar, cache, rows=r).cpu().numpy(), full[perm])
+    assert np.array_equal(D.lnl(dstar, cache, start=17, n=50).cpu().numpy(), full[17:67])
+
+
+def test_invalid_rows_do_not_poison(gpu):
+    st = STARS["cfg1"]
+    rows = st["rows"][:16].copy()
+    bad = rows.copy()
+    bad[1, 0] = np.nan       # P NaN
+    bad[3, 1] = 1.0          # e = 1
+    bad[5, 1] = 1.7          # e > 1
+    bad[7, 0] = 0.0          # P = 0
+    bad[9, 2] = np.inf       # omega inf
+    dstar = _device_star(st, "physical")
+    good = _gpu_lnl(dstar, rows)
+    got = _gpu_lnl(dstar, bad)
+    badidx = [1, 3, 5, 7, 9]
+    assert np.all(np.isnan(got[badidx]))
+    keep = np.setdiff1d(np.arange(16), badidx)
+    assert np.array_equal(got[keep], good[keep])
+
+
+@pytest.mark.parametrize("M,L", [(3, 1), (3, 2), (64, 3), (200, 2), (200, 4)])
+def test_shapes_vs_oracle(gpu, M, L):
+    rng = np.random.default_rng(M * 10 + L)
+    t = np.sort(rng.uniform(0, 1500, M))
+    err = rng.uniform(0.05, 1.0, M)
+    rv = rng.normal(0, 5, M) + 3 * np.sin(2 * np.pi * t / 77.0)
+    dt = t - t.min()
+    trend = np.vander(dt, N=L, increasing=True)[:, :L - 1] if L > 1 else np.zeros((M, 0))
+    mu = np.zeros(L)
+    lam = np.array([900.0, 1e4, 0.25, 1e-4][:L])
+    cols = syn.prior_columns(4096, seed=M + L, jitter=True)
+    rows = oracle.pack_rows(cols["P"], cols["e"], cols["omega"], cols["M0"], cols["s"])
+    for mode in ("physical", "reference"):
+        ost = oracle.OracleStar(t, rv, 1 / err ** 2, trend, mu, lam, t.min(),
+                                jitter_mode=1 if mode == "physical" else 0)
+        want, capped = oracle.batch_lnl(ost, rows, method="chol", n_threads=16,
+                                        return_capped=True)
+        dstar = D.DeviceStar(t, rv, 1 / err ** 2, trend, mu, lam, t.min(), jitter_mode=mode)
+        got = _gpu_lnl(dstar, rows)
+        err_ = rel_err(got[~capped], want[~capped])
+        print(f"M={M} L={L} {mode}: max rel {err_.max():.2e}")
+        assert err_.max() < RTOL_F64
+
+
+def test_linear_posterior_vs_oracle(gpu):
+    for name in ("cfg1", "ecsv", "doctest"):
+        st = STARS[name]
+        ost = oracle_star(st, 1)
+        _, capped = oracle.batch_lnl(ost, st["rows"], return_capped=True)
+        rows_idx = np.nonzero(~capped)[0][:64]
+        dstar = _device_star(st, "physical")
+        cache = D.PriorCache.from_packed(st["rows"])
+        a, cov, ll = D.linear_posterior(dstar, cache, rows_idx)
+        for k, i in enumerate(rows_idx):
+            ll_o, a_o, A_o = oracle.linear_posterior(ost, st["rows"][i])
+            assert np.allclose(a[k], a_o, rtol=1e-9, atol=1e-9 * np.abs(a_o).max())
+            assert np.allclose(cov[k], A_o, rtol=1e-9, atol=1e-12 * np.abs(A_o).max())
+            assert rel_err(ll[k], ll_o) < 1e-9

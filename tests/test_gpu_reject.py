@@ -1,0 +1,95 @@
+"""GPU rejection step (jk_pcg64_uniform / jk_reject through the C ABI):
+numpy-identical uniforms and the accepted index set of
+likelihood_helpers.py:107-109."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle
+from tests._golden import load
+from thejoker_amd import device as D
+
+pytestmark = pytest.mark.gpu
+PCG = load("pcg64.npz")
+
+
+def _state(i):
+    st = PCG[f"s{i}__state"]
+    inc = PCG[f"s{i}__inc"]
+    return (int(st[0]) << 64) | int(st[1]), (int(inc[0]) << 64) | int(inc[1])
+
+
+@pytest.mark.parametrize("i", range(5))
+def test_device_uniforms_bit_exact(gpu, i):
+    state, inc = _state(i)
+    u = D.uniforms((state, inc), 3000).cpu().numpy()
+    assert np.array_equal(u, PCG[f"s{i}__uniform"])
+    u2 = D.uniforms((state, inc), 64, offset=10 ** 7 + 3).cpu().numpy()
+    assert np.array_equal(u2, PCG[f"s{i}__uniform_after_1e7p3"])
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 4097, 1 << 20, (1 << 22) + 17])
+def test_device_uniforms_long_streams(gpu, n):
+    g = np.random.default_rng(n)
+    st = g.bit_generator.state["state"]
+    u = D.uniforms((st["state"], st["inc"]), n).cpu().numpy()
+    assert np.array_equal(u, g.uniform(size=n))
+
+
+@pytest.mark.parametrize("label", ["informative", "flat"])
+def test_reject_golden(gpu, label):
+    acc = load("accept_sets.npz")
+    ll = torch.as_tensor(acc[label + "__ll"], device="cuda")
+    st = np.random.default_rng(42).bit_generator.state["state"]
+    idx, n_pass, mx = D.reject(ll, (st["state"], st["inc"]), len(ll))
+    assert np.array_equal(idx, acc[label + "__idx"])
+    assert n_pass == len(acc[label + "__idx"])
+    assert mx == acc[label + "__ll"].max()
+    idx3, n3, _ = D.reject(ll, (st["state"], st["inc"]), 3)
+    assert np.array_equal(idx3, acc[label + "__idx"][:3]) and n3 == n_pass
+
+
+@pytest.mark.parametrize("n,scale", [(1 << 20, 1.0), (1 << 22, 3.0), ((1 << 21) + 5, 0.3),
+                                     (100, 0.01)])
+def test_reject_large_vs_oracle(gpu, n, scale):
+    """Many acceptances spread across blocks: ordered compaction, truncation
+    at several max_keep values, and a nonzero stream offset."""
+    g = np.random.default_rng(n)
+    ll_h = -scale * g.exponential(size=n)
+    ll = torch.as_tensor(ll_h, device="cuda")
+    st = np.random.default_rng(7).bit_generator.state["state"]
+    for offset in (0, 12345):
+        u = oracle.pcg64_uniform(st["state"], st["inc"], n, offset=offset)
+        want, n_want, _ = oracle.reject(ll_h, u)
+        for max_keep in (n, 256, 1):
+            idx, n_pass, _ = D.reject(ll, (st["state"], st["inc"]), max_keep, offset=offset)
+            assert n_pass == n_want
+            assert np.array_equal(idx, want[:max_keep])
+
+
+def test_reject_nan_and_inf(gpu):
+    st = np.random.default_rng(1).bit_generator.state["state"]
+    s = (st["state"], st["inc"])
+    ll = torch.as_tensor(np.array([0.0, np.nan, -1.0] * 1000), device="cuda")
+    idx, n, mx = D.reject(ll, s, 10)
+    assert n == 0 and np.isnan(mx)
+    ll = torch.as_tensor(np.array([np.inf, 0.0, -3.0]), device="cuda")
+    idx, n, mx = D.reject(ll, s, 10)
+    assert n == 0 and mx == np.inf
+    ll = torch.as_tensor(np.array([-np.inf, 0.0]), device="cuda")
+    idx, n, _ = D.reject(ll, s, 10)
+    assert list(idx) == [1]
+
+
+def test_host_generator_stays_in_sync(gpu):
+    """After the device draws n uniforms the host generator is advanced by n:
+    the next host draw equals numpy's after uniform(size=n)."""
+    g1 = np.random.default_rng(99)
+    g2 = np.random.default_rng(99)
+    n = 123457
+    ll = torch.zeros(n, dtype=torch.float64, device="cuda")
+    D.reject(ll, D.pcg64_state(g1), 5)
+    D.pcg64_advance_state(g1, n)
+    g2.uniform(size=n)
+    assert g1.uniform() == g2.uniform()
+    assert np.array_equal(g1.normal(size=10), g2.normal(size=10))

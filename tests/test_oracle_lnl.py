@@ -1,0 +1,123 @@
+"""Pins the oracle's marginal likelihood (fast_likelihood.pyx:255-469
+restated) against independent numpy dense-Gaussian vectors, 2-D quadrature of
+the un-marginalised likelihood, and its own O(N L^2) form."""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from tests._golden import lnl_stars, oracle_star, rel_err
+
+STARS = lnl_stars()
+
+# Tolerance against the dense golden vectors.  Any FP64 evaluation of the
+# phase 2 pi (t - t_ref)/P - M0 carries ~ulp(M) / (1 - e cos E) of anomaly
+# noise, ~1e-10 relative in lnL at |M| ~ 1e3; the "long" star (quadratic trend
+# over 3650 d, cond(B) ~ 1e10) also loses ~1e-8 relative in the N x N LU
+# log-determinant of the reference algorithm itself.  A dropped term, a wrong
+# sign or a transposed operand moves lnL by far more than either.
+DENSE_RTOL = {"long": 1e-6}
+
+
+@pytest.mark.parametrize("name", sorted(STARS))
+@pytest.mark.parametrize("mode", ["reference", "physical"])
+@pytest.mark.parametrize("method", ["reference", "chol"])
+def test_oracle_vs_dense_golden(name, mode, method):
+    st = STARS[name]
+    ost = oracle_star(st, jitter_mode=0 if mode == "reference" else 1)
+    ll, capped = oracle.batch_lnl(ost, st["rows"], method=method, return_capped=True)
+    want = st[f"ll_{mode}_mode"]
+    ok = ~capped
+    assert ok.sum() >= len(ok) - 3
+    err = rel_err(ll[ok], want[ok])
+    assert err.max() < DENSE_RTOL.get(name, 1e-9), (name, err.max(), np.argmax(err))
+
+
+def test_reference_mode_ignores_jitter():
+    # fast_likelihood.pyx:458 computes s_ivar and never reads it.
+    st = STARS["cfg1"]
+    ost = oracle_star(st, 0)
+    rows = st["rows"].copy()
+    a = oracle.batch_lnl(ost, rows)
+    rows[:, 4] = 0.0
+    b = oracle.batch_lnl(ost, rows)
+    assert np.array_equal(a, b)
+
+
+def test_modes_agree_at_zero_jitter():
+    st = STARS["ecsv"]
+    rows = st["rows"].copy()
+    rows[:, 4] = 0.0
+    a = oracle.batch_lnl(oracle_star(st, 0), rows)
+    b = oracle.batch_lnl(oracle_star(st, 1), rows)
+    assert rel_err(a, b).max() < 1e-13
+
+
+@pytest.mark.parametrize("i_row", [3, 10, 40])
+def test_marginal_equals_quadrature(i_row):
+    """lnL = ln of the integral over (K, v0) of prod_n N(y_n | K m_n + v0,
+    sigma_n^2) N(K | 0, 10^2) N(v0 | 0, 100^2) -- the definition the
+    closed form marginalises (py_likelihood.py:32-107)."""
+    from scipy import integrate
+
+    st = STARS["three"]
+    row = st["rows"][i_row]
+    ost = oracle_star(st, 1)
+    ll = oracle.batch_lnl(ost, row[None, :])[0]
+    m = oracle.rv_unit(st["t"], row[0], row[1], row[2], row[3], st["t_ref"])
+    var = st["err"] ** 2 + row[4] ** 2
+    y = st["rv"]
+    sK, sv = np.sqrt(st["lam"][0]), np.sqrt(st["lam"][1])
+    # Centre and scale the integration box on the posterior of (K, v0).
+    X = np.stack([m, np.ones_like(m)], axis=1)
+    G = np.diag(1 / st["lam"]) + X.T @ (X / var[:, None])
+    mean = np.linalg.solve(G, X.T @ (y / var))
+    sd = np.sqrt(np.diag(np.linalg.inv(G)))
+
+    def integrand(v0, K):
+        r = y - K * m - v0
+        lp = -0.5 * np.sum(r * r / var + np.log(2 * np.pi * var))
+        lp += -0.5 * (K / sK) ** 2 - np.log(sK) - 0.5 * np.log(2 * np.pi)
+        lp += -0.5 * (v0 / sv) ** 2 - np.log(sv) - 0.5 * np.log(2 * np.pi)
+        return np.exp(lp - ll)
+
+    val, _ = integrate.dblquad(integrand, mean[0] - 12 * sd[0], mean[0] + 12 * sd[0],
+                               mean[1] - 12 * sd[1], mean[1] + 12 * sd[1],
+                               epsabs=1e-12, epsrel=1e-11)
+    assert abs(np.log(val)) < 1e-8
+
+
+@pytest.mark.parametrize("name", ["cfg1", "ecsv", "long", "doctest"])
+def test_linear_posterior_vs_dense(name):
+    st = STARS[name]
+    ost = oracle_star(st, 1)
+    _, capped = oracle.batch_lnl(ost, st["rows"], return_capped=True)
+    for i in range(len(st["post_a"])):
+        if capped[i]:
+            continue
+        ll, a, A = oracle.linear_posterior(ost, st["rows"][i])
+        assert np.allclose(a, st["post_a"][i], rtol=1e-8, atol=1e-10 * np.abs(a).max())
+        assert np.allclose(A, st["post_cov"][i], rtol=1e-7, atol=1e-12 * np.abs(A).max())
+
+
+def test_fixed_companion_mass_clip_only_on_likelihood_path():
+    # Lambda_0 = min(max_K^2, ...) for lnL (fast_likelihood.pyx:461-464) but
+    # unclipped for the posterior (fast_likelihood.pyx:519-521): a tiny period
+    # makes the unclipped variance exceed max_K^2, so the two paths' lnL differ.
+    st = dict(STARS["doctest"])
+    ost = oracle_star(st, 0)
+    row = np.array([0.01, 0.5, 1.0, 2.0, 0.0])
+    ll_lik = oracle.batch_lnl(ost, row[None, :])[0]
+    ll_post, _, _ = oracle.linear_posterior(ost, row)
+    assert np.isfinite(ll_lik) and np.isfinite(ll_post)
+    assert ll_lik != ll_post
+    row2 = np.array([300.0, 0.2, 1.0, 2.0, 0.0])
+    assert abs(oracle.batch_lnl(ost, row2[None, :])[0] - oracle.linear_posterior(ost, row2)[0]) < 1e-9
+
+
+def test_threads_match_serial():
+    st = STARS["cfg1"]
+    ost = oracle_star(st, 1)
+    rows = np.tile(st["rows"], (8, 1))
+    a = oracle.batch_lnl(ost, rows, n_threads=1)
+    b = oracle.batch_lnl(ost, rows, n_threads=5)
+    assert np.array_equal(a, b)

@@ -1,0 +1,292 @@
+// jk_capi.hip -- extern "C" entry points of libjoker (declared in
+// include/joker.h).  Host-side validation, star construction and scratch
+// management; all arithmetic is in jk_kernels.hip.
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <mutex>
+#include <vector>
+
+#include "jk_internal.h"
+
+#define JK_ABI_VERSION 1
+
+namespace {
+
+std::mutex g_init_mu;
+bool g_inited[64] = {false};
+
+int check_hip(hipError_t e) { return e == hipSuccess ? JK_OK : JK_EHIP; }
+
+int current_device_inited()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -1;
+    if (dev < 0 || dev >= 64 || !g_inited[dev]) return -1;
+    return dev;
+}
+
+u128 make_u128(uint64_t hi, uint64_t lo)
+{
+    u128 r;
+    r.lo = lo;
+    r.hi = hi;
+    return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *jk_status_string(int status)
+{
+    switch (status) {
+    case JK_OK: return "ok";
+    case JK_EINVAL: return "invalid argument";
+    case JK_ENOMEM: return "device allocation failed";
+    case JK_EHIP: return "HIP runtime error";
+    case JK_ENODEV: return "no usable gfx950 device (call jk_init first)";
+    case JK_ERANGE: return "size exceeds the supported maximum";
+    default: return "unknown status";
+    }
+}
+
+int jk_abi_version(void) { return JK_ABI_VERSION; }
+
+int jk_init(int device)
+{
+    std::lock_guard<std::mutex> lock(g_init_mu);
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return JK_ENODEV;
+    if (device < 0 || device >= count || device >= 64) return JK_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return JK_EHIP;
+    if (g_inited[device]) return JK_OK;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return JK_EHIP;
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) return JK_ENODEV;
+    if (jk_upload_pcg_table(device) != hipSuccess) return JK_EHIP;
+    if (hipDeviceSynchronize() != hipSuccess) return JK_EHIP;
+    g_inited[device] = true;
+    return JK_OK;
+}
+
+int jk_star_create(int n, const double *t, const double *rv, const double *ivar, int L,
+                   const double *trend, const double *mu, const double *lambda_diag,
+                   double t_ref, int k_prior_mode, double sigma_K0, double P0, double max_K,
+                   int jitter_mode, jk_star **out)
+{
+    if (!out) return JK_EINVAL;
+    *out = nullptr;
+    if (n <= 0 || L < 1 || L > JK_LMAX || !t || !rv || !ivar || !mu || !lambda_diag)
+        return JK_EINVAL;
+    if (L > 1 && !trend) return JK_EINVAL;
+    if (jitter_mode != JK_JITTER_REFERENCE && jitter_mode != JK_JITTER_PHYSICAL) return JK_EINVAL;
+    if (k_prior_mode != JK_KPRIOR_NORMAL && k_prior_mode != JK_KPRIOR_FIXED_COMPANION_MASS)
+        return JK_EINVAL;
+    const int dev = current_device_inited();
+    if (dev < 0) return JK_ENODEV;
+    for (int i = 0; i < L; ++i)
+        if (!(lambda_diag[i] > 0.0) && !(i == 0 && k_prior_mode == JK_KPRIOR_FIXED_COMPANION_MASS))
+            return JK_EINVAL;
+    if (k_prior_mode == JK_KPRIOR_FIXED_COMPANION_MASS && !(P0 > 0.0)) return JK_EINVAL;
+
+    jk_star *st = new (std::nothrow) jk_star;
+    if (!st) return JK_ENOMEM;
+    memset(st, 0, sizeof(*st));
+    st->device = dev;
+    StarDev &d = st->dev;
+    d.n = n;
+    d.L = L;
+    d.jitter_mode = jitter_mode;
+    d.k_prior_mode = k_prior_mode;
+    d.sigK0sq = sigma_K0 * sigma_K0;
+    d.P0 = P0;
+    d.maxKsq = max_K * max_K;
+    d.mu0 = mu[0];
+    for (int i = 0; i < L; ++i) d.mu[i] = mu[i];
+    d.lam0 = lambda_diag[0];
+    d.lam_inv[0] = k_prior_mode == JK_KPRIOR_NORMAL ? 1.0 / lambda_diag[0] : 0.0;
+    d.ln_lam_trend = 0.0;
+    for (int i = 1; i < L; ++i) {
+        d.lam_inv[i] = 1.0 / lambda_diag[i];
+        d.ln_lam_trend += log(lambda_diag[i]);
+    }
+
+    // Host epoch arrays: dt, r' = y - T mu_trend, y, ivar, T (L-1 rows of n).
+    const int LT = L - 1;
+    std::vector<double> h((size_t)n * (4 + LT));
+    double *h_dt = h.data(), *h_rp = h_dt + n, *h_y = h_rp + n, *h_iv = h_y + n, *h_T = h_iv + n;
+    d.sum_ln_ivar = 0.0;
+    d.S_rr = 0.0;
+    for (int k = 0; k < n; ++k) {
+        if (!(ivar[k] > 0.0) || !isfinite(ivar[k]) || !isfinite(t[k]) || !isfinite(rv[k])) {
+            delete st;
+            return JK_EINVAL;
+        }
+        h_dt[k] = t[k] - t_ref;
+        double rp = rv[k];
+        for (int j = 0; j < LT; ++j) {
+            const double Tv = trend[(size_t)k * LT + j];
+            h_T[(size_t)j * n + k] = Tv;
+            rp -= Tv * mu[j + 1];
+        }
+        h_rp[k] = rp;
+        h_y[k] = rv[k];
+        h_iv[k] = ivar[k];
+        d.sum_ln_ivar += log(ivar[k]);
+        d.S_rr += ivar[k] * rp * rp;
+        for (int j = 0; j < LT; ++j) {
+            const double wT = ivar[k] * h_T[(size_t)j * n + k];
+            d.S_tr[j] += wT * rp;
+            d.S_ty[j] += wT * rv[k];
+            for (int q = 0; q <= j; ++q) {
+                d.S_tt[j * JK_LMAX + q] += wT * h_T[(size_t)q * n + k];
+            }
+        }
+    }
+    for (int j = 0; j < LT; ++j)
+        for (int q = 0; q < j; ++q) d.S_tt[q * JK_LMAX + j] = d.S_tt[j * JK_LMAX + q];
+
+    void *dmem = nullptr;
+    if (hipMalloc(&dmem, h.size() * sizeof(double)) != hipSuccess) {
+        delete st;
+        return JK_ENOMEM;
+    }
+    if (hipMemcpy(dmem, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(dmem);
+        delete st;
+        return JK_EHIP;
+    }
+    st->d_epochs = dmem;
+    double *base = (double *)dmem;
+    d.dt = base;
+    d.rp = base + n;
+    d.y = base + 2 * n;
+    d.ivar = base + 3 * n;
+    d.T = base + 4 * n;
+    *out = st;
+    return JK_OK;
+}
+
+int jk_star_destroy(jk_star *star)
+{
+    if (!star) return JK_EINVAL;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipSetDevice(star->device);
+    hipError_t e = hipFree(star->d_epochs);
+    (void)hipSetDevice(dev);
+    delete star;
+    return check_hip(e);
+}
+
+int jk_lnl_f64(const jk_star *star, int64_t n, const double *P, const double *e,
+               const double *omega, const double *M0, const double *s, const int64_t *rows,
+               double *ll_out, void *stream)
+{
+    if (!star || n < 0 || (n > 0 && (!P || !e || !omega || !M0 || !ll_out))) return JK_EINVAL;
+    if (star->dev.jitter_mode == JK_JITTER_PHYSICAL && n > 0 && !s) return JK_EINVAL;
+    if (current_device_inited() < 0) return JK_ENODEV;
+    return check_hip(jk_launch_lnl_f64(star, n, P, e, omega, M0, s, rows, ll_out,
+                                       (hipStream_t)stream));
+}
+
+int jk_lnl_mixed(const jk_star *star, int64_t n, const double *P, const float *e,
+                 const float *omega, const float *M0, const float *s, const int64_t *rows,
+                 double *ll_out, void *stream)
+{
+    if (!star || n < 0 || (n > 0 && (!P || !e || !omega || !M0 || !ll_out))) return JK_EINVAL;
+    if (star->dev.jitter_mode == JK_JITTER_PHYSICAL && n > 0 && !s) return JK_EINVAL;
+    if (current_device_inited() < 0) return JK_ENODEV;
+    return check_hip(jk_launch_lnl_mixed(star, n, P, e, omega, M0, s, rows, ll_out,
+                                         (hipStream_t)stream));
+}
+
+int jk_max_f64(const double *ll, int64_t n, double *max_out, void *stream)
+{
+    if (!ll || !max_out || n <= 0) return JK_EINVAL;
+    if (current_device_inited() < 0) return JK_ENODEV;
+    hipStream_t st = (hipStream_t)stream;
+    const int np = jk_max_partials_count(n);
+    double *partials = nullptr;
+    if (hipMallocAsync((void **)&partials, sizeof(double) * np, st) != hipSuccess) return JK_ENOMEM;
+    hipError_t err = jk_launch_max(ll, n, partials, np, max_out, st);
+    hipError_t err2 = hipFreeAsync(partials, st);
+    if (err != hipSuccess) return JK_EHIP;
+    return check_hip(err2);
+}
+
+static int reject_impl(const double *ll, int64_t n, const double *llmax_in, uint64_t state_hi,
+                       uint64_t state_lo, uint64_t inc_hi, uint64_t inc_lo, uint64_t offset,
+                       int64_t max_keep, int64_t *idx_out, int64_t *n_pass_out,
+                       double *llmax_out, hipStream_t st)
+{
+    if (!ll || n <= 0 || max_keep < 0 || !n_pass_out || (max_keep > 0 && !idx_out))
+        return JK_EINVAL;
+    if (!llmax_in && !llmax_out) return JK_EINVAL;
+    if (current_device_inited() < 0) return JK_ENODEV;
+    const int np = jk_max_partials_count(n);
+    const int64_t nwords = jk_reject_mask_words(n);
+    const int64_t nb = jk_reject_bsum_count(n);
+    const size_t bytes = sizeof(double) * np + sizeof(uint64_t) * nwords + sizeof(int64_t) * nb;
+    char *scratch = nullptr;
+    if (hipMallocAsync((void **)&scratch, bytes, st) != hipSuccess) return JK_ENOMEM;
+    double *partials = (double *)scratch;
+    uint64_t *mask = (uint64_t *)(scratch + sizeof(double) * np);
+    int64_t *bsum = (int64_t *)(scratch + sizeof(double) * np + sizeof(uint64_t) * nwords);
+    hipError_t err = hipSuccess;
+    const double *mx = llmax_in;
+    if (!mx) {
+        err = jk_launch_max(ll, n, partials, np, llmax_out, st);
+        mx = llmax_out;
+    }
+    if (err == hipSuccess)
+        err = jk_launch_reject(ll, n, mx, make_u128(state_hi, state_lo), make_u128(inc_hi, inc_lo),
+                               offset, max_keep, idx_out, n_pass_out, mask, bsum, st);
+    hipError_t err2 = hipFreeAsync(scratch, st);
+    if (err != hipSuccess) return JK_EHIP;
+    return check_hip(err2);
+}
+
+int jk_reject(const double *ll, int64_t n, uint64_t state_hi, uint64_t state_lo, uint64_t inc_hi,
+              uint64_t inc_lo, uint64_t stream_offset, int64_t max_keep, int64_t *idx_out,
+              int64_t *n_pass_out, double *llmax_out, void *stream)
+{
+    if (!llmax_out) return JK_EINVAL;
+    return reject_impl(ll, n, nullptr, state_hi, state_lo, inc_hi, inc_lo, stream_offset, max_keep,
+                       idx_out, n_pass_out, llmax_out, (hipStream_t)stream);
+}
+
+int jk_reject_with_max(const double *ll, int64_t n, const double *llmax, uint64_t state_hi,
+                       uint64_t state_lo, uint64_t inc_hi, uint64_t inc_lo, uint64_t stream_offset,
+                       int64_t max_keep, int64_t *idx_out, int64_t *n_pass_out, void *stream)
+{
+    if (!llmax) return JK_EINVAL;
+    return reject_impl(ll, n, llmax, state_hi, state_lo, inc_hi, inc_lo, stream_offset, max_keep,
+                       idx_out, n_pass_out, nullptr, (hipStream_t)stream);
+}
+
+int jk_pcg64_uniform(uint64_t state_hi, uint64_t state_lo, uint64_t inc_hi, uint64_t inc_lo,
+                     uint64_t offset, int64_t n, double *u_out, void *stream)
+{
+    if (n < 0 || (n > 0 && !u_out)) return JK_EINVAL;
+    if (current_device_inited() < 0) return JK_ENODEV;
+    return check_hip(jk_launch_uniform(make_u128(state_hi, state_lo), make_u128(inc_hi, inc_lo),
+                                       offset, n, u_out, (hipStream_t)stream));
+}
+
+int jk_linear_posterior(const jk_star *star, int64_t n, const double *P, const double *e,
+                        const double *omega, const double *M0, const double *s,
+                        const int64_t *rows, double *a_out, double *cov_out, double *ll_out,
+                        void *stream)
+{
+    if (!star || n < 0) return JK_EINVAL;
+    if (n > 0 && (!P || !e || !omega || !M0 || !a_out || !cov_out || !ll_out)) return JK_EINVAL;
+    if (star->dev.jitter_mode == JK_JITTER_PHYSICAL && n > 0 && !s) return JK_EINVAL;
+    if (current_device_inited() < 0) return JK_ENODEV;
+    return check_hip(jk_launch_linear_posterior(star, n, P, e, omega, M0, s, rows, a_out, cov_out,
+                                                ll_out, (hipStream_t)stream));
+}
+
+}  // extern "C"

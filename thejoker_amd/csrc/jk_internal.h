@@ -1,0 +1,49 @@
+// jk_internal.h -- host-side launch interface between the C ABI (jk_capi.hip)
+// and the kernels (jk_kernels.hip).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "jk_device.h"
+
+// Host mirror of one star: the StarDev block passed by value to kernels,
+// plus the device allocation that holds the epoch arrays it points into.
+struct jk_star {
+    StarDev dev;        // by-value kernel argument (pointers are device)
+    void *d_epochs;     // device allocation behind dev.dt / rp / y / ivar / T
+    int device;
+};
+
+// Jump table of numpy's PCG64 LCG: (A^(2^i), D_(2^i)) for i = 0..63, where
+// a jump by k steps maps state -> A^k state + inc * D_k.
+struct PcgJump {
+    u128 A[64];
+    u128 D[64];
+};
+
+// Host-side 128-bit helpers (same arithmetic as the device u128 functions).
+void jk_pcg_jump_coeffs(uint64_t k, u128 *A, u128 *D);
+
+hipError_t jk_launch_lnl_f64(const jk_star *st, int64_t n, const double *P, const double *e,
+                             const double *om, const double *M0, const double *s,
+                             const int64_t *rows, double *ll, hipStream_t stream);
+hipError_t jk_launch_lnl_mixed(const jk_star *st, int64_t n, const double *P, const float *e,
+                               const float *om, const float *M0, const float *s,
+                               const int64_t *rows, double *ll, hipStream_t stream);
+hipError_t jk_launch_max(const double *ll, int64_t n, double *partials, int n_partials,
+                         double *max_out, hipStream_t stream);
+int jk_max_partials_count(int64_t n);
+hipError_t jk_launch_reject(const double *ll, int64_t n, const double *llmax, u128 state, u128 inc,
+                            uint64_t offset, int64_t max_keep, int64_t *idx_out,
+                            int64_t *n_pass_out, uint64_t *mask_scratch,
+                            int64_t *bsum_scratch, hipStream_t stream);
+int64_t jk_reject_mask_words(int64_t n);
+int64_t jk_reject_bsum_count(int64_t n);
+hipError_t jk_launch_uniform(u128 state, u128 inc, uint64_t offset, int64_t n, double *u,
+                             hipStream_t stream);
+hipError_t jk_launch_linear_posterior(const jk_star *st, int64_t n, const double *P,
+                                      const double *e, const double *om, const double *M0,
+                                      const double *s, const int64_t *rows, double *a_out,
+                                      double *cov_out, double *ll_out, hipStream_t stream);
+hipError_t jk_upload_pcg_table(int device);

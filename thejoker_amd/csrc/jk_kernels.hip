@@ -1,0 +1,787 @@
+// jk_kernels.hip -- gfx950 kernels of The Joker's likelihood + rejection path.
+//
+//  k_lnl<L, PHYS, MIXED>   fused per-sample marginal lnL: phase, Kepler solve,
+//                          design column, moments, L x L Cholesky, lnL.  One
+//                          lane per prior sample; the star's epoch arrays are
+//                          wave-uniform and read through scalar loads.
+//  k_max_partial / final   NaN-propagating max (np.max).
+//  k_reject_flags          numpy-PCG64 uniforms generated in registers,
+//                          exp(ll - max) > u, one 64-bit ballot word per wave.
+//  k_reject_bsum/compact   order-preserving compaction of the ballot words.
+//  k_uniform               the raw uniforms (tests / callers).
+//  k_linear_posterior      a = G^-1 rhs and G^-1 for accepted rows.
+#include <type_traits>
+
+#include "jk_internal.h"
+
+#define JK_PI_HALF 1.57079632679489661923
+#define JK_PI_HI 3.141592653589793
+#define JK_PI_LO 1.2246467991473532e-16
+#define JK_PI_F_HI 3.1415927410125732f
+#define JK_PI_F_LO -8.742277657347586e-08f
+#define JK_LN2 0.69314718055994530942
+
+__constant__ PcgJump c_jump;
+
+// ---------------------------------------------------------------------------
+// sin and cos on [0, pi] from truncated Taylor series on [0, pi/2]:
+// E <= pi/2: y = E;  E > pi/2: y = pi - E (exact subtraction by Sterbenz,
+// plus the low part of pi), sin E = sin y, cos E = -cos y.
+// FP64: sin to y^21, cos to y^22 -- truncation below 1.3e-18 on [0, pi/2].
+// FP32: sin to y^13, cos to y^14 -- truncation below 7e-10.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void jk_sincos_0pi_f64(double E, double &s, double &c)
+{
+    const bool hi = E > JK_PI_HALF;
+    const double y = hi ? ((JK_PI_HI - E) + JK_PI_LO) : E;
+    const double y2 = y * y;
+    double ps = 1.9572941063391263e-20;
+    ps = fma(ps, y2, -8.22063524662433e-18);
+    ps = fma(ps, y2, 2.8114572543455206e-15);
+    ps = fma(ps, y2, -7.647163731819816e-13);
+    ps = fma(ps, y2, 1.6059043836821613e-10);
+    ps = fma(ps, y2, -2.505210838544172e-08);
+    ps = fma(ps, y2, 2.7557319223985893e-06);
+    ps = fma(ps, y2, -0.0001984126984126984);
+    ps = fma(ps, y2, 0.008333333333333333);
+    ps = fma(ps, y2, -0.16666666666666666);
+    s = fma(ps * y2, y, y);
+    double pc = -8.896791392450574e-22;
+    pc = fma(pc, y2, 4.110317623312165e-19);
+    pc = fma(pc, y2, -1.5619206968586225e-16);
+    pc = fma(pc, y2, 4.779477332387385e-14);
+    pc = fma(pc, y2, -1.1470745597729725e-11);
+    pc = fma(pc, y2, 2.08767569878681e-09);
+    pc = fma(pc, y2, -2.755731922398589e-07);
+    pc = fma(pc, y2, 2.48015873015873e-05);
+    pc = fma(pc, y2, -0.001388888888888889);
+    pc = fma(pc, y2, 0.041666666666666664);
+    pc = fma(pc, y2, -0.5);
+    c = fma(pc, y2, 1.0);
+    c = hi ? -c : c;
+}
+
+__device__ __forceinline__ void jk_sincos_0pi_f32(float E, float &s, float &c)
+{
+    const bool hi = E > 1.57079632679f;
+    const float y = hi ? ((JK_PI_F_HI - E) + JK_PI_F_LO) : E;
+    const float y2 = y * y;
+    float ps = 1.6059043836821613e-10f;
+    ps = fmaf(ps, y2, -2.505210838544172e-08f);
+    ps = fmaf(ps, y2, 2.7557319223985893e-06f);
+    ps = fmaf(ps, y2, -0.0001984126984126984f);
+    ps = fmaf(ps, y2, 0.008333333333333333f);
+    ps = fmaf(ps, y2, -0.16666666666666666f);
+    s = fmaf(ps * y2, y, y);
+    float pc = -1.1470745597729725e-11f;
+    pc = fmaf(pc, y2, 2.08767569878681e-09f);
+    pc = fmaf(pc, y2, -2.755731922398589e-07f);
+    pc = fmaf(pc, y2, 2.48015873015873e-05f);
+    pc = fmaf(pc, y2, -0.001388888888888889f);
+    pc = fmaf(pc, y2, 0.041666666666666664f);
+    pc = fmaf(pc, y2, -0.5f);
+    c = fmaf(pc, y2, 1.0f);
+    c = hi ? -c : c;
+}
+
+// Kepler in FP64 on ax = |x| in [0, pi] (see jk_kepler_f64 in jk_device.h for
+// the convergence argument); this variant evaluates sin/cos with the [0, pi]
+// polynomial pair above.
+__device__ __forceinline__ void jk_kepler_poly_f64(double x, double e, double &sinE, double &cosE)
+{
+    const double ax = fabs(x);
+    double E = fmin(fma(0.85, e, ax), JK_PI);
+    double s, c;
+    jk_sincos_0pi_f64(E, s, c);
+#pragma unroll 1
+    for (int it = 0; it < JK_KEPLER_MAXITER; ++it) {
+        const double fp = fma(-e, c, 1.0);
+        const double f = fma(-e, s, E - ax);
+        const double d = f / fp;
+        E -= d;
+        if (fabs(d) < JK_KEPLER_TOL) {
+            const double s1 = fma(-d, c, s);
+            c = fma(d, s, c);
+            s = s1;
+            break;
+        }
+        jk_sincos_0pi_f64(E, s, c);
+    }
+    sinE = x < 0.0 ? -s : s;
+    cosE = c;
+}
+
+// Kepler in FP32, same start and monotone Newton; stops when the step is
+// within 2^-22 of E (the FP32 iterate has stopped moving) or after 32 steps.
+__device__ __forceinline__ void jk_kepler_f32(float x, float e, float &sinE, float &cosE)
+{
+    const float ax = fabsf(x);
+    float E = fminf(fmaf(0.85f, e, ax), JK_PI_F_HI);
+    float s, c;
+    jk_sincos_0pi_f32(E, s, c);
+#pragma unroll 1
+    for (int it = 0; it < 32; ++it) {
+        const float fp = fmaf(-e, c, 1.0f);
+        const float f = fmaf(-e, s, E - ax);
+        const float d = f * __builtin_amdgcn_rcpf(fp);
+        E -= d;
+        if (fabsf(d) <= 2.4e-7f * E) {
+            const float s1 = fmaf(-d, c, s);
+            c = fmaf(d, s, c);
+            s = s1;
+            break;
+        }
+        jk_sincos_0pi_f32(E, s, c);
+    }
+    sinE = x < 0.0f ? -s : s;
+    cosE = c;
+}
+
+// ---------------------------------------------------------------------------
+// Fused marginal-likelihood kernel.
+// ---------------------------------------------------------------------------
+template <int L, bool PHYS, bool MIXED>
+__global__ __launch_bounds__(256) void k_lnl(const StarDev st, int64_t n,
+                                             const double *__restrict__ P,
+                                             const void *__restrict__ e_,
+                                             const void *__restrict__ om_,
+                                             const void *__restrict__ M0_,
+                                             const void *__restrict__ s_,
+                                             const int64_t *__restrict__ rows,
+                                             double *__restrict__ ll)
+{
+    using TF = typename std::conditional<MIXED, float, double>::type;
+    const TF *__restrict__ ecol = static_cast<const TF *>(e_);
+    const TF *__restrict__ omcol = static_cast<const TF *>(om_);
+    const TF *__restrict__ M0col = static_cast<const TF *>(M0_);
+    const TF *__restrict__ scol = static_cast<const TF *>(s_);
+
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t r = rows ? rows[i] : i;
+
+    const double Pv = P[r];
+    const double ev = (double)ecol[r];
+    const double omv = (double)omcol[r];
+    const double M0v = (double)M0col[r];
+    const double sv = PHYS ? (double)scol[r] : 0.0;
+
+    bool valid = isfinite(Pv) && Pv != 0.0 && ev >= 0.0 && ev < 1.0 && isfinite(omv) &&
+                 isfinite(M0v);
+    if (PHYS) valid = valid && isfinite(sv);
+    if (!valid) {
+        ll[i] = __builtin_nan("");
+        return;
+    }
+
+    double lam0;
+    if (st.k_prior_mode == JK_KPRIOR_FIXED_COMPANION_MASS)
+        lam0 = jk_k_variance(&st, Pv, ev, true);
+    else
+        lam0 = st.lam0;
+
+    const double nP = JK_TWO_PI / Pv;
+    const double s2 = sv * sv;
+
+    // Per-sample orbit constants.
+    double cw, sw, ecw, sq;
+    float cwf, swf, ecwf, sqf, ef;
+    if constexpr (MIXED) {
+        ef = (float)ev;
+        sincosf((float)omv, &swf, &cwf);
+        ecwf = ef * cwf;
+        sqf = sqrtf((1.0f - ef) * (1.0f + ef));
+    } else {
+        sincos(omv, &sw, &cw);
+        ecw = ev * cw;
+        sq = sqrt((1.0 - ev) * (1.0 + ev));
+    }
+
+    constexpr int LT = L > 1 ? L - 1 : 1;
+    double Smm = 0.0, Smr = 0.0;
+    double Smt[LT];
+    double Stt[LT * LT], Str[LT], Srr = 0.0;
+    double prod = 1.0;
+    int pexp = 0;
+#pragma unroll
+    for (int j = 0; j < LT; ++j) {
+        Smt[j] = 0.0;
+        Str[j] = 0.0;
+#pragma unroll
+        for (int k = 0; k < LT; ++k) Stt[j * LT + k] = 0.0;
+    }
+
+    const int N = st.n;
+#pragma unroll 2
+    for (int k = 0; k < N; ++k) {
+        const double dtk = st.dt[k];
+        const double x = jk_reduce_2pi(fma(nP, dtk, -M0v));
+        double m;
+        if constexpr (MIXED) {
+            float sE, cE;
+            jk_kepler_f32((float)x, ef, sE, cE);
+            const float den = __builtin_amdgcn_rcpf(fmaf(-ef, cE, 1.0f));
+            const float cf = (cE - ef) * den;
+            const float sf = sqf * sE * den;
+            m = (double)fmaf(cwf, cf, fmaf(-swf, sf, ecwf));
+        } else {
+            double sE, cE;
+            jk_kepler_poly_f64(x, ev, sE, cE);
+            m = jk_design_f64(sE, cE, ev, sq, cw, sw, ecw);
+        }
+        const double iv = st.ivar[k];
+        const double rp = st.rp[k];
+        double w;
+        if constexpr (PHYS) {
+            const double d = fma(s2, iv, 1.0);
+            w = iv / d;
+            int ex;
+            prod = frexp(prod * d, &ex);
+            pexp += ex;
+        } else {
+            w = iv;
+        }
+        const double wm = w * m;
+        Smm = fma(wm, m, Smm);
+        Smr = fma(wm, rp, Smr);
+        if constexpr (L > 1) {
+#pragma unroll
+            for (int j = 0; j < L - 1; ++j) {
+                const double Tj = st.T[j * N + k];
+                Smt[j] = fma(wm, Tj, Smt[j]);
+                if constexpr (PHYS) {
+                    const double wT = w * Tj;
+                    Str[j] = fma(wT, rp, Str[j]);
+#pragma unroll
+                    for (int q = 0; q <= j; ++q) Stt[j * LT + q] = fma(wT, st.T[q * N + k], Stt[j * LT + q]);
+                }
+            }
+        }
+        if constexpr (PHYS) Srr = fma(w * rp, rp, Srr);
+    }
+
+    // Assemble G (lower triangle used), u, chi0, log-constant.
+    double G[L * L], u[L];
+    const double mu0 = st.mu0;
+    G[0] = 1.0 / lam0 + Smm;
+    u[0] = fma(-mu0, Smm, Smr);
+    double chi0;
+    double sum_ln_w;
+    if constexpr (PHYS) {
+        chi0 = Srr;
+        sum_ln_w = st.sum_ln_ivar - (log(prod) + (double)pexp * JK_LN2);
+    } else {
+        chi0 = st.S_rr;
+        sum_ln_w = st.sum_ln_ivar;
+    }
+    chi0 = fma(mu0 * mu0, Smm, fma(-2.0 * mu0, Smr, chi0));
+    if constexpr (L > 1) {
+#pragma unroll
+        for (int j = 1; j < L; ++j) {
+            G[j * L] = Smt[j - 1];
+            G[j] = Smt[j - 1];
+            const double trj = PHYS ? Str[j - 1] : st.S_tr[j - 1];
+            u[j] = fma(-mu0, Smt[j - 1], trj);
+#pragma unroll
+            for (int q = 1; q <= j; ++q) {
+                const double tt = PHYS ? Stt[(j - 1) * LT + (q - 1)] : st.S_tt[(j - 1) * JK_LMAX + (q - 1)];
+                const double g = (q == j ? st.lam_inv[j] : 0.0) + tt;
+                G[j * L + q] = g;
+                G[q * L + j] = g;
+            }
+        }
+    }
+    const double lnc = (double)N * JK_LN_TWO_PI - sum_ln_w + log(lam0) + st.ln_lam_trend;
+    ll[i] = jk_finish_lnl<L>(G, u, chi0, lnc);
+}
+
+template <int L, bool PHYS, bool MIXED>
+static hipError_t launch_lnl_t(const jk_star *st, int64_t n, const double *P, const void *e,
+                               const void *om, const void *M0, const void *s,
+                               const int64_t *rows, double *ll, hipStream_t stream)
+{
+    const int64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL((k_lnl<L, PHYS, MIXED>), dim3((unsigned)blocks), dim3(256), 0, stream,
+                       st->dev, n, P, e, om, M0, s, rows, ll);
+    return hipGetLastError();
+}
+
+template <bool MIXED>
+static hipError_t launch_lnl_dispatch(const jk_star *st, int64_t n, const double *P, const void *e,
+                                      const void *om, const void *M0, const void *s,
+                                      const int64_t *rows, double *ll, hipStream_t stream)
+{
+    const bool phys = st->dev.jitter_mode == JK_JITTER_PHYSICAL;
+#define JK_CASE(LL)                                                                          \
+    case LL:                                                                                 \
+        return phys ? launch_lnl_t<LL, true, MIXED>(st, n, P, e, om, M0, s, rows, ll, stream) \
+                    : launch_lnl_t<LL, false, MIXED>(st, n, P, e, om, M0, s, rows, ll, stream);
+    switch (st->dev.L) {
+        JK_CASE(1)
+        JK_CASE(2)
+        JK_CASE(3)
+        JK_CASE(4)
+        JK_CASE(5)
+        JK_CASE(6)
+        JK_CASE(7)
+        JK_CASE(8)
+    default:
+        return hipErrorInvalidValue;
+    }
+#undef JK_CASE
+}
+
+hipError_t jk_launch_lnl_f64(const jk_star *st, int64_t n, const double *P, const double *e,
+                             const double *om, const double *M0, const double *s,
+                             const int64_t *rows, double *ll, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    return launch_lnl_dispatch<false>(st, n, P, e, om, M0, s, rows, ll, stream);
+}
+
+hipError_t jk_launch_lnl_mixed(const jk_star *st, int64_t n, const double *P, const float *e,
+                               const float *om, const float *M0, const float *s,
+                               const int64_t *rows, double *ll, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    return launch_lnl_dispatch<true>(st, n, P, e, om, M0, s, rows, ll, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Block-level helpers (256 threads = 4 wavefronts of 64).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_nanmax(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = jk_nanmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ int64_t wave_sum_i64(int64_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Inclusive scan across the wavefront.
+__device__ __forceinline__ int64_t wave_incl_scan_i64(int64_t v, int lane)
+{
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int64_t t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_max_partial(const double *__restrict__ ll, int64_t n,
+                                                     double *__restrict__ partials)
+{
+    __shared__ double red[4];
+    double m = -INFINITY;
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
+        m = jk_nanmax(m, ll[i]);
+    m = wave_nanmax(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        partials[blockIdx.x] = jk_nanmax(jk_nanmax(red[0], red[1]), jk_nanmax(red[2], red[3]));
+}
+
+__global__ __launch_bounds__(256) void k_max_final(const double *__restrict__ partials, int np,
+                                                   double *__restrict__ out)
+{
+    __shared__ double red[4];
+    double m = -INFINITY;
+    for (int i = threadIdx.x; i < np; i += 256) m = jk_nanmax(m, partials[i]);
+    m = wave_nanmax(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = jk_nanmax(jk_nanmax(red[0], red[1]), jk_nanmax(red[2], red[3]));
+}
+
+int jk_max_partials_count(int64_t n)
+{
+    int64_t b = (n + 255) / 256;
+    if (b > 1024) b = 1024;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+hipError_t jk_launch_max(const double *ll, int64_t n, double *partials, int n_partials,
+                         double *max_out, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_max_partial, dim3(n_partials), dim3(256), 0, stream, ll, n, partials);
+    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(256), 0, stream, partials, n_partials, max_out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// PCG64 uniforms and rejection.
+// Thread g of a grid of T threads owns indices g, g + T, g + 2T, ...; its
+// state starts at s_base (the state whose output is index 0) jumped by g
+// with the 2^i table, then advances by the (A_T, D_T) jump per iteration.
+// Waves step through 64-aligned index ranges, so one ballot word covers
+// indices [64 w, 64 w + 64).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ u128 pcg_jump_table(u128 s, uint64_t k, u128 inc)
+{
+#pragma unroll 1
+    for (int b = 0; k != 0; ++b, k >>= 1)
+        if (k & 1) s = pcg_affine(s, c_jump.A[b], c_jump.D[b], inc);
+    return s;
+}
+
+static int jk_rng_grid(int64_t n)
+{
+    int64_t b = (n + 256 * 16 - 1) / (256 * 16);
+    if (b > 4096) b = 4096;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+__global__ __launch_bounds__(256) void k_uniform(int64_t n, u128 s_base, u128 inc, u128 AT, u128 DT,
+                                                 double *__restrict__ u)
+{
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t T = (int64_t)gridDim.x * 256;
+    u128 s = pcg_jump_table(s_base, (uint64_t)g, inc);
+    for (int64_t i = g; i < n; i += T) {
+        u[i] = pcg_to_double(pcg_output(s));
+        s = pcg_affine(s, AT, DT, inc);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_reject_flags(const double *__restrict__ ll, int64_t n,
+                                                      const double *__restrict__ llmax,
+                                                      u128 s_base, u128 inc, u128 AT, u128 DT,
+                                                      uint64_t *__restrict__ mask)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t T = (int64_t)gridDim.x * 256;
+    const double mx = *llmax;
+    u128 s = pcg_jump_table(s_base, (uint64_t)g, inc);
+    for (int64_t base = g - lane; base < n; base += T) {
+        const int64_t i = base + lane;
+        bool keep = false;
+        if (i < n) {
+            const double uu = pcg_to_double(pcg_output(s));
+            keep = exp(ll[i] - mx) > uu;
+        }
+        const uint64_t word = __ballot(keep);
+        if (lane == 0) mask[base >> 6] = word;
+        s = pcg_affine(s, AT, DT, inc);
+    }
+}
+
+#define JK_WORDS_PER_BLOCK 4096   // 16 words per thread
+
+__global__ __launch_bounds__(256) void k_reject_bsum(const uint64_t *__restrict__ mask,
+                                                     int64_t nwords, int64_t *__restrict__ bsum)
+{
+    __shared__ int64_t red[4];
+    const int64_t w0 = (int64_t)blockIdx.x * JK_WORDS_PER_BLOCK;
+    int64_t c = 0;
+    for (int k = threadIdx.x; k < JK_WORDS_PER_BLOCK; k += 256) {
+        const int64_t w = w0 + k;
+        if (w < nwords) c += __popcll(mask[w]);
+    }
+    c = wave_sum_i64(c);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) bsum[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ __launch_bounds__(256) void k_reject_compact(const uint64_t *__restrict__ mask,
+                                                        int64_t nwords,
+                                                        const int64_t *__restrict__ bsum,
+                                                        int nblocks, int64_t max_keep,
+                                                        int64_t *__restrict__ idx_out,
+                                                        int64_t *__restrict__ n_pass_out)
+{
+    __shared__ int64_t red[4];
+    __shared__ int64_t s_off;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+
+    // Offset of this block = sum of the counts of the blocks before it; block
+    // 0 also publishes the total.
+    int64_t c = 0;
+    const int lim = blockIdx.x == 0 ? nblocks : (int)blockIdx.x;
+    for (int b = threadIdx.x; b < lim; b += 256) c += bsum[b];
+    c = wave_sum_i64(c);
+    if (lane == 0) red[wid] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int64_t tot = red[0] + red[1] + red[2] + red[3];
+        if (blockIdx.x == 0) {
+            *n_pass_out = tot;
+            s_off = 0;
+        } else {
+            s_off = tot;
+        }
+    }
+    __syncthreads();
+    const int64_t off = s_off;
+    if (off >= max_keep) return;   // uniform across the block
+
+    const int64_t w0 = (int64_t)blockIdx.x * JK_WORDS_PER_BLOCK + (int64_t)threadIdx.x * 16;
+    uint64_t words[16];
+    int64_t cnt = 0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int64_t w = w0 + q;
+        words[q] = w < nwords ? mask[w] : 0ull;
+        cnt += __popcll(words[q]);
+    }
+    // Exclusive block scan of cnt.
+    const int64_t incl = wave_incl_scan_i64(cnt, lane);
+    __syncthreads();
+    if (lane == 63) red[wid] = incl;
+    __syncthreads();
+    int64_t pre = incl - cnt;
+    for (int k = 0; k < wid; ++k) pre += red[k];
+    int64_t rank = off + pre;
+#pragma unroll 1
+    for (int q = 0; q < 16 && rank < max_keep; ++q) {
+        uint64_t word = words[q];
+        while (word != 0ull && rank < max_keep) {
+            const int bit = __builtin_ctzll(word);
+            idx_out[rank++] = ((w0 + q) << 6) + bit;
+            word &= word - 1ull;
+        }
+    }
+}
+
+int64_t jk_reject_mask_words(int64_t n) { return (n + 63) / 64; }
+int64_t jk_reject_bsum_count(int64_t n)
+{
+    return (jk_reject_mask_words(n) + JK_WORDS_PER_BLOCK - 1) / JK_WORDS_PER_BLOCK;
+}
+
+// ---------------------------------------------------------------------------
+// Host-side PCG64 jump arithmetic.
+// ---------------------------------------------------------------------------
+static u128 h_mul(u128 a, u128 b)
+{
+    unsigned __int128 x = ((unsigned __int128)a.hi << 64) | a.lo;
+    unsigned __int128 y = ((unsigned __int128)b.hi << 64) | b.lo;
+    unsigned __int128 z = x * y;
+    u128 r;
+    r.lo = (uint64_t)z;
+    r.hi = (uint64_t)(z >> 64);
+    return r;
+}
+
+static u128 h_add(u128 a, u128 b)
+{
+    unsigned __int128 x = ((unsigned __int128)a.hi << 64) | a.lo;
+    unsigned __int128 y = ((unsigned __int128)b.hi << 64) | b.lo;
+    unsigned __int128 z = x + y;
+    u128 r;
+    r.lo = (uint64_t)z;
+    r.hi = (uint64_t)(z >> 64);
+    return r;
+}
+
+static const u128 kPcgMult = {0x4385DF649FCCF645ULL, 0x2360ED051FC65DA4ULL};
+
+// A^k and D_k = sum_{j<k} A^j (mod 2^128).
+void jk_pcg_jump_coeffs(uint64_t k, u128 *A, u128 *D)
+{
+    u128 accA = {1, 0}, accD = {0, 0};
+    u128 curA = kPcgMult, curD = {1, 0};
+    const u128 one = {1, 0};
+    while (k) {
+        if (k & 1) {
+            // acc <- cur o acc:  A = curA accA,  D = curA accD + curD
+            accD = h_add(h_mul(curA, accD), curD);
+            accA = h_mul(curA, accA);
+        }
+        // cur <- cur o cur:  D = curA curD + curD = (curA + 1) curD
+        curD = h_mul(h_add(curA, one), curD);
+        curA = h_mul(curA, curA);
+        k >>= 1;
+    }
+    *A = accA;
+    *D = accD;
+}
+
+static u128 h_jump(u128 s, u128 inc, uint64_t k)
+{
+    u128 A, D;
+    jk_pcg_jump_coeffs(k, &A, &D);
+    return h_add(h_mul(A, s), h_mul(D, inc));
+}
+
+hipError_t jk_upload_pcg_table(int device)
+{
+    PcgJump tab;
+    for (int b = 0; b < 64; ++b) jk_pcg_jump_coeffs(1ull << b, &tab.A[b], &tab.D[b]);
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_jump), &tab, sizeof(tab), 0, hipMemcpyHostToDevice);
+}
+
+hipError_t jk_launch_uniform(u128 state, u128 inc, uint64_t offset, int64_t n, double *u,
+                             hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    const int grid = jk_rng_grid(n);
+    u128 AT, DT;
+    jk_pcg_jump_coeffs((uint64_t)grid * 256, &AT, &DT);
+    const u128 s_base = h_jump(state, inc, offset + 1);
+    hipLaunchKernelGGL(k_uniform, dim3(grid), dim3(256), 0, stream, n, s_base, inc, AT, DT, u);
+    return hipGetLastError();
+}
+
+hipError_t jk_launch_reject(const double *ll, int64_t n, const double *llmax, u128 state, u128 inc,
+                            uint64_t offset, int64_t max_keep, int64_t *idx_out,
+                            int64_t *n_pass_out, uint64_t *mask, int64_t *bsum,
+                            hipStream_t stream)
+{
+    const int grid = jk_rng_grid(n);
+    u128 AT, DT;
+    jk_pcg_jump_coeffs((uint64_t)grid * 256, &AT, &DT);
+    const u128 s_base = h_jump(state, inc, offset + 1);
+    const int64_t nwords = jk_reject_mask_words(n);
+    const int64_t nb = jk_reject_bsum_count(n);
+    hipLaunchKernelGGL(k_reject_flags, dim3(grid), dim3(256), 0, stream, ll, n, llmax, s_base, inc,
+                       AT, DT, mask);
+    hipLaunchKernelGGL(k_reject_bsum, dim3((unsigned)nb), dim3(256), 0, stream, mask, nwords, bsum);
+    hipLaunchKernelGGL(k_reject_compact, dim3((unsigned)nb), dim3(256), 0, stream, mask, nwords, bsum,
+                       (int)nb, max_keep, idx_out, n_pass_out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Linear-parameter posterior for a few rows (accepted samples): FP64, one
+// lane per row, runtime L <= JK_LMAX.  Lambda[0] is not clipped here
+// (fast_likelihood.pyx:519-521).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_linear_posterior(const StarDev st, int64_t n,
+                                                         const double *__restrict__ P,
+                                                         const double *__restrict__ E_,
+                                                         const double *__restrict__ OM,
+                                                         const double *__restrict__ MM,
+                                                         const double *__restrict__ S,
+                                                         const int64_t *__restrict__ rows,
+                                                         double *__restrict__ a_out,
+                                                         double *__restrict__ cov_out,
+                                                         double *__restrict__ ll_out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t r = rows ? rows[i] : i;
+    const int L = st.L, N = st.n;
+    const double Pv = P[r], ev = E_[r], omv = OM[r], M0v = MM[r];
+    const double sv = st.jitter_mode == JK_JITTER_PHYSICAL ? S[r] : 0.0;
+    double G[JK_LMAX * JK_LMAX], rhs[JK_LMAX], u[JK_LMAX];
+    for (int q = 0; q < JK_LMAX * JK_LMAX; ++q) G[q] = 0.0;
+    for (int q = 0; q < JK_LMAX; ++q) { rhs[q] = 0.0; u[q] = 0.0; }
+
+    const bool valid = isfinite(Pv) && Pv != 0.0 && ev >= 0.0 && ev < 1.0 && isfinite(omv) &&
+                       isfinite(M0v) && isfinite(sv);
+    if (!valid) {
+        for (int q = 0; q < L; ++q) a_out[i * L + q] = __builtin_nan("");
+        for (int q = 0; q < L * L; ++q) cov_out[i * L * L + q] = __builtin_nan("");
+        ll_out[i] = __builtin_nan("");
+        return;
+    }
+    const double lam0 = st.k_prior_mode == JK_KPRIOR_FIXED_COMPANION_MASS
+                            ? jk_k_variance(&st, Pv, ev, false)
+                            : st.lam0;
+    double sw, cw;
+    sincos(omv, &sw, &cw);
+    const double ecw = ev * cw, sq = sqrt((1.0 - ev) * (1.0 + ev));
+    const double nP = JK_TWO_PI / Pv;
+    double chi0 = 0.0, sum_ln_w = 0.0;
+    double x[JK_LMAX];
+    for (int k = 0; k < N; ++k) {
+        double sE, cE;
+        jk_kepler_f64(jk_reduce_2pi(fma(nP, st.dt[k], -M0v)), ev, sE, cE);
+        x[0] = jk_design_f64(sE, cE, ev, sq, cw, sw, ecw);
+        for (int j = 1; j < L; ++j) x[j] = st.T[(j - 1) * N + k];
+        const double iv = st.ivar[k];
+        const double w = st.jitter_mode == JK_JITTER_PHYSICAL ? iv / fma(sv * sv, iv, 1.0) : iv;
+        double rr = st.rp[k] - st.mu0 * x[0];
+        chi0 = fma(w * rr, rr, chi0);
+        sum_ln_w += log(w);
+        for (int a = 0; a < L; ++a) {
+            const double wx = w * x[a];
+            rhs[a] = fma(wx, st.y[k], rhs[a]);
+            u[a] = fma(wx, rr, u[a]);
+            for (int b = 0; b <= a; ++b) G[a * JK_LMAX + b] = fma(wx, x[b], G[a * JK_LMAX + b]);
+        }
+    }
+    double lnlam = log(lam0) + st.ln_lam_trend;
+    G[0] += 1.0 / lam0;
+    rhs[0] += st.mu0 / lam0;
+    for (int a = 1; a < L; ++a) {
+        G[a * JK_LMAX + a] += st.lam_inv[a];
+        rhs[a] += st.mu[a] * st.lam_inv[a];
+    }
+    // Cholesky G = R R^T (lower, in place).
+    double lndet = 0.0;
+    bool ok = true;
+    for (int j = 0; j < L && ok; ++j) {
+        double d = G[j * JK_LMAX + j];
+        for (int k = 0; k < j; ++k) d -= G[j * JK_LMAX + k] * G[j * JK_LMAX + k];
+        if (!(d > 0.0)) { ok = false; break; }
+        d = sqrt(d);
+        G[j * JK_LMAX + j] = d;
+        lndet += 2.0 * log(d);
+        for (int q = j + 1; q < L; ++q) {
+            double v = G[q * JK_LMAX + j];
+            for (int k = 0; k < j; ++k) v -= G[q * JK_LMAX + k] * G[j * JK_LMAX + k];
+            G[q * JK_LMAX + j] = v / d;
+        }
+    }
+    if (!ok) {
+        for (int q = 0; q < L; ++q) a_out[i * L + q] = __builtin_nan("");
+        for (int q = 0; q < L * L; ++q) cov_out[i * L * L + q] = __builtin_nan("");
+        ll_out[i] = INFINITY;
+        return;
+    }
+    // Rinv (lower) by forward substitution, then cov = Rinv^T Rinv.
+    double Ri[JK_LMAX * JK_LMAX];
+    for (int q = 0; q < JK_LMAX * JK_LMAX; ++q) Ri[q] = 0.0;
+    for (int c = 0; c < L; ++c) {
+        for (int a = c; a < L; ++a) {
+            double v = a == c ? 1.0 : 0.0;
+            for (int k = c; k < a; ++k) v -= G[a * JK_LMAX + k] * Ri[k * JK_LMAX + c];
+            Ri[a * JK_LMAX + c] = v / G[a * JK_LMAX + a];
+        }
+    }
+    for (int a = 0; a < L; ++a)
+        for (int b = 0; b < L; ++b) {
+            double v = 0.0;
+            for (int k = (a > b ? a : b); k < L; ++k) v += Ri[k * JK_LMAX + a] * Ri[k * JK_LMAX + b];
+            cov_out[i * L * L + a * L + b] = v;
+        }
+    // z = R^-1 u for chi2; a = cov rhs.
+    double zz = 0.0;
+    for (int a = 0; a < L; ++a) {
+        double v = 0.0;
+        for (int k = 0; k <= a; ++k) v += Ri[a * JK_LMAX + k] * u[k];
+        zz += v * v;
+    }
+    for (int a = 0; a < L; ++a) {
+        double v = 0.0;
+        for (int b = 0; b < L; ++b) v += cov_out[i * L * L + a * L + b] * rhs[b];
+        a_out[i * L + a] = v;
+    }
+    ll_out[i] = -0.5 * (chi0 - zz + (double)N * JK_LN_TWO_PI - sum_ln_w + lnlam + lndet);
+}
+
+hipError_t jk_launch_linear_posterior(const jk_star *st, int64_t n, const double *P,
+                                      const double *e, const double *om, const double *M0,
+                                      const double *s, const int64_t *rows, double *a_out,
+                                      double *cov_out, double *ll_out, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = (n + 63) / 64;
+    hipLaunchKernelGGL(k_linear_posterior, dim3((unsigned)blocks), dim3(64), 0, stream, st->dev, n,
+                       P, e, om, M0, s, rows, a_out, cov_out, ll_out);
+    return hipGetLastError();
+}

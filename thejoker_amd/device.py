@@ -1,0 +1,365 @@
+"""Device-side objects of the likelihood path.
+
+* :class:`DeviceStar` -- the per-star constants of CJokerHelper.__init__
+  (fast_likelihood.pyx:125-253) held on the GPU (a ``jk_star``).
+* :class:`PriorCache` -- the prior samples as device-resident
+  structure-of-arrays columns: an FP64 master (P, e, omega, M0, s) and an
+  FP32 copy of e, omega, M0, s for the mixed-precision kernel.  Replaces the
+  reference's HDF5 temp-file transport (utils.py:248-302,
+  multiproc_helpers.py:63-120).
+* :func:`lnl`, :func:`reject`, :func:`linear_posterior` -- thin wrappers of
+  the C ABI on torch's current HIP stream.
+
+torch is used for device memory and streams only; every arithmetic step runs
+in libjoker's HIP kernels.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from . import units as u
+from .samples import NONLINEAR_INTERNAL_UNITS, NONLINEAR_PACKED_ORDER, JokerSamples
+
+__all__ = ["DeviceStar", "PriorCache", "star_arrays", "lnl", "reject", "linear_posterior", "uniforms",
+           "pcg64_state", "pcg64_advance_state", "JITTER_MODES"]
+
+JITTER_MODES = {"reference": _lib.JK_JITTER_REFERENCE, "physical": _lib.JK_JITTER_PHYSICAL}
+
+_MASK64 = (1 << 64) - 1
+_MASK128 = (1 << 128) - 1
+_PCG_MULT = 0x2360ED051FC65DA44385DF649FCCF645
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise _lib.JokerLibError("no HIP device visible to torch: the likelihood path runs only "
+                                 "on a gfx950 GPU (there is no CPU fallback)")
+    return torch
+
+
+def _dptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def _hp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _stream(device):
+    torch = _torch()
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+# ---------------------------------------------------------------------------
+# numpy PCG64 state handling (the bit generator of np.random.default_rng).
+# ---------------------------------------------------------------------------
+def pcg64_state(rng):
+    """(state, inc) of a numpy Generator backed by PCG64."""
+    bg = rng.bit_generator
+    st = bg.state
+    if st.get("bit_generator") != "PCG64":
+        raise TypeError("device rejection sampling reproduces numpy's PCG64 stream; got a "
+                        f"{st.get('bit_generator')} generator (use np.random.default_rng)")
+    return int(st["state"]["state"]), int(st["state"]["inc"])
+
+
+def _lcg_jump(state, inc, k):
+    acc_mult, acc_plus = 1, 0
+    cur_mult, cur_plus = _PCG_MULT, inc
+    while k > 0:
+        if k & 1:
+            acc_mult = (acc_mult * cur_mult) & _MASK128
+            acc_plus = (acc_plus * cur_mult + cur_plus) & _MASK128
+        cur_plus = ((cur_mult + 1) * cur_plus) & _MASK128
+        cur_mult = (cur_mult * cur_mult) & _MASK128
+        k >>= 1
+    return (acc_mult * state + acc_plus) & _MASK128
+
+
+def pcg64_advance_state(rng, n):
+    """Advance `rng` by n 64-bit draws, exactly as n calls of
+    rng.uniform() would (the buffered 32-bit half, if any, is left alone)."""
+    bg = rng.bit_generator
+    st = bg.state
+    state, inc = int(st["state"]["state"]), int(st["state"]["inc"])
+    st["state"]["state"] = _lcg_jump(state, inc, int(n))
+    bg.state = st
+
+
+def _split(x):
+    return (x >> 64) & _MASK64, x & _MASK64
+
+
+# ---------------------------------------------------------------------------
+class DeviceStar:
+    """One star's constants on the device.
+
+    Parameters mirror CJokerHelper.__init__ (fast_likelihood.pyx:125-253):
+    t, rv, ivar (n,), trend (n, L-1), mu and lam (L,) in M_T row order
+    (K, v0, offsets..., v1, ...), t_ref; k_prior_mode/sigma_K0/P0/max_K for
+    the FixedCompanionMass K prior; jitter_mode "physical" or "reference".
+    """
+
+    def __init__(self, t, rv, ivar, trend, mu, lam, t_ref, k_prior_mode=_lib.JK_KPRIOR_NORMAL,
+                 sigma_K0=0.0, P0=1.0, max_K=np.inf, jitter_mode="physical", device=0):
+        self.device = device
+        lib = _lib.ensure_device(device)
+        self._lib = lib
+        self.t = np.ascontiguousarray(t, dtype=np.float64)
+        self.rv = np.ascontiguousarray(rv, dtype=np.float64)
+        self.ivar = np.ascontiguousarray(ivar, dtype=np.float64)
+        n = self.t.size
+        self.mu = np.ascontiguousarray(mu, dtype=np.float64)
+        self.lam = np.ascontiguousarray(lam, dtype=np.float64)
+        L = self.mu.size
+        trend = np.asarray(trend, dtype=np.float64)
+        if trend.shape != (n, L - 1):
+            raise ValueError(f"Invalid design matrix shape: {trend.shape}, expected: {(n, L - 1)}")
+        self.trend = np.ascontiguousarray(trend) if trend.size else np.zeros(1)
+        if jitter_mode not in JITTER_MODES:
+            raise ValueError(f"jitter_mode must be one of {list(JITTER_MODES)}")
+        self.jitter_mode = jitter_mode
+        self.k_prior_mode = int(k_prior_mode)
+        self.t_ref = float(t_ref)
+        self.sigma_K0, self.P0, self.max_K = float(sigma_K0), float(P0), float(max_K)
+        h = ctypes.c_void_p()
+        _lib.check(lib.jk_star_create(n, _hp(self.t), _hp(self.rv), _hp(self.ivar), L,
+                                      _hp(self.trend), _hp(self.mu), _hp(self.lam), self.t_ref,
+                                      self.k_prior_mode, self.sigma_K0, self.P0, self.max_K,
+                                      JITTER_MODES[jitter_mode], ctypes.byref(h)),
+                   "jk_star_create")
+        self._h = h
+
+    @property
+    def n(self):
+        return self.t.size
+
+    @property
+    def L(self):
+        return self.mu.size
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.jk_star_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @classmethod
+    def from_data_prior(cls, data, prior, trend_M, jitter_mode="physical", device=0):
+        """Star constants from RVData + JokerPrior + trend_M (see
+        :func:`star_arrays`)."""
+        a = star_arrays(data, prior, trend_M)
+        return cls(a["t"], a["rv"], a["ivar"], a["trend"], a["mu"], a["lam"], a["t_ref"],
+                   a["k_prior_mode"], a["sigma_K0"], a["P0"], a["max_K"],
+                   jitter_mode=jitter_mode, device=device)
+
+
+def star_arrays(data, prior, trend_M):
+    """The arrays CJokerHelper.__init__ derives from (data, prior, trend_M)
+    (fast_likelihood.pyx:134-252): times, RVs and 1/sigma^2 in the data's RV
+    unit, and the prior mean / variance of the linear parameters in M_T row
+    order -- K at 0, v0 at 1, the v0 offsets at 2.., v_i (i >= 1) after them
+    -- in internal units.  For a FixedCompanionMass K prior, sigma_K0, P0
+    [day] and max_K are returned instead of a fixed Lambda[0]."""
+    from .distributions import FixedCompanionMass
+
+    rv_unit = data.rv.unit
+    n_off = prior.n_offsets
+    L = 1 + prior.poly_trend + n_off
+    mu = np.zeros(L)
+    lam = np.zeros(L)
+    internal = linear_internal_units(prior, rv_unit)
+    Kp = prior.pars["K"]
+    kmode, sK0, P0, maxK = _lib.JK_KPRIOR_NORMAL, 0.0, 1.0, np.inf
+    if isinstance(Kp, FixedCompanionMass):
+        kmode = _lib.JK_KPRIOR_FIXED_COMPANION_MASS
+        sK0 = Kp.sigma_K0.to_value(rv_unit)
+        P0 = Kp.P0.to_value(u.day)
+        maxK = Kp.max_K.to_value(rv_unit)
+        mu[0] = Kp.mu * Kp.unit.to(rv_unit)
+        lam[0] = 1.0
+    else:
+        m0, s0 = Kp.mean_std(rv_unit)
+        mu[0], lam[0] = m0, s0 ** 2
+    for i, name in enumerate(prior._v_trend_names):
+        m_, s_ = prior.pars[name].mean_std(internal[name])
+        j = 1 if i == 0 else 1 + n_off + i
+        mu[j], lam[j] = m_, s_ ** 2
+    for i in range(n_off):
+        m_, s_ = prior.pars[f"dv0_{i + 1}"].mean_std(rv_unit)
+        mu[2 + i], lam[2 + i] = m_, s_ ** 2
+    return dict(t=np.asarray(data._t_bmjd, dtype=np.float64),
+                rv=np.asarray(data.rv.value, dtype=np.float64), ivar=data.ivar_value(),
+                trend=np.asarray(trend_M, dtype=np.float64), mu=mu, lam=lam,
+                t_ref=float(data._t_ref_bmjd), k_prior_mode=kmode, sigma_K0=sK0, P0=P0,
+                max_K=maxK)
+
+
+def linear_internal_units(prior, rv_unit):
+    """The internal units of CJokerHelper (fast_likelihood.pyx:134-148), in
+    packed order: P, e, omega, M0, s, K, v0, dv0_1..., v1, v2, ..."""
+    units = dict(NONLINEAR_INTERNAL_UNITS)
+    units["s"] = rv_unit
+    units["K"] = rv_unit
+    units["v0"] = rv_unit
+    for i in range(prior.n_offsets):
+        units[f"dv0_{i + 1}"] = rv_unit
+    for i, name in enumerate(prior._v_trend_names):
+        units[name] = rv_unit / u.day ** i
+    return units
+
+
+# ---------------------------------------------------------------------------
+class PriorCache:
+    """Prior samples resident in HBM as SoA columns.
+
+    FP64 master: P [day], e, omega [rad], M0 [rad], s [rv unit] (40 B/row);
+    FP32 copy of e, omega, M0, s (16 B/row) for the mixed-precision kernel.
+    `ln_prior` (optional) stays on the host.
+    """
+
+    def __init__(self, P, e, omega, M0, s=None, ln_prior=None, device=0, fp32_copy=True):
+        torch = _torch()
+        _lib.ensure_device(device)
+        self.device = device
+        dev = torch.device("cuda", device)
+        n = int(np.size(P))
+        cols = [P, e, omega, M0, np.zeros(n) if s is None else s]
+        self.P, self.e, self.omega, self.M0, self.s = [
+            torch.as_tensor(np.ascontiguousarray(c, dtype=np.float64)).to(dev) for c in cols]
+        self.fp32 = None
+        if fp32_copy:
+            self.fp32 = tuple(c.to(torch.float32) for c in (self.e, self.omega, self.M0, self.s))
+        self.ln_prior = None if ln_prior is None else np.asarray(ln_prior, dtype=np.float64)
+
+    def __len__(self):
+        return int(self.P.shape[0])
+
+    @classmethod
+    def from_packed(cls, packed, ln_prior=None, device=0, fp32_copy=True):
+        packed = np.asarray(packed, dtype=np.float64)
+        return cls(packed[:, 0], packed[:, 1], packed[:, 2], packed[:, 3], packed[:, 4],
+                   ln_prior=ln_prior, device=device, fp32_copy=fp32_copy)
+
+    @classmethod
+    def from_samples(cls, samples, rv_unit=u.km / u.s, device=0, fp32_copy=True):
+        units = dict(NONLINEAR_INTERNAL_UNITS)
+        units["s"] = rv_unit
+        packed, _ = samples.pack(units=units, names=NONLINEAR_PACKED_ORDER)
+        lnp = np.asarray(samples["ln_prior"].value) if "ln_prior" in samples else None
+        return cls.from_packed(packed, ln_prior=lnp, device=device, fp32_copy=fp32_copy)
+
+    def host_rows(self, idx):
+        """(len(idx), 5) float64 rows P, e, omega, M0, s copied to the host."""
+        torch = _torch()
+        it = torch.as_tensor(np.asarray(idx, dtype=np.int64), device=self.P.device)
+        cols = [c.index_select(0, it) for c in (self.P, self.e, self.omega, self.M0, self.s)]
+        return torch.stack(cols, dim=1).cpu().numpy()
+
+    def nbytes(self):
+        b = 8 * 5 * len(self)
+        if self.fp32 is not None:
+            b += 4 * 4 * len(self)
+        return b
+
+
+# ---------------------------------------------------------------------------
+def lnl(star, cache, rows=None, n=None, precision="f64", out=None, start=0):
+    """Marginal lnL of cache rows (all, `rows` (device int64 tensor) or
+    [start, start + n)) into a device float64 tensor."""
+    torch = _torch()
+    lib = _lib.ensure_device(cache.device)
+    if rows is not None:
+        n = int(rows.shape[0])
+    elif n is None:
+        n = len(cache) - start
+    if out is None:
+        out = torch.empty(n, dtype=torch.float64, device=cache.P.device)
+    if n == 0:
+        return out
+    off64 = 8 * start
+    off32 = 4 * start
+    if rows is not None:
+        off64 = off32 = 0
+    P = ctypes.c_void_p(cache.P.data_ptr() + off64)
+    if precision == "f64":
+        cols = [ctypes.c_void_p(c.data_ptr() + off64) for c in (cache.e, cache.omega, cache.M0, cache.s)]
+        fn, what = lib.jk_lnl_f64, "jk_lnl_f64"
+    elif precision == "mixed":
+        if cache.fp32 is None:
+            raise ValueError("this PriorCache has no FP32 copy")
+        cols = [ctypes.c_void_p(c.data_ptr() + off32) for c in cache.fp32]
+        fn, what = lib.jk_lnl_mixed, "jk_lnl_mixed"
+    else:
+        raise ValueError("precision must be 'f64' or 'mixed'")
+    _lib.check(fn(star.handle, n, P, *cols, _dptr(rows), _dptr(out), _stream(cache.device)), what)
+    return out
+
+
+def uniforms(rng_state, n, offset=0, device=0):
+    """numpy-identical uniforms of a PCG64 state (tests and callers)."""
+    torch = _torch()
+    lib = _lib.ensure_device(device)
+    state, inc = rng_state
+    out = torch.empty(int(n), dtype=torch.float64, device=torch.device("cuda", device))
+    _lib.check(lib.jk_pcg64_uniform(*_split(state), *_split(inc), int(offset), int(n), _dptr(out),
+                                    _stream(device)), "jk_pcg64_uniform")
+    return out
+
+
+def reject(ll, rng_state, max_keep, offset=0, llmax=None, device=0):
+    """Rejection step on a device lnL tensor.  Returns (idx (np.int64, ascending,
+    <= max_keep), n_pass, llmax)."""
+    torch = _torch()
+    lib = _lib.ensure_device(device)
+    n = int(ll.shape[0])
+    if n == 0:
+        raise ValueError("zero-size array to reduction operation maximum which has no identity")
+    dev = ll.device
+    max_keep = int(min(max_keep, n))
+    idx = torch.empty(max(max_keep, 1), dtype=torch.int64, device=dev)
+    npass = torch.zeros(1, dtype=torch.int64, device=dev)
+    state, inc = rng_state
+    if llmax is None:
+        mx = torch.empty(1, dtype=torch.float64, device=dev)
+        _lib.check(lib.jk_reject(_dptr(ll), n, *_split(state), *_split(inc), int(offset), max_keep,
+                                 _dptr(idx), _dptr(npass), _dptr(mx), _stream(device)), "jk_reject")
+    else:
+        mx = llmax
+        _lib.check(lib.jk_reject_with_max(_dptr(ll), n, _dptr(mx), *_split(state), *_split(inc),
+                                          int(offset), max_keep, _dptr(idx), _dptr(npass),
+                                          _stream(device)), "jk_reject_with_max")
+    n_pass = int(npass.item())
+    k = min(n_pass, max_keep)
+    return idx[:k].cpu().numpy(), n_pass, float(mx.item())
+
+
+def linear_posterior(star, cache, rows):
+    """(a (k, L), cov (k, L, L), ll (k,)) for cache rows `rows` (host ints)."""
+    torch = _torch()
+    lib = _lib.ensure_device(cache.device)
+    k = len(rows)
+    L = star.L
+    dev = cache.P.device
+    if k == 0:
+        return np.zeros((0, L)), np.zeros((0, L, L)), np.zeros(0)
+    r = torch.as_tensor(np.asarray(rows, dtype=np.int64), device=dev)
+    a = torch.empty((k, L), dtype=torch.float64, device=dev)
+    cov = torch.empty((k, L, L), dtype=torch.float64, device=dev)
+    ll = torch.empty(k, dtype=torch.float64, device=dev)
+    _lib.check(lib.jk_linear_posterior(star.handle, k, _dptr(cache.P), _dptr(cache.e),
+                                       _dptr(cache.omega), _dptr(cache.M0), _dptr(cache.s),
+                                       _dptr(r), _dptr(a), _dptr(cov), _dptr(ll),
+                                       _stream(cache.device)), "jk_linear_posterior")
+    return a.cpu().numpy(), cov.cpu().numpy(), ll.cpu().numpy()

@@ -1,0 +1,357 @@
+"""TheJoker: the drop-in sampler API (thejoker/thejoker.py:23-370).
+
+Same methods, signatures and defaults as the reference --
+``marginal_ln_likelihood``, ``rejection_sample``,
+``iterative_rejection_sample`` -- with the per-sample likelihood, the
+rejection step and the linear-parameter posterior running in libjoker's gfx950
+kernels on device-resident prior samples.  The numpy random stream is
+consumed exactly as the reference consumes it:
+
+* rejection uniforms: ``rng.uniform(size=n)`` (likelihood_helpers.py:107,
+  multiproc_helpers.py:256/373) -- generated on the device from the
+  generator's PCG64 state, after which the host generator is advanced by n;
+* ``randomize_prior_order``: ``rng.choice`` on the host first
+  (multiproc_helpers.py:247, 352);
+* linear parameters: ``rng.multivariate_normal`` per accepted row on the host
+  (fast_likelihood.pyx:529-530), from ``rng`` itself on the in-memory path
+  (likelihood_helpers.py:69-88) or from per-batch children of
+  ``rng.bit_generator._seed_seq.spawn`` on the default path
+  (multiproc_helpers.py:49-54, 150-182).
+
+``in_memory`` keeps its meaning for the iteration rule: the in-memory path
+grows batches with safety_factor 1 (likelihood_helpers.py:145), the default
+path with 4 (multiproc_helpers.py:334).  The HDF5 temp-file and process-pool
+transport of the default path is replaced by the device-resident cache.
+"""
+import logging
+import os
+
+import numpy as np
+
+from . import units as u
+from .data import RVData
+from .device import (DeviceStar, PriorCache, linear_internal_units, lnl, pcg64_advance_state,
+                     pcg64_state, reject)
+from .likelihood_helpers import batch_tasks, next_batch_size, validate_prepare_data
+from .prior import JokerPrior
+from .samples import NONLINEAR_PACKED_ORDER, JokerSamples
+
+__all__ = ["TheJoker"]
+
+logger = logging.getLogger("thejoker")
+
+
+class TheJoker:
+    """
+    Parameters
+    ----------
+    prior : JokerPrior
+    pool : object with ``map`` and ``close`` (optional)
+        Accepted for API compatibility; computation runs on the GPU.  Its
+        ``size`` sets the default number of batches, which (as in the
+        reference) decides how many child generators draw the linear
+        parameters on the default path.
+    rng : numpy.random.Generator (optional)
+    tempfile_path : str (optional)
+    jitter_mode : {"physical", "reference"}
+        "physical": C = diag(sigma^2 + s^2) (py_likelihood.py:17-29).
+        "reference": the jitter is ignored, as in fast_likelihood.pyx:458.
+        They agree whenever s = 0 (the default prior).
+    precision : {"f64", "mixed"}
+        "f64": the plain FP64 kernel.  "mixed": FP32 Kepler / design column
+        with FP64 phase and accumulation; every row within 40 of the maximum
+        lnL is then re-evaluated in FP64, so the accepted set equals the FP64
+        one.
+    device : int
+        HIP device ordinal.
+    """
+
+    def __init__(self, prior, pool=None, rng=None, tempfile_path=None, jitter_mode="physical",
+                 precision="f64", device=0, random_state=None):
+        if random_state is not None and rng is None:
+            rng = random_state
+        if pool is not None and (not hasattr(pool, "map") or not hasattr(pool, "close")):
+            raise TypeError("Input pool object must have .map() and .close() methods. We "
+                            "recommend using `schwimmbad` pools.")
+        self.pool = pool
+        if rng is None:
+            rng = np.random.default_rng()
+        elif not isinstance(rng, np.random.Generator):
+            raise TypeError("The input random number generator must be a numpy.random.Generator "
+                            "instance.")
+        self.rng = rng
+        if not isinstance(prior, JokerPrior):
+            raise TypeError("The input prior must be a JokerPrior instance.")
+        self.prior = prior
+        if tempfile_path is None:
+            self._tempfile_path = os.path.expanduser("~/.thejoker/")
+        else:
+            self._tempfile_path = os.path.abspath(os.path.expanduser(tempfile_path))
+        if jitter_mode not in ("physical", "reference"):
+            raise ValueError("jitter_mode must be 'physical' or 'reference'")
+        if precision not in ("f64", "mixed"):
+            raise ValueError("precision must be 'f64' or 'mixed'")
+        self.jitter_mode = jitter_mode
+        self.precision = precision
+        self.device = device
+
+    @property
+    def tempfile_path(self):
+        os.makedirs(self._tempfile_path, exist_ok=True)
+        return self._tempfile_path
+
+    # ------------------------------------------------------------------
+    def _make_joker_helper(self, data):
+        """thejoker.py:87-91: validate data, build the star on the device."""
+        all_data, ids, trend_M = validate_prepare_data(data, self.prior.poly_trend,
+                                                       self.prior.n_offsets)
+        star = DeviceStar.from_data_prior(all_data, self.prior, trend_M,
+                                          jitter_mode=self.jitter_mode, device=self.device)
+        star.data = all_data
+        star.internal_units = linear_internal_units(self.prior, all_data.rv.unit)
+        return star
+
+    def _cache(self, star, prior_samples, return_logprobs=False):
+        """Prior samples (JokerSamples, PriorCache or .npz filename) as a
+        device-resident PriorCache."""
+        if isinstance(prior_samples, PriorCache):
+            return prior_samples
+        if isinstance(prior_samples, str):
+            prior_samples = JokerSamples.read(prior_samples)
+        if isinstance(prior_samples, JokerSamples):
+            return PriorCache.from_samples(prior_samples, rv_unit=star.data.rv.unit,
+                                           device=self.device,
+                                           fp32_copy=self.precision == "mixed")
+        if isinstance(prior_samples, np.ndarray):
+            return PriorCache.from_packed(prior_samples, device=self.device,
+                                          fp32_copy=self.precision == "mixed")
+        raise TypeError("prior_samples must either be a string filename specifying a cache file "
+                        "containing prior samples, or must be a JokerSamples instance, not: "
+                        f"{type(prior_samples)}")
+
+    def _lnl_rows(self, star, cache, rows=None, start=0, n=None):
+        """Device lnL at the current precision; mixed runs are re-evaluated in
+        FP64 on every row within 40 of the maximum."""
+        ll = lnl(star, cache, rows=rows, start=start, n=n, precision=self.precision)
+        if self.precision == "mixed":
+            ll = _polish(star, cache, ll, rows=rows, start=start)
+        return ll
+
+    def _n_batches(self, n_batches):
+        if n_batches is not None:
+            return int(n_batches)
+        size = getattr(self.pool, "size", 1) if self.pool is not None else 1
+        return max(1, int(size or 0))
+
+    def _full_samples(self, star, cache, rows, rng, n_linear_samples, spawn_batches=None):
+        """Linear parameters of accepted rows (fast_likelihood.pyx:471-545) and
+        JokerSamples.unpack (samples.py:447-478)."""
+        rows = np.asarray(rows, dtype=np.int64)
+        a, cov, _ = _linear_posterior(star, cache, rows)
+        packed = cache.host_rows(rows) if len(rows) else np.zeros((0, 5))
+        L = star.L
+        n_pars = 5 + L
+        raw = np.zeros((len(rows), n_linear_samples, n_pars))
+        if spawn_batches is None:
+            gens = [(np.arange(len(rows)), rng)]
+        else:
+            tasks = batch_tasks(len(rows), n_batches=spawn_batches, arr=np.arange(len(rows)))
+            from numpy.random import PCG64, Generator
+            sg = rng.bit_generator._seed_seq.spawn(len(tasks))
+            gens = [(tasks[i][0], Generator(PCG64(sg[i]))) for i in range(len(tasks))]
+        for sel, g in gens:
+            for j in np.asarray(sel, dtype=np.int64):
+                lin = g.multivariate_normal(a[j], cov[j], size=n_linear_samples)
+                raw[j, :, :5] = packed[j]
+                raw[j, :, 5:] = lin
+        raw = raw.reshape(len(rows) * n_linear_samples, n_pars)
+        return JokerSamples.unpack(raw, star.internal_units, t_ref=star.data.t_ref,
+                                   poly_trend=self.prior.poly_trend,
+                                   n_offsets=self.prior.n_offsets)
+
+    # ------------------------------------------------------------------
+    def marginal_ln_likelihood(self, data, prior_samples, n_batches=None, in_memory=False):
+        """thejoker.py:93-138: lnL at every prior sample (np.ndarray)."""
+        star = self._make_joker_helper(data)
+        cache = self._cache(star, prior_samples)
+        return self._lnl_rows(star, cache).cpu().numpy()
+
+    def rejection_sample(self, data, prior_samples, n_prior_samples=None,
+                         max_posterior_samples=None, n_linear_samples=1, return_logprobs=False,
+                         return_all_logprobs=False, n_batches=None, randomize_prior_order=False,
+                         in_memory=False):
+        """thejoker.py:140-257."""
+        import torch
+
+        star = self._make_joker_helper(data)
+        if isinstance(prior_samples, (int, np.integer)) and not isinstance(prior_samples, bool):
+            prior_samples = self.prior.sample(size=int(prior_samples),
+                                              return_logprobs=return_logprobs)
+        cache = self._cache(star, prior_samples)
+        n_total = len(cache)
+        rng = self.rng
+
+        if in_memory:
+            n_use = n_total
+            rows = None
+            ln_prior = cache.ln_prior if return_logprobs else None
+        else:
+            if return_logprobs and cache.ln_prior is None:
+                raise RuntimeError("return_logprobs=True but ln_prior values not found in prior "
+                                   "cache: make sure you generate prior samples with prior.sample "
+                                   "(..., return_logprobs=True) before saving the prior samples.")
+            if n_prior_samples is None:
+                n_use = n_total
+            elif n_prior_samples > n_total:
+                raise ValueError("Number of prior samples to use is greater than the number of "
+                                 "prior samples passed, or cached to a filename specified. "
+                                 f"n_prior_samples={n_prior_samples} vs. "
+                                 f"n_total_samples={n_total}")
+            else:
+                n_use = int(n_prior_samples)
+            rows = None
+            if randomize_prior_order:
+                rows = rng.choice(n_total, size=n_use, replace=False)
+            ln_prior = cache.ln_prior
+        if max_posterior_samples is None:
+            max_posterior_samples = n_use
+
+        if rows is not None:
+            d_rows = torch.as_tensor(np.asarray(rows, dtype=np.int64), device=cache.P.device)
+            lls = self._lnl_rows(star, cache, rows=d_rows)
+        else:
+            lls = self._lnl_rows(star, cache, start=0, n=n_use)
+
+        good, _, _ = reject(lls, pcg64_state(rng), max_posterior_samples, device=self.device)
+        pcg64_advance_state(rng, n_use)
+        full_idx = rows[good] if rows is not None else good
+
+        samples = self._full_samples(star, cache, full_idx, rng, n_linear_samples,
+                                     spawn_batches=None if in_memory else self._n_batches(n_batches))
+        lls_h = lls.cpu().numpy()
+        if in_memory:
+            if ln_prior is not None and ln_prior is not False:
+                samples["ln_prior"] = ln_prior[good]
+                samples["ln_likelihood"] = lls_h[good]
+        elif return_logprobs:
+            samples["ln_likelihood"] = lls_h[good]
+            samples["ln_prior"] = cache.ln_prior[full_idx]
+        if return_all_logprobs:
+            return samples, lls_h
+        return samples
+
+    def iterative_rejection_sample(self, data, prior_samples, n_requested_samples,
+                                   max_prior_samples=None, n_linear_samples=1,
+                                   return_logprobs=False, n_batches=None,
+                                   randomize_prior_order=False, init_batch_size=None,
+                                   growth_factor=128, in_memory=False):
+        """thejoker.py:259-370, with the loop of likelihood_helpers.py:130-229
+        (in_memory=True) or multiproc_helpers.py:289-427 (default)."""
+        import torch
+
+        star = self._make_joker_helper(data)
+        cache = self._cache(star, prior_samples)
+        rng = self.rng
+        n_total = len(cache)
+        if in_memory:
+            safety_factor = 1
+            max_prior = n_total
+            all_idx = None
+        else:
+            if return_logprobs and cache.ln_prior is None:
+                raise RuntimeError("return_logprobs=True but ln_prior values not found in prior "
+                                   "cache: make sure you generate prior samples with prior.sample "
+                                   "(..., return_logprobs=True) before saving the prior samples.")
+            safety_factor = 4
+            max_prior = n_total if max_prior_samples is None else int(max_prior_samples)
+            all_idx = None
+            if randomize_prior_order:
+                all_idx = rng.choice(n_total, size=max_prior, replace=False)
+
+        maxiter = 128
+        n_process = growth_factor * n_requested_samples if init_batch_size is None else init_batch_size
+        if n_process > max_prior:
+            raise ValueError("Prior sample library not big enough! For iterative sampling, you "
+                             "have to have at least growth_factor * n_requested_samples = "
+                             f"{growth_factor * n_requested_samples} samples in the prior samples "
+                             f"cache file. You have, or have limited to, {max_prior} samples.")
+
+        dev = cache.P.device
+        all_ll = torch.empty(max_prior, dtype=torch.float64, device=dev)
+        d_all_idx = None
+        if all_idx is not None:
+            d_all_idx = torch.as_tensor(np.asarray(all_idx, dtype=np.int64), device=dev)
+        start_idx = 0
+        n_evals = 0
+        good = None
+        for i in range(maxiter):
+            logger.log(1, f"iteration {i}, computing {n_process} likelihoods")
+            n_now = min(n_process, max_prior - start_idx)
+            if d_all_idx is not None:
+                part = self._lnl_rows(star, cache, rows=d_all_idx[start_idx:start_idx + n_now])
+            else:
+                part = self._lnl_rows(star, cache, start=start_idx, n=n_now)
+            all_ll[n_evals:n_evals + n_now] = part
+            n_evals += n_now
+            view = all_ll[:n_evals]
+            if in_memory and not bool(torch.isfinite(view).all()):
+                raise RuntimeError(f"There are NaN or Inf likelihood values in iteration step {i}!")
+            good, n_good, _ = reject(view, pcg64_state(rng), n_requested_samples,
+                                    device=self.device)
+            pcg64_advance_state(rng, n_evals)
+            if n_good == 0:
+                raise RuntimeError("Failed to find any good samples!")
+            logger.log(1, f"{n_good} good samples after rejection sampling")
+            if n_good >= n_requested_samples:
+                logger.log(1, "Enough samples found!")
+                break
+            start_idx += n_process
+            n_process = next_batch_size(n_requested_samples, n_good, n_evals, start_idx,
+                                        max_prior, safety_factor)
+            if n_process <= 0:
+                break
+        else:
+            raise RuntimeError("Hit maximum number of iterations!")
+
+        good = good[:n_requested_samples]
+        full_idx = all_idx[good] if all_idx is not None else good
+        samples = self._full_samples(star, cache, full_idx, rng, n_linear_samples,
+                                     spawn_batches=None if in_memory else self._n_batches(n_batches))
+        if return_logprobs:
+            ll_h = all_ll[:n_evals].cpu().numpy()
+            if in_memory:
+                if cache.ln_prior is not None:
+                    samples["ln_prior"] = cache.ln_prior[full_idx]
+                    samples["ln_likelihood"] = ll_h[good]
+            else:
+                samples["ln_likelihood"] = ll_h[good]
+                samples["ln_prior"] = cache.ln_prior[full_idx]
+        return samples
+
+
+def _linear_posterior(star, cache, rows):
+    from .device import linear_posterior
+    return linear_posterior(star, cache, rows)
+
+
+# Re-evaluation window of the mixed path: rows whose mixed lnL is within this
+# of the mixed maximum are recomputed in FP64.  exp(-40) = 4e-18 is below the
+# smallest nonzero uniform (2^-53 = 1.1e-16), and the mixed-vs-FP64 error is
+# orders of magnitude below the margin.
+POLISH_WINDOW = 40.0
+
+
+def _polish(star, cache, ll, rows=None, start=0):
+    import torch
+
+    mx = torch.max(ll)
+    if not bool(torch.isfinite(mx)):
+        # NaN or +inf anywhere: recompute everything in FP64 so the maximum
+        # and every comparison are FP64's.
+        return lnl(star, cache, rows=rows, start=start, n=int(ll.shape[0]), precision="f64")
+    live = torch.nonzero(ll >= mx - POLISH_WINDOW).flatten()
+    if live.numel() == 0:
+        return ll
+    src = live + start if rows is None else rows.index_select(0, live)
+    ll[live] = lnl(star, cache, rows=src, precision="f64")
+    return ll
