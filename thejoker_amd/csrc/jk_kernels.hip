@@ -138,6 +138,103 @@ __device__ __forceinline__ void jk_kepler_f32(float x, float e, float &sinE, flo
 }
 
 // ---------------------------------------------------------------------------
+// Non-iterative Kepler solve (Markley 1995, "Kepler equation solver",
+// Celest. Mech. Dyn. Astron. 63, 101): a Pade-based cubic starter accurate to
+// 4.4e-4 rad for every e in [0, 1), followed by one fifth-order correction
+//   d3 = -f0 / (f1 - f0 f2 / (2 f1))
+//   d4 = -f0 / (f1 + d3 f2 / 2 + d3^2 f3 / 6)
+//   d5 = -f0 / (f1 + d4 f2 / 2 + d4^2 f3 / 6 + d4^3 f4 / 24)
+// with f0 = E1 - e sin E1 - M, f1 = 1 - e cos E1, f2 = e sin E1,
+// f3 = e cos E1, f4 = -e sin E1.  From a 4.4e-4 start the correction leaves
+// an error below the conditioning of the problem (ulp(M) / (1 - e cos E)),
+// i.e. the same answer a converged Newton iteration gives.  The starter needs
+// only ~1e-4 accuracy, so it runs in FP32 with the hardware sqrt / log2 /
+// exp2; (sin E, cos E) at E = E1 + d5 come from rotating (sin E1, cos E1) by
+// d5 with fourth-order series (|d5| <= 4.4e-4, truncation < 2e-19).
+// M is taken in [0, pi].  Per-sample constants: ome = 1 - e and
+// a1 = 1.6 pi / ((1 + e) (pi^2 - 6)).
+// ---------------------------------------------------------------------------
+#define JK_MARKLEY_A0 7.6516382901912925f   // 3 pi^2 / (pi^2 - 6)
+#define JK_MARKLEY_A1N 1.29898246041084f    // 1.6 pi / (pi^2 - 6)
+
+__device__ __forceinline__ float jk_markley_starter(float M, float e, float ome, float a1)
+{
+    const float alpha = fmaf(a1, JK_PI_F_HI - M, JK_MARKLEY_A0);
+    const float d = fmaf(alpha, e, 3.0f * ome);
+    const float M2 = M * M;
+    const float ad = alpha * d;
+    const float q = fmaf(2.0f * ad, ome, -M2);
+    const float r = M * fmaf(3.0f * ad, d - ome, M2);
+    const float q2 = q * q;
+    const float x = fabsf(r) + __builtin_amdgcn_sqrtf(fmaxf(fmaf(q2, q, r * r), 0.0f));
+    const float w = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * 0.6666666667f);
+    const float den = fmaf(w, w + q, q2);
+    return fmaf(2.0f * r * w, __builtin_amdgcn_rcpf(den), M) * __builtin_amdgcn_rcpf(d);
+}
+
+// 1/x in FP64 from the hardware reciprocal refined by two Newton steps.
+__device__ __forceinline__ double jk_rcp2(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+
+__device__ __forceinline__ void jk_kepler_markley_f64(double x, double e, float ef, float omef,
+                                                      float a1f, double &sinE, double &cosE)
+{
+    const double ax = fabs(x);
+    const double E1 = (double)jk_markley_starter((float)ax, ef, omef, a1f);
+    double s1, c1;
+    jk_sincos_0pi_f64(E1, s1, c1);
+    const double es = e * s1, ec = e * c1;
+    const double f0 = (E1 - ax) - es;
+    const double f1 = 1.0 - ec;
+    const double rf1 = __builtin_amdgcn_rcp(f1);
+    const double d3 = -f0 * __builtin_amdgcn_rcp(fma(-0.5 * f0 * es, rf1, f1));
+    const double d4 = -f0 * __builtin_amdgcn_rcp(fma(d3, fma(d3, ec * (1.0 / 6.0), 0.5 * es), f1));
+    const double den5 = fma(d4, fma(d4, fma(d4, -es * (1.0 / 24.0), ec * (1.0 / 6.0)), 0.5 * es), f1);
+    const double d5 = -f0 * jk_rcp2(den5);
+    double s, c;
+    if (fabs(d5) < 1e-3) {
+        const double d2 = d5 * d5;
+        const double cd = fma(d2, fma(d2, 1.0 / 24.0, -0.5), 1.0);
+        const double sd = d5 * fma(d2, -1.0 / 6.0, 1.0);
+        s = fma(s1, cd, c1 * sd);
+        c = fma(c1, cd, -s1 * sd);
+    } else {
+        // Outside the starter's error bound (not reached for e in [0, 1)):
+        // fall back to the bracketed Newton iteration.
+        jk_kepler_poly_f64(ax, e, s, c);
+    }
+    sinE = x < 0.0 ? -s : s;
+    cosE = c;
+}
+
+__device__ __forceinline__ void jk_kepler_markley_f32(float x, float e, float ome, float a1,
+                                                      float &sinE, float &cosE)
+{
+    const float ax = fabsf(x);
+    const float E1 = jk_markley_starter(ax, e, ome, a1);
+    float s1, c1;
+    jk_sincos_0pi_f32(E1, s1, c1);
+    const float es = e * s1, ec = e * c1;
+    const float f0 = (E1 - ax) - es;
+    const float f1 = 1.0f - ec;
+    const float rf1 = __builtin_amdgcn_rcpf(f1);
+    const float d3 = -f0 * __builtin_amdgcn_rcpf(fmaf(-0.5f * f0 * es, rf1, f1));
+    const float d4 = -f0 * __builtin_amdgcn_rcpf(fmaf(d3, fmaf(d3, ec * (1.0f / 6.0f), 0.5f * es), f1));
+    const float d5 = -f0 * __builtin_amdgcn_rcpf(
+                               fmaf(d4, fmaf(d4, fmaf(d4, -es * (1.0f / 24.0f), ec * (1.0f / 6.0f)), 0.5f * es), f1));
+    const float d2 = d5 * d5;
+    const float cd = fmaf(d2, -0.5f, 1.0f);
+    const float sd = d5 * fmaf(d2, -1.0f / 6.0f, 1.0f);
+    const float s = fmaf(s1, cd, c1 * sd);
+    cosE = fmaf(c1, cd, -s1 * sd);
+    sinE = x < 0.0f ? -s : s;
+}
+
+// ---------------------------------------------------------------------------
 // Fused marginal-likelihood kernel.
 // ---------------------------------------------------------------------------
 template <int L, bool PHYS, bool MIXED>
@@ -185,12 +282,15 @@ __global__ __launch_bounds__(256) void k_lnl(const StarDev st, int64_t n,
 
     // Per-sample orbit constants.
     double cw, sw, ecw, sq;
-    float cwf, swf, ecwf, sqf, ef;
+    float cwf, swf, ecwf, sqf;
+    // FP32 copies for the Kepler starter (e capped below 1 in FP32).
+    const float ef = fminf((float)ev, 0.99999994f);
+    const float omef = 1.0f - ef;
+    const float a1f = JK_MARKLEY_A1N / (1.0f + ef);
     if constexpr (MIXED) {
-        ef = (float)ev;
         sincosf((float)omv, &swf, &cwf);
         ecwf = ef * cwf;
-        sqf = sqrtf((1.0f - ef) * (1.0f + ef));
+        sqf = sqrtf(omef * (1.0f + ef));
     } else {
         sincos(omv, &sw, &cw);
         ecw = ev * cw;
@@ -219,22 +319,23 @@ __global__ __launch_bounds__(256) void k_lnl(const StarDev st, int64_t n,
         double m;
         if constexpr (MIXED) {
             float sE, cE;
-            jk_kepler_f32((float)x, ef, sE, cE);
+            jk_kepler_markley_f32((float)x, ef, omef, a1f, sE, cE);
             const float den = __builtin_amdgcn_rcpf(fmaf(-ef, cE, 1.0f));
             const float cf = (cE - ef) * den;
             const float sf = sqf * sE * den;
             m = (double)fmaf(cwf, cf, fmaf(-swf, sf, ecwf));
         } else {
             double sE, cE;
-            jk_kepler_poly_f64(x, ev, sE, cE);
-            m = jk_design_f64(sE, cE, ev, sq, cw, sw, ecw);
+            jk_kepler_markley_f64(x, ev, ef, omef, a1f, sE, cE);
+            const double den = jk_rcp2(fma(-ev, cE, 1.0));
+            m = fma(cw, (cE - ev) * den, fma(-sw, sq * sE * den, ecw));
         }
         const double iv = st.ivar[k];
         const double rp = st.rp[k];
         double w;
         if constexpr (PHYS) {
             const double d = fma(s2, iv, 1.0);
-            w = iv / d;
+            w = iv * jk_rcp2(d);
             int ex;
             prod = frexp(prod * d, &ex);
             pexp += ex;

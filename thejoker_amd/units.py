@@ -24,6 +24,8 @@ class Unit:
     """scale x (m^a s^b rad^c); `dims` = (a, b, c)."""
 
     __slots__ = ("scale", "dims", "name")
+    # numpy defers binary operators to the Unit (array * u.km -> Quantity).
+    __array_ufunc__ = None
 
     def __init__(self, scale, dims, name=None):
         self.scale = float(scale)
@@ -106,6 +108,7 @@ class Quantity:
     """A value (scalar or numpy array) with a Unit."""
 
     __array_priority__ = 1000
+    __array_ufunc__ = None
 
     def __init__(self, value, unit=None):
         if isinstance(value, Quantity):
