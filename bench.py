@@ -77,6 +77,9 @@ def main():
     ap.add_argument("--log2-rows", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (the "
+                         "reference dispatch for scaling FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the likelihood path here runs on a single MI355X "
@@ -89,6 +92,16 @@ def main():
 
     from thejoker_amd import device as D
     from thejoker_amd import synthetic as syn
+
+    if args.copy_calibration:
+        nbytes = args.copy_calibration << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": nbytes, "dispatches": 3}), flush=True)
+        return
 
     n = 1 << args.log2_rows
     log(f"[bench] building workload: M=16 star, {n} prior rows")

@@ -105,6 +105,19 @@ int jk_lnl_mixed(const jk_star *star, int64_t n, const double *P, const float *e
                  const float *omega, const float *M0, const float *s,
                  const int64_t *rows, double *ll_out, void *stream);
 
+/* FP64 re-evaluation of the live set after a mixed-precision pass: every
+ * entry of ll_inout (n rows, as in jk_lnl_mixed) that is not below
+ * max(ll) - window -- all of them when the maximum is NaN or infinite -- is
+ * recomputed with the FP64 kernel in place; *n_polished_out (device int64)
+ * receives how many.  With window >= 40, exp(lnL - max) of every row left in
+ * mixed precision is below the smallest nonzero uniform (2^-53), so the
+ * rejection step's accepted set equals the plain FP64 path's.  No reference
+ * counterpart (the reference has one precision). */
+int jk_polish_f64(const jk_star *star, int64_t n, const double *P, const double *e,
+                  const double *omega, const double *M0, const double *s,
+                  const int64_t *rows, double *ll_inout, double window,
+                  int64_t *n_polished_out, void *stream);
+
 /* NaN-propagating maximum of ll[0..n) into *max_out (device), as np.max at
  * likelihood_helpers.py:108.  n must be > 0. */
 int jk_max_f64(const double *ll, int64_t n, double *max_out, void *stream);
@@ -147,6 +160,75 @@ int jk_linear_posterior(const jk_star *star, int64_t n, const double *P, const d
                         const double *omega, const double *M0, const double *s,
                         const int64_t *rows, double *a_out, double *cov_out, double *ll_out,
                         void *stream);
+
+/* ------------------------------------------------------------------------
+ * Many stars in one launch.  No reference counterpart: the reference handles
+ * one star per CJokerHelper (fast_likelihood.pyx:70) and parallelises only
+ * over prior rows (multiproc_helpers.py:17-60).
+ * ------------------------------------------------------------------------ */
+
+typedef struct jk_batch jk_batch;
+
+/* A unit of work of jk_lnl_batch: rows [row_begin, row_begin + n_rows) of the
+ * prior cache (or of `rows`) against star `star`, written to
+ * ll_out[out_offset .. out_offset + n_rows). */
+typedef struct {
+    int32_t star;
+    int32_t n_rows;
+    int64_t row_begin;
+    int64_t out_offset;
+} jk_work_item;
+
+/* One rejection segment of jk_reject_batch: ll[offset .. offset + n) with its
+ * own numpy PCG64 state (state/inc as in jk_reject, skipping stream_offset
+ * draws); kept indices (relative to offset, ascending, at most max_keep)
+ * go to idx_out[idx_offset ..]. */
+typedef struct {
+    int64_t offset;
+    int64_t n;
+    int64_t max_keep;
+    int64_t idx_offset;
+    uint64_t state_hi, state_lo, inc_hi, inc_lo;
+    uint64_t stream_offset;
+} jk_segment;
+
+/* Gather stars (all with the same L and jitter mode) into a device-resident
+ * star table; stars[i] is addressed as star index i by work items.  The
+ * jk_star objects may be destroyed afterwards. */
+int jk_batch_create(const jk_star *const *stars, int n_stars, jk_batch **out);
+int jk_batch_destroy(jk_batch *batch);
+
+/* Persistent work-queue launch over device-resident work items.
+ * precision 0: FP64 (e, omega, M0, s are double columns); 1: mixed (float
+ * columns).  block: threads per workgroup (64, 128 or 256); grid: workgroups
+ * (0 = 8 per compute unit).  workspace: NULL or 256 device bytes (the queue
+ * counter; pass one to keep the call free of allocation). */
+int jk_lnl_batch(const jk_batch *batch, int precision, int64_t n_items,
+                 const jk_work_item *items, const double *P, const void *e, const void *omega,
+                 const void *M0, const void *s, const int64_t *rows, double *ll_out,
+                 int block, int grid, void *workspace, void *stream);
+
+/* Segmented rejection step (jk_reject per segment, one launch sequence for
+ * all): segs is a HOST array of n_seg descriptors; n_pass_out[i] and
+ * llmax_out[i] (device) receive segment i's pass count and maximum. */
+int jk_reject_batch(const double *ll, int n_seg, const jk_segment *segs, int64_t *idx_out,
+                    int64_t *n_pass_out, double *llmax_out, void *stream);
+
+/* Fill rows [row0, row0 + n) of a device prior cache (arrays indexed from
+ * 0 for row row0) with draws of the default nonlinear prior
+ * (prior.py:410-493): P log-uniform on [P_min, P_max] (distributions.py:
+ * 24-28), e from the inverse CDF given as n_tab + 1 equally spaced quantiles
+ * (device array; e.g. of Kipping13Global's Beta(0.867, 3.03),
+ * distributions.py:263-268), omega and M0 uniform on [0, 2 pi),
+ * ln s ~ N(ln_s_mu, ln_s_sigma) or s = 0 when ln_s_sigma < 0.  Draws come
+ * from a counter-based Philox4x32-10 stream keyed by `seed`, so row r is the
+ * same whatever chunk it is generated in.  The FP32 columns are optional
+ * (all four or none).  No reference counterpart: the reference draws priors
+ * on the host through pymc (prior.py:297-407). */
+int jk_prior_fill(int64_t n, int64_t row0, uint64_t seed, double P_min, double P_max,
+                  const double *e_quantiles, int n_tab, double ln_s_mu, double ln_s_sigma,
+                  double *P, double *e, double *omega, double *M0, double *s, float *e32,
+                  float *omega32, float *M032, float *s32, void *stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,130 @@
+"""Many-star launches (jk_lnl_batch, jk_reject_batch, jk_polish_f64) against
+the single-star entry points and the oracle: same bits alone and in a mix,
+independent of work-item order and launch shape."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle
+from tests._golden import rel_err
+from thejoker_amd import batch as B
+from thejoker_amd import device as D
+from thejoker_amd import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+
+
+def _stars(n_stars=24, seed=3, L=3, mode="physical"):
+    out = []
+    for st in syn.many_stars(n_stars, seed=seed):
+        trend = st.trend(poly_trend=L - 1)
+        lam = np.array([900.0, 1e4, 0.25, 1e-4][:L])
+        out.append((st, D.DeviceStar(st.t, st.rv, st.ivar(), trend, np.zeros(L), lam, st.t_ref,
+                                     jitter_mode=mode)))
+    return out
+
+
+@pytest.fixture(scope="module")
+def setup(gpu):
+    stars = _stars()
+    cols = syn.prior_columns(1 << 14, seed=9, jitter=True)
+    cache = D.PriorCache(cols["P"], cols["e"], cols["omega"], cols["M0"], cols["s"])
+    return stars, cache, cols
+
+
+@pytest.mark.parametrize("precision", ["f64", "mixed"])
+@pytest.mark.parametrize("shape", [B.LaunchShape(256, 0, 2048), B.LaunchShape(64, 37, 777),
+                                   B.LaunchShape(128, 1, 4096)])
+def test_batch_equals_single_star_launches(setup, precision, shape):
+    stars, cache, _ = setup
+    batch = B.StarBatch([d for _, d in stars])
+    n = len(cache)
+    items, off, total = B.work_items([n] * len(stars), shape.tile)
+    rng = np.random.default_rng(0)
+    items = items[rng.permutation(len(items))]
+    d_items = B.upload_items(items)
+    ll = B.lnl_batch(batch, cache, d_items, len(items), total, precision, shape).cpu().numpy()
+    for s, (_, dstar) in enumerate(stars):
+        single = D.lnl(dstar, cache, precision=precision).cpu().numpy()
+        assert np.array_equal(ll[off[s]:off[s] + n], single), s
+
+
+def test_batch_vs_oracle(setup):
+    stars, cache, cols = setup
+    batch = B.StarBatch([d for _, d in stars])
+    n = len(cache)
+    items, off, total = B.work_items([n] * len(stars), 1024)
+    ll = B.lnl_batch(batch, cache, B.upload_items(items), len(items), total, "f64").cpu().numpy()
+    rows = oracle.pack_rows(cols["P"], cols["e"], cols["omega"], cols["M0"], cols["s"])
+    for s in (0, 5, 11):
+        st, dstar = stars[s]
+        ost = oracle.OracleStar(st.t, st.rv, st.ivar(), st.trend(poly_trend=2), np.zeros(3),
+                                np.array([900.0, 1e4, 0.25]), st.t_ref, jitter_mode=1)
+        want, capped = oracle.batch_lnl(ost, rows, method="chol", n_threads=16, return_capped=True)
+        assert rel_err(ll[off[s]:off[s] + n][~capped], want[~capped]).max() < 1e-10
+
+
+def test_partial_row_ranges(setup):
+    """Stars evaluated on different row ranges (the serving pattern)."""
+    stars, cache, _ = setup
+    batch = B.StarBatch([d for _, d in stars[:5]])
+    begins = np.array([0, 100, 4000, 9999, 16000])
+    sizes = np.array([50, 3000, 1, 6000, 384])
+    items, off, total = B.work_items(sizes, 512, row_begin=begins)
+    ll = B.lnl_batch(batch, cache, B.upload_items(items), len(items), total, "f64").cpu().numpy()
+    for s in range(5):
+        single = D.lnl(stars[s][1], cache, start=int(begins[s]), n=int(sizes[s])).cpu().numpy()
+        assert np.array_equal(ll[off[s]:off[s] + sizes[s]], single)
+
+
+def test_reject_batch_equals_single(gpu):
+    g = np.random.default_rng(1)
+    sizes = np.array([1, 63, 64, 65, 5000, 300000, 1 << 18, 77])
+    offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    ll_h = np.concatenate([-g.exponential(scale=sc, size=n) for n, sc in
+                           zip(sizes, [1, 0.5, 2, 0.01, 3, 1, 0.2, 10])])
+    ll_h[offsets[3] + 5] = np.nan
+    ll = torch.as_tensor(ll_h, device="cuda")
+    states = [D.pcg64_state(np.random.default_rng(100 + i)) for i in range(len(sizes))]
+    stream_off = np.array([0, 5, 0, 123456, 0, 7, 1 << 30, 3], dtype=np.uint64)
+    for max_keep in (1, 256, 1 << 20):
+        idx, n_pass, mx = B.reject_batch(ll, offsets, sizes, states, max_keep,
+                                         stream_offsets=stream_off)
+        for i in range(len(sizes)):
+            seg = ll[offsets[i]:offsets[i] + sizes[i]]
+            want, npw, mxw = D.reject(seg, states[i], max_keep, offset=int(stream_off[i]))
+            assert np.array_equal(idx[i], want), i
+            assert n_pass[i] == npw
+            assert (np.isnan(mx[i]) and np.isnan(mxw)) or mx[i] == mxw
+
+
+def test_polish_makes_live_set_fp64(setup):
+    stars, cache, _ = setup
+    for _, dstar in stars[:4]:
+        f64 = D.lnl(dstar, cache, precision="f64")
+        mix = D.lnl(dstar, cache, precision="mixed")
+        m0 = mix.cpu().numpy()
+        live = m0 >= m0.max() - 40
+        cnt = D.polish(dstar, cache, mix)
+        f, m = f64.cpu().numpy(), mix.cpu().numpy()
+        assert int(cnt.item()) == live.sum() > 0
+        assert np.array_equal(m[live], f[live])
+        assert m.max() == f.max()
+
+
+def test_prior_fill_distribution_and_determinism(gpu):
+    from scipy import stats
+    c = D.PriorCache.generate(1 << 20, seed=7, P_min=1.0, P_max=16384.0, ln_s=(-3.0, 1.0))
+    c2 = D.PriorCache.generate(1 << 16, seed=7, P_min=1.0, P_max=16384.0, ln_s=(-3.0, 1.0))
+    P = c.P.cpu().numpy()
+    e = c.e.cpu().numpy()
+    assert np.array_equal(P[:1 << 16], c2.P.cpu().numpy())      # row r depends on (seed, r)
+    assert P.min() >= 1.0 and P.max() <= 16384.0
+    assert stats.kstest(np.log(P), stats.uniform(0, np.log(16384.0)).cdf).pvalue > 1e-3
+    assert stats.kstest(e, stats.beta(0.867, 3.03).cdf).pvalue > 1e-3
+    assert stats.kstest(c.omega.cpu().numpy(), stats.uniform(0, 2 * np.pi).cdf).pvalue > 1e-3
+    ls = np.log(c.s.cpu().numpy())
+    assert stats.kstest(ls, stats.norm(-3, 1).cdf).pvalue > 1e-3
+    assert np.array_equal(c.fp32[0].cpu().numpy(), e.astype(np.float32))
+    z = D.PriorCache.generate(1000, seed=1, ln_s=None)
+    assert np.all(z.s.cpu().numpy() == 0.0)

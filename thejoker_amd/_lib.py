@@ -40,11 +40,22 @@ SIGNATURES = [
     ("jk_lnl_f64", ctypes.c_int, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("jk_lnl_mixed", ctypes.c_int, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("jk_max_f64", ctypes.c_int, [_vp, _i64, _vp, _vp]),
+    ("jk_polish_f64", ctypes.c_int,
+     [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _vp, _vp]),
     ("jk_reject", ctypes.c_int,
      [_vp, _i64, _u64, _u64, _u64, _u64, _u64, _i64, _vp, _vp, _vp, _vp]),
     ("jk_reject_with_max", ctypes.c_int,
      [_vp, _i64, _vp, _u64, _u64, _u64, _u64, _u64, _i64, _vp, _vp, _vp]),
     ("jk_pcg64_uniform", ctypes.c_int, [_u64, _u64, _u64, _u64, _u64, _i64, _vp, _vp]),
+    ("jk_batch_create", ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(_vp)]),
+    ("jk_batch_destroy", ctypes.c_int, [_vp]),
+    ("jk_lnl_batch", ctypes.c_int,
+     [_vp, ctypes.c_int, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
+      _vp, _vp]),
+    ("jk_reject_batch", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]),
+    ("jk_prior_fill", ctypes.c_int,
+     [_i64, _i64, _u64, ctypes.c_double, ctypes.c_double, _vp, ctypes.c_int, ctypes.c_double,
+      ctypes.c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("jk_linear_posterior", ctypes.c_int,
      [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
 ]

@@ -1,0 +1,267 @@
+"""Many stars against one device-resident prior cache.
+
+The reference evaluates one star at a time (TheJoker._make_joker_helper ->
+CJokerHelper, thejoker.py:87-91).  Here a set of stars with the same number
+of linear parameters is gathered into a device star table (``jk_batch``) and
+evaluated by one persistent work-queue launch over (star, row tile) items
+(``jk_lnl_batch``); the rejection step of every star runs as one segmented
+launch sequence (``jk_reject_batch``), each star with its own numpy PCG64
+stream.  :func:`autotune` picks the launch shape (threads per workgroup,
+workgroups, rows per work item) by timing candidates on the actual workload.
+"""
+import ctypes
+import time
+
+import numpy as np
+
+from . import _lib
+from .device import DeviceStar, _dptr, _split, _stream, _torch, pcg64_advance_state, pcg64_state
+
+__all__ = ["WORK_ITEM_DTYPE", "SEGMENT_DTYPE", "StarBatch", "work_items", "lnl_batch",
+           "reject_batch", "autotune", "LaunchShape"]
+
+WORK_ITEM_DTYPE = np.dtype([("star", "<i4"), ("n_rows", "<i4"), ("row_begin", "<i8"),
+                            ("out_offset", "<i8")])
+SEGMENT_DTYPE = np.dtype([("offset", "<i8"), ("n", "<i8"), ("max_keep", "<i8"),
+                          ("idx_offset", "<i8"), ("state_hi", "<u8"), ("state_lo", "<u8"),
+                          ("inc_hi", "<u8"), ("inc_lo", "<u8"), ("stream_offset", "<u8")])
+
+
+class LaunchShape:
+    """Threads per workgroup, workgroups (0 = 8 per CU), rows per work item."""
+
+    def __init__(self, block=256, grid=0, tile=2048):
+        self.block, self.grid, self.tile = int(block), int(grid), int(tile)
+
+    def __repr__(self):
+        return f"LaunchShape(block={self.block}, grid={self.grid}, tile={self.tile})"
+
+    def as_dict(self):
+        return {"block": self.block, "grid": self.grid, "tile": self.tile}
+
+
+class StarBatch:
+    """Device star table of DeviceStar objects (same L and jitter mode)."""
+
+    def __init__(self, stars, device=0):
+        lib = _lib.ensure_device(device)
+        self._lib = lib
+        self.device = device
+        self.stars = list(stars)
+        arr = (ctypes.c_void_p * len(self.stars))(*[s.handle.value for s in self.stars])
+        h = ctypes.c_void_p()
+        _lib.check(lib.jk_batch_create(arr, len(self.stars), ctypes.byref(h)), "jk_batch_create")
+        self._h = h
+        self.M = np.array([s.n for s in self.stars], dtype=np.int64)
+        self.L = self.stars[0].L
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __len__(self):
+        return len(self.stars)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.jk_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def work_items(n_rows, tile, row_begin=None, order=None):
+    """Work items covering rows [row_begin[s], row_begin[s] + n_rows[s]) of
+    each star s, `tile` rows per item, outputs packed star after star.
+    `order` (optional) permutes the items (e.g. heaviest first).
+    Returns (items, out_offsets, total)."""
+    n_rows = np.asarray(n_rows, dtype=np.int64)
+    if row_begin is None:
+        row_begin = np.zeros_like(n_rows)
+    out_off = np.concatenate([[0], np.cumsum(n_rows)[:-1]]).astype(np.int64)
+    counts = (n_rows + tile - 1) // tile
+    items = np.zeros(int(counts.sum()), dtype=WORK_ITEM_DTYPE)
+    k = 0
+    for s, (n, c) in enumerate(zip(n_rows, counts)):
+        j = np.arange(c, dtype=np.int64)
+        items["star"][k:k + c] = s
+        items["n_rows"][k:k + c] = np.minimum(tile, n - j * tile)
+        items["row_begin"][k:k + c] = row_begin[s] + j * tile
+        items["out_offset"][k:k + c] = out_off[s] + j * tile
+        k += c
+    if order is not None:
+        items = items[order]
+    return items, out_off, int(n_rows.sum())
+
+
+def heaviest_first(items, M):
+    """Order work items by decreasing cost (rows x epochs), so the queue ends
+    on short items."""
+    cost = items["n_rows"].astype(np.int64) * np.asarray(M)[items["star"]]
+    return np.argsort(-cost, kind="stable")
+
+
+def upload_items(items, device=0):
+    torch = _torch()
+    raw = np.frombuffer(items.tobytes(), dtype=np.uint8)
+    return torch.as_tensor(raw.copy(), device=torch.device("cuda", device))
+
+
+def lnl_batch(batch, cache, d_items, n_items, n_out, precision="mixed", shape=None, out=None,
+              rows=None, workspace=None):
+    """lnL of every work item into a device float64 tensor of n_out entries."""
+    torch = _torch()
+    lib = _lib.ensure_device(cache.device)
+    shape = shape or LaunchShape()
+    if out is None:
+        out = torch.empty(n_out, dtype=torch.float64, device=cache.P.device)
+    if precision == "f64":
+        cols, prec = (cache.e, cache.omega, cache.M0, cache.s), 0
+    elif precision == "mixed":
+        cols, prec = cache.fp32, 1
+    else:
+        raise ValueError("precision must be 'f64' or 'mixed'")
+    _lib.check(lib.jk_lnl_batch(batch.handle, prec, int(n_items), _dptr(d_items), _dptr(cache.P),
+                                *[_dptr(c) for c in cols], _dptr(rows), _dptr(out),
+                                shape.block, shape.grid, _dptr(workspace),
+                                _stream(cache.device)), "jk_lnl_batch")
+    return out
+
+
+def reject_batch(ll, offsets, sizes, rng_states, max_keep, stream_offsets=None, device=0):
+    """Segmented rejection step.  rng_states: list of (state, inc).  Returns
+    (list of index arrays, n_pass array, llmax array)."""
+    torch = _torch()
+    lib = _lib.ensure_device(device)
+    n_seg = len(sizes)
+    segs = np.zeros(n_seg, dtype=SEGMENT_DTYPE)
+    segs["offset"] = offsets
+    segs["n"] = sizes
+    mk = np.minimum(np.broadcast_to(np.asarray(max_keep, dtype=np.int64), (n_seg,)), sizes)
+    segs["max_keep"] = mk
+    segs["idx_offset"] = np.concatenate([[0], np.cumsum(mk)[:-1]])
+    for i, (st, inc) in enumerate(rng_states):
+        segs["state_hi"][i], segs["state_lo"][i] = _split(int(st))
+        segs["inc_hi"][i], segs["inc_lo"][i] = _split(int(inc))
+    if stream_offsets is not None:
+        segs["stream_offset"] = stream_offsets
+    dev = ll.device
+    idx = torch.empty(max(int(mk.sum()), 1), dtype=torch.int64, device=dev)
+    npass = torch.zeros(n_seg, dtype=torch.int64, device=dev)
+    mx = torch.empty(n_seg, dtype=torch.float64, device=dev)
+    _lib.check(lib.jk_reject_batch(_dptr(ll), n_seg, segs.ctypes.data_as(ctypes.c_void_p),
+                                   _dptr(idx), _dptr(npass), _dptr(mx), _stream(device)),
+               "jk_reject_batch")
+    n_pass = npass.cpu().numpy()
+    idx_h = idx.cpu().numpy()
+    out = []
+    for i in range(n_seg):
+        k = int(min(n_pass[i], mk[i]))
+        out.append(idx_h[segs["idx_offset"][i]:segs["idx_offset"][i] + k].copy())
+    return out, n_pass, mx.cpu().numpy()
+
+
+def autotune(batch, cache, n_rows, precision="mixed", candidates=None, sample_stars=64,
+             reps=3, device=0):
+    """Time candidate launch shapes on a subset of the workload (the first
+    `sample_stars` stars); return (best LaunchShape, {repr(shape): ms})."""
+    torch = _torch()
+    n_rows = np.asarray(n_rows, dtype=np.int64)
+    if candidates is None:
+        props = torch.cuda.get_device_properties(device)
+        cus = props.multi_processor_count
+        candidates = [LaunchShape(b, g * cus, t) for b in (256, 128) for g in (4, 8, 16)
+                      for t in (1024, 4096)]
+    sub = min(sample_stars, len(batch))
+    results = {}
+    best, best_ms = None, np.inf
+    ws = torch.zeros(64, dtype=torch.int32, device=cache.P.device)
+    for shape in candidates:
+        items, _, total = work_items(n_rows[:sub], shape.tile)
+        items = items[heaviest_first(items, batch.M)]
+        d_items = upload_items(items, device)
+        out = torch.empty(total, dtype=torch.float64, device=cache.P.device)
+        lnl_batch(batch, cache, d_items, len(items), total, precision, shape, out=out, workspace=ws)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            lnl_batch(batch, cache, d_items, len(items), total, precision, shape, out=out,
+                      workspace=ws)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        results[repr(shape)] = ms
+        if ms < best_ms:
+            best, best_ms = shape, ms
+    return best, results
+
+
+def rejection_sample_many(datas, prior, prior_samples, rngs, max_posterior_samples=256,
+                          n_linear_samples=1, precision="mixed", jitter_mode="physical",
+                          shape=None, device=0, return_logprobs=False, return_all_logprobs=False):
+    """TheJoker.rejection_sample(..., in_memory=True) for many stars at once.
+
+    datas: RVData (or per-survey lists) per star; rngs: one numpy Generator
+    per star (consumed exactly as rejection_sample_inmem consumes it,
+    likelihood_helpers.py:91-127); prior_samples: a PriorCache (or
+    JokerSamples) shared by all stars.  Returns one JokerSamples per star
+    (and the per-star lnL arrays with return_all_logprobs)."""
+    from .device import PriorCache, linear_internal_units, polish, star_arrays
+    from .likelihood_helpers import validate_prepare_data
+    from .thejoker import POLISH_WINDOW, TheJoker
+
+    torch = _torch()
+    stars, units, all_data = [], [], []
+    rv_unit = None
+    for d in datas:
+        dd, ids, trend = validate_prepare_data(d, prior.poly_trend, prior.n_offsets)
+        if rv_unit is None:
+            rv_unit = dd.rv.unit
+        elif not dd.rv.unit.is_equivalent(rv_unit) or dd.rv.unit != rv_unit:
+            raise ValueError("all stars of a batch must share one RV unit")
+        a = star_arrays(dd, prior, trend)
+        st = DeviceStar(a["t"], a["rv"], a["ivar"], a["trend"], a["mu"], a["lam"], a["t_ref"],
+                        a["k_prior_mode"], a["sigma_K0"], a["P0"], a["max_K"],
+                        jitter_mode=jitter_mode, device=device)
+        st.data = dd
+        st.internal_units = linear_internal_units(prior, dd.rv.unit)
+        stars.append(st)
+        all_data.append(dd)
+    if isinstance(prior_samples, PriorCache):
+        cache = prior_samples
+    else:
+        cache = PriorCache.from_samples(prior_samples, rv_unit=rv_unit, device=device,
+                                        fp32_copy=precision == "mixed")
+    n = len(cache)
+    batch = StarBatch(stars, device=device)
+    shape = shape or LaunchShape()
+    items, off, total = work_items([n] * len(stars), shape.tile)
+    items = items[heaviest_first(items, batch.M)]
+    ll = lnl_batch(batch, cache, upload_items(items, device), len(items), total, precision, shape)
+    if precision == "mixed":
+        for s, st in enumerate(stars):
+            polish(st, cache, ll[int(off[s]):int(off[s]) + n], window=POLISH_WINDOW)
+    states = [pcg64_state(g) for g in rngs]
+    good, _, _ = reject_batch(ll, off, np.full(len(stars), n), states, max_posterior_samples,
+                              device=device)
+    for g in rngs:
+        pcg64_advance_state(g, n)
+    ll_h = ll.cpu().numpy() if (return_logprobs or return_all_logprobs) else None
+    helper = TheJoker(prior, rng=np.random.default_rng(0), jitter_mode=jitter_mode,
+                      precision=precision, device=device)
+    out = []
+    for s, st in enumerate(stars):
+        samp = helper._full_samples(st, cache, good[s], rngs[s], n_linear_samples)
+        if return_logprobs:
+            seg = ll_h[off[s]:off[s] + n]
+            if cache.ln_prior is not None:
+                samp["ln_prior"] = cache.ln_prior[good[s]]
+            samp["ln_likelihood"] = seg[good[s]]
+        if return_all_logprobs:
+            out.append((samp, ll_h[off[s]:off[s] + n].copy()))
+        else:
+            out.append(samp)
+    return out

@@ -16,11 +16,14 @@ SOURCES = ["jk_kernels.hip", "jk_capi.hip"]
 HEADERS = ["jk_device.h", "jk_internal.h"]
 
 # Packed-FP32 VALU forms stay disabled: every HIP object is compiled with
-# -packed-fp32-ops switched off.
+# -packed-fp32-ops switched off.  Contraction of a*b+c is off so that every
+# kernel that inlines the same per-row arithmetic rounds identically (the
+# fused multiply-adds the kernels want are written as fma() calls).
 HIP_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
     "-std=c++17",
+    "-ffp-contract=off",
     "-fPIC",
     "-Wall",
     "-Wno-unused-variable",

@@ -289,4 +289,196 @@ int jk_linear_posterior(const jk_star *star, int64_t n, const double *P, const d
                                                 ll_out, (hipStream_t)stream));
 }
 
+int jk_polish_f64(const jk_star *star, int64_t n, const double *P, const double *e,
+                  const double *omega, const double *M0, const double *s, const int64_t *rows,
+                  double *ll_inout, double window, int64_t *n_polished_out, void *stream)
+{
+    if (!star || n < 0 || !(window >= 0.0)) return JK_EINVAL;
+    if (n == 0) return JK_OK;
+    if (!P || !e || !omega || !M0 || !ll_inout || !n_polished_out) return JK_EINVAL;
+    if (star->dev.jitter_mode == JK_JITTER_PHYSICAL && !s) return JK_EINVAL;
+    if (current_device_inited() < 0) return JK_ENODEV;
+    hipStream_t st = (hipStream_t)stream;
+    void *scratch = nullptr;
+    if (hipMallocAsync(&scratch, (size_t)jk_polish_scratch_bytes(n), st) != hipSuccess)
+        return JK_ENOMEM;
+    hipError_t err = jk_launch_polish(star, n, P, e, omega, M0, s, rows, ll_inout, window, scratch,
+                                      n_polished_out, st);
+    hipError_t err2 = hipFreeAsync(scratch, st);
+    if (err != hipSuccess) return JK_EHIP;
+    return check_hip(err2);
+}
+
+int jk_batch_create(const jk_star *const *stars, int n_stars, jk_batch **out)
+{
+    if (!out || !stars || n_stars <= 0) return JK_EINVAL;
+    *out = nullptr;
+    const int dev = current_device_inited();
+    if (dev < 0) return JK_ENODEV;
+    const int L = stars[0] ? stars[0]->dev.L : 0;
+    const int jm = stars[0] ? stars[0]->dev.jitter_mode : 0;
+    size_t n_epoch_doubles = 0;
+    for (int i = 0; i < n_stars; ++i) {
+        if (!stars[i] || stars[i]->dev.L != L || stars[i]->dev.jitter_mode != jm) return JK_EINVAL;
+        n_epoch_doubles += (size_t)stars[i]->dev.n * (4 + L - 1);
+    }
+    jk_batch *b = new (std::nothrow) jk_batch;
+    if (!b) return JK_ENOMEM;
+    memset(b, 0, sizeof(*b));
+    b->n_stars = n_stars;
+    b->L = L;
+    b->jitter_mode = jm;
+    b->device = dev;
+    if (hipMalloc((void **)&b->d_stars, sizeof(StarDev) * n_stars) != hipSuccess ||
+        hipMalloc(&b->d_epochs, sizeof(double) * n_epoch_doubles) != hipSuccess) {
+        (void)hipFree(b->d_stars);
+        delete b;
+        return JK_ENOMEM;
+    }
+    std::vector<StarDev> h(n_stars);
+    double *base = (double *)b->d_epochs;
+    size_t off = 0;
+    for (int i = 0; i < n_stars; ++i) {
+        h[i] = stars[i]->dev;
+        const size_t cnt = (size_t)h[i].n * (4 + L - 1);
+        if (hipMemcpy(base + off, stars[i]->d_epochs, sizeof(double) * cnt,
+                      hipMemcpyDeviceToDevice) != hipSuccess) {
+            (void)hipFree(b->d_stars);
+            (void)hipFree(b->d_epochs);
+            delete b;
+            return JK_EHIP;
+        }
+        double *p = base + off;
+        const int n = h[i].n;
+        h[i].dt = p;
+        h[i].rp = p + n;
+        h[i].y = p + 2 * n;
+        h[i].ivar = p + 3 * n;
+        h[i].T = p + 4 * n;
+        off += cnt;
+    }
+    if (hipMemcpy(b->d_stars, h.data(), sizeof(StarDev) * n_stars, hipMemcpyHostToDevice) !=
+        hipSuccess) {
+        (void)hipFree(b->d_stars);
+        (void)hipFree(b->d_epochs);
+        delete b;
+        return JK_EHIP;
+    }
+    *out = b;
+    return JK_OK;
+}
+
+int jk_batch_destroy(jk_batch *b)
+{
+    if (!b) return JK_EINVAL;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipSetDevice(b->device);
+    hipError_t e1 = hipFree(b->d_stars);
+    hipError_t e2 = hipFree(b->d_epochs);
+    (void)hipSetDevice(dev);
+    delete b;
+    return (e1 == hipSuccess && e2 == hipSuccess) ? JK_OK : JK_EHIP;
+}
+
+int jk_lnl_batch(const jk_batch *batch, int precision, int64_t n_items, const jk_work_item *items,
+                 const double *P, const void *e, const void *omega, const void *M0, const void *s,
+                 const int64_t *rows, double *ll_out, int block, int grid, void *workspace,
+                 void *stream)
+{
+    if (!batch || n_items < 0 || (precision != 0 && precision != 1)) return JK_EINVAL;
+    if (n_items == 0) return JK_OK;
+    if (!items || !P || !e || !omega || !M0 || !ll_out) return JK_EINVAL;
+    if (batch->jitter_mode == JK_JITTER_PHYSICAL && !s) return JK_EINVAL;
+    if (block != 64 && block != 128 && block != 256) return JK_EINVAL;
+    if (n_items > (int64_t)0xFFFFFFF0u) return JK_ERANGE;
+    const int dev = current_device_inited();
+    if (dev < 0) return JK_ENODEV;
+    if (grid <= 0) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return JK_EHIP;
+        grid = 8 * cus;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    void *ws = workspace;
+    if (!ws && hipMallocAsync(&ws, 256, st) != hipSuccess) return JK_ENOMEM;
+    hipError_t err = jk_launch_lnl_batch(batch, precision, n_items, items, P, e, omega, M0, s, rows,
+                                         ll_out, block, grid, (unsigned *)ws, st);
+    hipError_t err2 = workspace ? hipSuccess : hipFreeAsync(ws, st);
+    if (err != hipSuccess) return JK_EHIP;
+    return check_hip(err2);
+}
+
+int jk_reject_batch(const double *ll, int n_seg, const jk_segment *segs, int64_t *idx_out,
+                    int64_t *n_pass_out, double *llmax_out, void *stream)
+{
+    if (!ll || n_seg <= 0 || !segs || !n_pass_out || !llmax_out) return JK_EINVAL;
+    if (current_device_inited() < 0) return JK_ENODEV;
+    std::vector<SegDev> h(n_seg);
+    int64_t max_n = 1, max_nb = 1, mask_words = 0, bsum_n = 0;
+    bool any_keep = false;
+    for (int i = 0; i < n_seg; ++i) {
+        const jk_segment &g = segs[i];
+        if (g.n <= 0 || g.offset < 0 || g.max_keep < 0 || g.idx_offset < 0) return JK_EINVAL;
+        any_keep = any_keep || g.max_keep > 0;
+        SegDev &d = h[i];
+        d.off = g.offset;
+        d.n = g.n;
+        d.max_keep = g.max_keep;
+        d.idx_off = g.idx_offset;
+        d.mask_off = mask_words;
+        d.nb = jk_reject_bsum_count(g.n);
+        d.bsum_off = bsum_n;
+        d.max_off = 0;
+        mask_words += jk_reject_mask_words(g.n);
+        bsum_n += d.nb;
+        if (g.n > max_n) max_n = g.n;
+        if (d.nb > max_nb) max_nb = d.nb;
+        u128 state = make_u128(g.state_hi, g.state_lo);
+        d.inc = make_u128(g.inc_hi, g.inc_lo);
+        jk_host_pcg_jump(state, d.inc, g.stream_offset + 1, &d.s_base);
+    }
+    if (any_keep && !idx_out) return JK_EINVAL;
+    int pb = (int)((max_n + 4095) / 4096);
+    if (pb > 64) pb = 64;
+    if (pb < 1) pb = 1;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t seg_bytes = sizeof(SegDev) * n_seg;
+    const size_t seg_pad = ((seg_bytes + 255) / 256) * 256;
+    const size_t bytes = seg_pad + sizeof(double) * (size_t)pb * n_seg +
+                         sizeof(uint64_t) * mask_words + sizeof(int64_t) * bsum_n;
+    char *scratch = nullptr;
+    if (hipMallocAsync((void **)&scratch, bytes, st) != hipSuccess) return JK_ENOMEM;
+    SegDev *d_segs = (SegDev *)scratch;
+    double *partials = (double *)(scratch + seg_pad);
+    uint64_t *mask = (uint64_t *)(partials + (size_t)pb * n_seg);
+    int64_t *bsum = (int64_t *)(mask + mask_words);
+    hipError_t err = hipMemcpyAsync(d_segs, h.data(), seg_bytes, hipMemcpyHostToDevice, st);
+    if (err == hipSuccess)
+        err = jk_launch_reject_batch(ll, n_seg, d_segs, max_n, max_nb, partials, pb, mask, bsum,
+                                     idx_out, n_pass_out, llmax_out, st);
+    // The host descriptor array must outlive the queued copy.
+    hipError_t err3 = hipStreamSynchronize(st);
+    hipError_t err2 = hipFreeAsync(scratch, st);
+    if (err != hipSuccess || err3 != hipSuccess) return JK_EHIP;
+    return check_hip(err2);
+}
+
+int jk_prior_fill(int64_t n, int64_t row0, uint64_t seed, double P_min, double P_max,
+                  const double *e_quantiles, int n_tab, double ln_s_mu, double ln_s_sigma,
+                  double *P, double *e, double *omega, double *M0, double *s, float *e32,
+                  float *omega32, float *M032, float *s32, void *stream)
+{
+    if (n < 0 || row0 < 0 || !(P_min > 0.0) || !(P_max > P_min) || n_tab < 1 || !e_quantiles)
+        return JK_EINVAL;
+    if (n > 0 && (!P || !e || !omega || !M0 || !s)) return JK_EINVAL;
+    const bool any32 = e32 || omega32 || M032 || s32;
+    if (any32 && !(e32 && omega32 && M032 && s32)) return JK_EINVAL;
+    if (current_device_inited() < 0) return JK_ENODEV;
+    return check_hip(jk_launch_prior_fill(n, row0, seed, P_min, P_max, e_quantiles, n_tab, ln_s_mu,
+                                          ln_s_sigma, P, e, omega, M0, s, e32, omega32, M032, s32,
+                                          (hipStream_t)stream));
+}
+
 }  // extern "C"

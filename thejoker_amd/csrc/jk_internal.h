@@ -15,6 +15,23 @@ struct jk_star {
     int device;
 };
 
+// Device-resident table of stars for the batched launch.
+struct jk_batch {
+    StarDev *d_stars;
+    int n_stars;
+    int L;
+    int jitter_mode;
+    int device;
+    void *d_epochs;     // epoch arrays of all stars (StarDev pointers point here)
+};
+
+// Internal segment descriptor of the segmented rejection step.
+struct SegDev {
+    int64_t off, n, max_keep, idx_off;
+    int64_t mask_off, bsum_off, nb, max_off;
+    u128 s_base, inc;
+};
+
 // Jump table of numpy's PCG64 LCG: (A^(2^i), D_(2^i)) for i = 0..63, where
 // a jump by k steps maps state -> A^k state + inc * D_k.
 struct PcgJump {
@@ -47,3 +64,24 @@ hipError_t jk_launch_linear_posterior(const jk_star *st, int64_t n, const double
                                       const double *s, const int64_t *rows, double *a_out,
                                       double *cov_out, double *ll_out, hipStream_t stream);
 hipError_t jk_upload_pcg_table(int device);
+int64_t jk_polish_scratch_bytes(int64_t n);
+hipError_t jk_launch_lnl_batch(const jk_batch *b, int precision, int64_t n_items,
+                               const jk_work_item *items, const double *P, const void *e,
+                               const void *om, const void *M0, const void *s,
+                               const int64_t *rows, double *ll, int block, int grid,
+                               unsigned *counter, hipStream_t stream);
+hipError_t jk_launch_reject_batch(const double *ll, int n_seg, const SegDev *d_segs,
+                                  int64_t max_n, int64_t max_nb, double *partials, int pb,
+                                  uint64_t *mask, int64_t *bsum, int64_t *idx_out,
+                                  int64_t *n_pass_out, double *llmax_out, hipStream_t stream);
+void jk_host_pcg_jump(u128 state, u128 inc, uint64_t k, u128 *out);
+hipError_t jk_launch_prior_fill(int64_t n, int64_t row0, uint64_t seed, double P_min, double P_max,
+                                const double *e_tab, int n_tab, double s_mu, double s_sigma,
+                                double *P, double *e, double *om, double *M0, double *s,
+                                float *e32, float *om32, float *M032, float *s32,
+                                hipStream_t stream);
+int jk_seg_flag_blocks(int64_t max_n);
+hipError_t jk_launch_polish(const jk_star *st, int64_t n, const double *P, const double *e,
+                            const double *om, const double *M0, const double *s,
+                            const int64_t *rows, double *ll, double window, void *scratch,
+                            int64_t *n_polished, hipStream_t stream);

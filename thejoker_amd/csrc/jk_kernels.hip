@@ -238,24 +238,18 @@ __device__ __forceinline__ void jk_kepler_markley_f32(float x, float e, float om
 // Fused marginal-likelihood kernel.
 // ---------------------------------------------------------------------------
 template <int L, bool PHYS, bool MIXED>
-__global__ __launch_bounds__(256) void k_lnl(const StarDev st, int64_t n,
+__device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
                                              const double *__restrict__ P,
                                              const void *__restrict__ e_,
                                              const void *__restrict__ om_,
                                              const void *__restrict__ M0_,
-                                             const void *__restrict__ s_,
-                                             const int64_t *__restrict__ rows,
-                                             double *__restrict__ ll)
+                                             const void *__restrict__ s_)
 {
     using TF = typename std::conditional<MIXED, float, double>::type;
     const TF *__restrict__ ecol = static_cast<const TF *>(e_);
     const TF *__restrict__ omcol = static_cast<const TF *>(om_);
     const TF *__restrict__ M0col = static_cast<const TF *>(M0_);
     const TF *__restrict__ scol = static_cast<const TF *>(s_);
-
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int64_t r = rows ? rows[i] : i;
 
     const double Pv = P[r];
     const double ev = (double)ecol[r];
@@ -266,10 +260,7 @@ __global__ __launch_bounds__(256) void k_lnl(const StarDev st, int64_t n,
     bool valid = isfinite(Pv) && Pv != 0.0 && ev >= 0.0 && ev < 1.0 && isfinite(omv) &&
                  isfinite(M0v);
     if (PHYS) valid = valid && isfinite(sv);
-    if (!valid) {
-        ll[i] = __builtin_nan("");
-        return;
-    }
+    if (!valid) return __builtin_nan("");
 
     double lam0;
     if (st.k_prior_mode == JK_KPRIOR_FIXED_COMPANION_MASS)
@@ -393,7 +384,48 @@ __global__ __launch_bounds__(256) void k_lnl(const StarDev st, int64_t n,
         }
     }
     const double lnc = (double)N * JK_LN_TWO_PI - sum_ln_w + log(lam0) + st.ln_lam_trend;
-    ll[i] = jk_finish_lnl<L>(G, u, chi0, lnc);
+    return jk_finish_lnl<L>(G, u, chi0, lnc);
+}
+
+// Dense launch: ll[i] = lnL(row rows[i] or i), one lane per row.
+template <int L, bool PHYS, bool MIXED>
+__global__ __launch_bounds__(256) void k_lnl(const StarDev st, int64_t n,
+                                             const double *__restrict__ P,
+                                             const void *__restrict__ e_,
+                                             const void *__restrict__ om_,
+                                             const void *__restrict__ M0_,
+                                             const void *__restrict__ s_,
+                                             const int64_t *__restrict__ rows,
+                                             double *__restrict__ ll)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t r = rows ? rows[i] : i;
+    ll[i] = jk_lnl_row<L, PHYS, MIXED>(st, r, P, e_, om_, M0_, s_);
+}
+
+// Scatter launch over a device-resident index list: for j < *count,
+// i = idx[j], ll[i] = lnL(row rows[i] or i).  Grid-stride, so the host does
+// not need the count.
+template <int L, bool PHYS>
+__global__ __launch_bounds__(256) void k_lnl_scatter_f64(const StarDev st,
+                                                         const int64_t *__restrict__ count,
+                                                         const int64_t *__restrict__ idx,
+                                                         const double *__restrict__ P,
+                                                         const void *__restrict__ e_,
+                                                         const void *__restrict__ om_,
+                                                         const void *__restrict__ M0_,
+                                                         const void *__restrict__ s_,
+                                                         const int64_t *__restrict__ rows,
+                                                         double *__restrict__ ll)
+{
+    const int64_t cnt = *count;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += stride) {
+        const int64_t i = idx[j];
+        const int64_t r = rows ? rows[i] : i;
+        ll[i] = jk_lnl_row<L, PHYS, false>(st, r, P, e_, om_, M0_, s_);
+    }
 }
 
 template <int L, bool PHYS, bool MIXED>
@@ -578,6 +610,27 @@ __global__ __launch_bounds__(256) void k_reject_flags(const double *__restrict__
     }
 }
 
+// Rows to re-evaluate in FP64 after a mixed-precision pass: every row not
+// below max - window, and every row when the maximum itself is not finite.
+__global__ __launch_bounds__(256) void k_window_flags(const double *__restrict__ ll, int64_t n,
+                                                      const double *__restrict__ llmax,
+                                                      double window,
+                                                      uint64_t *__restrict__ mask)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t T = (int64_t)gridDim.x * 256;
+    const double mx = *llmax;
+    const bool all = !isfinite(mx);
+    const double thr = mx - window;
+    for (int64_t base = g - lane; base < n; base += T) {
+        const int64_t i = base + lane;
+        const bool keep = i < n && (all || !(ll[i] < thr));
+        const uint64_t word = __ballot(keep);
+        if (lane == 0) mask[base >> 6] = word;
+    }
+}
+
 #define JK_WORDS_PER_BLOCK 4096   // 16 words per thread
 
 __global__ __launch_bounds__(256) void k_reject_bsum(const uint64_t *__restrict__ mask,
@@ -752,6 +805,440 @@ hipError_t jk_launch_reject(const double *ll, int64_t n, const double *llmax, u1
     hipLaunchKernelGGL(k_reject_bsum, dim3((unsigned)nb), dim3(256), 0, stream, mask, nwords, bsum);
     hipLaunchKernelGGL(k_reject_compact, dim3((unsigned)nb), dim3(256), 0, stream, mask, nwords, bsum,
                        (int)nb, max_keep, idx_out, n_pass_out);
+    return hipGetLastError();
+}
+
+template <int L, bool PHYS>
+static void launch_scatter_t(const jk_star *st, const int64_t *count, const int64_t *idx, int grid,
+                             const double *P, const void *e, const void *om, const void *M0,
+                             const void *s, const int64_t *rows, double *ll, hipStream_t stream)
+{
+    hipLaunchKernelGGL((k_lnl_scatter_f64<L, PHYS>), dim3(grid), dim3(256), 0, stream, st->dev,
+                       count, idx, P, e, om, M0, s, rows, ll);
+}
+
+int64_t jk_polish_scratch_bytes(int64_t n)
+{
+    return (int64_t)sizeof(double) * (jk_max_partials_count(n) + 1) +
+           (int64_t)sizeof(uint64_t) * jk_reject_mask_words(n) +
+           (int64_t)sizeof(int64_t) * (jk_reject_bsum_count(n) + n + 1);
+}
+
+hipError_t jk_launch_polish(const jk_star *st, int64_t n, const double *P, const double *e,
+                            const double *om, const double *M0, const double *s,
+                            const int64_t *rows, double *ll, double window, void *scratch,
+                            int64_t *n_polished, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    const int np = jk_max_partials_count(n);
+    const int64_t nwords = jk_reject_mask_words(n);
+    const int64_t nb = jk_reject_bsum_count(n);
+    char *p = (char *)scratch;
+    double *partials = (double *)p;
+    double *mx = partials + np;
+    uint64_t *mask = (uint64_t *)(mx + 1);
+    int64_t *bsum = (int64_t *)(mask + nwords);
+    int64_t *idx = bsum + nb;
+    int64_t *count = n_polished;
+    hipError_t err = jk_launch_max(ll, n, partials, np, mx, stream);
+    if (err != hipSuccess) return err;
+    const int grid = jk_rng_grid(n);
+    hipLaunchKernelGGL(k_window_flags, dim3(grid), dim3(256), 0, stream, ll, n, mx, window, mask);
+    hipLaunchKernelGGL(k_reject_bsum, dim3((unsigned)nb), dim3(256), 0, stream, mask, nwords, bsum);
+    hipLaunchKernelGGL(k_reject_compact, dim3((unsigned)nb), dim3(256), 0, stream, mask, nwords, bsum,
+                       (int)nb, n, idx, count);
+    int64_t sgrid = (n + 255) / 256;
+    if (sgrid > 2048) sgrid = 2048;
+    const bool phys = st->dev.jitter_mode == JK_JITTER_PHYSICAL;
+#define JK_SCASE(LL)                                                                                 \
+    case LL:                                                                                         \
+        if (phys)                                                                                    \
+            launch_scatter_t<LL, true>(st, count, idx, (int)sgrid, P, e, om, M0, s, rows, ll, stream); \
+        else                                                                                         \
+            launch_scatter_t<LL, false>(st, count, idx, (int)sgrid, P, e, om, M0, s, rows, ll, stream); \
+        break;
+    switch (st->dev.L) {
+        JK_SCASE(1)
+        JK_SCASE(2)
+        JK_SCASE(3)
+        JK_SCASE(4)
+        JK_SCASE(5)
+        JK_SCASE(6)
+        JK_SCASE(7)
+        JK_SCASE(8)
+    default:
+        return hipErrorInvalidValue;
+    }
+#undef JK_SCASE
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Many stars per launch: a persistent work queue.  Each workgroup takes the
+// next (star, row tile) item from a device counter, reads the star's
+// constants through scalar loads (the item index is made wave-uniform with
+// readfirstlane) and evaluates the tile one lane per row.  Every workgroup
+// exits once the counter passes n_items.
+// ---------------------------------------------------------------------------
+__global__ void k_zero_u32(unsigned *p) { *p = 0u; }
+
+template <int L, bool PHYS, bool MIXED>
+__global__ __launch_bounds__(256) void k_lnl_batch(const StarDev *__restrict__ stars,
+                                                   const jk_work_item *__restrict__ items,
+                                                   int64_t n_items, unsigned *__restrict__ counter,
+                                                   const double *__restrict__ P,
+                                                   const void *__restrict__ e_,
+                                                   const void *__restrict__ om_,
+                                                   const void *__restrict__ M0_,
+                                                   const void *__restrict__ s_,
+                                                   const int64_t *__restrict__ rows,
+                                                   double *__restrict__ ll)
+{
+    __shared__ unsigned s_item;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
+        __syncthreads();
+        const int64_t it = (int64_t)__builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
+        if (it >= n_items) break;
+        const jk_work_item w = items[it];
+        const StarDev &st = stars[w.star];
+        for (int j = threadIdx.x; j < w.n_rows; j += blockDim.x) {
+            const int64_t q = w.row_begin + j;
+            const int64_t r = rows ? rows[q] : q;
+            ll[w.out_offset + j] = jk_lnl_row<L, PHYS, MIXED>(st, r, P, e_, om_, M0_, s_);
+        }
+    }
+}
+
+template <int L, bool PHYS, bool MIXED>
+static void launch_batch_t(const jk_batch *b, int64_t n_items, const jk_work_item *items,
+                           const double *P, const void *e, const void *om, const void *M0,
+                           const void *s, const int64_t *rows, double *ll, int block, int grid,
+                           unsigned *counter, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, stream, counter);
+    hipLaunchKernelGGL((k_lnl_batch<L, PHYS, MIXED>), dim3(grid), dim3(block), 0, stream,
+                       b->d_stars, items, n_items, counter, P, e, om, M0, s, rows, ll);
+}
+
+template <bool MIXED>
+static hipError_t launch_batch_dispatch(const jk_batch *b, int64_t n_items,
+                                        const jk_work_item *items, const double *P,
+                                        const void *e, const void *om, const void *M0,
+                                        const void *s, const int64_t *rows, double *ll,
+                                        int block, int grid, unsigned *counter,
+                                        hipStream_t stream)
+{
+    const bool phys = b->jitter_mode == JK_JITTER_PHYSICAL;
+#define JK_BCASE(LL)                                                                              \
+    case LL:                                                                                      \
+        if (phys)                                                                                 \
+            launch_batch_t<LL, true, MIXED>(b, n_items, items, P, e, om, M0, s, rows, ll, block,   \
+                                            grid, counter, stream);                               \
+        else                                                                                      \
+            launch_batch_t<LL, false, MIXED>(b, n_items, items, P, e, om, M0, s, rows, ll, block,  \
+                                             grid, counter, stream);                              \
+        break;
+    switch (b->L) {
+        JK_BCASE(1)
+        JK_BCASE(2)
+        JK_BCASE(3)
+        JK_BCASE(4)
+        JK_BCASE(5)
+        JK_BCASE(6)
+        JK_BCASE(7)
+        JK_BCASE(8)
+    default:
+        return hipErrorInvalidValue;
+    }
+#undef JK_BCASE
+    return hipGetLastError();
+}
+
+hipError_t jk_launch_lnl_batch(const jk_batch *b, int precision, int64_t n_items,
+                               const jk_work_item *items, const double *P, const void *e,
+                               const void *om, const void *M0, const void *s,
+                               const int64_t *rows, double *ll, int block, int grid,
+                               unsigned *counter, hipStream_t stream)
+{
+    if (n_items <= 0) return hipSuccess;
+    if (precision == 0)
+        return launch_batch_dispatch<false>(b, n_items, items, P, e, om, M0, s, rows, ll, block,
+                                            grid, counter, stream);
+    return launch_batch_dispatch<true>(b, n_items, items, P, e, om, M0, s, rows, ll, block, grid,
+                                       counter, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Segmented rejection: the steps of jk_launch_reject with blockIdx.y picking
+// the segment.  Blocks past a segment's extent exit at once.
+// ---------------------------------------------------------------------------
+#define JK_SEG_MAX_BLOCKS 64
+
+__global__ __launch_bounds__(256) void k_seg_max_partial(const double *__restrict__ ll,
+                                                         const SegDev *__restrict__ segs,
+                                                         double *__restrict__ partials, int pb)
+{
+    __shared__ double red[4];
+    const SegDev sg = segs[blockIdx.y];
+    double m = -INFINITY;
+    const int64_t stride = (int64_t)pb * 256;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < sg.n; i += stride)
+        m = jk_nanmax(m, ll[sg.off + i]);
+    m = wave_nanmax(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        partials[(int64_t)blockIdx.y * pb + blockIdx.x] =
+            jk_nanmax(jk_nanmax(red[0], red[1]), jk_nanmax(red[2], red[3]));
+}
+
+__global__ __launch_bounds__(64) void k_seg_max_final(const double *__restrict__ partials, int pb,
+                                                      double *__restrict__ llmax)
+{
+    double m = -INFINITY;
+    for (int i = threadIdx.x; i < pb; i += 64) m = jk_nanmax(m, partials[(int64_t)blockIdx.x * pb + i]);
+    m = wave_nanmax(m);
+    if (threadIdx.x == 0) llmax[blockIdx.x] = m;
+}
+
+__global__ __launch_bounds__(256) void k_seg_flags(const double *__restrict__ ll,
+                                                   const SegDev *__restrict__ segs,
+                                                   const double *__restrict__ llmax, u128 AT,
+                                                   u128 DT, uint64_t *__restrict__ mask)
+{
+    const SegDev sg = segs[blockIdx.y];
+    const int lane = threadIdx.x & 63;
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t T = (int64_t)gridDim.x * 256;
+    if (g - lane >= sg.n) return;
+    const double mx = llmax[blockIdx.y];
+    u128 s = pcg_jump_table(sg.s_base, (uint64_t)g, sg.inc);
+    for (int64_t base = g - lane; base < sg.n; base += T) {
+        const int64_t i = base + lane;
+        bool keep = false;
+        if (i < sg.n) {
+            const double uu = pcg_to_double(pcg_output(s));
+            keep = exp(ll[sg.off + i] - mx) > uu;
+        }
+        const uint64_t word = __ballot(keep);
+        if (lane == 0) mask[sg.mask_off + (base >> 6)] = word;
+        s = pcg_affine(s, AT, DT, sg.inc);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_seg_bsum(const SegDev *__restrict__ segs,
+                                                  const uint64_t *__restrict__ mask,
+                                                  int64_t *__restrict__ bsum)
+{
+    __shared__ int64_t red[4];
+    const SegDev sg = segs[blockIdx.y];
+    if ((int64_t)blockIdx.x >= sg.nb) return;
+    const int64_t nwords = (sg.n + 63) / 64;
+    const int64_t w0 = (int64_t)blockIdx.x * JK_WORDS_PER_BLOCK;
+    int64_t c = 0;
+    for (int k = threadIdx.x; k < JK_WORDS_PER_BLOCK; k += 256) {
+        const int64_t w = w0 + k;
+        if (w < nwords) c += __popcll(mask[sg.mask_off + w]);
+    }
+    c = wave_sum_i64(c);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) bsum[sg.bsum_off + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ __launch_bounds__(256) void k_seg_compact(const SegDev *__restrict__ segs,
+                                                     const uint64_t *__restrict__ mask,
+                                                     const int64_t *__restrict__ bsum,
+                                                     int64_t *__restrict__ idx_out,
+                                                     int64_t *__restrict__ n_pass_out)
+{
+    __shared__ int64_t red[4];
+    __shared__ int64_t s_off;
+    const SegDev sg = segs[blockIdx.y];
+    if ((int64_t)blockIdx.x >= sg.nb) return;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int64_t nwords = (sg.n + 63) / 64;
+    int64_t c = 0;
+    const int64_t lim = blockIdx.x == 0 ? sg.nb : (int64_t)blockIdx.x;
+    for (int64_t b = threadIdx.x; b < lim; b += 256) c += bsum[sg.bsum_off + b];
+    c = wave_sum_i64(c);
+    if (lane == 0) red[wid] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int64_t tot = red[0] + red[1] + red[2] + red[3];
+        if (blockIdx.x == 0) {
+            n_pass_out[blockIdx.y] = tot;
+            s_off = 0;
+        } else {
+            s_off = tot;
+        }
+    }
+    __syncthreads();
+    const int64_t off = s_off;
+    if (off >= sg.max_keep) return;
+    const int64_t w0 = (int64_t)blockIdx.x * JK_WORDS_PER_BLOCK + (int64_t)threadIdx.x * 16;
+    uint64_t words[16];
+    int64_t cnt = 0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int64_t w = w0 + q;
+        words[q] = w < nwords ? mask[sg.mask_off + w] : 0ull;
+        cnt += __popcll(words[q]);
+    }
+    const int64_t incl = wave_incl_scan_i64(cnt, lane);
+    __syncthreads();
+    if (lane == 63) red[wid] = incl;
+    __syncthreads();
+    int64_t pre = incl - cnt;
+    for (int k = 0; k < wid; ++k) pre += red[k];
+    int64_t rank = off + pre;
+#pragma unroll 1
+    for (int q = 0; q < 16 && rank < sg.max_keep; ++q) {
+        uint64_t word = words[q];
+        while (word != 0ull && rank < sg.max_keep) {
+            const int bit = __builtin_ctzll(word);
+            idx_out[sg.idx_off + rank++] = ((w0 + q) << 6) + bit;
+            word &= word - 1ull;
+        }
+    }
+}
+
+int jk_seg_flag_blocks(int64_t max_n)
+{
+    int64_t b = (max_n + 256 * 16 - 1) / (256 * 16);
+    if (b > JK_SEG_MAX_BLOCKS) b = JK_SEG_MAX_BLOCKS;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+hipError_t jk_launch_reject_batch(const double *ll, int n_seg, const SegDev *d_segs,
+                                  int64_t max_n, int64_t max_nb, double *partials, int pb,
+                                  uint64_t *mask, int64_t *bsum, int64_t *idx_out,
+                                  int64_t *n_pass_out, double *llmax_out, hipStream_t stream)
+{
+    if (n_seg <= 0) return hipSuccess;
+    const int fb = jk_seg_flag_blocks(max_n);
+    u128 AT, DT;
+    jk_pcg_jump_coeffs((uint64_t)fb * 256, &AT, &DT);
+    hipLaunchKernelGGL(k_seg_max_partial, dim3(pb, n_seg), dim3(256), 0, stream, ll, d_segs,
+                       partials, pb);
+    hipLaunchKernelGGL(k_seg_max_final, dim3(n_seg), dim3(64), 0, stream, partials, pb, llmax_out);
+    hipLaunchKernelGGL(k_seg_flags, dim3(fb, n_seg), dim3(256), 0, stream, ll, d_segs, llmax_out,
+                       AT, DT, mask);
+    hipLaunchKernelGGL(k_seg_bsum, dim3((unsigned)max_nb, n_seg), dim3(256), 0, stream, d_segs,
+                       mask, bsum);
+    hipLaunchKernelGGL(k_seg_compact, dim3((unsigned)max_nb, n_seg), dim3(256), 0, stream, d_segs,
+                       mask, bsum, idx_out, n_pass_out);
+    return hipGetLastError();
+}
+
+void jk_host_pcg_jump(u128 state, u128 inc, uint64_t k, u128 *out) { *out = h_jump(state, inc, k); }
+
+// ---------------------------------------------------------------------------
+// On-device prior cache fill.  Philox4x32-10 (Salmon, Moraes, Dror & Shaw,
+// "Parallel random numbers: as easy as 1, 2, 3", SC 2011): counter = (row,
+// block, 0, 0), key = seed; three blocks per row give six 53-bit uniforms.
+//   P     = exp(u0 (ln P_max - ln P_min) + ln P_min)    (log-uniform)
+//   e     = inverse Beta CDF, linear interpolation in a table of n_tab + 1
+//           equally spaced quantiles built on the host
+//   omega = 2 pi u2,  M0 = 2 pi u3
+//   s     = exp(mu + sigma z), z = sqrt(-2 ln(1 - u4)) cos(2 pi u5), or 0
+// FP64 master columns and their FP32 copy are written in one pass.
+// ---------------------------------------------------------------------------
+struct Philox4 {
+    uint32_t v[4];
+};
+
+__device__ __forceinline__ Philox4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                                 uint32_t k0, uint32_t k1)
+{
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t hi0 = __umulhi(M0, c0), lo0 = M0 * c0;
+        const uint32_t hi1 = __umulhi(M1, c2), lo1 = M1 * c2;
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += W0;
+        k1 += W1;
+    }
+    Philox4 o;
+    o.v[0] = c0;
+    o.v[1] = c1;
+    o.v[2] = c2;
+    o.v[3] = c3;
+    return o;
+}
+
+__device__ __forceinline__ double u53(uint32_t a, uint32_t b)
+{
+    return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+__global__ __launch_bounds__(256) void k_prior_fill(int64_t n, int64_t row0, uint32_t k0,
+                                                    uint32_t k1, double lnPmin, double lnPspan,
+                                                    const double *__restrict__ e_tab, int n_tab,
+                                                    double s_mu, double s_sigma,
+                                                    double *__restrict__ P, double *__restrict__ e,
+                                                    double *__restrict__ om,
+                                                    double *__restrict__ M0,
+                                                    double *__restrict__ s, float *__restrict__ e32,
+                                                    float *__restrict__ om32,
+                                                    float *__restrict__ M032,
+                                                    float *__restrict__ s32)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t row = (uint64_t)(row0 + i);
+    const uint32_t lo = (uint32_t)row, hi = (uint32_t)(row >> 32);
+    const Philox4 a = philox4x32_10(lo, hi, 0u, 0u, k0, k1);
+    const Philox4 b = philox4x32_10(lo, hi, 1u, 0u, k0, k1);
+    const Philox4 c = philox4x32_10(lo, hi, 2u, 0u, k0, k1);
+    const double u0 = u53(a.v[0], a.v[1]), u1 = u53(a.v[2], a.v[3]);
+    const double u2 = u53(b.v[0], b.v[1]), u3 = u53(b.v[2], b.v[3]);
+    const double u4 = u53(c.v[0], c.v[1]), u5 = u53(c.v[2], c.v[3]);
+    const double Pv = exp(fma(u0, lnPspan, lnPmin));
+    const double x = u1 * n_tab;
+    int k = (int)x;
+    if (k >= n_tab) k = n_tab - 1;
+    const double f = x - k;
+    const double ev = fma(f, e_tab[k + 1] - e_tab[k], e_tab[k]);
+    const double omv = JK_TWO_PI * u2, M0v = JK_TWO_PI * u3;
+    double sv = 0.0;
+    if (s_sigma >= 0.0) {
+        const double z = sqrt(-2.0 * log1p(-u4)) * cos(JK_TWO_PI * u5);
+        sv = exp(fma(s_sigma, z, s_mu));
+    }
+    P[i] = Pv;
+    e[i] = ev;
+    om[i] = omv;
+    M0[i] = M0v;
+    s[i] = sv;
+    if (e32) {
+        e32[i] = (float)ev;
+        om32[i] = (float)omv;
+        M032[i] = (float)M0v;
+        s32[i] = (float)sv;
+    }
+}
+
+hipError_t jk_launch_prior_fill(int64_t n, int64_t row0, uint64_t seed, double P_min, double P_max,
+                                const double *e_tab, int n_tab, double s_mu, double s_sigma,
+                                double *P, double *e, double *om, double *M0, double *s,
+                                float *e32, float *om32, float *M032, float *s32,
+                                hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    const double lnPmin = log(P_min);
+    const double span = log(P_max) - lnPmin;
+    hipLaunchKernelGGL(k_prior_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, n,
+                       row0, (uint32_t)seed, (uint32_t)(seed >> 32), lnPmin, span, e_tab, n_tab,
+                       s_mu, s_sigma, P, e, om, M0, s, e32, om32, M032, s32);
     return hipGetLastError();
 }
 

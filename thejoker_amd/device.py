@@ -21,7 +21,8 @@ from . import _lib
 from . import units as u
 from .samples import NONLINEAR_INTERNAL_UNITS, NONLINEAR_PACKED_ORDER, JokerSamples
 
-__all__ = ["DeviceStar", "PriorCache", "star_arrays", "lnl", "reject", "linear_posterior", "uniforms",
+__all__ = ["DeviceStar", "PriorCache", "star_arrays", "lnl", "polish", "reject",
+           "linear_posterior", "uniforms",
            "pcg64_state", "pcg64_advance_state", "JITTER_MODES"]
 
 JITTER_MODES = {"reference": _lib.JK_JITTER_REFERENCE, "physical": _lib.JK_JITTER_PHYSICAL}
@@ -260,6 +261,40 @@ class PriorCache:
         lnp = np.asarray(samples["ln_prior"].value) if "ln_prior" in samples else None
         return cls.from_packed(packed, ln_prior=lnp, device=device, fp32_copy=fp32_copy)
 
+    @classmethod
+    def generate(cls, n, seed=0, P_min=2.0, P_max=2048.0, e_ab=(0.867, 3.03), ln_s=None,
+                 device=0, fp32_copy=True, n_tab=1 << 16):
+        """n prior rows drawn on the device (jk_prior_fill): P log-uniform on
+        [P_min, P_max] days, e ~ Beta(e_ab) (inverse CDF table), omega and M0
+        uniform on [0, 2 pi), ln s ~ N(ln_s) km/s or s = 0.  Row r depends
+        only on (seed, r)."""
+        from scipy.special import betaincinv
+
+        torch = _torch()
+        lib = _lib.ensure_device(device)
+        dev = torch.device("cuda", device)
+        self = cls.__new__(cls)
+        self.device = device
+        self.ln_prior = None
+        n = int(n)
+        q = np.linspace(0.0, 1.0, int(n_tab) + 1)
+        tab = betaincinv(e_ab[0], e_ab[1], q)
+        tab[0], tab[-1] = 0.0, min(tab[-1], np.nextafter(1.0, 0.0))
+        d_tab = torch.as_tensor(tab, device=dev)
+        self.P, self.e, self.omega, self.M0, self.s = [
+            torch.empty(n, dtype=torch.float64, device=dev) for _ in range(5)]
+        self.fp32 = None
+        f32 = [None] * 4
+        if fp32_copy:
+            self.fp32 = tuple(torch.empty(n, dtype=torch.float32, device=dev) for _ in range(4))
+            f32 = list(self.fp32)
+        mu, sig = (0.0, -1.0) if ln_s is None else (float(ln_s[0]), float(ln_s[1]))
+        _lib.check(lib.jk_prior_fill(n, 0, int(seed), float(P_min), float(P_max), _dptr(d_tab),
+                                     int(n_tab), mu, sig, _dptr(self.P), _dptr(self.e),
+                                     _dptr(self.omega), _dptr(self.M0), _dptr(self.s),
+                                     *[_dptr(x) for x in f32], _stream(device)), "jk_prior_fill")
+        return self
+
     def host_rows(self, idx):
         """(len(idx), 5) float64 rows P, e, omega, M0, s copied to the host."""
         torch = _torch()
@@ -305,6 +340,24 @@ def lnl(star, cache, rows=None, n=None, precision="f64", out=None, start=0):
         raise ValueError("precision must be 'f64' or 'mixed'")
     _lib.check(fn(star.handle, n, P, *cols, _dptr(rows), _dptr(out), _stream(cache.device)), what)
     return out
+
+
+def polish(star, cache, ll, rows=None, start=0, window=40.0):
+    """FP64 re-evaluation, in place, of every entry of `ll` (a mixed-precision
+    lnL of cache rows `rows` or [start, start + len(ll))) not below
+    max(ll) - window.  Returns a device int64 tensor holding the count."""
+    torch = _torch()
+    lib = _lib.ensure_device(cache.device)
+    n = int(ll.shape[0])
+    count = torch.zeros(1, dtype=torch.int64, device=cache.P.device)
+    if n == 0:
+        return count
+    off = 0 if rows is not None else 8 * start
+    cols = [ctypes.c_void_p(c.data_ptr() + off) for c in
+            (cache.P, cache.e, cache.omega, cache.M0, cache.s)]
+    _lib.check(lib.jk_polish_f64(star.handle, n, *cols, _dptr(rows), _dptr(ll), float(window),
+                                 _dptr(count), _stream(cache.device)), "jk_polish_f64")
+    return count
 
 
 def uniforms(rng_state, n, offset=0, device=0):

@@ -251,74 +251,29 @@ class TheJoker:
 
         star = self._make_joker_helper(data)
         cache = self._cache(star, prior_samples)
-        rng = self.rng
-        n_total = len(cache)
-        if in_memory:
-            safety_factor = 1
-            max_prior = n_total
-            all_idx = None
-        else:
-            if return_logprobs and cache.ln_prior is None:
-                raise RuntimeError("return_logprobs=True but ln_prior values not found in prior "
-                                   "cache: make sure you generate prior samples with prior.sample "
-                                   "(..., return_logprobs=True) before saving the prior samples.")
-            safety_factor = 4
-            max_prior = n_total if max_prior_samples is None else int(max_prior_samples)
-            all_idx = None
-            if randomize_prior_order:
-                all_idx = rng.choice(n_total, size=max_prior, replace=False)
-
-        maxiter = 128
-        n_process = growth_factor * n_requested_samples if init_batch_size is None else init_batch_size
-        if n_process > max_prior:
-            raise ValueError("Prior sample library not big enough! For iterative sampling, you "
-                             "have to have at least growth_factor * n_requested_samples = "
-                             f"{growth_factor * n_requested_samples} samples in the prior samples "
-                             f"cache file. You have, or have limited to, {max_prior} samples.")
-
-        dev = cache.P.device
-        all_ll = torch.empty(max_prior, dtype=torch.float64, device=dev)
-        d_all_idx = None
-        if all_idx is not None:
-            d_all_idx = torch.as_tensor(np.asarray(all_idx, dtype=np.int64), device=dev)
-        start_idx = 0
-        n_evals = 0
-        good = None
-        for i in range(maxiter):
-            logger.log(1, f"iteration {i}, computing {n_process} likelihoods")
-            n_now = min(n_process, max_prior - start_idx)
-            if d_all_idx is not None:
-                part = self._lnl_rows(star, cache, rows=d_all_idx[start_idx:start_idx + n_now])
+        if not in_memory and return_logprobs and cache.ln_prior is None:
+            raise RuntimeError("return_logprobs=True but ln_prior values not found in prior "
+                               "cache: make sure you generate prior samples with prior.sample "
+                               "(..., return_logprobs=True) before saving the prior samples.")
+        it = IterativeRejection(len(cache), n_requested_samples, self.rng, in_memory=in_memory,
+                                max_prior_samples=max_prior_samples,
+                                randomize_prior_order=randomize_prior_order,
+                                init_batch_size=init_batch_size, growth_factor=growth_factor,
+                                device=cache.P.device)
+        all_ll = torch.empty(it.max_prior, dtype=torch.float64, device=cache.P.device)
+        while not it.done:
+            rows, start, n_now = it.next_batch()
+            part = all_ll[it.n_evals:it.n_evals + n_now]
+            if rows is not None:
+                part.copy_(self._lnl_rows(star, cache, rows=rows))
             else:
-                part = self._lnl_rows(star, cache, start=start_idx, n=n_now)
-            all_ll[n_evals:n_evals + n_now] = part
-            n_evals += n_now
-            view = all_ll[:n_evals]
-            if in_memory and not bool(torch.isfinite(view).all()):
-                raise RuntimeError(f"There are NaN or Inf likelihood values in iteration step {i}!")
-            good, n_good, _ = reject(view, pcg64_state(rng), n_requested_samples,
-                                    device=self.device)
-            pcg64_advance_state(rng, n_evals)
-            if n_good == 0:
-                raise RuntimeError("Failed to find any good samples!")
-            logger.log(1, f"{n_good} good samples after rejection sampling")
-            if n_good >= n_requested_samples:
-                logger.log(1, "Enough samples found!")
-                break
-            start_idx += n_process
-            n_process = next_batch_size(n_requested_samples, n_good, n_evals, start_idx,
-                                        max_prior, safety_factor)
-            if n_process <= 0:
-                break
-        else:
-            raise RuntimeError("Hit maximum number of iterations!")
-
-        good = good[:n_requested_samples]
-        full_idx = all_idx[good] if all_idx is not None else good
-        samples = self._full_samples(star, cache, full_idx, rng, n_linear_samples,
+                part.copy_(self._lnl_rows(star, cache, start=start, n=n_now))
+            it.absorb(all_ll[:it.n_evals + n_now], self.device)
+        full_idx, good = it.result()
+        samples = self._full_samples(star, cache, full_idx, self.rng, n_linear_samples,
                                      spawn_batches=None if in_memory else self._n_batches(n_batches))
         if return_logprobs:
-            ll_h = all_ll[:n_evals].cpu().numpy()
+            ll_h = all_ll[:it.n_evals].cpu().numpy()
             if in_memory:
                 if cache.ln_prior is not None:
                     samples["ln_prior"] = cache.ln_prior[full_idx]
@@ -327,6 +282,98 @@ class TheJoker:
                 samples["ln_likelihood"] = ll_h[good]
                 samples["ln_prior"] = cache.ln_prior[full_idx]
         return samples
+
+
+class IterativeRejection:
+    """State of one iterative rejection run -- the loop of
+    likelihood_helpers.py:130-229 (in_memory, safety_factor 1) and
+    multiproc_helpers.py:289-427 (default, safety_factor 4, optional
+    randomised prior order, max_prior_samples).  Each round evaluates the next
+    batch of rows, then redraws uniforms for every lnL so far, so the rows and
+    the random stream consumed depend only on this star's lnL values.
+
+    Use: ``while not it.done: rows, start, n = it.next_batch(); <lnL of those
+    rows appended to the star's lnL buffer>; it.absorb(ll_so_far)``."""
+
+    maxiter = 128
+
+    def __init__(self, n_total, n_requested, rng, in_memory=False, max_prior_samples=None,
+                 randomize_prior_order=False, init_batch_size=None, growth_factor=128,
+                 device=None):
+        self.rng = rng
+        self.n_requested = int(n_requested)
+        self.in_memory = in_memory
+        if in_memory:
+            self.safety_factor = 1
+            self.max_prior = int(n_total)
+            self.all_idx = None
+        else:
+            self.safety_factor = 4
+            self.max_prior = int(n_total) if max_prior_samples is None else int(max_prior_samples)
+            self.all_idx = None
+            if randomize_prior_order:
+                self.all_idx = rng.choice(int(n_total), size=self.max_prior, replace=False)
+        self.n_process = (growth_factor * self.n_requested if init_batch_size is None
+                          else int(init_batch_size))
+        if self.n_process > self.max_prior:
+            raise ValueError("Prior sample library not big enough! For iterative sampling, you "
+                             "have to have at least growth_factor * n_requested_samples = "
+                             f"{growth_factor * self.n_requested} samples in the prior samples "
+                             f"cache file. You have, or have limited to, {self.max_prior} "
+                             "samples.")
+        self._d_all_idx = None
+        if self.all_idx is not None:
+            import torch
+            self._d_all_idx = torch.as_tensor(np.asarray(self.all_idx, dtype=np.int64),
+                                              device=device)
+        self.start_idx = 0
+        self.n_evals = 0
+        self.iteration = 0
+        self.good = None
+        self.n_good = 0
+        self.done = False
+
+    def next_batch(self):
+        """(device rows or None, first cache row, count) of the next round."""
+        n_now = min(self.n_process, self.max_prior - self.start_idx)
+        if self._d_all_idx is not None:
+            return self._d_all_idx[self.start_idx:self.start_idx + n_now], 0, n_now
+        return None, self.start_idx, n_now
+
+    def absorb(self, ll_so_far, device=0):
+        """Rejection over every lnL evaluated so far (a device tensor of
+        n_evals + the new rows), then the growth rule."""
+        import torch
+
+        i = self.iteration
+        self.n_evals = int(ll_so_far.shape[0])
+        logger.log(1, f"iteration {i}, computed {self.n_evals} likelihoods")
+        if self.in_memory and not bool(torch.isfinite(ll_so_far).all()):
+            raise RuntimeError(f"There are NaN or Inf likelihood values in iteration step {i}!")
+        good, n_good, _ = reject(ll_so_far, pcg64_state(self.rng), self.n_requested, device=device)
+        pcg64_advance_state(self.rng, self.n_evals)
+        self.good, self.n_good = good, n_good
+        self.iteration += 1
+        if n_good == 0:
+            raise RuntimeError("Failed to find any good samples!")
+        logger.log(1, f"{n_good} good samples after rejection sampling")
+        if n_good >= self.n_requested:
+            self.done = True
+            return
+        self.start_idx += self.n_process
+        self.n_process = next_batch_size(self.n_requested, n_good, self.n_evals, self.start_idx,
+                                         self.max_prior, self.safety_factor)
+        if self.n_process <= 0:
+            self.done = True
+            return
+        if self.iteration >= self.maxiter:
+            raise RuntimeError("Hit maximum number of iterations!")
+
+    def result(self):
+        """(cache rows of the kept samples, their positions in the lnL buffer)."""
+        good = self.good[:self.n_requested]
+        full_idx = self.all_idx[good] if self.all_idx is not None else good
+        return full_idx, good
 
 
 def _linear_posterior(star, cache, rows):
@@ -342,16 +389,6 @@ POLISH_WINDOW = 40.0
 
 
 def _polish(star, cache, ll, rows=None, start=0):
-    import torch
-
-    mx = torch.max(ll)
-    if not bool(torch.isfinite(mx)):
-        # NaN or +inf anywhere: recompute everything in FP64 so the maximum
-        # and every comparison are FP64's.
-        return lnl(star, cache, rows=rows, start=start, n=int(ll.shape[0]), precision="f64")
-    live = torch.nonzero(ll >= mx - POLISH_WINDOW).flatten()
-    if live.numel() == 0:
-        return ll
-    src = live + start if rows is None else rows.index_select(0, live)
-    ll[live] = lnl(star, cache, rows=src, precision="f64")
+    from .device import polish
+    polish(star, cache, ll, rows=rows, start=start, window=POLISH_WINDOW)
     return ll
