@@ -1,24 +1,34 @@
 """Benchmark of The Joker's likelihood + rejection path on one MI355X.
 
-Default workload (N = 1): the single-star plain-FP64 case -- one star with
+One JSON line.  `value` is the single-star plain-FP64 workload: one star with
 M = 16 epochs over 1000 d, 2^24 prior rows (P log-uniform on [2, 2048] d,
 e ~ Beta(0.867, 3.03), omega, M0 ~ U[0, 2 pi), ln s ~ N(-3, 1) km/s),
 K ~ N(0, 30), v0 ~ N(0, 100), physical jitter mode.  One step = the FP64
-marginal-likelihood kernel over all 2^24 device-resident rows plus the
-on-device rejection step (max, numpy-PCG64 uniforms, ordered compaction of
-at most 256 indices) and the read-back of the accepted indices.
+marginal-likelihood kernel over all 2^24 device-resident rows, the on-device
+rejection step (max, numpy-PCG64 uniforms, ordered compaction of at most 256
+indices) and the read-back of the accepted indices; value = prior-samples x
+epochs / s of the step.
 
-value = prior-samples x epochs / s of the whole step.  The JSON line also
-carries:
-  roofline      -- the likelihood kernel's algorithmic FLOP rate (HIP events
-                   around its launches on the launch stream) against the
-                   FP64 vector peak;
-  cpu_baseline  -- the oracle's restatement of the reference algorithm
-                   (LU + Woodbury, fast_likelihood.pyx:255-469) on the host
-                   cores over a bounded sample of the same rows;
-  mixed         -- the mixed-precision kernel on the same rows: its rate and
-                   its error against FP64 (max |dlnL| over all rows and over
-                   the live set, acceptance flips after FP64 re-evaluation).
+Further legs (each an object of the same line; --workload picks one):
+  mixed     -- the mixed-precision kernel on the same rows, its error
+               against FP64 (max |dlnL| all rows / live set);
+  batched   -- 1024 stars, M ~ U{3..64}, baselines U[200, 3000] d, half
+               binaries, poly_trend = 2, 2^20 shared rows: one autotuned
+               persistent launch, mixed vs FP64 rates, errors, acceptance
+               flips after FP64 re-evaluation of the live sets;
+  serving   -- Poisson arrivals at 200 stars/s, M log-uniform on [3, 200],
+               2^26-row device cache, iterative_rejection_sample to 256
+               samples per star under continuous batching: p50 / p99
+               latency, throughput, flips against FP64 runs of the same
+               stars alone;
+  stress    -- one star, M = 50 over 3650 d, P log-uniform on [1, 16384] d,
+               2^29 rows generated and kept in HBM, evaluated in 2^26-row
+               chunks in FP64 and mixed: rates, HBM GB/s, errors, flips.
+roofline: the FP64 likelihood kernel's algorithmic FLOP rate (HIP events
+around its launches on the launch stream) against the FP64 vector peak;
+traffic from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary.
+cpu_baseline: the oracle's restatement of the reference algorithm (LU +
+Woodbury, fast_likelihood.pyx:255-469) on the host cores, bounded sample.
 """
 import argparse
 import json
@@ -33,14 +43,16 @@ if ROOT not in sys.path:
 METRIC = ("prior-samples×epochs/s and p50/p99 star latency, mixed vs FP64 at "
           "|ΔlnL|≤1e-3")
 # Algorithmic FLOPs (weights: add/mul 1, FMA 2, div/sqrt 4, sin/cos/exp/log 8;
-# Kepler = starter + 4 Newton steps): per sample-epoch with sampled jitter, and
-# per sample for L = 2.
+# Kepler = starter + 4 Newton steps) per sample-epoch with sampled jitter, and
+# per sample for L = 2 / 3.
 F_EPOCH_JITTER = 176
-F_EPOCH_NOJITTER = 156
-F_SAMPLE_L2 = 90
+F_EPOCH_L3 = 184
+F_SAMPLE = {2: 90, 3: 128}
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 BYTES_PER_SAMPLE_F64 = 56   # 5 x f64 read + lnL write + lnL read by the rejection step
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_lnl_f64_single_star.json")
+WINDOW = 40.0
 
 
 def log(msg):
@@ -69,14 +81,323 @@ def cpu_baseline(star, rows, mu, lam, n_sample):
                       f"restated in C (-O3), {cores} threads, {dt:.2f} s wall"}
 
 
+def timed(fn, reps, torch):
+    """Mean ms of fn() over reps launches, HIP events on the current stream."""
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        fn()
+    ev1.record()
+    torch.cuda.synchronize()
+    return ev0.elapsed_time(ev1) / reps
+
+
+# ---------------------------------------------------------------------------
+def leg_single(args, torch, np, D, syn):
+    n = 1 << args.log2_rows
+    log(f"[bench] single: M=16 star, {n} prior rows")
+    star = syn.single_star()
+    cols = syn.prior_columns(n, seed=0, jitter=True)
+    mu, lam = np.zeros(2), np.array([900.0, 1e4])
+    dstar = D.DeviceStar(star.t, star.rv, star.ivar(), star.trend(), mu, lam, star.t_ref,
+                         jitter_mode="physical", device=0)
+    cache = D.PriorCache(cols["P"], cols["e"], cols["omega"], cols["M0"], cols["s"], device=0,
+                         fp32_copy=True)
+    rng = np.random.default_rng(42)
+    ll = torch.empty(n, dtype=torch.float64, device="cuda")
+
+    def step():
+        D.lnl(dstar, cache, precision="f64", out=ll)
+        idx, n_pass, mx = D.reject(ll, D.pcg64_state(rng), 256)
+        D.pcg64_advance_state(rng, n)
+        return idx, n_pass
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        idx, n_pass = step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_step = dt / args.steps * 1e3
+    value = n * star.M / (dt / args.steps)
+    log(f"[bench] single: {ms_step:.3f} ms/step, {value:.4e} sample-epochs/s, accepted {n_pass}")
+
+    reps = max(10, args.steps)
+    k_ms = timed(lambda: D.lnl(dstar, cache, precision="f64", out=ll), reps, torch)
+    flops = n * (star.M * F_EPOCH_JITTER + F_SAMPLE[2])
+    achieved = flops / (k_ms * 1e-3) / 1e12
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            tr = json.load(f)
+        if tr.get("rows") == n:
+            traffic = tr["hbm_bytes_per_launch"]
+    roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+                "kernel": "k_lnl<2,true,false> (FP64)", "kernel_ms": k_ms,
+                "flops_per_launch": flops,
+                "hbm_gbs_prior_stream": n * BYTES_PER_SAMPLE_F64 / (k_ms * 1e-3) / 1e9}
+    log(f"[bench] single: lnL kernel {k_ms:.3f} ms -> {achieved:.2f} TFLOP/s algorithmic "
+        f"({100 * achieved / PEAK_FP64_TFLOPS:.1f}% of FP64 peak)")
+
+    out = {"metric": METRIC, "value": value, "unit": "sample-epochs/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": "single star, plain FP64 likelihood + rejection: M=16 epochs, "
+                                  f"2^{args.log2_rows} prior rows, ln s ~ N(-3,1), physical "
+                                  "jitter, K~N(0,30), v0~N(0,100), max_posterior_samples=256",
+                      "epochs": star.M, "prior_rows": n, "linear_params": 2,
+                      "parallelism": "single GPU"},
+           "roofline": roofline}
+
+    # mixed precision on the same rows
+    ll32 = torch.empty(n, dtype=torch.float64, device="cuda")
+    D.lnl(dstar, cache, precision="mixed", out=ll32)
+    m_ms = timed(lambda: D.lnl(dstar, cache, precision="mixed", out=ll32), reps, torch)
+    D.lnl(dstar, cache, precision="f64", out=ll)
+    ll_h, m_h = ll.cpu().numpy(), ll32.cpu().numpy()
+    d = np.abs(m_h - ll_h)
+    live = ll_h >= ll_h.max() - WINDOW
+    g1, g2 = np.random.default_rng(7), np.random.default_rng(7)
+    D.polish(dstar, cache, ll32, window=WINDOW)
+    a_f, _, _ = D.reject(ll, D.pcg64_state(g1), n)
+    a_m, _, _ = D.reject(ll32, D.pcg64_state(g2), n)
+    out["mixed"] = {"kernel_ms": m_ms, "sample_epochs_per_s": n * star.M / (m_ms * 1e-3),
+                    "speedup_vs_f64_kernel": k_ms / m_ms, "max_abs_dlnl_all": float(d.max()),
+                    "max_abs_dlnl_live": float(d[live].max()), "live_rows": int(live.sum()),
+                    "accept_flips_after_fp64_window": int(len(np.setxor1d(a_f, a_m)))}
+    log(f"[bench] mixed: {m_ms:.3f} ms ({k_ms / m_ms:.2f}x), max|dlnL| all {d.max():.2e} "
+        f"live {d[live].max():.2e}, flips {out['mixed']['accept_flips_after_fp64_window']}")
+
+    if not args.no_cpu:
+        rows = np.ascontiguousarray(np.stack([cols[k] for k in ("P", "e", "omega", "M0", "s")],
+                                             axis=1))
+        n_cpu = min(n, 1 << 21)
+        out["cpu_baseline"] = cpu_baseline(star, rows, mu, lam, n_cpu)
+        log(f"[bench] cpu {out['cpu_baseline']['value']:.3e} sample-epochs/s")
+    del cache, ll, ll32
+    torch.cuda.empty_cache()
+    return out
+
+
+# ---------------------------------------------------------------------------
+def _l3_star(D, np, st):
+    return D.DeviceStar(st.t, st.rv, st.ivar(), st.trend(poly_trend=2), np.zeros(3),
+                        np.array([900.0, 1e4, 0.25]), st.t_ref, jitter_mode="physical", device=0)
+
+
+def leg_batched(args, torch, np, D, syn):
+    from thejoker_amd import batch as B
+
+    n_stars, n = 1024, 1 << 20
+    log(f"[bench] batched: {n_stars} stars x {n} rows")
+    stars = syn.many_stars(n_stars, seed=3)
+    dstars = [_l3_star(D, np, st) for st in stars]
+    cols = syn.prior_columns(n, seed=0, jitter=True)
+    cache = D.PriorCache(cols["P"], cols["e"], cols["omega"], cols["M0"], cols["s"], device=0)
+    batch = B.StarBatch(dstars)
+    M = batch.M
+    work = int(M.sum()) * n
+    t0 = time.perf_counter()
+    shape, table = B.autotune(batch, cache, [n] * n_stars, "mixed", sample_stars=128)
+    tune_s = time.perf_counter() - t0
+    log(f"[bench] batched: autotuned {shape} in {tune_s:.1f} s")
+    items, off, total = B.work_items([n] * n_stars, shape.tile)
+    items = items[B.heaviest_first(items, M)]
+    d_items = B.upload_items(items)
+    ws = torch.zeros(64, dtype=torch.int32, device="cuda")
+    llm = torch.empty(total, dtype=torch.float64, device="cuda")
+    llf = torch.empty(total, dtype=torch.float64, device="cuda")
+    run_m = lambda: B.lnl_batch(batch, cache, d_items, len(items), total, "mixed", shape,
+                                out=llm, workspace=ws)
+    run_f = lambda: B.lnl_batch(batch, cache, d_items, len(items), total, "f64", shape,
+                                out=llf, workspace=ws)
+    run_m()
+    run_f()
+    m_ms = timed(run_m, 3, torch)
+    f_ms = timed(run_f, 2, torch)
+    # errors before the FP64 window, per star
+    dmax_all, dmax_live = 0.0, 0.0
+    for s in range(n_stars):
+        a = llf[off[s]:off[s] + n]
+        b = llm[off[s]:off[s] + n]
+        dd = (a - b).abs()
+        live = a >= a.max() - WINDOW
+        dmax_all = max(dmax_all, float(dd.max()))
+        dmax_live = max(dmax_live, float(dd[live].max()))
+    sizes = np.full(n_stars, n)
+    states = lambda: [D.pcg64_state(np.random.default_rng(1000 + s)) for s in range(n_stars)]
+
+    def step():
+        run_m()
+        B.polish_batch(batch, cache, llm, off, sizes, window=WINDOW)
+        return B.reject_batch(llm, off, sizes, states(), 256)
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(2):
+        good_m, np_m, _ = step()
+    torch.cuda.synchronize()
+    e2e_ms = (time.perf_counter() - t0) / 2 * 1e3
+    good_f, np_f, _ = B.reject_batch(llf, off, sizes, states(), 256)
+    flips = sum(len(np.setxor1d(x, y)) for x, y in zip(good_m, good_f))
+    res = {"stars": n_stars, "prior_rows": n, "epochs_total": int(M.sum()), "linear_params": 3,
+           "launch_shape": shape.as_dict(), "autotune_s": tune_s,
+           "autotune_ms": {k: round(v, 3) for k, v in table.items()},
+           "mixed_lnl_ms": m_ms, "f64_lnl_ms": f_ms,
+           "mixed_sample_epochs_per_s": work / (m_ms * 1e-3),
+           "f64_sample_epochs_per_s": work / (f_ms * 1e-3),
+           "mixed_speedup_vs_f64": f_ms / m_ms,
+           "mixed_end_to_end_ms": e2e_ms,
+           "mixed_end_to_end_sample_epochs_per_s": work / (e2e_ms * 1e-3),
+           "max_abs_dlnl_all": dmax_all, "max_abs_dlnl_live": dmax_live,
+           "accept_flips_after_fp64_window": int(flips),
+           "accepted_total": int(np.minimum(np_f, 256).sum())}
+    log(f"[bench] batched: mixed {m_ms:.2f} ms ({res['mixed_sample_epochs_per_s']:.3e}/s), f64 "
+        f"{f_ms:.2f} ms, e2e {e2e_ms:.2f} ms, |dlnL| all {dmax_all:.2e} live {dmax_live:.2e}, "
+        f"flips {flips}")
+    del llm, llf, cache, batch
+    torch.cuda.empty_cache()
+    return res
+
+
+# ---------------------------------------------------------------------------
+def leg_serving(args, torch, np, D, syn):
+    import thejoker_amd as tj
+    from thejoker_amd import serving as S
+    from thejoker_amd import units as u
+
+    n_req = args.serving_requests
+    log(f"[bench] serving: {n_req} requests at 200/s, 2^26-row cache")
+    cache = D.PriorCache.generate(1 << 26, seed=4, P_min=2.0, P_max=2048.0, ln_s=(-3.0, 1.0))
+    prior = tj.JokerPrior.default(
+        P_min=2 * u.day, P_max=2048 * u.day,
+        sigma_v=[100 * u.km / u.s, 0.5 * u.km / u.s / u.day], poly_trend=2,
+        pars={"K": tj.distributions.Normal(0.0, 30.0, unit=u.km / u.s),
+              "s": tj.distributions.LogNormal(-3.0, 1.0, unit=u.km / u.s)})
+    times, stars = syn.arrivals(n_req, rate=200.0, seed=4)
+    datas = [st.to_rvdata() for st in stars]
+    # warm up the launch paths on a few requests
+    wsrv = S.JokerServer(prior, cache, precision="mixed", max_active=8)
+    S.run_poisson(wsrv, np.zeros(4), datas[:4], [np.random.default_rng(s) for s in range(4)])
+    del wsrv
+    srv = S.JokerServer(prior, cache, precision="mixed", max_active=args.serving_active,
+                        hbm_budget_bytes=8 << 30)
+    rngs = [np.random.default_rng(10_000 + i) for i in range(n_req)]
+    t0 = time.perf_counter()
+    done = S.run_poisson(srv, times, datas, rngs)
+    wall = time.perf_counter() - t0
+    lat = np.array([r.latency for r in done if r.error is None])
+    errs = [r for r in done if r.error is not None]
+    res = {"requests": n_req, "rate_per_s": 200.0, "completed": int(len(lat)),
+           "failed": len(errs), "p50_latency_s": float(np.percentile(lat, 50)),
+           "p99_latency_s": float(np.percentile(lat, 99)), "mean_latency_s": float(lat.mean()),
+           "wall_s": wall, "stars_per_s": len(lat) / wall,
+           "sample_epochs": int(srv.sample_epochs),
+           "sample_epochs_per_s": srv.sample_epochs / wall, "steps": srv.steps,
+           "max_active": args.serving_active, "prior_rows": len(cache)}
+    log(f"[bench] serving: p50 {res['p50_latency_s'] * 1e3:.1f} ms p99 "
+        f"{res['p99_latency_s'] * 1e3:.1f} ms, {res['stars_per_s']:.1f} stars/s, "
+        f"{res['sample_epochs_per_s']:.3e} sample-epochs/s, {srv.steps} steps")
+    # capacity: the whole stream submitted at once
+    srv2 = S.JokerServer(prior, cache, precision="mixed", max_active=args.serving_active,
+                         hbm_budget_bytes=8 << 30)
+    rngs2 = [np.random.default_rng(10_000 + i) for i in range(n_req)]
+    t0 = time.perf_counter()
+    done2 = S.run_poisson(srv2, np.zeros(n_req), datas, rngs2)
+    wall2 = time.perf_counter() - t0
+    res["saturated"] = {"wall_s": wall2, "stars_per_s": n_req / wall2,
+                        "sample_epochs_per_s": srv2.sample_epochs / wall2}
+    # the same stars alone in plain FP64: identical accepted rows?
+    flips, checked = 0, 0
+    for i in range(0, n_req, max(1, n_req // 16)):
+        if done[i].error is not None:
+            continue
+        joker = tj.TheJoker(prior, rng=np.random.default_rng(10_000 + i), precision="f64")
+        alone = joker.iterative_rejection_sample(datas[i], cache, n_requested_samples=256)
+        a, b = alone["P"].value, done[i].result["P"].value
+        flips += int(len(np.setxor1d(a, b)))
+        checked += 1
+    res["fp64_alone_checked"] = checked
+    res["accept_flips_vs_fp64_alone"] = flips
+    log(f"[bench] serving: saturated {res['saturated']['stars_per_s']:.1f} stars/s; flips vs "
+        f"FP64 alone on {checked} stars: {flips}")
+    del cache
+    torch.cuda.empty_cache()
+    return res
+
+
+# ---------------------------------------------------------------------------
+def leg_stress(args, torch, np, D, syn):
+    log2 = args.stress_log2_rows
+    n = 1 << log2
+    chunk = min(n, 1 << 26)
+    log(f"[bench] stress: M=50 star, 2^{log2} rows in 2^{chunk.bit_length() - 1}-row chunks")
+    star = syn.single_star(seed=5, M=50, baseline=3650.0)
+    dstar = D.DeviceStar(star.t, star.rv, star.ivar(), star.trend(), np.zeros(2),
+                         np.array([900.0, 1e4]), star.t_ref, jitter_mode="physical", device=0)
+    t0 = time.perf_counter()
+    cache = D.PriorCache.generate(n, seed=5, P_min=1.0, P_max=16384.0, ln_s=(-3.0, 1.0))
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+    llf = torch.empty(n, dtype=torch.float64, device="cuda")
+    llm = torch.empty(n, dtype=torch.float64, device="cuda")
+
+    def run(prec, out):
+        for s0 in range(0, n, chunk):
+            D.lnl(dstar, cache, precision=prec, start=s0, n=min(chunk, n - s0), out=out[s0:s0 + chunk])
+
+    run("f64", llf)
+    run("mixed", llm)
+    f_ms = timed(lambda: run("f64", llf), 2, torch)
+    m_ms = timed(lambda: run("mixed", llm), 2, torch)
+    work = n * star.M
+    d = (llf - llm).abs()
+    mx = llf.max()
+    live = llf >= mx - WINDOW
+    dmax_all, dmax_live = float(d.max()), float(d[live].max())
+    n_live = int(live.sum())
+    del d, live
+    D.polish(dstar, cache, llm, window=WINDOW)
+    a_f, n_f, _ = D.reject(llf, D.pcg64_state(np.random.default_rng(5)), 256)
+    a_m, n_m, _ = D.reject(llm, D.pcg64_state(np.random.default_rng(5)), 256)
+    flops = n * (star.M * F_EPOCH_JITTER + F_SAMPLE[2])
+    res = {"prior_rows": n, "epochs": star.M, "generate_s": gen_s,
+           "f64_ms": f_ms, "mixed_ms": m_ms,
+           "f64_sample_epochs_per_s": work / (f_ms * 1e-3),
+           "mixed_sample_epochs_per_s": work / (m_ms * 1e-3),
+           "mixed_speedup_vs_f64": f_ms / m_ms,
+           "f64_tflops_algorithmic": flops / (f_ms * 1e-3) / 1e12,
+           "f64_frac_fp64_peak": flops / (f_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+           "f64_hbm_gbs": n * 48 / (f_ms * 1e-3) / 1e9,
+           "mixed_hbm_gbs": n * 32 / (m_ms * 1e-3) / 1e9,
+           "max_abs_dlnl_all": dmax_all, "max_abs_dlnl_live": dmax_live, "live_rows": n_live,
+           "accepted": int(min(n_f, 256)),
+           "accept_flips_after_fp64_window": int(len(np.setxor1d(a_f, a_m)))}
+    log(f"[bench] stress: f64 {f_ms:.1f} ms mixed {m_ms:.1f} ms ({f_ms / m_ms:.2f}x), "
+        f"|dlnL| all {dmax_all:.2e} live {dmax_live:.2e}, flips "
+        f"{res['accept_flips_after_fp64_window']}")
+    del llf, llm, cache
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--log2-rows", type=int, default=24)
+    ap.add_argument("--workload", default="all",
+                    choices=["all", "single", "batched", "serving", "stress"])
+    ap.add_argument("--serving-requests", type=int, default=1000)
+    ap.add_argument("--serving-active", type=int, default=64)
+    ap.add_argument("--stress-log2-rows", type=int, default=29)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (the "
                          "reference dispatch for scaling FETCH_SIZE / WRITE_SIZE counters)")
@@ -103,95 +424,15 @@ def main():
         print(json.dumps({"copy_calibration_bytes": nbytes, "dispatches": 3}), flush=True)
         return
 
-    n = 1 << args.log2_rows
-    log(f"[bench] building workload: M=16 star, {n} prior rows")
-    star = syn.single_star()
-    cols = syn.prior_columns(n, seed=0, jitter=True)
-    mu, lam = np.zeros(2), np.array([900.0, 1e4])
-    dstar = D.DeviceStar(star.t, star.rv, star.ivar(), star.trend(), mu, lam, star.t_ref,
-                         jitter_mode="physical", device=0)
-    cache = D.PriorCache(cols["P"], cols["e"], cols["omega"], cols["M0"], cols["s"], device=0,
-                         fp32_copy=not args.no_mixed)
-    rng = np.random.default_rng(42)
-    ll = torch.empty(n, dtype=torch.float64, device="cuda")
-
-    def step():
-        D.lnl(dstar, cache, precision="f64", out=ll)
-        idx, n_pass, mx = D.reject(ll, D.pcg64_state(rng), 256)
-        D.pcg64_advance_state(rng, n)
-        return idx, n_pass
-
-    log(f"[bench] warmup {args.warmup}")
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    log(f"[bench] timing {args.steps} steps")
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        idx, n_pass = step()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    ms_step = dt / args.steps * 1e3
-    value = n * star.M / (dt / args.steps)
-    log(f"[bench] {ms_step:.3f} ms/step, {value:.4e} sample-epochs/s, accepted {n_pass}")
-
-    # Kernel time of the likelihood launch alone (HIP events on the launch stream).
-    reps = max(5, args.steps)
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record()
-    for _ in range(reps):
-        D.lnl(dstar, cache, precision="f64", out=ll)
-    ev1.record()
-    torch.cuda.synchronize()
-    k_ms = ev0.elapsed_time(ev1) / reps
-    flops = n * (star.M * F_EPOCH_JITTER + F_SAMPLE_L2)
-    achieved = flops / (k_ms * 1e-3) / 1e12
-    roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                "kernel": "k_lnl<2,true,false> (FP64)", "kernel_ms": k_ms,
-                "flops_per_launch": flops,
-                "hbm_gbs_prior_stream": n * BYTES_PER_SAMPLE_F64 / (k_ms * 1e-3) / 1e9}
-    log(f"[bench] lnL kernel {k_ms:.3f} ms -> {achieved:.2f} TFLOP/s algorithmic "
-        f"({100 * achieved / PEAK_FP64_TFLOPS:.1f}% of FP64 peak)")
-
-    out = {"metric": METRIC, "value": value, "unit": "sample-epochs/s", "n_gpus": 1,
-           "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-           "data": "synthetic",
-           "config": {"workload": "single star, plain FP64 likelihood + rejection: M=16 epochs, "
-                                  f"2^{args.log2_rows} prior rows, ln s ~ N(-3,1), physical "
-                                  "jitter, K~N(0,30), v0~N(0,100), max_posterior_samples=256",
-                      "epochs": star.M, "prior_rows": n, "linear_params": 2,
-                      "parallelism": "single GPU"},
-           "roofline": roofline}
-
-    if not args.no_mixed:
-        ll32 = torch.empty(n, dtype=torch.float64, device="cuda")
-        D.lnl(dstar, cache, precision="mixed", out=ll32)
-        torch.cuda.synchronize()
-        ev0.record()
-        for _ in range(reps):
-            D.lnl(dstar, cache, precision="mixed", out=ll32)
-        ev1.record()
-        torch.cuda.synchronize()
-        m_ms = ev0.elapsed_time(ev1) / reps
-        D.lnl(dstar, cache, precision="f64", out=ll)
-        diff = (ll32 - ll).abs()
-        live = ll >= ll.max() - 40
-        out["mixed"] = {"kernel_ms": m_ms, "sample_epochs_per_s": n * star.M / (m_ms * 1e-3),
-                        "speedup_vs_f64_kernel": k_ms / m_ms,
-                        "max_abs_dlnl_all": float(diff.max()),
-                        "max_abs_dlnl_live": float(diff[live].max()),
-                        "live_rows": int(live.sum())}
-        log(f"[bench] mixed kernel {m_ms:.3f} ms ({k_ms / m_ms:.2f}x), max|dlnL| all "
-            f"{out['mixed']['max_abs_dlnl_all']:.2e} live {out['mixed']['max_abs_dlnl_live']:.2e}")
-
-    if not args.no_cpu:
-        rows = np.stack([cols[k] for k in ("P", "e", "omega", "M0", "s")], axis=1)
-        n_cpu = min(n, 1 << 21)
-        log(f"[bench] cpu baseline on {n_cpu} rows")
-        out["cpu_baseline"] = cpu_baseline(star, np.ascontiguousarray(rows), mu, lam, n_cpu)
-        log(f"[bench] cpu {out['cpu_baseline']['value']:.3e} sample-epochs/s")
+    out = leg_single(args, torch, np, D, syn) if args.workload in ("all", "single") else {
+        "metric": METRIC, "value": None, "unit": "sample-epochs/s", "n_gpus": 1}
+    legs = {"batched": leg_batched, "serving": leg_serving, "stress": leg_stress}
+    for name, fn in legs.items():
+        if args.workload in ("all", name):
+            out[name] = fn(args, torch, np, D, syn)
+    if "serving" in out and out.get("value"):
+        out["serving"]["sample_epochs_per_s_vs_single_f64"] = (
+            out["serving"]["saturated"]["sample_epochs_per_s"] / out["value"])
     print(json.dumps(out), flush=True)
 
 

@@ -214,6 +214,26 @@ int jk_lnl_batch(const jk_batch *batch, int precision, int64_t n_items,
 int jk_reject_batch(const double *ll, int n_seg, const jk_segment *segs, int64_t *idx_out,
                     int64_t *n_pass_out, double *llmax_out, void *stream);
 
+/* One segment of jk_polish_batch: ll[offset .. offset + n) holds the mixed
+ * lnL of star `star` (star-table index) on cache rows row_begin .. (or
+ * rows[row_begin ..] when a rows array is given). */
+typedef struct {
+    int64_t offset;
+    int64_t n;
+    int32_t star;
+    int32_t reserved;
+    int64_t row_begin;
+} jk_polish_segment;
+
+/* jk_polish_f64 for many segments in one launch sequence: each segment's
+ * entries not below its own maximum - window are recomputed in FP64 in
+ * place; n_polished_out[i] (device) counts segment i's.  segs is a HOST
+ * array; FP64 columns as in jk_lnl_f64. */
+int jk_polish_batch(const jk_batch *batch, int n_seg, const jk_polish_segment *segs,
+                    const double *P, const double *e, const double *omega, const double *M0,
+                    const double *s, const int64_t *rows, double *ll_inout, double window,
+                    int64_t *n_polished_out, void *stream);
+
 /* Fill rows [row0, row0 + n) of a device prior cache (arrays indexed from
  * 0 for row row0) with draws of the default nonlinear prior
  * (prior.py:410-493): P log-uniform on [P_min, P_max] (distributions.py:

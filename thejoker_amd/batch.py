@@ -17,11 +17,14 @@ import numpy as np
 from . import _lib
 from .device import DeviceStar, _dptr, _split, _stream, _torch, pcg64_advance_state, pcg64_state
 
-__all__ = ["WORK_ITEM_DTYPE", "SEGMENT_DTYPE", "StarBatch", "work_items", "lnl_batch",
-           "reject_batch", "autotune", "LaunchShape"]
+__all__ = ["WORK_ITEM_DTYPE", "SEGMENT_DTYPE", "POLISH_SEGMENT_DTYPE", "StarBatch",
+           "work_items", "lnl_batch", "reject_batch", "polish_batch", "autotune", "LaunchShape",
+           "rejection_sample_many"]
 
 WORK_ITEM_DTYPE = np.dtype([("star", "<i4"), ("n_rows", "<i4"), ("row_begin", "<i8"),
                             ("out_offset", "<i8")])
+POLISH_SEGMENT_DTYPE = np.dtype([("offset", "<i8"), ("n", "<i8"), ("star", "<i4"),
+                                 ("reserved", "<i4"), ("row_begin", "<i8")])
 SEGMENT_DTYPE = np.dtype([("offset", "<i8"), ("n", "<i8"), ("max_keep", "<i8"),
                           ("idx_offset", "<i8"), ("state_hi", "<u8"), ("state_lo", "<u8"),
                           ("inc_hi", "<u8"), ("inc_lo", "<u8"), ("stream_offset", "<u8")])
@@ -165,6 +168,29 @@ def reject_batch(ll, offsets, sizes, rng_states, max_keep, stream_offsets=None, 
     return out, n_pass, mx.cpu().numpy()
 
 
+def polish_batch(batch, cache, ll, offsets, sizes, stars=None, row_begins=None, rows=None,
+                 window=40.0, device=0):
+    """jk_polish_batch: FP64 re-evaluation of every segment's window.  Segment
+    i is ll[offsets[i] .. + sizes[i]) of star stars[i] (default i) on cache
+    rows row_begins[i] .. (default 0).  Returns a device int64 tensor of
+    per-segment counts."""
+    torch = _torch()
+    lib = _lib.ensure_device(device)
+    n_seg = len(sizes)
+    segs = np.zeros(n_seg, dtype=POLISH_SEGMENT_DTYPE)
+    segs["offset"] = offsets
+    segs["n"] = sizes
+    segs["star"] = np.arange(n_seg) if stars is None else stars
+    segs["row_begin"] = 0 if row_begins is None else row_begins
+    count = torch.zeros(n_seg, dtype=torch.int64, device=ll.device)
+    _lib.check(lib.jk_polish_batch(batch.handle, n_seg, segs.ctypes.data_as(ctypes.c_void_p),
+                                   _dptr(cache.P), _dptr(cache.e), _dptr(cache.omega),
+                                   _dptr(cache.M0), _dptr(cache.s), _dptr(rows), _dptr(ll),
+                                   float(window), _dptr(count), _stream(device)),
+               "jk_polish_batch")
+    return count
+
+
 def autotune(batch, cache, n_rows, precision="mixed", candidates=None, sample_stars=64,
              reps=3, device=0):
     """Time candidate launch shapes on a subset of the workload (the first
@@ -209,7 +235,7 @@ def rejection_sample_many(datas, prior, prior_samples, rngs, max_posterior_sampl
     likelihood_helpers.py:91-127); prior_samples: a PriorCache (or
     JokerSamples) shared by all stars.  Returns one JokerSamples per star
     (and the per-star lnL arrays with return_all_logprobs)."""
-    from .device import PriorCache, linear_internal_units, polish, star_arrays
+    from .device import PriorCache, linear_internal_units, star_arrays
     from .likelihood_helpers import validate_prepare_data
     from .thejoker import POLISH_WINDOW, TheJoker
 
@@ -242,8 +268,8 @@ def rejection_sample_many(datas, prior, prior_samples, rngs, max_posterior_sampl
     items = items[heaviest_first(items, batch.M)]
     ll = lnl_batch(batch, cache, upload_items(items, device), len(items), total, precision, shape)
     if precision == "mixed":
-        for s, st in enumerate(stars):
-            polish(st, cache, ll[int(off[s]):int(off[s]) + n], window=POLISH_WINDOW)
+        polish_batch(batch, cache, ll, off, np.full(len(stars), n), window=POLISH_WINDOW,
+                     device=device)
     states = [pcg64_state(g) for g in rngs]
     good, _, _ = reject_batch(ll, off, np.full(len(stars), n), states, max_posterior_samples,
                               device=device)

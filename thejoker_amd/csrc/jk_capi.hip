@@ -465,6 +465,67 @@ int jk_reject_batch(const double *ll, int n_seg, const jk_segment *segs, int64_t
     return check_hip(err2);
 }
 
+int jk_polish_batch(const jk_batch *batch, int n_seg, const jk_polish_segment *segs,
+                    const double *P, const double *e, const double *omega, const double *M0,
+                    const double *s, const int64_t *rows, double *ll_inout, double window,
+                    int64_t *n_polished_out, void *stream)
+{
+    if (!batch || n_seg <= 0 || !segs || !ll_inout || !n_polished_out || !(window >= 0.0))
+        return JK_EINVAL;
+    if (!P || !e || !omega || !M0) return JK_EINVAL;
+    if (batch->jitter_mode == JK_JITTER_PHYSICAL && !s) return JK_EINVAL;
+    if (current_device_inited() < 0) return JK_ENODEV;
+    std::vector<SegDev> h(n_seg);
+    int64_t max_n = 1, max_nb = 1, mask_words = 0, bsum_n = 0, idx_n = 0;
+    for (int i = 0; i < n_seg; ++i) {
+        const jk_polish_segment &g = segs[i];
+        if (g.n <= 0 || g.offset < 0 || g.row_begin < 0 || g.star < 0 || g.star >= batch->n_stars)
+            return JK_EINVAL;
+        SegDev &d = h[i];
+        memset(&d, 0, sizeof(d));
+        d.off = g.offset;
+        d.n = g.n;
+        d.max_keep = g.n;
+        d.idx_off = idx_n;
+        d.mask_off = mask_words;
+        d.nb = jk_reject_bsum_count(g.n);
+        d.bsum_off = bsum_n;
+        d.star = g.star;
+        d.row0 = g.row_begin;
+        idx_n += g.n;
+        mask_words += jk_reject_mask_words(g.n);
+        bsum_n += d.nb;
+        if (g.n > max_n) max_n = g.n;
+        if (d.nb > max_nb) max_nb = d.nb;
+    }
+    int pb = (int)((max_n + 4095) / 4096);
+    if (pb > 64) pb = 64;
+    if (pb < 1) pb = 1;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t seg_bytes = sizeof(SegDev) * n_seg;
+    const size_t seg_pad = ((seg_bytes + 255) / 256) * 256;
+    const size_t bytes = seg_pad + sizeof(double) * ((size_t)pb * n_seg + n_seg) +
+                         sizeof(uint64_t) * mask_words + sizeof(int64_t) * (bsum_n + idx_n);
+    char *scratch = nullptr;
+    if (hipMallocAsync((void **)&scratch, bytes, st) != hipSuccess) return JK_ENOMEM;
+    SegDev *d_segs = (SegDev *)scratch;
+    double *partials = (double *)(scratch + seg_pad);
+    double *llmax = partials + (size_t)pb * n_seg;
+    uint64_t *mask = (uint64_t *)(llmax + n_seg);
+    int64_t *bsum = (int64_t *)(mask + mask_words);
+    int64_t *idx = bsum + bsum_n;
+    hipError_t err = hipMemcpyAsync(d_segs, h.data(), seg_bytes, hipMemcpyHostToDevice, st);
+    if (err == hipSuccess)
+        err = jk_launch_polish_batch(batch, n_seg, d_segs, max_n, max_nb, partials, pb, mask, bsum,
+                                     idx, llmax, P, e, omega, M0, s, rows, ll_inout, window,
+                                     n_polished_out, st);
+    // The host descriptor array must outlive the queued copy.
+    hipError_t err3 = hipStreamSynchronize(st);
+    hipError_t err2 = hipFreeAsync(scratch, st);
+    if (err != hipSuccess || err3 != hipSuccess) return JK_EHIP;
+    return check_hip(err2);
+}
+
 int jk_prior_fill(int64_t n, int64_t row0, uint64_t seed, double P_min, double P_max,
                   const double *e_quantiles, int n_tab, double ln_s_mu, double ln_s_sigma,
                   double *P, double *e, double *omega, double *M0, double *s, float *e32,

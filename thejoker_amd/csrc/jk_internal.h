@@ -30,6 +30,7 @@ struct SegDev {
     int64_t off, n, max_keep, idx_off;
     int64_t mask_off, bsum_off, nb, max_off;
     u128 s_base, inc;
+    int64_t star, row0;   // segmented polish: star-table index, first cache row
 };
 
 // Jump table of numpy's PCG64 LCG: (A^(2^i), D_(2^i)) for i = 0..63, where
@@ -75,6 +76,13 @@ hipError_t jk_launch_reject_batch(const double *ll, int n_seg, const SegDev *d_s
                                   uint64_t *mask, int64_t *bsum, int64_t *idx_out,
                                   int64_t *n_pass_out, double *llmax_out, hipStream_t stream);
 void jk_host_pcg_jump(u128 state, u128 inc, uint64_t k, u128 *out);
+hipError_t jk_launch_polish_batch(const jk_batch *b, int n_seg, const SegDev *d_segs,
+                                  int64_t max_n, int64_t max_nb, double *partials, int pb,
+                                  uint64_t *mask, int64_t *bsum, int64_t *idx, double *llmax,
+                                  const double *P, const double *e, const double *om,
+                                  const double *M0, const double *s, const int64_t *rows,
+                                  double *ll, double window, int64_t *n_polished,
+                                  hipStream_t stream);
 hipError_t jk_launch_prior_fill(int64_t n, int64_t row0, uint64_t seed, double P_min, double P_max,
                                 const double *e_tab, int n_tab, double s_mu, double s_sigma,
                                 double *P, double *e, double *om, double *M0, double *s,

@@ -1134,6 +1134,113 @@ hipError_t jk_launch_reject_batch(const double *ll, int n_seg, const SegDev *d_s
     return hipGetLastError();
 }
 
+// Segmented FP64 re-evaluation window (jk_polish_f64 per segment): flags,
+// the segmented compaction above, then a scatter of FP64 lnL over each
+// segment's selected entries.
+__global__ __launch_bounds__(256) void k_seg_window_flags(const double *__restrict__ ll,
+                                                          const SegDev *__restrict__ segs,
+                                                          const double *__restrict__ llmax,
+                                                          double window,
+                                                          uint64_t *__restrict__ mask)
+{
+    const SegDev sg = segs[blockIdx.y];
+    const int lane = threadIdx.x & 63;
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t T = (int64_t)gridDim.x * 256;
+    if (g - lane >= sg.n) return;
+    const double mx = llmax[blockIdx.y];
+    const bool all = !isfinite(mx);
+    const double thr = mx - window;
+    for (int64_t base = g - lane; base < sg.n; base += T) {
+        const int64_t i = base + lane;
+        const bool keep = i < sg.n && (all || !(ll[sg.off + i] < thr));
+        const uint64_t word = __ballot(keep);
+        if (lane == 0) mask[sg.mask_off + (base >> 6)] = word;
+    }
+}
+
+template <int L, bool PHYS>
+__global__ __launch_bounds__(256) void k_seg_scatter_f64(const StarDev *__restrict__ stars,
+                                                         const SegDev *__restrict__ segs,
+                                                         const int64_t *__restrict__ count,
+                                                         const int64_t *__restrict__ idx,
+                                                         const double *__restrict__ P,
+                                                         const void *__restrict__ e_,
+                                                         const void *__restrict__ om_,
+                                                         const void *__restrict__ M0_,
+                                                         const void *__restrict__ s_,
+                                                         const int64_t *__restrict__ rows,
+                                                         double *__restrict__ ll)
+{
+    const SegDev sg = segs[blockIdx.y];
+    const int64_t cnt = count[blockIdx.y];
+    const StarDev &st = stars[sg.star];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += stride) {
+        const int64_t p = idx[sg.idx_off + j];
+        const int64_t q = sg.row0 + p;
+        const int64_t r = rows ? rows[q] : q;
+        ll[sg.off + p] = jk_lnl_row<L, PHYS, false>(st, r, P, e_, om_, M0_, s_);
+    }
+}
+
+template <int L, bool PHYS>
+static void launch_seg_scatter_t(const jk_batch *b, int n_seg, const SegDev *d_segs,
+                                 const int64_t *count, const int64_t *idx, int gx, const double *P,
+                                 const void *e, const void *om, const void *M0, const void *s,
+                                 const int64_t *rows, double *ll, hipStream_t stream)
+{
+    hipLaunchKernelGGL((k_seg_scatter_f64<L, PHYS>), dim3(gx, n_seg), dim3(256), 0, stream,
+                       b->d_stars, d_segs, count, idx, P, e, om, M0, s, rows, ll);
+}
+
+hipError_t jk_launch_polish_batch(const jk_batch *b, int n_seg, const SegDev *d_segs,
+                                  int64_t max_n, int64_t max_nb, double *partials, int pb,
+                                  uint64_t *mask, int64_t *bsum, int64_t *idx, double *llmax,
+                                  const double *P, const double *e, const double *om,
+                                  const double *M0, const double *s, const int64_t *rows,
+                                  double *ll, double window, int64_t *n_polished,
+                                  hipStream_t stream)
+{
+    if (n_seg <= 0) return hipSuccess;
+    const int fb = jk_seg_flag_blocks(max_n);
+    hipLaunchKernelGGL(k_seg_max_partial, dim3(pb, n_seg), dim3(256), 0, stream, ll, d_segs,
+                       partials, pb);
+    hipLaunchKernelGGL(k_seg_max_final, dim3(n_seg), dim3(64), 0, stream, partials, pb, llmax);
+    hipLaunchKernelGGL(k_seg_window_flags, dim3(fb, n_seg), dim3(256), 0, stream, ll, d_segs, llmax,
+                       window, mask);
+    hipLaunchKernelGGL(k_seg_bsum, dim3((unsigned)max_nb, n_seg), dim3(256), 0, stream, d_segs,
+                       mask, bsum);
+    hipLaunchKernelGGL(k_seg_compact, dim3((unsigned)max_nb, n_seg), dim3(256), 0, stream, d_segs,
+                       mask, bsum, idx, n_polished);
+    int gx = (int)((max_n + 255) / 256);
+    if (gx > 16) gx = 16;
+    const bool phys = b->jitter_mode == JK_JITTER_PHYSICAL;
+#define JK_PCASE(LL)                                                                             \
+    case LL:                                                                                     \
+        if (phys)                                                                                \
+            launch_seg_scatter_t<LL, true>(b, n_seg, d_segs, n_polished, idx, gx, P, e, om, M0, s, \
+                                           rows, ll, stream);                                    \
+        else                                                                                     \
+            launch_seg_scatter_t<LL, false>(b, n_seg, d_segs, n_polished, idx, gx, P, e, om, M0,   \
+                                            s, rows, ll, stream);                                \
+        break;
+    switch (b->L) {
+        JK_PCASE(1)
+        JK_PCASE(2)
+        JK_PCASE(3)
+        JK_PCASE(4)
+        JK_PCASE(5)
+        JK_PCASE(6)
+        JK_PCASE(7)
+        JK_PCASE(8)
+    default:
+        return hipErrorInvalidValue;
+    }
+#undef JK_PCASE
+    return hipGetLastError();
+}
+
 void jk_host_pcg_jump(u128 state, u128 inc, uint64_t k, u128 *out) { *out = h_jump(state, inc, k); }
 
 // ---------------------------------------------------------------------------

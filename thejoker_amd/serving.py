@@ -2,27 +2,63 @@
 
 Each request is one star run through ``iterative_rejection_sample``
 (thejoker.py:259-370) against a shared :class:`PriorCache`.  Requests are
-admitted while active slots and the HBM budget allow; every step evaluates
+admitted while active slots and the HBM arena allow.  Every step evaluates
 the next batch of rows of *every* active request in one persistent
-``jk_lnl_batch`` launch (stars of different epoch counts side by side), then
-runs each request's FP64 re-evaluation window and rejection step, advances
-its iteration state (:class:`thejoker_amd.thejoker.IterativeRejection`, the
-same code the single-star API runs) and retires finished requests while new
-ones are admitted.  A request's result is bit-for-bit the one
-``TheJoker.iterative_rejection_sample`` returns for that star alone: the
-batched lnL is bit-identical to the single-star kernel, and the random
-stream of each request is its own.
+``jk_lnl_batch`` launch (stars of different epoch counts side by side)
+writing straight into each request's region of one HBM lnL arena, then
+re-evaluates every request's live window in FP64 (``jk_polish_batch``) and
+runs every request's rejection step with its own numpy PCG64 stream
+(``jk_reject_batch``).  The iteration state of each request is
+:class:`thejoker_amd.thejoker.IterativeRejection` -- the code the single-star
+API runs -- and the batched lnL is bit-identical to the single-star kernel,
+so a request's result is exactly what ``TheJoker.iterative_rejection_sample``
+returns for that star alone.
 """
 import time
 
 import numpy as np
 
-from .batch import LaunchShape, StarBatch, lnl_batch, upload_items, work_items
-from .device import DeviceStar, _torch, linear_internal_units, polish, star_arrays
+from .batch import LaunchShape, StarBatch, lnl_batch, polish_batch, reject_batch, upload_items, \
+    work_items
+from .device import DeviceStar, _torch, linear_internal_units, pcg64_state, star_arrays
 from .likelihood_helpers import validate_prepare_data
 from .thejoker import POLISH_WINDOW, IterativeRejection, TheJoker
 
-__all__ = ["Request", "JokerServer", "run_poisson"]
+__all__ = ["Request", "RangeAllocator", "JokerServer", "run_poisson"]
+
+
+class RangeAllocator:
+    """First-fit allocator of [0, capacity) element ranges."""
+
+    def __init__(self, capacity):
+        self.capacity = int(capacity)
+        self.free = [(0, self.capacity)]
+
+    def alloc(self, n):
+        n = int(n)
+        for k, (a, b) in enumerate(self.free):
+            if b - a >= n:
+                if b - a == n:
+                    self.free.pop(k)
+                else:
+                    self.free[k] = (a + n, b)
+                return a
+        return None
+
+    def release(self, start, n):
+        a, b = int(start), int(start) + int(n)
+        self.free.append((a, b))
+        self.free.sort()
+        merged = []
+        for x, y in self.free:
+            if merged and merged[-1][1] == x:
+                merged[-1] = (merged[-1][0], y)
+            else:
+                merged.append((x, y))
+        self.free = merged
+
+    def used(self):
+        return self.capacity - sum(b - a for a, b in self.free)
 
 
 class Request:
@@ -35,10 +71,11 @@ class Request:
         self.t_done = None
         self.star = None
         self.it = None
-        self.ll = None
+        self.region = None      # (start, capacity) in the arena
         self.result = None
         self.error = None
         self.rows_evaluated = 0
+        self.sample_epochs = 0
 
     @property
     def latency(self):
@@ -53,28 +90,27 @@ class JokerServer:
     cache : PriorCache
         Device-resident prior rows shared by every request.
     n_requested : int
-        Posterior samples per star (iterative_rejection_sample's
-        n_requested_samples).
+        Posterior samples per star (n_requested_samples).
     precision : "mixed" or "f64"
     max_active : int
         Requests evaluated together.
     hbm_budget_bytes : int
-        Device memory the per-request lnL buffers may hold together.
+        Size of the lnL arena the active requests share.
     in_memory : bool
         Iteration rule of the in-memory path (safety factor 1) or of the
         default path (safety factor 4).
     """
 
     def __init__(self, prior, cache, n_requested=256, precision="mixed", jitter_mode="physical",
-                 max_active=64, hbm_budget_bytes=16 << 30, in_memory=False, growth_factor=128,
+                 max_active=64, hbm_budget_bytes=4 << 30, in_memory=False, growth_factor=128,
                  n_linear_samples=1, shape=None, device=0, clock=time.perf_counter):
+        torch = _torch()
         self.prior = prior
         self.cache = cache
         self.n_requested = int(n_requested)
         self.precision = precision
         self.jitter_mode = jitter_mode
         self.max_active = int(max_active)
-        self.budget = int(hbm_budget_bytes)
         self.in_memory = in_memory
         self.growth_factor = growth_factor
         self.n_linear_samples = n_linear_samples
@@ -85,37 +121,48 @@ class JokerServer:
         self.pending = []
         self.active = []
         self.finished = []
-        self.reserved = 0
         self.steps = 0
         self.sample_epochs = 0
+        self.arena = torch.empty(int(hbm_budget_bytes) // 8, dtype=torch.float64,
+                                 device=cache.P.device)
+        self.alloc = RangeAllocator(self.arena.shape[0])
         self._batch = None
         self._batch_key = None
+        self._ws = torch.zeros(64, dtype=torch.int32, device=cache.P.device)
         self._helper = TheJoker(prior, rng=np.random.default_rng(0), jitter_mode=jitter_mode,
                                 precision=precision, device=device)
-        torch = _torch()
-        self._ws = torch.zeros(64, dtype=torch.int32, device=cache.P.device)
 
     def now(self):
         return self.clock() - self.t0
 
     def submit(self, data, rng, rid=None, arrival=None):
-        r = Request(len(self.finished) + len(self.active) + len(self.pending) if rid is None else rid,
-                    data, rng, self.now() if arrival is None else arrival)
+        if rid is None:
+            rid = len(self.finished) + len(self.active) + len(self.pending)
+        r = Request(rid, data, rng, self.now() if arrival is None else arrival)
         self.pending.append(r)
         return r
 
+    def busy(self):
+        return bool(self.pending or self.active)
+
     # ------------------------------------------------------------------
-    def _need_bytes(self, n_rows):
-        return 8 * int(n_rows)
+    def _fail(self, r, exc):
+        r.error = exc
+        r.t_done = self.now()
+        if r.region is not None:
+            self.alloc.release(*r.region)
+            r.region = None
+        self.finished.append(r)
 
     def _admit(self):
-        torch = _torch()
         while self.pending and len(self.active) < self.max_active:
             r = self.pending[0]
             first = self.growth_factor * self.n_requested
-            if self.reserved + self._need_bytes(first) > self.budget and self.active:
-                break
+            start = self.alloc.alloc(first)
+            if start is None:
+                break                      # wait for memory
             self.pending.pop(0)
+            r.region = (start, first)
             try:
                 dd, ids, trend = validate_prepare_data(r.data, self.prior.poly_trend,
                                                        self.prior.n_offsets)
@@ -129,26 +176,28 @@ class JokerServer:
                                           in_memory=self.in_memory,
                                           growth_factor=self.growth_factor,
                                           device=self.cache.P.device)
-            except Exception as exc:  # a bad request fails alone
-                r.error = exc
-                r.t_done = self.now()
-                self.finished.append(r)
+            except Exception as exc:       # a bad request fails alone
+                self._fail(r, exc)
                 continue
-            r.ll = torch.empty(first, dtype=torch.float64, device=self.cache.P.device)
-            self.reserved += self._need_bytes(first)
             r.t_admit = self.now()
             self.active.append(r)
 
-    def _grow(self, r, need):
-        torch = _torch()
-        if need <= r.ll.shape[0]:
-            return
-        cap = max(need, 2 * r.ll.shape[0])
-        cap = min(cap, r.it.max_prior)
-        new = torch.empty(cap, dtype=torch.float64, device=r.ll.device)
-        new[:r.it.n_evals] = r.ll[:r.it.n_evals]
-        self.reserved += self._need_bytes(cap - r.ll.shape[0])
-        r.ll = new
+    def _ensure_region(self, r, need):
+        """Grow r's arena region to hold `need` values; False if the arena is
+        full for now."""
+        start, cap = r.region
+        if need <= cap:
+            return True
+        new_cap = min(max(need, 2 * cap), r.it.max_prior)
+        new = self.alloc.alloc(new_cap)
+        if new is None:
+            return False
+        n = r.it.n_evals
+        if n:
+            self.arena[new:new + n] = self.arena[start:start + n]
+        self.alloc.release(start, cap)
+        r.region = (new, new_cap)
+        return True
 
     def step(self):
         """Admit, evaluate one round of every active request, retire the
@@ -156,48 +205,49 @@ class JokerServer:
         self._admit()
         if not self.active:
             return []
-        torch = _torch()
-        act = self.active
-        key = tuple(id(r) for r in act)
-        if key != self._batch_key:
-            self._batch = StarBatch([r.star for r in act], device=self.device)
-            self._batch_key = key
-        rows_l, begins, sizes = [], [], []
-        for r in act:
+        plan = []
+        for r in self.active:
             rows, start, n_now = r.it.next_batch()
-            self._grow(r, r.it.n_evals + n_now)
-            rows_l.append(rows)
-            begins.append(start)
-            sizes.append(n_now)
-        sizes = np.array(sizes, dtype=np.int64)
-        if any(x is not None for x in rows_l):
-            # Randomised prior order: gather the rows of every request.
-            row_arr = torch.cat([x if x is not None else
-                                 torch.arange(b, b + n, device=self.cache.P.device)
-                                 for x, b, n in zip(rows_l, begins, sizes)])
-            items, off, total = work_items(sizes, self.shape.tile,
-                                           row_begin=np.concatenate([[0], np.cumsum(sizes)[:-1]]))
-        else:
-            row_arr = None
-            items, off, total = work_items(sizes, self.shape.tile, row_begin=np.array(begins))
-        out = lnl_batch(self._batch, self.cache, upload_items(items, self.device), len(items),
-                        total, self.precision, self.shape, rows=row_arr, workspace=self._ws)
-        self.sample_epochs += int(np.sum(sizes * self._batch.M))
+            if self._ensure_region(r, r.it.n_evals + n_now):
+                plan.append((r, start, n_now))
+        if not plan:
+            # Every active request is waiting to grow and the arena is full:
+            # fail the one holding the largest region so the rest can go on.
+            r = max(self.active, key=lambda q: q.region[1])
+            self.active.remove(r)
+            self._fail(r, MemoryError("lnL arena too small for this request's next batch"))
+            return [r]
+        key = tuple(id(r) for r, _, _ in plan)
+        if key != self._batch_key:
+            self._batch = StarBatch([r.star for r, _, _ in plan], device=self.device)
+            self._batch_key = key
+        sizes = np.array([n for _, _, n in plan], dtype=np.int64)
+        begins = np.array([b for _, b, _ in plan], dtype=np.int64)
+        dst = np.array([r.region[0] + r.it.n_evals for r, _, _ in plan], dtype=np.int64)
+        items, off, total = work_items(sizes, self.shape.tile, row_begin=begins)
+        # write each request's rows straight into its arena region
+        items["out_offset"] += np.repeat(dst - off, (sizes + self.shape.tile - 1) // self.shape.tile)
+        lnl_batch(self._batch, self.cache, upload_items(items, self.device), len(items), 0,
+                  self.precision, self.shape, out=self.arena, workspace=self._ws)
+        M = self._batch.M
+        self.sample_epochs += int(np.sum(sizes * M))
+        seg_off = np.array([r.region[0] for r, _, _ in plan], dtype=np.int64)
+        seg_n = np.array([r.it.n_evals + n for r, _, n in plan], dtype=np.int64)
+        if self.precision == "mixed":
+            polish_batch(self._batch, self.cache, self.arena, seg_off, seg_n, window=POLISH_WINDOW,
+                         device=self.device)
+        good, n_pass, llmax = reject_batch(self.arena, seg_off, seg_n,
+                                           [pcg64_state(r.rng) for r, _, _ in plan],
+                                           self.n_requested, device=self.device)
         done = []
-        for k, r in enumerate(act):
-            n0 = r.it.n_evals
-            n_now = int(sizes[k])
-            r.ll[n0:n0 + n_now] = out[int(off[k]):int(off[k]) + n_now]
+        for k, (r, _, n_now) in enumerate(plan):
             r.rows_evaluated += n_now
-            view = r.ll[:n0 + n_now]
+            r.sample_epochs += int(n_now * M[k])
             try:
-                if self.precision == "mixed":
-                    if r.it.all_idx is None:
-                        polish(r.star, self.cache, view, start=0, window=POLISH_WINDOW)
-                    else:
-                        polish(r.star, self.cache, view, rows=r.it._d_all_idx[:n0 + n_now],
-                               window=POLISH_WINDOW)
-                r.it.absorb(view, self.device)
+                if self.in_memory and not np.isfinite(llmax[k]):
+                    raise RuntimeError("There are NaN or Inf likelihood values in iteration step "
+                                       f"{r.it.iteration}!")
+                r.it.update(seg_n[k], good[k], n_pass[k])
             except Exception as exc:
                 r.error = exc
                 r.it.done = True
@@ -205,20 +255,17 @@ class JokerServer:
                 done.append(r)
         for r in done:
             self.active.remove(r)
-            self.reserved -= self._need_bytes(r.ll.shape[0])
             if r.error is None:
                 full_idx, _ = r.it.result()
                 r.result = self._helper._full_samples(r.star, self.cache, full_idx, r.rng,
                                                       self.n_linear_samples,
                                                       spawn_batches=None if self.in_memory else 1)
-            r.ll = None
+            self.alloc.release(*r.region)
+            r.region = None
             r.t_done = self.now()
             self.finished.append(r)
         self.steps += 1
         return done
-
-    def busy(self):
-        return bool(self.pending or self.active)
 
 
 def run_poisson(server, arrival_times, datas, rngs, max_wall=None):

@@ -345,14 +345,22 @@ class IterativeRejection:
         n_evals + the new rows), then the growth rule."""
         import torch
 
-        i = self.iteration
-        self.n_evals = int(ll_so_far.shape[0])
-        logger.log(1, f"iteration {i}, computed {self.n_evals} likelihoods")
+        n_evals = int(ll_so_far.shape[0])
         if self.in_memory and not bool(torch.isfinite(ll_so_far).all()):
-            raise RuntimeError(f"There are NaN or Inf likelihood values in iteration step {i}!")
+            raise RuntimeError(f"There are NaN or Inf likelihood values in iteration step "
+                               f"{self.iteration}!")
         good, n_good, _ = reject(ll_so_far, pcg64_state(self.rng), self.n_requested, device=device)
+        self.update(n_evals, good, n_good)
+
+    def update(self, n_evals, good, n_good):
+        """Bookkeeping after the rejection step of a round whose lnL buffer now
+        holds n_evals values and whose uniforms were drawn from self.rng
+        (advanced here by n_evals)."""
+        i = self.iteration
+        self.n_evals = int(n_evals)
         pcg64_advance_state(self.rng, self.n_evals)
-        self.good, self.n_good = good, n_good
+        logger.log(1, f"iteration {i}, computed {self.n_evals} likelihoods")
+        self.good, self.n_good = good, int(n_good)
         self.iteration += 1
         if n_good == 0:
             raise RuntimeError("Failed to find any good samples!")
