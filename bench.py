@@ -303,29 +303,48 @@ def leg_serving(args, torch, np, D, syn):
     log(f"[bench] serving: p50 {res['p50_latency_s'] * 1e3:.1f} ms p99 "
         f"{res['p99_latency_s'] * 1e3:.1f} ms, {res['stars_per_s']:.1f} stars/s, "
         f"{res['sample_epochs_per_s']:.3e} sample-epochs/s, {srv.steps} steps")
-    # capacity: the whole stream submitted at once
+    # capacity: the whole stream submitted at once; phases timed with a
+    # device synchronisation between them
     srv2 = S.JokerServer(prior, cache, precision="mixed", max_active=args.serving_active,
-                         hbm_budget_bytes=8 << 30)
+                         hbm_budget_bytes=8 << 30, profile=True)
     rngs2 = [np.random.default_rng(10_000 + i) for i in range(n_req)]
     t0 = time.perf_counter()
-    done2 = S.run_poisson(srv2, np.zeros(n_req), datas, rngs2)
+    S.run_poisson(srv2, np.zeros(n_req), datas, rngs2)
     wall2 = time.perf_counter() - t0
     res["saturated"] = {"wall_s": wall2, "stars_per_s": n_req / wall2,
-                        "sample_epochs_per_s": srv2.sample_epochs / wall2}
-    # the same stars alone in plain FP64: identical accepted rows?
-    flips, checked = 0, 0
-    for i in range(0, n_req, max(1, n_req // 16)):
+                        "sample_epochs_per_s": srv2.sample_epochs / wall2,
+                        "steps": srv2.steps,
+                        "phase_s": {k: round(v, 4) for k, v in srv2.phase_s.items()}}
+    log(f"[bench] serving: saturated {res['saturated']['stars_per_s']:.1f} stars/s, phases "
+        f"{res['saturated']['phase_s']}")
+    # The plain path: the same stars one at a time through
+    # TheJoker(precision="f64").iterative_rejection_sample.  Same rows, so
+    # the same sample-epochs as in the server; also checks for flips.
+    flips, checked, se, t_alone = 0, 0, 0, 0.0
+    for i in range(0, n_req, max(1, n_req // args.serving_plain)):
         if done[i].error is not None:
             continue
         joker = tj.TheJoker(prior, rng=np.random.default_rng(10_000 + i), precision="f64")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
         alone = joker.iterative_rejection_sample(datas[i], cache, n_requested_samples=256)
+        torch.cuda.synchronize()
+        t_alone += time.perf_counter() - t0
+        se += done[i].sample_epochs
         a, b = alone["P"].value, done[i].result["P"].value
         flips += int(len(np.setxor1d(a, b)))
         checked += 1
+    res["plain_f64_one_at_a_time"] = {"stars": checked, "wall_s": t_alone,
+                                      "sample_epochs_per_s": se / t_alone,
+                                      "stars_per_s": checked / t_alone}
+    res["speedup_vs_plain_f64_one_at_a_time"] = res["saturated"]["sample_epochs_per_s"] / (
+        se / t_alone)
     res["fp64_alone_checked"] = checked
     res["accept_flips_vs_fp64_alone"] = flips
-    log(f"[bench] serving: saturated {res['saturated']['stars_per_s']:.1f} stars/s; flips vs "
-        f"FP64 alone on {checked} stars: {flips}")
+    log(f"[bench] serving: plain FP64 one star at a time {se / t_alone:.3e} sample-epochs/s "
+        f"({checked / t_alone:.1f} stars/s) -> serving "
+        f"{res['speedup_vs_plain_f64_one_at_a_time']:.1f}x; flips vs FP64 alone on {checked} "
+        f"stars: {flips}")
     del cache
     torch.cuda.empty_cache()
     return res
@@ -396,6 +415,8 @@ def main():
                     choices=["all", "single", "batched", "serving", "stress"])
     ap.add_argument("--serving-requests", type=int, default=1000)
     ap.add_argument("--serving-active", type=int, default=64)
+    ap.add_argument("--serving-plain", type=int, default=64,
+                    help="stars of the serving stream also run one at a time in plain FP64")
     ap.add_argument("--stress-log2-rows", type=int, default=29)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",

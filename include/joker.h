@@ -106,16 +106,19 @@ int jk_lnl_mixed(const jk_star *star, int64_t n, const double *P, const float *e
                  const int64_t *rows, double *ll_out, void *stream);
 
 /* FP64 re-evaluation of the live set after a mixed-precision pass: every
- * entry of ll_inout (n rows, as in jk_lnl_mixed) that is not below
- * max(ll) - window -- all of them when the maximum is NaN or infinite -- is
- * recomputed with the FP64 kernel in place; *n_polished_out (device int64)
- * receives how many.  With window >= 40, exp(lnL - max) of every row left in
+ * entry i >= first of ll_inout (n rows, as in jk_lnl_mixed) that is not below
+ * max(ll[0..n)) - window -- all of them when the maximum is NaN or infinite --
+ * is recomputed with the FP64 kernel in place; *n_polished_out (device int64)
+ * receives how many.  Entries before `first` are taken as already FP64 (an
+ * earlier call polished them against a maximum no larger than today's, so
+ * none of them can have entered the window since): they count toward the
+ * maximum only.  With window >= 40, exp(lnL - max) of every row left in
  * mixed precision is below the smallest nonzero uniform (2^-53), so the
  * rejection step's accepted set equals the plain FP64 path's.  No reference
  * counterpart (the reference has one precision). */
 int jk_polish_f64(const jk_star *star, int64_t n, const double *P, const double *e,
                   const double *omega, const double *M0, const double *s,
-                  const int64_t *rows, double *ll_inout, double window,
+                  const int64_t *rows, double *ll_inout, double window, int64_t first,
                   int64_t *n_polished_out, void *stream);
 
 /* NaN-propagating maximum of ll[0..n) into *max_out (device), as np.max at
@@ -198,6 +201,18 @@ typedef struct {
 int jk_batch_create(const jk_star *const *stars, int n_stars, jk_batch **out);
 int jk_batch_destroy(jk_batch *batch);
 
+/* A star table of `capacity` empty slots for stars with L linear parameters
+ * and jitter mode `jitter_mode`, filled and refilled by jk_batch_set_star:
+ * the form a request loop uses, where stars come and go between launches.
+ * No work item may name a slot that was never set. */
+int jk_batch_create_slots(int capacity, int L, int jitter_mode, jk_batch **out);
+
+/* Point slot `slot` of a table made by jk_batch_create_slots at `star` (same
+ * L, jitter mode and device).  The star's device arrays are referenced, not
+ * copied, so the star must outlive every launch that reads the slot.  The
+ * update is ordered on `stream` after the work already queued there. */
+int jk_batch_set_star(jk_batch *batch, int slot, const jk_star *star, void *stream);
+
 /* Persistent work-queue launch over device-resident work items.
  * precision 0: FP64 (e, omega, M0, s are double columns); 1: mixed (float
  * columns).  block: threads per workgroup (64, 128 or 256); grid: workgroups
@@ -214,15 +229,17 @@ int jk_lnl_batch(const jk_batch *batch, int precision, int64_t n_items,
 int jk_reject_batch(const double *ll, int n_seg, const jk_segment *segs, int64_t *idx_out,
                     int64_t *n_pass_out, double *llmax_out, void *stream);
 
-/* One segment of jk_polish_batch: ll[offset .. offset + n) holds the mixed
- * lnL of star `star` (star-table index) on cache rows row_begin .. (or
- * rows[row_begin ..] when a rows array is given). */
+/* One segment of jk_polish_batch: ll[offset .. offset + n) holds the lnL of
+ * star `star` (star-table index) on cache rows row_begin .. (or
+ * rows[row_begin ..] when a rows array is given); entries from `first` on are
+ * mixed precision, earlier ones already FP64 (as in jk_polish_f64). */
 typedef struct {
     int64_t offset;
     int64_t n;
     int32_t star;
     int32_t reserved;
     int64_t row_begin;
+    int64_t first;
 } jk_polish_segment;
 
 /* jk_polish_f64 for many segments in one launch sequence: each segment's

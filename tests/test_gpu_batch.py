@@ -128,3 +128,65 @@ def test_prior_fill_distribution_and_determinism(gpu):
     assert np.array_equal(c.fp32[0].cpu().numpy(), e.astype(np.float32))
     z = D.PriorCache.generate(1000, seed=1, ln_s=None)
     assert np.all(z.s.cpu().numpy() == 0.0)
+
+
+def test_polish_first_skips_settled_rows(setup):
+    """Polishing rows [k, n) against the maximum of all n rows, after rows
+    [0, k) were polished on their own, gives the full polish."""
+    stars, cache, _ = setup
+    n = len(cache)
+    for _, dstar in stars[:4]:
+        full = D.lnl(dstar, cache, precision="mixed")
+        D.polish(dstar, cache, full)
+        for k in (1, 4097, n // 2, n - 1):
+            inc = D.lnl(dstar, cache, precision="mixed")
+            D.polish(dstar, cache, inc[:k])
+            before = inc[:k].clone()
+            cnt = D.polish(dstar, cache, inc, first=k)
+            assert torch.equal(inc[:k], before)
+            f, g = full.cpu().numpy(), inc.cpu().numpy()
+            live = f >= f.max() - 40
+            assert np.array_equal(g[live], f[live]), k
+            assert g.max() == f.max()
+            assert int(cnt.item()) <= n - k
+
+
+def test_polish_batch_first(setup):
+    stars, cache, _ = setup
+    batch = B.StarBatch([d for _, d in stars[:6]])
+    n = len(cache)
+    items, off, total = B.work_items([n] * 6, 1024)
+    ll = B.lnl_batch(batch, cache, B.upload_items(items), len(items), total, "mixed")
+    first = np.array([0, 1, 5000, n // 3, n - 64, n - 1])
+    for s in range(6):
+        k = int(first[s])
+        if k:
+            D.polish(stars[s][1], cache, ll[off[s]:off[s] + k])
+    B.polish_batch(batch, cache, ll, off, np.full(6, n), first=first)
+    for s, (_, dstar) in enumerate(stars[:6]):
+        ref = D.lnl(dstar, cache, precision="mixed")
+        D.polish(dstar, cache, ref)
+        r, g = ref.cpu().numpy(), ll[off[s]:off[s] + n].cpu().numpy()
+        live = r >= r.max() - 40
+        assert np.array_equal(g[live], r[live]), s
+
+
+def test_slot_table_equals_single_star_launches(setup):
+    """A slot table filled in scrambled order, then with one slot refilled,
+    gives each star's single-star lnL."""
+    stars, cache, _ = setup
+    table = B.StarBatch.slots(8, 3)
+    assign = {5: 0, 0: 1, 7: 2, 2: 3}
+    for slot, s in assign.items():
+        table.set(slot, stars[s][1])
+    n = 5000
+    for _ in range(2):
+        slots = np.array(sorted(assign))
+        items, off, total = B.work_items([n] * len(slots), 512, row_begin=[100] * len(slots),
+                                         stars=slots)
+        ll = B.lnl_batch(table, cache, B.upload_items(items), len(items), total, "f64")
+        for k, slot in enumerate(slots):
+            single = D.lnl(stars[assign[slot]][1], cache, start=100, n=n).cpu().numpy()
+            assert np.array_equal(ll[off[k]:off[k] + n].cpu().numpy(), single), slot
+        assign[7] = 9
+        table.set(7, stars[9][1])

@@ -221,3 +221,21 @@ def test_units_roundtrip():
     assert u.as_quantity(3.0, u.km / u.s).to_value(u.m / u.s) == 3000.0
     with pytest.raises(u.UnitsError):
         (1 * u.day).to_value(u.km)
+
+
+@pytest.mark.parametrize("L", [1, 2, 3, 5, 8])
+@pytest.mark.parametrize("n,size", [(1, 1), (7, 3), (256, 1), (3000, 2)])
+def test_mvn_draws_equal_row_by_row(L, n, size):
+    """Stacked linear-parameter draws give the values and the generator state
+    of rng.multivariate_normal called row by row (fast_likelihood.pyx:529-530)."""
+    from thejoker_amd.thejoker import mvn_draws
+    g0 = np.random.default_rng(L * 1000 + n)
+    A = g0.normal(size=(n, L, L))
+    covs = A @ np.swapaxes(A, 1, 2) + 1e-3 * np.eye(L)
+    covs[::5] *= 1e4
+    means = g0.normal(size=(n, L)) * 50
+    g1, g2 = np.random.default_rng(42), np.random.default_rng(42)
+    want = np.stack([g1.multivariate_normal(means[j], covs[j], size=size) for j in range(n)])
+    got = mvn_draws(g2, means, covs, size)
+    assert np.array_equal(got, want)
+    assert g1.bit_generator.state == g2.bit_generator.state

@@ -10,7 +10,7 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libjoker.so")
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 JK_OK = 0
 JK_JITTER_REFERENCE = 0
@@ -41,7 +41,7 @@ SIGNATURES = [
     ("jk_lnl_mixed", ctypes.c_int, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("jk_max_f64", ctypes.c_int, [_vp, _i64, _vp, _vp]),
     ("jk_polish_f64", ctypes.c_int,
-     [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _vp, _vp]),
+     [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _i64, _vp, _vp]),
     ("jk_reject", ctypes.c_int,
      [_vp, _i64, _u64, _u64, _u64, _u64, _u64, _i64, _vp, _vp, _vp, _vp]),
     ("jk_reject_with_max", ctypes.c_int,
@@ -49,6 +49,9 @@ SIGNATURES = [
     ("jk_pcg64_uniform", ctypes.c_int, [_u64, _u64, _u64, _u64, _u64, _i64, _vp, _vp]),
     ("jk_batch_create", ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(_vp)]),
     ("jk_batch_destroy", ctypes.c_int, [_vp]),
+    ("jk_batch_create_slots", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                              ctypes.POINTER(_vp)]),
+    ("jk_batch_set_star", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
     ("jk_lnl_batch", ctypes.c_int,
      [_vp, ctypes.c_int, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
       _vp, _vp]),

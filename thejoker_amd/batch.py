@@ -15,7 +15,8 @@ import time
 import numpy as np
 
 from . import _lib
-from .device import DeviceStar, _dptr, _split, _stream, _torch, pcg64_advance_state, pcg64_state
+from .device import (DeviceStar, _dptr, _split, _stream, _torch, pcg64_advance_state, pcg64_state,
+                     to_host)
 
 __all__ = ["WORK_ITEM_DTYPE", "SEGMENT_DTYPE", "POLISH_SEGMENT_DTYPE", "StarBatch",
            "work_items", "lnl_batch", "reject_batch", "polish_batch", "autotune", "LaunchShape",
@@ -24,7 +25,7 @@ __all__ = ["WORK_ITEM_DTYPE", "SEGMENT_DTYPE", "POLISH_SEGMENT_DTYPE", "StarBatc
 WORK_ITEM_DTYPE = np.dtype([("star", "<i4"), ("n_rows", "<i4"), ("row_begin", "<i8"),
                             ("out_offset", "<i8")])
 POLISH_SEGMENT_DTYPE = np.dtype([("offset", "<i8"), ("n", "<i8"), ("star", "<i4"),
-                                 ("reserved", "<i4"), ("row_begin", "<i8")])
+                                 ("reserved", "<i4"), ("row_begin", "<i8"), ("first", "<i8")])
 SEGMENT_DTYPE = np.dtype([("offset", "<i8"), ("n", "<i8"), ("max_keep", "<i8"),
                           ("idx_offset", "<i8"), ("state_hi", "<u8"), ("state_lo", "<u8"),
                           ("inc_hi", "<u8"), ("inc_lo", "<u8"), ("stream_offset", "<u8")])
@@ -58,6 +59,32 @@ class StarBatch:
         self.M = np.array([s.n for s in self.stars], dtype=np.int64)
         self.L = self.stars[0].L
 
+    @classmethod
+    def slots(cls, capacity, L, jitter_mode="physical", device=0):
+        """An empty table of `capacity` slots (jk_batch_create_slots), filled
+        with :meth:`set`; a slot keeps its DeviceStar alive until replaced."""
+        from .device import JITTER_MODES
+        lib = _lib.ensure_device(device)
+        self = cls.__new__(cls)
+        self._lib = lib
+        self.device = device
+        self.stars = [None] * int(capacity)
+        h = ctypes.c_void_p()
+        _lib.check(lib.jk_batch_create_slots(int(capacity), int(L), JITTER_MODES[jitter_mode],
+                                             ctypes.byref(h)), "jk_batch_create_slots")
+        self._h = h
+        self.M = np.zeros(int(capacity), dtype=np.int64)
+        self.L = int(L)
+        return self
+
+    def set(self, slot, star):
+        """Point slot `slot` at DeviceStar `star` (jk_batch_set_star), ordered
+        after the work already queued on the device's stream."""
+        _lib.check(self._lib.jk_batch_set_star(self._h, int(slot), star.handle,
+                                                _stream(self.device)), "jk_batch_set_star")
+        self.stars[slot] = star
+        self.M[slot] = star.n
+
     @property
     def handle(self):
         return self._h
@@ -77,25 +104,24 @@ class StarBatch:
             pass
 
 
-def work_items(n_rows, tile, row_begin=None, order=None):
+def work_items(n_rows, tile, row_begin=None, order=None, stars=None):
     """Work items covering rows [row_begin[s], row_begin[s] + n_rows[s]) of
     each star s, `tile` rows per item, outputs packed star after star.
+    `stars` (optional) gives each star's star-table index (default s);
     `order` (optional) permutes the items (e.g. heaviest first).
     Returns (items, out_offsets, total)."""
     n_rows = np.asarray(n_rows, dtype=np.int64)
-    if row_begin is None:
-        row_begin = np.zeros_like(n_rows)
+    row_begin = np.zeros_like(n_rows) if row_begin is None else np.asarray(row_begin, np.int64)
     out_off = np.concatenate([[0], np.cumsum(n_rows)[:-1]]).astype(np.int64)
     counts = (n_rows + tile - 1) // tile
-    items = np.zeros(int(counts.sum()), dtype=WORK_ITEM_DTYPE)
-    k = 0
-    for s, (n, c) in enumerate(zip(n_rows, counts)):
-        j = np.arange(c, dtype=np.int64)
-        items["star"][k:k + c] = s
-        items["n_rows"][k:k + c] = np.minimum(tile, n - j * tile)
-        items["row_begin"][k:k + c] = row_begin[s] + j * tile
-        items["out_offset"][k:k + c] = out_off[s] + j * tile
-        k += c
+    pos = np.repeat(np.arange(len(n_rows)), counts)
+    first = np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.int64)
+    j = np.arange(int(counts.sum()), dtype=np.int64) - first[pos]
+    items = np.zeros(len(pos), dtype=WORK_ITEM_DTYPE)
+    items["star"] = pos if stars is None else np.asarray(stars)[pos]
+    items["n_rows"] = np.minimum(tile, n_rows[pos] - j * tile)
+    items["row_begin"] = row_begin[pos] + j * tile
+    items["out_offset"] = out_off[pos] + j * tile
     if order is not None:
         items = items[order]
     return items, out_off, int(n_rows.sum())
@@ -152,28 +178,30 @@ def reject_batch(ll, offsets, sizes, rng_states, max_keep, stream_offsets=None, 
         segs["inc_hi"][i], segs["inc_lo"][i] = _split(int(inc))
     if stream_offsets is not None:
         segs["stream_offset"] = stream_offsets
-    dev = ll.device
-    idx = torch.empty(max(int(mk.sum()), 1), dtype=torch.int64, device=dev)
-    npass = torch.zeros(n_seg, dtype=torch.int64, device=dev)
-    mx = torch.empty(n_seg, dtype=torch.float64, device=dev)
+    # one device block [n_pass (n_seg), max (n_seg), idx...], one copy back
+    res = torch.empty(2 * n_seg + max(int(mk.sum()), 1), dtype=torch.int64, device=ll.device)
+    p = res.data_ptr()
     _lib.check(lib.jk_reject_batch(_dptr(ll), n_seg, segs.ctypes.data_as(ctypes.c_void_p),
-                                   _dptr(idx), _dptr(npass), _dptr(mx), _stream(device)),
+                                   ctypes.c_void_p(p + 16 * n_seg), ctypes.c_void_p(p),
+                                   ctypes.c_void_p(p + 8 * n_seg), _stream(device)),
                "jk_reject_batch")
-    n_pass = npass.cpu().numpy()
-    idx_h = idx.cpu().numpy()
+    h = to_host(res)
+    n_pass = h[:n_seg].copy()
+    mx = h[n_seg:2 * n_seg].view(np.float64).copy()
+    idx_h = h[2 * n_seg:]
     out = []
     for i in range(n_seg):
         k = int(min(n_pass[i], mk[i]))
         out.append(idx_h[segs["idx_offset"][i]:segs["idx_offset"][i] + k].copy())
-    return out, n_pass, mx.cpu().numpy()
+    return out, n_pass, mx
 
 
 def polish_batch(batch, cache, ll, offsets, sizes, stars=None, row_begins=None, rows=None,
-                 window=40.0, device=0):
+                 window=40.0, first=None, device=0):
     """jk_polish_batch: FP64 re-evaluation of every segment's window.  Segment
     i is ll[offsets[i] .. + sizes[i]) of star stars[i] (default i) on cache
-    rows row_begins[i] .. (default 0).  Returns a device int64 tensor of
-    per-segment counts."""
+    rows row_begins[i] .. (default 0), mixed precision from entry first[i]
+    (default 0) on.  Returns a device int64 tensor of per-segment counts."""
     torch = _torch()
     lib = _lib.ensure_device(device)
     n_seg = len(sizes)
@@ -182,6 +210,7 @@ def polish_batch(batch, cache, ll, offsets, sizes, stars=None, row_begins=None, 
     segs["n"] = sizes
     segs["star"] = np.arange(n_seg) if stars is None else stars
     segs["row_begin"] = 0 if row_begins is None else row_begins
+    segs["first"] = 0 if first is None else first
     count = torch.zeros(n_seg, dtype=torch.int64, device=ll.device)
     _lib.check(lib.jk_polish_batch(batch.handle, n_seg, segs.ctypes.data_as(ctypes.c_void_p),
                                    _dptr(cache.P), _dptr(cache.e), _dptr(cache.omega),

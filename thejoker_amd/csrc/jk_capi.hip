@@ -10,7 +10,7 @@
 
 #include "jk_internal.h"
 
-#define JK_ABI_VERSION 1
+#define JK_ABI_VERSION 2
 
 namespace {
 
@@ -66,6 +66,7 @@ int jk_init(int device)
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return JK_EHIP;
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) return JK_ENODEV;
     if (jk_upload_pcg_table(device) != hipSuccess) return JK_EHIP;
+    if (jk_upload_sincos_table(device) != hipSuccess) return JK_EHIP;
     if (hipDeviceSynchronize() != hipSuccess) return JK_EHIP;
     g_inited[device] = true;
     return JK_OK;
@@ -291,9 +292,10 @@ int jk_linear_posterior(const jk_star *star, int64_t n, const double *P, const d
 
 int jk_polish_f64(const jk_star *star, int64_t n, const double *P, const double *e,
                   const double *omega, const double *M0, const double *s, const int64_t *rows,
-                  double *ll_inout, double window, int64_t *n_polished_out, void *stream)
+                  double *ll_inout, double window, int64_t first, int64_t *n_polished_out,
+                  void *stream)
 {
-    if (!star || n < 0 || !(window >= 0.0)) return JK_EINVAL;
+    if (!star || n < 0 || !(window >= 0.0) || first < 0) return JK_EINVAL;
     if (n == 0) return JK_OK;
     if (!P || !e || !omega || !M0 || !ll_inout || !n_polished_out) return JK_EINVAL;
     if (star->dev.jitter_mode == JK_JITTER_PHYSICAL && !s) return JK_EINVAL;
@@ -302,8 +304,8 @@ int jk_polish_f64(const jk_star *star, int64_t n, const double *P, const double 
     void *scratch = nullptr;
     if (hipMallocAsync(&scratch, (size_t)jk_polish_scratch_bytes(n), st) != hipSuccess)
         return JK_ENOMEM;
-    hipError_t err = jk_launch_polish(star, n, P, e, omega, M0, s, rows, ll_inout, window, scratch,
-                                      n_polished_out, st);
+    hipError_t err = jk_launch_polish(star, n, P, e, omega, M0, s, rows, ll_inout, window, first,
+                                      scratch, n_polished_out, st);
     hipError_t err2 = hipFreeAsync(scratch, st);
     if (err != hipSuccess) return JK_EHIP;
     return check_hip(err2);
@@ -366,6 +368,51 @@ int jk_batch_create(const jk_star *const *stars, int n_stars, jk_batch **out)
     }
     *out = b;
     return JK_OK;
+}
+
+int jk_batch_create_slots(int capacity, int L, int jitter_mode, jk_batch **out)
+{
+    if (!out || capacity <= 0 || L < 1 || L > JK_LMAX) return JK_EINVAL;
+    if (jitter_mode != JK_JITTER_REFERENCE && jitter_mode != JK_JITTER_PHYSICAL) return JK_EINVAL;
+    *out = nullptr;
+    const int dev = current_device_inited();
+    if (dev < 0) return JK_ENODEV;
+    jk_batch *b = new (std::nothrow) jk_batch;
+    if (!b) return JK_ENOMEM;
+    memset(b, 0, sizeof(*b));
+    b->n_stars = capacity;
+    b->L = L;
+    b->jitter_mode = jitter_mode;
+    b->device = dev;
+    if (hipMalloc((void **)&b->d_stars, sizeof(StarDev) * capacity) != hipSuccess) {
+        delete b;
+        return JK_ENOMEM;
+    }
+    // Empty slots hold a zero-epoch star; no work item should name one.
+    std::vector<StarDev> h(capacity);
+    memset(h.data(), 0, sizeof(StarDev) * capacity);
+    for (int i = 0; i < capacity; ++i) {
+        h[i].L = L;
+        h[i].jitter_mode = jitter_mode;
+    }
+    if (hipMemcpy(b->d_stars, h.data(), sizeof(StarDev) * capacity, hipMemcpyHostToDevice) !=
+        hipSuccess) {
+        (void)hipFree(b->d_stars);
+        delete b;
+        return JK_EHIP;
+    }
+    *out = b;
+    return JK_OK;
+}
+
+int jk_batch_set_star(jk_batch *b, int slot, const jk_star *star, void *stream)
+{
+    if (!b || !star || slot < 0 || slot >= b->n_stars || b->d_epochs) return JK_EINVAL;
+    if (star->dev.L != b->L || star->dev.jitter_mode != b->jitter_mode) return JK_EINVAL;
+    if (star->device != b->device) return JK_EINVAL;
+    if (current_device_inited() < 0) return JK_ENODEV;
+    return check_hip(hipMemcpyAsync(b->d_stars + slot, &star->dev, sizeof(StarDev),
+                                    hipMemcpyHostToDevice, (hipStream_t)stream));
 }
 
 int jk_batch_destroy(jk_batch *b)
@@ -479,7 +526,8 @@ int jk_polish_batch(const jk_batch *batch, int n_seg, const jk_polish_segment *s
     int64_t max_n = 1, max_nb = 1, mask_words = 0, bsum_n = 0, idx_n = 0;
     for (int i = 0; i < n_seg; ++i) {
         const jk_polish_segment &g = segs[i];
-        if (g.n <= 0 || g.offset < 0 || g.row_begin < 0 || g.star < 0 || g.star >= batch->n_stars)
+        if (g.n <= 0 || g.offset < 0 || g.row_begin < 0 || g.star < 0 ||
+            g.star >= batch->n_stars || g.first < 0)
             return JK_EINVAL;
         SegDev &d = h[i];
         memset(&d, 0, sizeof(d));
@@ -492,6 +540,7 @@ int jk_polish_batch(const jk_batch *batch, int n_seg, const jk_polish_segment *s
         d.bsum_off = bsum_n;
         d.star = g.star;
         d.row0 = g.row_begin;
+        d.first = g.first;
         idx_n += g.n;
         mask_words += jk_reject_mask_words(g.n);
         bsum_n += d.nb;

@@ -31,6 +31,7 @@ struct SegDev {
     int64_t mask_off, bsum_off, nb, max_off;
     u128 s_base, inc;
     int64_t star, row0;   // segmented polish: star-table index, first cache row
+    int64_t first;        // segmented polish: first entry still in mixed precision
 };
 
 // Jump table of numpy's PCG64 LCG: (A^(2^i), D_(2^i)) for i = 0..63, where
@@ -65,6 +66,7 @@ hipError_t jk_launch_linear_posterior(const jk_star *st, int64_t n, const double
                                       const double *s, const int64_t *rows, double *a_out,
                                       double *cov_out, double *ll_out, hipStream_t stream);
 hipError_t jk_upload_pcg_table(int device);
+hipError_t jk_upload_sincos_table(int device);
 int64_t jk_polish_scratch_bytes(int64_t n);
 hipError_t jk_launch_lnl_batch(const jk_batch *b, int precision, int64_t n_items,
                                const jk_work_item *items, const double *P, const void *e,
@@ -91,5 +93,5 @@ hipError_t jk_launch_prior_fill(int64_t n, int64_t row0, uint64_t seed, double P
 int jk_seg_flag_blocks(int64_t max_n);
 hipError_t jk_launch_polish(const jk_star *st, int64_t n, const double *P, const double *e,
                             const double *om, const double *M0, const double *s,
-                            const int64_t *rows, double *ll, double window, void *scratch,
-                            int64_t *n_polished, hipStream_t stream);
+                            const int64_t *rows, double *ll, double window, int64_t first,
+                            void *scratch, int64_t *n_polished, hipStream_t stream);

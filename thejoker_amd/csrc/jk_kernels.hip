@@ -10,6 +10,7 @@
 //  k_reject_bsum/compact   order-preserving compaction of the ballot words.
 //  k_uniform               the raw uniforms (tests / callers).
 //  k_linear_posterior      a = G^-1 rhs and G^-1 for accepted rows.
+#include <cmath>
 #include <type_traits>
 
 #include "jk_internal.h"
@@ -22,6 +23,41 @@
 #define JK_LN2 0.69314718055994530942
 
 __constant__ PcgJump c_jump;
+
+// (sin g_i, cos g_i) on the grid g_i = i * JK_TAB_H, i = 0 .. JK_TAB_N, over
+// [0, pi].  JK_TAB_H is pi / 256 rounded to 44 significant bits, so i * H is
+// exact in FP64.  Uploaded once per device by jk_init; the kernels that
+// evaluate the FP64 likelihood stage it in LDS.
+#define JK_TAB_N 256
+#define JK_TAB_H 0x1.921fb54442e00p-7
+#define JK_TAB_INV_H_F 81.48733f
+__device__ double2 g_sincos_tab[JK_TAB_N + 1];
+
+__device__ __forceinline__ void jk_stage_tab(double2 *s_tab)
+{
+    for (int i = threadIdx.x; i <= JK_TAB_N; i += blockDim.x) s_tab[i] = g_sincos_tab[i];
+    __syncthreads();
+}
+
+// sin and cos in FP64 of an FP32 angle Ef in [0, pi] (a little outside is
+// fine): Ef = g_i + d with g_i the grid point below, |d| <= 1.01 H, then
+//   sin Ef = sin g_i cos d + cos g_i sin d,  cos Ef = cos g_i cos d - sin g_i sin d
+// with sin d to d^5 and cos d to d^6 (truncation below 1e-17).  d is exact:
+// Ef has 24 significant bits and g_i 53, and they agree in their leading bits.
+// The index is clamped into the table whatever Ef is.
+__device__ __forceinline__ void jk_sincos_tab(float Ef, const double2 *__restrict__ tab,
+                                              double &s, double &c)
+{
+    int idx = (int)(Ef * JK_TAB_INV_H_F);
+    idx = min(max(idx, 0), JK_TAB_N);
+    const double2 t = tab[idx];
+    const double d = (double)Ef - (double)idx * JK_TAB_H;
+    const double d2 = d * d;
+    const double sd = fma(d * d2, fma(d2, 1.0 / 120.0, -1.0 / 6.0), d);
+    const double cd = fma(d2, fma(d2, fma(d2, -1.0 / 720.0, 1.0 / 24.0), -0.5), 1.0);
+    s = fma(t.x, cd, t.y * sd);
+    c = fma(t.y, cd, -(t.x * sd));
+}
 
 // ---------------------------------------------------------------------------
 // sin and cos on [0, pi] from truncated Taylor series on [0, pi/2]:
@@ -138,19 +174,23 @@ __device__ __forceinline__ void jk_kepler_f32(float x, float e, float &sinE, flo
 }
 
 // ---------------------------------------------------------------------------
-// Non-iterative Kepler solve (Markley 1995, "Kepler equation solver",
-// Celest. Mech. Dyn. Astron. 63, 101): a Pade-based cubic starter accurate to
-// 4.4e-4 rad for every e in [0, 1), followed by one fifth-order correction
-//   d3 = -f0 / (f1 - f0 f2 / (2 f1))
-//   d4 = -f0 / (f1 + d3 f2 / 2 + d3^2 f3 / 6)
-//   d5 = -f0 / (f1 + d4 f2 / 2 + d4^2 f3 / 6 + d4^3 f4 / 24)
-// with f0 = E1 - e sin E1 - M, f1 = 1 - e cos E1, f2 = e sin E1,
-// f3 = e cos E1, f4 = -e sin E1.  From a 4.4e-4 start the correction leaves
-// an error below the conditioning of the problem (ulp(M) / (1 - e cos E)),
-// i.e. the same answer a converged Newton iteration gives.  The starter needs
-// only ~1e-4 accuracy, so it runs in FP32 with the hardware sqrt / log2 /
-// exp2; (sin E, cos E) at E = E1 + d5 come from rotating (sin E1, cos E1) by
-// d5 with fourth-order series (|d5| <= 4.4e-4, truncation < 2e-19).
+// Non-iterative Kepler solve.
+// Starter: Markley (1995, "Kepler equation solver", Celest. Mech. Dyn.
+// Astron. 63, 101), a Pade-based cubic accurate to 4.4e-4 rad for every e in
+// [0, 1) and asymptotically exact as e -> 1, M -> 0.  It needs only that
+// accuracy, so it runs in FP32 with the hardware sqrt / log2 / exp2 / rcp.
+// Correction: with f(E) = E - e sin E - M expanded about the starter E1,
+//   f(E1 + d) = f0 + f1 d + f2 d^2/2 + f3 d^3/6 + f4 d^4/24 + ...,
+//   f0 = E1 - e sin E1 - M, f1 = 1 - e cos E1, f2 = e sin E1, f3 = e cos E1,
+//   f4 = -f2,
+// the root of the quartic is taken by series reversion in t = -f0 / f1:
+//   d = t - b2 t^2 + (2 b2^2 - b3) t^3 + (-5 b2^3 + 5 b2 b3 - b4) t^4
+// with b_k = f_k / (k! f1).  This is the fifth-order step of Markley's
+// scheme computed with one reciprocal instead of three; from the starter's
+// error it leaves an error below the conditioning of the problem
+// (ulp(M) / (1 - e cos E)), the answer a converged Newton iteration gives.
+// (sin E, cos E) at E = E1 + d come from rotating (sin E1, cos E1) by d with
+// fourth-order series (|d| <= 4.4e-4, truncation < 2e-19).
 // M is taken in [0, pi].  Per-sample constants: ome = 1 - e and
 // a1 = 1.6 pi / ((1 + e) (pi^2 - 6)).
 // ---------------------------------------------------------------------------
@@ -169,42 +209,56 @@ __device__ __forceinline__ float jk_markley_starter(float M, float e, float ome,
     const float x = fabsf(r) + __builtin_amdgcn_sqrtf(fmaxf(fmaf(q2, q, r * r), 0.0f));
     const float w = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * 0.6666666667f);
     const float den = fmaf(w, w + q, q2);
-    return fmaf(2.0f * r * w, __builtin_amdgcn_rcpf(den), M) * __builtin_amdgcn_rcpf(d);
+    // (2 r w / den + M) / d with a single reciprocal
+    return fmaf(2.0f * r, w, M * den) * __builtin_amdgcn_rcpf(den * d);
 }
 
-// 1/x in FP64 from the hardware reciprocal refined by two Newton steps.
+// 1/x in FP64 from the hardware reciprocal refined by Newton steps.
+__device__ __forceinline__ double jk_rcp1(double x)
+{
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+
 __device__ __forceinline__ double jk_rcp2(double x)
 {
-    double r = __builtin_amdgcn_rcp(x);
-    r = fma(r, fma(-x, r, 1.0), r);
+    const double r = jk_rcp1(x);
     return fma(r, fma(-x, r, 1.0), r);
 }
 
 __device__ __forceinline__ void jk_kepler_markley_f64(double x, double e, float ef, float omef,
-                                                      float a1f, double &sinE, double &cosE)
+                                                      float a1f, double dmax,
+                                                      const double2 *__restrict__ tab,
+                                                      double &sinE, double &cosE)
 {
     const double ax = fabs(x);
-    const double E1 = (double)jk_markley_starter((float)ax, ef, omef, a1f);
+    const float E1f = jk_markley_starter((float)ax, ef, omef, a1f);
+    const double E1 = (double)E1f;
     double s1, c1;
-    jk_sincos_0pi_f64(E1, s1, c1);
+    jk_sincos_tab(E1f, tab, s1, c1);
     const double es = e * s1, ec = e * c1;
     const double f0 = (E1 - ax) - es;
-    const double f1 = 1.0 - ec;
-    const double rf1 = __builtin_amdgcn_rcp(f1);
-    const double d3 = -f0 * __builtin_amdgcn_rcp(fma(-0.5 * f0 * es, rf1, f1));
-    const double d4 = -f0 * __builtin_amdgcn_rcp(fma(d3, fma(d3, ec * (1.0 / 6.0), 0.5 * es), f1));
-    const double den5 = fma(d4, fma(d4, fma(d4, -es * (1.0 / 24.0), ec * (1.0 / 6.0)), 0.5 * es), f1);
-    const double d5 = -f0 * jk_rcp2(den5);
+    // One Newton step makes r good to the square of the hardware estimate's
+    // error, far below what d (<= 4.4e-4) needs.
+    const double r = jk_rcp1(1.0 - ec);
+    const double t = -f0 * r;
+    const double b2 = 0.5 * es * r;
+    const double b3 = ec * r * (1.0 / 6.0);
+    const double b22 = b2 * b2;
+    const double c3 = fma(2.0, b22, -b3);
+    const double c4 = b2 * fma(5.0, b3 - b22, 1.0 / 12.0);   // -b4 = b2 / 12
+    const double d = t * fma(t, fma(t, fma(t, c4, c3), -b2), 1.0);
     double s, c;
-    if (fabs(d5) < 1e-3) {
-        const double d2 = d5 * d5;
+    if (fabs(d) < dmax) {
+        const double d2 = d * d;
         const double cd = fma(d2, fma(d2, 1.0 / 24.0, -0.5), 1.0);
-        const double sd = d5 * fma(d2, -1.0 / 6.0, 1.0);
+        const double sd = d * fma(d2, -1.0 / 6.0, 1.0);
         s = fma(s1, cd, c1 * sd);
         c = fma(c1, cd, -s1 * sd);
     } else {
-        // Outside the starter's error bound (not reached for e in [0, 1)):
-        // fall back to the bracketed Newton iteration.
+        // Outside the starter's error bound, or e so close to 1 that its
+        // FP32 copy misleads the starter (dmax < 0): fall back to the
+        // bracketed Newton iteration.
         jk_kepler_poly_f64(ax, e, s, c);
     }
     sinE = x < 0.0 ? -s : s;
@@ -220,15 +274,17 @@ __device__ __forceinline__ void jk_kepler_markley_f32(float x, float e, float om
     jk_sincos_0pi_f32(E1, s1, c1);
     const float es = e * s1, ec = e * c1;
     const float f0 = (E1 - ax) - es;
-    const float f1 = 1.0f - ec;
-    const float rf1 = __builtin_amdgcn_rcpf(f1);
-    const float d3 = -f0 * __builtin_amdgcn_rcpf(fmaf(-0.5f * f0 * es, rf1, f1));
-    const float d4 = -f0 * __builtin_amdgcn_rcpf(fmaf(d3, fmaf(d3, ec * (1.0f / 6.0f), 0.5f * es), f1));
-    const float d5 = -f0 * __builtin_amdgcn_rcpf(
-                               fmaf(d4, fmaf(d4, fmaf(d4, -es * (1.0f / 24.0f), ec * (1.0f / 6.0f)), 0.5f * es), f1));
-    const float d2 = d5 * d5;
+    const float r = __builtin_amdgcn_rcpf(1.0f - ec);
+    const float t = -f0 * r;
+    const float b2 = 0.5f * es * r;
+    const float b3 = ec * r * (1.0f / 6.0f);
+    const float b22 = b2 * b2;
+    const float c3 = fmaf(2.0f, b22, -b3);
+    const float c4 = b2 * fmaf(5.0f, b3 - b22, 1.0f / 12.0f);
+    const float d = t * fmaf(t, fmaf(t, fmaf(t, c4, c3), -b2), 1.0f);
+    const float d2 = d * d;
     const float cd = fmaf(d2, -0.5f, 1.0f);
-    const float sd = d5 * fmaf(d2, -1.0f / 6.0f, 1.0f);
+    const float sd = d * fmaf(d2, -1.0f / 6.0f, 1.0f);
     const float s = fmaf(s1, cd, c1 * sd);
     cosE = fmaf(c1, cd, -s1 * sd);
     sinE = x < 0.0f ? -s : s;
@@ -243,7 +299,8 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
                                              const void *__restrict__ e_,
                                              const void *__restrict__ om_,
                                              const void *__restrict__ M0_,
-                                             const void *__restrict__ s_)
+                                             const void *__restrict__ s_,
+                                             const double2 *__restrict__ tab)
 {
     using TF = typename std::conditional<MIXED, float, double>::type;
     const TF *__restrict__ ecol = static_cast<const TF *>(e_);
@@ -272,7 +329,7 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
     const double s2 = sv * sv;
 
     // Per-sample orbit constants.
-    double cw, sw, ecw, sq;
+    double cw, ecw, swsq;
     float cwf, swf, ecwf, sqf;
     // FP32 copies for the Kepler starter (e capped below 1 in FP32).
     const float ef = fminf((float)ev, 0.99999994f);
@@ -283,10 +340,14 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
         ecwf = ef * cwf;
         sqf = sqrtf(omef * (1.0f + ef));
     } else {
+        double sw;
         sincos(omv, &sw, &cw);
         ecw = ev * cw;
-        sq = sqrt((1.0 - ev) * (1.0 + ev));
+        swsq = sw * sqrt((1.0 - ev) * (1.0 + ev));
     }
+    // The FP32 starter sees e rounded to FP32; within 1e-7 of 1 that can put
+    // it outside the correction's reach, so such samples take the fallback.
+    const double dmax = ev > 0.9999999 ? -1.0 : 1e-3;
 
     constexpr int LT = L > 1 ? L - 1 : 1;
     double Smm = 0.0, Smr = 0.0;
@@ -317,9 +378,10 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
             m = (double)fmaf(cwf, cf, fmaf(-swf, sf, ecwf));
         } else {
             double sE, cE;
-            jk_kepler_markley_f64(x, ev, ef, omef, a1f, sE, cE);
+            jk_kepler_markley_f64(x, ev, ef, omef, a1f, dmax, tab, sE, cE);
+            // m = [cos w (cos E - e) - sin w sqrt(1-e^2) sin E] / (1 - e cos E) + e cos w
             const double den = jk_rcp2(fma(-ev, cE, 1.0));
-            m = fma(cw, (cE - ev) * den, fma(-sw, sq * sE * den, ecw));
+            m = fma(fma(cw, cE - ev, -swsq * sE), den, ecw);
         }
         const double iv = st.ivar[k];
         const double rp = st.rp[k];
@@ -398,10 +460,12 @@ __global__ __launch_bounds__(256) void k_lnl(const StarDev st, int64_t n,
                                              const int64_t *__restrict__ rows,
                                              double *__restrict__ ll)
 {
+    __shared__ double2 s_tab[JK_TAB_N + 1];
+    if constexpr (!MIXED) jk_stage_tab(s_tab);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int64_t r = rows ? rows[i] : i;
-    ll[i] = jk_lnl_row<L, PHYS, MIXED>(st, r, P, e_, om_, M0_, s_);
+    ll[i] = jk_lnl_row<L, PHYS, MIXED>(st, r, P, e_, om_, M0_, s_, s_tab);
 }
 
 // Scatter launch over a device-resident index list: for j < *count,
@@ -419,12 +483,14 @@ __global__ __launch_bounds__(256) void k_lnl_scatter_f64(const StarDev st,
                                                          const int64_t *__restrict__ rows,
                                                          double *__restrict__ ll)
 {
+    __shared__ double2 s_tab[JK_TAB_N + 1];
+    jk_stage_tab(s_tab);
     const int64_t cnt = *count;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += stride) {
         const int64_t i = idx[j];
         const int64_t r = rows ? rows[i] : i;
-        ll[i] = jk_lnl_row<L, PHYS, false>(st, r, P, e_, om_, M0_, s_);
+        ll[i] = jk_lnl_row<L, PHYS, false>(st, r, P, e_, om_, M0_, s_, s_tab);
     }
 }
 
@@ -613,6 +679,7 @@ __global__ __launch_bounds__(256) void k_reject_flags(const double *__restrict__
 // Rows to re-evaluate in FP64 after a mixed-precision pass: every row not
 // below max - window, and every row when the maximum itself is not finite.
 __global__ __launch_bounds__(256) void k_window_flags(const double *__restrict__ ll, int64_t n,
+                                                      int64_t first,
                                                       const double *__restrict__ llmax,
                                                       double window,
                                                       uint64_t *__restrict__ mask)
@@ -625,7 +692,7 @@ __global__ __launch_bounds__(256) void k_window_flags(const double *__restrict__
     const double thr = mx - window;
     for (int64_t base = g - lane; base < n; base += T) {
         const int64_t i = base + lane;
-        const bool keep = i < n && (all || !(ll[i] < thr));
+        const bool keep = i < n && i >= first && (all || !(ll[i] < thr));
         const uint64_t word = __ballot(keep);
         if (lane == 0) mask[base >> 6] = word;
     }
@@ -777,6 +844,19 @@ hipError_t jk_upload_pcg_table(int device)
     return hipMemcpyToSymbol(HIP_SYMBOL(c_jump), &tab, sizeof(tab), 0, hipMemcpyHostToDevice);
 }
 
+// The sin / cos grid of jk_sincos_tab, from 80-bit long double on the host
+// rounded once to FP64.
+hipError_t jk_upload_sincos_table(int device)
+{
+    double2 tab[JK_TAB_N + 1];
+    for (int i = 0; i <= JK_TAB_N; ++i) {
+        const long double g = (long double)i * (long double)JK_TAB_H;
+        tab[i] = make_double2((double)sinl(g), (double)cosl(g));
+    }
+    return hipMemcpyToSymbol(HIP_SYMBOL(g_sincos_tab), tab, sizeof(tab), 0,
+                             hipMemcpyHostToDevice);
+}
+
 hipError_t jk_launch_uniform(u128 state, u128 inc, uint64_t offset, int64_t n, double *u,
                              hipStream_t stream)
 {
@@ -826,8 +906,8 @@ int64_t jk_polish_scratch_bytes(int64_t n)
 
 hipError_t jk_launch_polish(const jk_star *st, int64_t n, const double *P, const double *e,
                             const double *om, const double *M0, const double *s,
-                            const int64_t *rows, double *ll, double window, void *scratch,
-                            int64_t *n_polished, hipStream_t stream)
+                            const int64_t *rows, double *ll, double window, int64_t first,
+                            void *scratch, int64_t *n_polished, hipStream_t stream)
 {
     if (n <= 0) return hipSuccess;
     const int np = jk_max_partials_count(n);
@@ -843,7 +923,8 @@ hipError_t jk_launch_polish(const jk_star *st, int64_t n, const double *P, const
     hipError_t err = jk_launch_max(ll, n, partials, np, mx, stream);
     if (err != hipSuccess) return err;
     const int grid = jk_rng_grid(n);
-    hipLaunchKernelGGL(k_window_flags, dim3(grid), dim3(256), 0, stream, ll, n, mx, window, mask);
+    hipLaunchKernelGGL(k_window_flags, dim3(grid), dim3(256), 0, stream, ll, n, first, mx, window,
+                       mask);
     hipLaunchKernelGGL(k_reject_bsum, dim3((unsigned)nb), dim3(256), 0, stream, mask, nwords, bsum);
     hipLaunchKernelGGL(k_reject_compact, dim3((unsigned)nb), dim3(256), 0, stream, mask, nwords, bsum,
                        (int)nb, n, idx, count);
@@ -895,6 +976,8 @@ __global__ __launch_bounds__(256) void k_lnl_batch(const StarDev *__restrict__ s
                                                    double *__restrict__ ll)
 {
     __shared__ unsigned s_item;
+    __shared__ double2 s_tab[JK_TAB_N + 1];
+    if constexpr (!MIXED) jk_stage_tab(s_tab);
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
         __syncthreads();
@@ -906,7 +989,7 @@ __global__ __launch_bounds__(256) void k_lnl_batch(const StarDev *__restrict__ s
         for (int j = threadIdx.x; j < w.n_rows; j += blockDim.x) {
             const int64_t q = w.row_begin + j;
             const int64_t r = rows ? rows[q] : q;
-            ll[w.out_offset + j] = jk_lnl_row<L, PHYS, MIXED>(st, r, P, e_, om_, M0_, s_);
+            ll[w.out_offset + j] = jk_lnl_row<L, PHYS, MIXED>(st, r, P, e_, om_, M0_, s_, s_tab);
         }
     }
 }
@@ -1153,7 +1236,7 @@ __global__ __launch_bounds__(256) void k_seg_window_flags(const double *__restri
     const double thr = mx - window;
     for (int64_t base = g - lane; base < sg.n; base += T) {
         const int64_t i = base + lane;
-        const bool keep = i < sg.n && (all || !(ll[sg.off + i] < thr));
+        const bool keep = i < sg.n && i >= sg.first && (all || !(ll[sg.off + i] < thr));
         const uint64_t word = __ballot(keep);
         if (lane == 0) mask[sg.mask_off + (base >> 6)] = word;
     }
@@ -1172,6 +1255,8 @@ __global__ __launch_bounds__(256) void k_seg_scatter_f64(const StarDev *__restri
                                                          const int64_t *__restrict__ rows,
                                                          double *__restrict__ ll)
 {
+    __shared__ double2 s_tab[JK_TAB_N + 1];
+    jk_stage_tab(s_tab);
     const SegDev sg = segs[blockIdx.y];
     const int64_t cnt = count[blockIdx.y];
     const StarDev &st = stars[sg.star];
@@ -1180,7 +1265,7 @@ __global__ __launch_bounds__(256) void k_seg_scatter_f64(const StarDev *__restri
         const int64_t p = idx[sg.idx_off + j];
         const int64_t q = sg.row0 + p;
         const int64_t r = rows ? rows[q] : q;
-        ll[sg.off + p] = jk_lnl_row<L, PHYS, false>(st, r, P, e_, om_, M0_, s_);
+        ll[sg.off + p] = jk_lnl_row<L, PHYS, false>(st, r, P, e_, om_, M0_, s_, s_tab);
     }
 }
 

@@ -342,10 +342,11 @@ def lnl(star, cache, rows=None, n=None, precision="f64", out=None, start=0):
     return out
 
 
-def polish(star, cache, ll, rows=None, start=0, window=40.0):
-    """FP64 re-evaluation, in place, of every entry of `ll` (a mixed-precision
-    lnL of cache rows `rows` or [start, start + len(ll))) not below
-    max(ll) - window.  Returns a device int64 tensor holding the count."""
+def polish(star, cache, ll, rows=None, start=0, window=40.0, first=0):
+    """FP64 re-evaluation, in place, of every entry ll[i], i >= first, of `ll`
+    (the lnL of cache rows `rows` or [start, start + len(ll)), mixed precision
+    from `first` on) not below max(ll) - window.  Returns a device int64
+    tensor holding the count."""
     torch = _torch()
     lib = _lib.ensure_device(cache.device)
     n = int(ll.shape[0])
@@ -356,7 +357,7 @@ def polish(star, cache, ll, rows=None, start=0, window=40.0):
     cols = [ctypes.c_void_p(c.data_ptr() + off) for c in
             (cache.P, cache.e, cache.omega, cache.M0, cache.s)]
     _lib.check(lib.jk_polish_f64(star.handle, n, *cols, _dptr(rows), _dptr(ll), float(window),
-                                 _dptr(count), _stream(cache.device)), "jk_polish_f64")
+                                 int(first), _dptr(count), _stream(cache.device)), "jk_polish_f64")
     return count
 
 
@@ -381,21 +382,34 @@ def reject(ll, rng_state, max_keep, offset=0, llmax=None, device=0):
         raise ValueError("zero-size array to reduction operation maximum which has no identity")
     dev = ll.device
     max_keep = int(min(max_keep, n))
-    idx = torch.empty(max(max_keep, 1), dtype=torch.int64, device=dev)
-    npass = torch.zeros(1, dtype=torch.int64, device=dev)
+    # one device block [n_pass, max, idx...] and one copy back to pinned memory
+    res = torch.empty(2 + max(max_keep, 1), dtype=torch.int64, device=dev)
+    p = res.data_ptr()
     state, inc = rng_state
     if llmax is None:
-        mx = torch.empty(1, dtype=torch.float64, device=dev)
         _lib.check(lib.jk_reject(_dptr(ll), n, *_split(state), *_split(inc), int(offset), max_keep,
-                                 _dptr(idx), _dptr(npass), _dptr(mx), _stream(device)), "jk_reject")
+                                 ctypes.c_void_p(p + 16), ctypes.c_void_p(p),
+                                 ctypes.c_void_p(p + 8), _stream(device)), "jk_reject")
     else:
-        mx = llmax
-        _lib.check(lib.jk_reject_with_max(_dptr(ll), n, _dptr(mx), *_split(state), *_split(inc),
-                                          int(offset), max_keep, _dptr(idx), _dptr(npass),
-                                          _stream(device)), "jk_reject_with_max")
-    n_pass = int(npass.item())
+        _lib.check(lib.jk_reject_with_max(_dptr(ll), n, _dptr(llmax), *_split(state), *_split(inc),
+                                          int(offset), max_keep, ctypes.c_void_p(p + 16),
+                                          ctypes.c_void_p(p), _stream(device)),
+                   "jk_reject_with_max")
+        res[1:2].copy_(llmax.reshape(1).view(torch.int64))
+    h = to_host(res)
+    n_pass = int(h[0])
     k = min(n_pass, max_keep)
-    return idx[:k].cpu().numpy(), n_pass, float(mx.item())
+    return h[2:2 + k].copy(), n_pass, float(h[1:2].view(np.float64)[0])
+
+
+def to_host(t):
+    """Device tensor -> numpy array through a pinned staging buffer (one DMA
+    copy ordered on the current stream, then a wait for that stream)."""
+    torch = _torch()
+    h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+    h.copy_(t, non_blocking=True)
+    torch.cuda.current_stream(t.device).synchronize()
+    return h.numpy()
 
 
 def linear_posterior(star, cache, rows):
@@ -408,11 +422,15 @@ def linear_posterior(star, cache, rows):
     if k == 0:
         return np.zeros((0, L)), np.zeros((0, L, L)), np.zeros(0)
     r = torch.as_tensor(np.asarray(rows, dtype=np.int64), device=dev)
-    a = torch.empty((k, L), dtype=torch.float64, device=dev)
-    cov = torch.empty((k, L, L), dtype=torch.float64, device=dev)
-    ll = torch.empty(k, dtype=torch.float64, device=dev)
+    # a, cov and lnL in one device block, one copy back
+    res = torch.empty(k * (L + L * L + 1), dtype=torch.float64, device=dev)
+    p = res.data_ptr()
     _lib.check(lib.jk_linear_posterior(star.handle, k, _dptr(cache.P), _dptr(cache.e),
                                        _dptr(cache.omega), _dptr(cache.M0), _dptr(cache.s),
-                                       _dptr(r), _dptr(a), _dptr(cov), _dptr(ll),
+                                       _dptr(r), ctypes.c_void_p(p),
+                                       ctypes.c_void_p(p + 8 * k * L),
+                                       ctypes.c_void_p(p + 8 * k * (L + L * L)),
                                        _stream(cache.device)), "jk_linear_posterior")
-    return a.cpu().numpy(), cov.cpu().numpy(), ll.cpu().numpy()
+    h = to_host(res)
+    return (h[:k * L].reshape(k, L).copy(), h[k * L:k * (L + L * L)].reshape(k, L, L).copy(),
+            h[k * (L + L * L):].copy())

@@ -2,24 +2,25 @@
 
 Each request is one star run through ``iterative_rejection_sample``
 (thejoker.py:259-370) against a shared :class:`PriorCache`.  Requests are
-admitted while active slots and the HBM arena allow.  Every step evaluates
-the next batch of rows of *every* active request in one persistent
-``jk_lnl_batch`` launch (stars of different epoch counts side by side)
-writing straight into each request's region of one HBM lnL arena, then
-re-evaluates every request's live window in FP64 (``jk_polish_batch``) and
-runs every request's rejection step with its own numpy PCG64 stream
-(``jk_reject_batch``).  The iteration state of each request is
-:class:`thejoker_amd.thejoker.IterativeRejection` -- the code the single-star
-API runs -- and the batched lnL is bit-identical to the single-star kernel,
-so a request's result is exactly what ``TheJoker.iterative_rejection_sample``
-returns for that star alone.
+admitted while active slots and the HBM arena allow; each admitted star takes
+a slot of one device star table (``jk_batch_create_slots``) until it
+finishes.  Every step evaluates the next batch of rows of *every* active
+request in one persistent ``jk_lnl_batch`` launch (stars of different epoch
+counts side by side) writing straight into each request's region of one HBM
+lnL arena, then re-evaluates the new rows of every request's live window in
+FP64 (``jk_polish_batch``) and runs every request's rejection step with its
+own numpy PCG64 stream (``jk_reject_batch``).  The iteration state of each
+request is :class:`thejoker_amd.thejoker.IterativeRejection` -- the code the
+single-star API runs -- and the batched lnL is bit-identical to the
+single-star kernel, so a request's result is exactly what
+``TheJoker.iterative_rejection_sample`` returns for that star alone.
 """
 import time
 
 import numpy as np
 
-from .batch import LaunchShape, StarBatch, lnl_batch, polish_batch, reject_batch, upload_items, \
-    work_items
+from .batch import LaunchShape, StarBatch, heaviest_first, lnl_batch, polish_batch, reject_batch, \
+    upload_items, work_items
 from .device import DeviceStar, _torch, linear_internal_units, pcg64_state, star_arrays
 from .likelihood_helpers import validate_prepare_data
 from .thejoker import POLISH_WINDOW, IterativeRejection, TheJoker
@@ -72,6 +73,7 @@ class Request:
         self.star = None
         self.it = None
         self.region = None      # (start, capacity) in the arena
+        self.slot = None        # star-table slot
         self.result = None
         self.error = None
         self.rows_evaluated = 0
@@ -99,11 +101,18 @@ class JokerServer:
     in_memory : bool
         Iteration rule of the in-memory path (safety factor 1) or of the
         default path (safety factor 4).
+    profile : bool
+        Synchronise the device between the phases of every step and add each
+        phase's wall time to ``self.phase_s`` (admit, plan, lnl, polish,
+        reject, update, finish).
     """
+
+    PHASES = ("admit", "plan", "lnl", "polish", "reject", "update", "finish")
 
     def __init__(self, prior, cache, n_requested=256, precision="mixed", jitter_mode="physical",
                  max_active=64, hbm_budget_bytes=4 << 30, in_memory=False, growth_factor=128,
-                 n_linear_samples=1, shape=None, device=0, clock=time.perf_counter):
+                 n_linear_samples=1, shape=None, device=0, clock=time.perf_counter,
+                 profile=False):
         torch = _torch()
         self.prior = prior
         self.cache = cache
@@ -126,9 +135,13 @@ class JokerServer:
         self.arena = torch.empty(int(hbm_budget_bytes) // 8, dtype=torch.float64,
                                  device=cache.P.device)
         self.alloc = RangeAllocator(self.arena.shape[0])
-        self._batch = None
-        self._batch_key = None
+        self._table = StarBatch.slots(self.max_active, 1 + prior.poly_trend + prior.n_offsets,
+                                      jitter_mode, device=device)
+        self._free_slots = list(range(self.max_active - 1, -1, -1))
         self._ws = torch.zeros(64, dtype=torch.int32, device=cache.P.device)
+        self.profile = profile
+        self.phase_s = dict.fromkeys(self.PHASES, 0.0)
+        self._t_mark = None
         self._helper = TheJoker(prior, rng=np.random.default_rng(0), jitter_mode=jitter_mode,
                                 precision=precision, device=device)
 
@@ -145,13 +158,29 @@ class JokerServer:
     def busy(self):
         return bool(self.pending or self.active)
 
+    def _mark(self, phase=None):
+        """Profiling: charge the time since the last mark to `phase`."""
+        if not self.profile:
+            return
+        _torch().cuda.synchronize()
+        t = time.perf_counter()
+        if phase is not None and self._t_mark is not None:
+            self.phase_s[phase] += t - self._t_mark
+        self._t_mark = t
+
     # ------------------------------------------------------------------
-    def _fail(self, r, exc):
-        r.error = exc
-        r.t_done = self.now()
+    def _release(self, r):
         if r.region is not None:
             self.alloc.release(*r.region)
             r.region = None
+        if r.slot is not None:
+            self._free_slots.append(r.slot)
+            r.slot = None
+
+    def _fail(self, r, exc):
+        r.error = exc
+        r.t_done = self.now()
+        self._release(r)
         self.finished.append(r)
 
     def _admit(self):
@@ -176,6 +205,8 @@ class JokerServer:
                                           in_memory=self.in_memory,
                                           growth_factor=self.growth_factor,
                                           device=self.cache.P.device)
+                r.slot = self._free_slots.pop()
+                self._table.set(r.slot, r.star)
             except Exception as exc:       # a bad request fails alone
                 self._fail(r, exc)
                 continue
@@ -202,7 +233,9 @@ class JokerServer:
     def step(self):
         """Admit, evaluate one round of every active request, retire the
         finished.  Returns the requests finished in this step."""
+        self._mark()
         self._admit()
+        self._mark("admit")
         if not self.active:
             return []
         plan = []
@@ -217,28 +250,33 @@ class JokerServer:
             self.active.remove(r)
             self._fail(r, MemoryError("lnL arena too small for this request's next batch"))
             return [r]
-        key = tuple(id(r) for r, _, _ in plan)
-        if key != self._batch_key:
-            self._batch = StarBatch([r.star for r, _, _ in plan], device=self.device)
-            self._batch_key = key
+        slots = np.array([r.slot for r, _, _ in plan], dtype=np.int64)
         sizes = np.array([n for _, _, n in plan], dtype=np.int64)
         begins = np.array([b for _, b, _ in plan], dtype=np.int64)
-        dst = np.array([r.region[0] + r.it.n_evals for r, _, _ in plan], dtype=np.int64)
-        items, off, total = work_items(sizes, self.shape.tile, row_begin=begins)
-        # write each request's rows straight into its arena region
-        items["out_offset"] += np.repeat(dst - off, (sizes + self.shape.tile - 1) // self.shape.tile)
-        lnl_batch(self._batch, self.cache, upload_items(items, self.device), len(items), 0,
-                  self.precision, self.shape, out=self.arena, workspace=self._ws)
-        M = self._batch.M
-        self.sample_epochs += int(np.sum(sizes * M))
+        before = np.array([r.it.n_evals for r, _, _ in plan], dtype=np.int64)
         seg_off = np.array([r.region[0] for r, _, _ in plan], dtype=np.int64)
-        seg_n = np.array([r.it.n_evals + n for r, _, n in plan], dtype=np.int64)
+        items, off, total = work_items(sizes, self.shape.tile, row_begin=begins, stars=slots)
+        # write each request's rows straight into its arena region
+        counts = (sizes + self.shape.tile - 1) // self.shape.tile
+        items["out_offset"] += np.repeat(seg_off + before - off, counts)
+        items = items[heaviest_first(items, self._table.M)]
+        d_items = upload_items(items, self.device)
+        self._mark("plan")
+        lnl_batch(self._table, self.cache, d_items, len(items), 0, self.precision, self.shape,
+                  out=self.arena, workspace=self._ws)
+        M = self._table.M[slots]
+        self.sample_epochs += int(np.sum(sizes * M))
+        seg_n = before + sizes
+        self._mark("lnl")
         if self.precision == "mixed":
-            polish_batch(self._batch, self.cache, self.arena, seg_off, seg_n, window=POLISH_WINDOW,
-                         device=self.device)
+            # rows evaluated in earlier steps are FP64 where it matters already
+            polish_batch(self._table, self.cache, self.arena, seg_off, seg_n, stars=slots,
+                         window=POLISH_WINDOW, first=before, device=self.device)
+        self._mark("polish")
         good, n_pass, llmax = reject_batch(self.arena, seg_off, seg_n,
                                            [pcg64_state(r.rng) for r, _, _ in plan],
                                            self.n_requested, device=self.device)
+        self._mark("reject")
         done = []
         for k, (r, _, n_now) in enumerate(plan):
             r.rows_evaluated += n_now
@@ -253,6 +291,7 @@ class JokerServer:
                 r.it.done = True
             if r.it.done:
                 done.append(r)
+        self._mark("update")
         for r in done:
             self.active.remove(r)
             if r.error is None:
@@ -260,11 +299,11 @@ class JokerServer:
                 r.result = self._helper._full_samples(r.star, self.cache, full_idx, r.rng,
                                                       self.n_linear_samples,
                                                       spawn_batches=None if self.in_memory else 1)
-            self.alloc.release(*r.region)
-            r.region = None
+            self._release(r)
             r.t_done = self.now()
             self.finished.append(r)
         self.steps += 1
+        self._mark("finish")
         return done
 
 

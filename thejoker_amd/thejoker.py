@@ -160,10 +160,10 @@ class TheJoker:
             sg = rng.bit_generator._seed_seq.spawn(len(tasks))
             gens = [(tasks[i][0], Generator(PCG64(sg[i]))) for i in range(len(tasks))]
         for sel, g in gens:
-            for j in np.asarray(sel, dtype=np.int64):
-                lin = g.multivariate_normal(a[j], cov[j], size=n_linear_samples)
-                raw[j, :, :5] = packed[j]
-                raw[j, :, 5:] = lin
+            sel = np.asarray(sel, dtype=np.int64)
+            if len(sel):
+                raw[sel, :, :5] = packed[sel][:, None, :]
+                raw[sel, :, 5:] = mvn_draws(g, a[sel], cov[sel], n_linear_samples)
         raw = raw.reshape(len(rows) * n_linear_samples, n_pars)
         return JokerSamples.unpack(raw, star.internal_units, t_ref=star.data.t_ref,
                                    poly_trend=self.prior.poly_trend,
@@ -387,6 +387,23 @@ class IterativeRejection:
 def _linear_posterior(star, cache, rows):
     from .device import linear_posterior
     return linear_posterior(star, cache, rows)
+
+
+def mvn_draws(rng, means, covs, size):
+    """rng.multivariate_normal(means[j], covs[j], size=size) for every j in
+    order, as one stacked computation (fast_likelihood.pyx:529-530 draws row
+    by row).  Generator.multivariate_normal (method "svd") factors
+    cov = u diag(s) vh, draws standard normals of shape (size, L) and returns
+    mean + z @ (u sqrt(s))^T; the stacked SVD and product run the same LAPACK
+    and BLAS kernels per matrix, and the normals are drawn in the same order,
+    so the values and the generator's final state are the same."""
+    means = np.asarray(means, dtype=np.float64)
+    covs = np.asarray(covs, dtype=np.float64)
+    n, L = means.shape
+    u, s, _ = np.linalg.svd(covs)
+    factor = u * np.sqrt(s)[:, None, :]
+    z = rng.standard_normal((n, size, L))
+    return means[:, None, :] + np.matmul(z, np.swapaxes(factor, 1, 2))
 
 
 # Re-evaluation window of the mixed path: rows whose mixed lnL is within this
