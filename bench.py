@@ -107,8 +107,8 @@ def leg_single(args, torch, np, D, syn):
     ll = torch.empty(n, dtype=torch.float64, device="cuda")
 
     def step():
-        D.lnl(dstar, cache, precision="f64", out=ll)
-        idx, n_pass, mx = D.reject(ll, D.pcg64_state(rng), 256)
+        # lnL of every row and the rejection step (max reduced in the lnL launch)
+        _, idx, n_pass, mx = D.lnl_reject(dstar, cache, D.pcg64_state(rng), 256, out=ll)
         D.pcg64_advance_state(rng, n)
         return idx, n_pass
 

@@ -146,6 +146,18 @@ int jk_reject_with_max(const double *ll, int64_t n, const double *llmax,
                        uint64_t stream_offset, int64_t max_keep, int64_t *idx_out,
                        int64_t *n_pass_out, void *stream);
 
+/* One rejection-sampling pass of one star in a single call: the plain FP64
+ * lnL of n prior rows into ll_out (exactly as jk_lnl_f64) followed by the
+ * rejection step over them (exactly as jk_reject, same arguments and
+ * outputs), with max(ll) reduced inside the likelihood launch instead of by
+ * a second pass over ll_out.  Replaces marginal_ln_likelihood_inmem plus the
+ * rejection step of rejection_sample_inmem (likelihood_helpers.py:104-109). */
+int jk_lnl_reject_f64(const jk_star *star, int64_t n, const double *P, const double *e,
+                      const double *omega, const double *M0, const double *s,
+                      const int64_t *rows, double *ll_out, uint64_t state_hi, uint64_t state_lo,
+                      uint64_t inc_hi, uint64_t inc_lo, uint64_t stream_offset, int64_t max_keep,
+                      int64_t *idx_out, int64_t *n_pass_out, double *llmax_out, void *stream);
+
 /* numpy-identical uniforms u[i] = output offset+i of the PCG64 stream
  * (Generator.uniform, likelihood_helpers.py:107), for tests and callers that
  * want the draws themselves. */

@@ -93,3 +93,35 @@ def test_host_generator_stays_in_sync(gpu):
     g2.uniform(size=n)
     assert g1.uniform() == g2.uniform()
     assert np.array_equal(g1.normal(size=10), g2.normal(size=10))
+
+
+@pytest.mark.parametrize("n", [1, 255, 256, 257, 70_001, 1 << 20])
+def test_lnl_reject_equals_separate_calls(gpu, n):
+    """jk_lnl_reject_f64 (maximum reduced inside the likelihood launch) gives
+    the lnL of jk_lnl_f64 and the rejection of jk_reject, bit for bit."""
+    from thejoker_amd import synthetic as syn
+    st = syn.single_star()
+    cols = syn.prior_columns(n + 300, seed=n, jitter=True)
+    cols["e"][5 % n] = 1.5                     # one invalid row: NaN propagates
+    star = D.DeviceStar(st.t, st.rv, st.ivar(), st.trend(), np.zeros(2), np.array([900.0, 1e4]),
+                        st.t_ref)
+    cache = D.PriorCache(cols["P"], cols["e"], cols["omega"], cols["M0"], cols["s"])
+    for nan_row in (False, True):
+        if nan_row and n < 6:
+            continue
+        start = 0 if nan_row else 300
+        g1, g2 = np.random.default_rng(9), np.random.default_rng(9)
+        want_ll = D.lnl(star, cache, start=start, n=n)
+        want = D.reject(want_ll, D.pcg64_state(g1), 256)
+        ll, idx, n_pass, mx = D.lnl_reject(star, cache, D.pcg64_state(g2), 256, start=start, n=n)
+        assert np.array_equal(ll.cpu().numpy(), want_ll.cpu().numpy(), equal_nan=True)
+        assert np.array_equal(idx, want[0]) and n_pass == want[1]
+        assert (np.isnan(mx) and np.isnan(want[2])) or mx == want[2]
+        assert np.isnan(mx) == nan_row
+    rows = torch.as_tensor(np.random.default_rng(1).permutation(n + 300)[:n], device="cuda")
+    want_ll = D.lnl(star, cache, rows=rows)
+    want = D.reject(want_ll, D.pcg64_state(np.random.default_rng(3)), 10)
+    ll, idx, n_pass, mx = D.lnl_reject(star, cache, D.pcg64_state(np.random.default_rng(3)), 10,
+                                       rows=rows)
+    assert np.array_equal(ll.cpu().numpy(), want_ll.cpu().numpy(), equal_nan=True)
+    assert np.array_equal(idx, want[0]) and n_pass == want[1]

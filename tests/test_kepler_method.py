@@ -1,0 +1,119 @@
+"""The non-iterative Kepler method of the gfx950 kernels, emulated on the CPU
+in numpy (FP32 starter rounded through numpy float32, FP64 table sincos and
+one-reciprocal series correction) against a long-double Newton solution: the
+error stays within a small multiple of the conditioning ulp(M) / (1 - e cos E)
+for e up to 1 - 1e-6 and M down to 1e-12, as a converged Newton iteration's
+does.  Pins the method, not the device code (the GPU parity tests do that)."""
+import numpy as np
+import pytest
+
+f32 = np.float32
+LD = np.longdouble
+A0 = f32(7.6516382901912925)     # 3 pi^2 / (pi^2 - 6)
+A1N = f32(1.29898246041084)      # 1.6 pi / (pi^2 - 6)
+PIF = f32(np.pi)
+NT = 256
+_m, _ex = np.frexp(np.pi / NT)
+H = float(np.ldexp(np.round(_m * 2 ** 44), _ex - 44))
+TS = np.sin(np.arange(NT + 1) * LD(H)).astype(np.float64)
+TC = np.cos(np.arange(NT + 1) * LD(H)).astype(np.float64)
+
+
+def starter(M, e):
+    """Markley's cubic starter in FP32, one reciprocal (as the kernel)."""
+    ome = f32(1) - e
+    a1 = A1N / (f32(1) + e)
+    alpha = a1 * (PIF - M) + A0
+    d = alpha * e + f32(3) * ome
+    M2 = M * M
+    ad = alpha * d
+    q = f32(2) * ad * ome - M2
+    r = M * (f32(3) * ad * (d - ome) + M2)
+    q2 = q * q
+    x = np.abs(r) + np.sqrt(np.maximum(q2 * q + r * r, f32(0)))
+    w = np.exp2(np.log2(x) * f32(2.0 / 3.0))
+    den = w * (w + q) + q2
+    return ((f32(2) * r * w + M * den) / (den * d)).astype(f32)
+
+
+def table_sincos(Ef):
+    idx = np.clip((Ef * f32(NT / np.pi)).astype(np.int32), 0, NT)
+    d = Ef.astype(np.float64) - idx * H
+    d2 = d * d
+    sd = d + d * d2 * (d2 / 120 - 1 / 6)
+    cd = 1 + d2 * (-0.5 + d2 * (1 / 24 - d2 / 720))
+    return TS[idx] * cd + TC[idx] * sd, TC[idx] * cd - TS[idx] * sd
+
+
+def solve(M, e):
+    """E, sin E, cos E for M in [0, pi] (the kernel's sequence of steps)."""
+    E1f = starter(M.astype(f32), np.minimum(e.astype(f32), f32(0.99999994)))
+    E1 = E1f.astype(np.float64)
+    s1, c1 = table_sincos(E1f)
+    es, ec = e * s1, e * c1
+    f0 = (E1 - M) - es
+    r = 1.0 / (1.0 - ec)
+    t = -f0 * r
+    b2 = 0.5 * es * r
+    b3 = ec * r / 6
+    b22 = b2 * b2
+    c3 = 2 * b22 - b3
+    c4 = b2 * (5 * (b3 - b22) + 1 / 12)
+    d = t * (1 + t * (-b2 + t * (c3 + t * c4)))
+    d2 = d * d
+    cd = 1 + d2 * (-0.5 + d2 / 24)
+    sd = d * (1 - d2 / 6)
+    return E1 + d, s1 * cd + c1 * sd, c1 * cd - s1 * sd, d
+
+
+def newton_ld(M, e, E):
+    E = E.astype(LD)
+    Ml, el = M.astype(LD), e.astype(LD)
+    for _ in range(6):
+        E = E - (E - el * np.sin(E) - Ml) / (1 - el * np.cos(E))
+    return E
+
+
+@pytest.mark.parametrize("e0", [0.0, 0.3, 0.7, 0.9, 0.99, 0.999, 0.9999, 0.99999, 1 - 1e-6])
+def test_series_correction_reaches_conditioning(e0):
+    rng = np.random.default_rng(int(e0 * 1e6))
+    k = 40_000
+    M = np.concatenate([np.exp(rng.uniform(np.log(1e-12), np.log(np.pi), k)),
+                        rng.uniform(0, np.pi, k), [0.0, np.pi]])
+    e = np.full(M.shape, e0)
+    E, s, c, d = solve(M, e)
+    T = newton_ld(M, e, E)
+    eps = np.finfo(float).eps
+    cond = np.maximum(eps * (np.abs(T) + e * np.abs(np.sin(T)) + M) / (1 - e * np.cos(T)), 1e-300)
+    assert np.all(np.abs(d) < 1e-3)                      # no fallback needed
+    assert (np.abs((E - T).astype(float)) / cond).max() < 2.5
+    assert np.abs((s - np.sin(T)).astype(float)).max() < 4 * eps + 2.5 * cond.max()
+    assert np.abs((c - np.cos(T)).astype(float)).max() < 4 * eps + 2.5 * cond.max()
+
+
+def test_table_sincos_accuracy():
+    Ef = np.random.default_rng(0).uniform(0, np.pi, 200_000).astype(f32)
+    s, c = table_sincos(Ef)
+    x = Ef.astype(LD)
+    assert np.abs((s - np.sin(x)).astype(float)).max() < 2.5e-16
+    assert np.abs((c - np.cos(x)).astype(float)).max() < 2.5e-16
+
+
+def test_truncated_series_would_fail():
+    """Dropping the t^4 term of the reversion (a plausible slip) breaks the
+    bound at e = 0.99, so the test above can see such a mistake."""
+    rng = np.random.default_rng(5)
+    M = np.concatenate([rng.uniform(0, np.pi, 50_000),
+                        np.exp(rng.uniform(np.log(1e-9), 0.0, 50_000))])
+    e = np.full(M.shape, 0.99)
+    E1f = starter(M.astype(f32), e.astype(f32))
+    E1 = E1f.astype(np.float64)
+    s1, c1 = table_sincos(E1f)
+    es, ec = e * s1, e * c1
+    t = -((E1 - M) - es) / (1 - ec)
+    b2 = 0.5 * es / (1 - ec)
+    b3 = ec / (1 - ec) / 6
+    E = E1 + t * (1 + t * (-b2 + t * (2 * b2 * b2 - b3)))
+    T = newton_ld(M, e, E)
+    cond = np.finfo(float).eps * (np.abs(T) + e * np.abs(np.sin(T)) + M) / (1 - e * np.cos(T))
+    assert (np.abs((E - T).astype(float)) / cond).max() > 10

@@ -44,6 +44,9 @@ SIGNATURES = [
      [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _i64, _vp, _vp]),
     ("jk_reject", ctypes.c_int,
      [_vp, _i64, _u64, _u64, _u64, _u64, _u64, _i64, _vp, _vp, _vp, _vp]),
+    ("jk_lnl_reject_f64", ctypes.c_int,
+     [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _u64, _u64, _u64, _u64, _i64, _vp, _vp,
+      _vp, _vp]),
     ("jk_reject_with_max", ctypes.c_int,
      [_vp, _i64, _vp, _u64, _u64, _u64, _u64, _u64, _i64, _vp, _vp, _vp]),
     ("jk_pcg64_uniform", ctypes.c_int, [_u64, _u64, _u64, _u64, _u64, _i64, _vp, _vp]),

@@ -146,6 +146,8 @@ int jk_star_create(int n, const double *t, const double *rv, const double *ivar,
             }
         }
     }
+    d.ln_c = (double)n * JK_LN_TWO_PI - d.sum_ln_ivar + d.ln_lam_trend +
+             (k_prior_mode == JK_KPRIOR_NORMAL ? log(lambda_diag[0]) : 0.0);
     for (int j = 0; j < LT; ++j)
         for (int q = 0; q < j; ++q) d.S_tt[q * JK_LMAX + j] = d.S_tt[j * JK_LMAX + q];
 
@@ -266,6 +268,30 @@ int jk_reject_with_max(const double *ll, int64_t n, const double *llmax, uint64_
     if (!llmax) return JK_EINVAL;
     return reject_impl(ll, n, llmax, state_hi, state_lo, inc_hi, inc_lo, stream_offset, max_keep,
                        idx_out, n_pass_out, nullptr, (hipStream_t)stream);
+}
+
+int jk_lnl_reject_f64(const jk_star *star, int64_t n, const double *P, const double *e,
+                      const double *omega, const double *M0, const double *s,
+                      const int64_t *rows, double *ll_out, uint64_t state_hi, uint64_t state_lo,
+                      uint64_t inc_hi, uint64_t inc_lo, uint64_t stream_offset, int64_t max_keep,
+                      int64_t *idx_out, int64_t *n_pass_out, double *llmax_out, void *stream)
+{
+    if (!star || n <= 0 || !P || !e || !omega || !M0 || !ll_out) return JK_EINVAL;
+    if (star->dev.jitter_mode == JK_JITTER_PHYSICAL && !s) return JK_EINVAL;
+    if (max_keep < 0 || !n_pass_out || !llmax_out || (max_keep > 0 && !idx_out)) return JK_EINVAL;
+    if ((n + 255) / 256 > (int64_t)0x7fffffff) return JK_ERANGE;
+    if (current_device_inited() < 0) return JK_ENODEV;
+    hipStream_t st = (hipStream_t)stream;
+    void *scratch = nullptr;
+    if (hipMallocAsync(&scratch, (size_t)jk_lnl_reject_scratch_bytes(n), st) != hipSuccess)
+        return JK_ENOMEM;
+    hipError_t err = jk_launch_lnl_reject_f64(star, n, P, e, omega, M0, s, rows, ll_out,
+                                              make_u128(state_hi, state_lo),
+                                              make_u128(inc_hi, inc_lo), stream_offset, max_keep,
+                                              idx_out, n_pass_out, llmax_out, scratch, st);
+    hipError_t err2 = hipFreeAsync(scratch, st);
+    if (err != hipSuccess) return JK_EHIP;
+    return check_hip(err2);
 }
 
 int jk_pcg64_uniform(uint64_t state_hi, uint64_t state_lo, uint64_t inc_hi, uint64_t inc_lo,

@@ -44,6 +44,9 @@ struct StarDev {
     double lam_inv[JK_LMAX];   // 1 / Lambda_i (entry 0 only for Normal K)
     double ln_lam_trend;       // sum_{i>=1} ln Lambda_i
     double sum_ln_ivar;        // sum_n ln ivar_n
+    // N ln 2pi - sum_n ln ivar_n + sum_{i>=1} ln Lambda_i, plus ln Lambda_0
+    // for the Normal K prior: the sample-independent part of the log-constant.
+    double ln_c;
     // Trend-only moments with w = ivar (the reference jitter mode, where they
     // do not depend on the sample): S_tt[j][k] = sum ivar T_j T_k,
     // S_tr[j] = sum ivar T_j r'_n, S_rr = sum ivar r'^2, r' = y - T mu_trend.
@@ -147,37 +150,45 @@ struct Moments {
 
 // Solve the L x L system and return lnL.
 //   G = [[1/lam0 + Smm, Smt], [Smt, diag(lam_inv) + Stt]],
-//   u = [Smr - mu0 Smm, Str - mu0 Smt],  chi0 = Srr - 2 mu0 Smr + mu0^2 Smm.
-// lnc = N ln 2pi - sum ln w + sum ln Lambda.
+//   u = [Smr - mu0 Smm, Str - mu0 Smt],  chi0 = Srr - 2 mu0 Smr + mu0^2 Smm,
+//   lnL = -1/2 [chi0 - u^T G^-1 u + lnc + ln(scale det G)]
+// where lnc + ln(scale) = N ln 2pi - sum ln w + sum ln Lambda: the sample-
+// dependent factors (the jitter product, a per-sample Lambda_0) arrive as
+// `scale` and share the one logarithm of the determinant.
 // Returns +inf if the Cholesky factorisation breaks down, as the reference
 // returns INF when its LU of Ainv fails (fast_likelihood.pyx:283-291,381-382).
 template <int L>
 __device__ __forceinline__ double jk_finish_lnl(const double *G /* LxL, lower used */,
-                                                const double *u, double chi0, double lnc)
+                                                const double *u, double chi0, double lnc,
+                                                double scale)
 {
     if constexpr (L == 1) {
         const double g = G[0];
         if (!(g > 0.0)) return INFINITY;
-        return -0.5 * (chi0 - u[0] * u[0] / g + lnc + log(g));
+        return -0.5 * (chi0 - u[0] * u[0] / g + lnc + log(scale * g));
     } else if constexpr (L == 2) {
         const double g00 = G[0], g10 = G[2], g11 = G[3];
         const double det = fma(g00, g11, -g10 * g10);
         if (!(g00 > 0.0) || !(det > 0.0)) return INFINITY;
         const double q = (fma(g11, u[0] * u[0], g00 * u[1] * u[1]) - 2.0 * g10 * u[0] * u[1]) / det;
-        return -0.5 * (chi0 - q + lnc + log(det));
+        return -0.5 * (chi0 - q + lnc + log(scale * det));
     } else {
         double R[L * L];
         double z[L];
-        double lndet = 0.0, zz = 0.0;
+        double zz = 0.0;
+        double detm = scale;   // product of the pivots, renormalised
+        int dexp = 0;
 #pragma unroll
         for (int j = 0; j < L; ++j) {
             double d = G[j * L + j];
 #pragma unroll
             for (int k = 0; k < j; ++k) d -= R[j * L + k] * R[j * L + k];
             if (!(d > 0.0)) return INFINITY;
+            int ex;
+            detm = frexp(detm * d, &ex);
+            dexp += ex;
             d = sqrt(d);
             R[j * L + j] = d;
-            lndet += log(d);
             const double inv = 1.0 / d;
 #pragma unroll
             for (int i = j + 1; i < L; ++i) {
@@ -195,7 +206,7 @@ __device__ __forceinline__ double jk_finish_lnl(const double *G /* LxL, lower us
             z[i] = v / R[i * L + i];
             zz += z[i] * z[i];
         }
-        return -0.5 * (chi0 - zz + lnc + 2.0 * lndet);
+        return -0.5 * (chi0 - zz + lnc + (log(detm) + (double)dexp * 0.69314718055994530942));
     }
 }
 

@@ -67,6 +67,13 @@ hipError_t jk_launch_linear_posterior(const jk_star *st, int64_t n, const double
                                       double *cov_out, double *ll_out, hipStream_t stream);
 hipError_t jk_upload_pcg_table(int device);
 hipError_t jk_upload_sincos_table(int device);
+int64_t jk_lnl_reject_scratch_bytes(int64_t n);
+hipError_t jk_launch_lnl_reject_f64(const jk_star *st, int64_t n, const double *P, const double *e,
+                                    const double *om, const double *M0, const double *s,
+                                    const int64_t *rows, double *ll, u128 state, u128 inc,
+                                    uint64_t offset, int64_t max_keep, int64_t *idx_out,
+                                    int64_t *n_pass_out, double *llmax_out, void *scratch,
+                                    hipStream_t stream);
 int64_t jk_polish_scratch_bytes(int64_t n);
 hipError_t jk_launch_lnl_batch(const jk_batch *b, int precision, int64_t n_items,
                                const jk_work_item *items, const double *P, const void *e,

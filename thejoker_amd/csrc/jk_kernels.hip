@@ -23,6 +23,10 @@
 #define JK_LN2 0.69314718055994530942
 
 __constant__ PcgJump c_jump;
+// (A^t, D_t) for t = 0 .. 255: the jump of thread t of a 256-thread workgroup
+// from its workgroup's first state.
+__constant__ u128 c_tid_A[256];
+__constant__ u128 c_tid_D[256];
 
 // (sin g_i, cos g_i) on the grid g_i = i * JK_TAB_H, i = 0 .. JK_TAB_N, over
 // [0, pi].  JK_TAB_H is pi / 256 rounded to 44 significant bits, so i * H is
@@ -37,6 +41,26 @@ __device__ __forceinline__ void jk_stage_tab(double2 *s_tab)
 {
     for (int i = threadIdx.x; i <= JK_TAB_N; i += blockDim.x) s_tab[i] = g_sincos_tab[i];
     __syncthreads();
+}
+
+// sin and cos in FP64 of any finite FP64 angle x: Cody-Waite reduction to
+// y in [-pi, pi], then the grid as in jk_sincos_tab on |y| (exact d = |y| - g_i
+// by Sterbenz's lemma), then the sign of y.
+__device__ __forceinline__ void jk_sincos_tab_f64(double x, const double2 *__restrict__ tab,
+                                                  double &s, double &c)
+{
+    const double y = jk_reduce_2pi(x);
+    const double ay = fabs(y);
+    int idx = (int)(ay * (JK_TAB_N / JK_PI));
+    idx = min(max(idx, 0), JK_TAB_N);
+    const double2 t = tab[idx];
+    const double d = ay - (double)idx * JK_TAB_H;
+    const double d2 = d * d;
+    const double sd = fma(d * d2, fma(d2, 1.0 / 120.0, -1.0 / 6.0), d);
+    const double cd = fma(d2, fma(d2, fma(d2, -1.0 / 720.0, 1.0 / 24.0), -0.5), 1.0);
+    const double sa = fma(t.x, cd, t.y * sd);
+    s = y < 0.0 ? -sa : sa;
+    c = fma(t.y, cd, -(t.x * sd));
 }
 
 // sin and cos in FP64 of an FP32 angle Ef in [0, pi] (a little outside is
@@ -325,7 +349,7 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
     else
         lam0 = st.lam0;
 
-    const double nP = JK_TWO_PI / Pv;
+    const double nP = JK_TWO_PI * jk_rcp2(Pv);
     const double s2 = sv * sv;
 
     // Per-sample orbit constants.
@@ -334,14 +358,15 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
     // FP32 copies for the Kepler starter (e capped below 1 in FP32).
     const float ef = fminf((float)ev, 0.99999994f);
     const float omef = 1.0f - ef;
-    const float a1f = JK_MARKLEY_A1N / (1.0f + ef);
+    const float a1f = JK_MARKLEY_A1N * __builtin_amdgcn_rcpf(1.0f + ef);
+    double sw;
+    jk_sincos_tab_f64(omv, tab, sw, cw);
     if constexpr (MIXED) {
-        sincosf((float)omv, &swf, &cwf);
+        swf = (float)sw;
+        cwf = (float)cw;
         ecwf = ef * cwf;
         sqf = sqrtf(omef * (1.0f + ef));
     } else {
-        double sw;
-        sincos(omv, &sw, &cw);
         ecw = ev * cw;
         swsq = sw * sqrt((1.0 - ev) * (1.0 + ev));
     }
@@ -417,17 +442,20 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
     // Assemble G (lower triangle used), u, chi0, log-constant.
     double G[L * L], u[L];
     const double mu0 = st.mu0;
-    G[0] = 1.0 / lam0 + Smm;
+    const bool fcm = st.k_prior_mode == JK_KPRIOR_FIXED_COMPANION_MASS;
+    G[0] = (fcm ? jk_rcp2(lam0) : st.lam_inv[0]) + Smm;
     u[0] = fma(-mu0, Smm, Smr);
-    double chi0;
-    double sum_ln_w;
+    // log-constant: st.ln_c holds everything but the sample's jitter product
+    // prod(1 + s^2 ivar_n) = prod * 2^pexp and, for FixedCompanionMass, ln lam0
+    double chi0, lnc = st.ln_c, scale = 1.0;
     if constexpr (PHYS) {
         chi0 = Srr;
-        sum_ln_w = st.sum_ln_ivar - (log(prod) + (double)pexp * JK_LN2);
+        lnc += (double)pexp * JK_LN2;
+        scale = prod;
     } else {
         chi0 = st.S_rr;
-        sum_ln_w = st.sum_ln_ivar;
     }
+    if (fcm) scale *= lam0;
     chi0 = fma(mu0 * mu0, Smm, fma(-2.0 * mu0, Smr, chi0));
     if constexpr (L > 1) {
 #pragma unroll
@@ -445,11 +473,41 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
             }
         }
     }
-    const double lnc = (double)N * JK_LN_TWO_PI - sum_ln_w + log(lam0) + st.ln_lam_trend;
-    return jk_finish_lnl<L>(G, u, chi0, lnc);
+    return jk_finish_lnl<L>(G, u, chi0, lnc, scale);
 }
 
-// Dense launch: ll[i] = lnL(row rows[i] or i), one lane per row.
+// ---------------------------------------------------------------------------
+// Block-level helpers (256 threads = 4 wavefronts of 64).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_nanmax(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = jk_nanmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ int64_t wave_sum_i64(int64_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Inclusive scan across the wavefront.
+__device__ __forceinline__ int64_t wave_incl_scan_i64(int64_t v, int lane)
+{
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int64_t t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+// Dense launch: ll[i] = lnL(row rows[i] or i), one lane per row.  With
+// blockmax != nullptr each workgroup also writes the NaN-propagating maximum
+// of its rows to blockmax[blockIdx.x], so the rejection step need not read
+// the lnL array a second time to find the maximum.
 template <int L, bool PHYS, bool MIXED>
 __global__ __launch_bounds__(256) void k_lnl(const StarDev st, int64_t n,
                                              const double *__restrict__ P,
@@ -458,14 +516,26 @@ __global__ __launch_bounds__(256) void k_lnl(const StarDev st, int64_t n,
                                              const void *__restrict__ M0_,
                                              const void *__restrict__ s_,
                                              const int64_t *__restrict__ rows,
-                                             double *__restrict__ ll)
+                                             double *__restrict__ ll,
+                                             double *__restrict__ blockmax)
 {
     __shared__ double2 s_tab[JK_TAB_N + 1];
-    if constexpr (!MIXED) jk_stage_tab(s_tab);
+    __shared__ double red[4];
+    jk_stage_tab(s_tab);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int64_t r = rows ? rows[i] : i;
-    ll[i] = jk_lnl_row<L, PHYS, MIXED>(st, r, P, e_, om_, M0_, s_, s_tab);
+    double v = -INFINITY;
+    if (i < n) {
+        const int64_t r = rows ? rows[i] : i;
+        v = jk_lnl_row<L, PHYS, MIXED>(st, r, P, e_, om_, M0_, s_, s_tab);
+        ll[i] = v;
+    }
+    if (blockmax) {
+        v = wave_nanmax(v);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            blockmax[blockIdx.x] = jk_nanmax(jk_nanmax(red[0], red[1]), jk_nanmax(red[2], red[3]));
+    }
 }
 
 // Scatter launch over a device-resident index list: for j < *count,
@@ -497,24 +567,26 @@ __global__ __launch_bounds__(256) void k_lnl_scatter_f64(const StarDev st,
 template <int L, bool PHYS, bool MIXED>
 static hipError_t launch_lnl_t(const jk_star *st, int64_t n, const double *P, const void *e,
                                const void *om, const void *M0, const void *s,
-                               const int64_t *rows, double *ll, hipStream_t stream)
+                               const int64_t *rows, double *ll, double *blockmax,
+                               hipStream_t stream)
 {
     const int64_t blocks = (n + 255) / 256;
     hipLaunchKernelGGL((k_lnl<L, PHYS, MIXED>), dim3((unsigned)blocks), dim3(256), 0, stream,
-                       st->dev, n, P, e, om, M0, s, rows, ll);
+                       st->dev, n, P, e, om, M0, s, rows, ll, blockmax);
     return hipGetLastError();
 }
 
 template <bool MIXED>
 static hipError_t launch_lnl_dispatch(const jk_star *st, int64_t n, const double *P, const void *e,
                                       const void *om, const void *M0, const void *s,
-                                      const int64_t *rows, double *ll, hipStream_t stream)
+                                      const int64_t *rows, double *ll, double *bm,
+                                      hipStream_t stream)
 {
     const bool phys = st->dev.jitter_mode == JK_JITTER_PHYSICAL;
-#define JK_CASE(LL)                                                                          \
-    case LL:                                                                                 \
-        return phys ? launch_lnl_t<LL, true, MIXED>(st, n, P, e, om, M0, s, rows, ll, stream) \
-                    : launch_lnl_t<LL, false, MIXED>(st, n, P, e, om, M0, s, rows, ll, stream);
+#define JK_CASE(LL)                                                                               \
+    case LL:                                                                                      \
+        return phys ? launch_lnl_t<LL, true, MIXED>(st, n, P, e, om, M0, s, rows, ll, bm, stream)  \
+                    : launch_lnl_t<LL, false, MIXED>(st, n, P, e, om, M0, s, rows, ll, bm, stream);
     switch (st->dev.L) {
         JK_CASE(1)
         JK_CASE(2)
@@ -535,7 +607,7 @@ hipError_t jk_launch_lnl_f64(const jk_star *st, int64_t n, const double *P, cons
                              const int64_t *rows, double *ll, hipStream_t stream)
 {
     if (n <= 0) return hipSuccess;
-    return launch_lnl_dispatch<false>(st, n, P, e, om, M0, s, rows, ll, stream);
+    return launch_lnl_dispatch<false>(st, n, P, e, om, M0, s, rows, ll, nullptr, stream);
 }
 
 hipError_t jk_launch_lnl_mixed(const jk_star *st, int64_t n, const double *P, const float *e,
@@ -543,36 +615,9 @@ hipError_t jk_launch_lnl_mixed(const jk_star *st, int64_t n, const double *P, co
                                const int64_t *rows, double *ll, hipStream_t stream)
 {
     if (n <= 0) return hipSuccess;
-    return launch_lnl_dispatch<true>(st, n, P, e, om, M0, s, rows, ll, stream);
+    return launch_lnl_dispatch<true>(st, n, P, e, om, M0, s, rows, ll, nullptr, stream);
 }
 
-// ---------------------------------------------------------------------------
-// Block-level helpers (256 threads = 4 wavefronts of 64).
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ double wave_nanmax(double v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = jk_nanmax(v, __shfl_xor(v, off, 64));
-    return v;
-}
-
-__device__ __forceinline__ int64_t wave_sum_i64(int64_t v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// Inclusive scan across the wavefront.
-__device__ __forceinline__ int64_t wave_incl_scan_i64(int64_t v, int lane)
-{
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int64_t t = __shfl_up(v, off, 64);
-        if (lane >= off) v += t;
-    }
-    return v;
-}
 
 __global__ __launch_bounds__(256) void k_max_partial(const double *__restrict__ ll, int64_t n,
                                                      double *__restrict__ partials)
@@ -633,6 +678,19 @@ __device__ __forceinline__ u128 pcg_jump_table(u128 s, uint64_t k, u128 inc)
     return s;
 }
 
+// State of thread g = blockIdx.x * 256 + threadIdx.x (256-thread workgroups):
+// the workgroup's jump by blockIdx.x * 256 is the same for every lane, so its
+// loop does not diverge; the lane's own jump by threadIdx.x is one step from
+// the per-lane table.  Equal to pcg_jump_table(s, g, inc).
+__device__ __forceinline__ u128 pcg_thread_start(u128 s, u128 inc)
+{
+    uint64_t k = blockIdx.x;
+#pragma unroll 1
+    for (int b = 8; k != 0; ++b, k >>= 1)
+        if (k & 1) s = pcg_affine(s, c_jump.A[b], c_jump.D[b], inc);
+    return pcg_affine(s, c_tid_A[threadIdx.x], c_tid_D[threadIdx.x], inc);
+}
+
 static int jk_rng_grid(int64_t n)
 {
     int64_t b = (n + 256 * 16 - 1) / (256 * 16);
@@ -646,7 +704,7 @@ __global__ __launch_bounds__(256) void k_uniform(int64_t n, u128 s_base, u128 in
 {
     const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t T = (int64_t)gridDim.x * 256;
-    u128 s = pcg_jump_table(s_base, (uint64_t)g, inc);
+    u128 s = pcg_thread_start(s_base, inc);
     for (int64_t i = g; i < n; i += T) {
         u[i] = pcg_to_double(pcg_output(s));
         s = pcg_affine(s, AT, DT, inc);
@@ -662,7 +720,7 @@ __global__ __launch_bounds__(256) void k_reject_flags(const double *__restrict__
     const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t T = (int64_t)gridDim.x * 256;
     const double mx = *llmax;
-    u128 s = pcg_jump_table(s_base, (uint64_t)g, inc);
+    u128 s = pcg_thread_start(s_base, inc);
     for (int64_t base = g - lane; base < n; base += T) {
         const int64_t i = base + lane;
         bool keep = false;
@@ -841,7 +899,14 @@ hipError_t jk_upload_pcg_table(int device)
 {
     PcgJump tab;
     for (int b = 0; b < 64; ++b) jk_pcg_jump_coeffs(1ull << b, &tab.A[b], &tab.D[b]);
-    return hipMemcpyToSymbol(HIP_SYMBOL(c_jump), &tab, sizeof(tab), 0, hipMemcpyHostToDevice);
+    hipError_t err =
+        hipMemcpyToSymbol(HIP_SYMBOL(c_jump), &tab, sizeof(tab), 0, hipMemcpyHostToDevice);
+    if (err != hipSuccess) return err;
+    u128 tA[256], tD[256];
+    for (int t = 0; t < 256; ++t) jk_pcg_jump_coeffs((uint64_t)t, &tA[t], &tD[t]);
+    err = hipMemcpyToSymbol(HIP_SYMBOL(c_tid_A), tA, sizeof(tA), 0, hipMemcpyHostToDevice);
+    if (err != hipSuccess) return err;
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_tid_D), tD, sizeof(tD), 0, hipMemcpyHostToDevice);
 }
 
 // The sin / cos grid of jk_sincos_tab, from 80-bit long double on the host
@@ -886,6 +951,35 @@ hipError_t jk_launch_reject(const double *ll, int64_t n, const double *llmax, u1
     hipLaunchKernelGGL(k_reject_compact, dim3((unsigned)nb), dim3(256), 0, stream, mask, nwords, bsum,
                        (int)nb, max_keep, idx_out, n_pass_out);
     return hipGetLastError();
+}
+
+// FP64 lnL plus the rejection step of the same rows, with the maximum reduced
+// inside the likelihood launch (one partial per workgroup).
+int64_t jk_lnl_reject_scratch_bytes(int64_t n)
+{
+    const int64_t blocks = (n + 255) / 256;
+    return (int64_t)sizeof(double) * (blocks + 1) +
+           (int64_t)sizeof(uint64_t) * jk_reject_mask_words(n) +
+           (int64_t)sizeof(int64_t) * jk_reject_bsum_count(n);
+}
+
+hipError_t jk_launch_lnl_reject_f64(const jk_star *st, int64_t n, const double *P, const double *e,
+                                    const double *om, const double *M0, const double *s,
+                                    const int64_t *rows, double *ll, u128 state, u128 inc,
+                                    uint64_t offset, int64_t max_keep, int64_t *idx_out,
+                                    int64_t *n_pass_out, double *llmax_out, void *scratch,
+                                    hipStream_t stream)
+{
+    const int64_t blocks = (n + 255) / 256;
+    double *partials = (double *)scratch;
+    uint64_t *mask = (uint64_t *)(partials + blocks + 1);
+    int64_t *bsum = (int64_t *)(mask + jk_reject_mask_words(n));
+    hipError_t err = launch_lnl_dispatch<false>(st, n, P, e, om, M0, s, rows, ll, partials, stream);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(256), 0, stream, partials, (int)blocks,
+                       llmax_out);
+    return jk_launch_reject(ll, n, llmax_out, state, inc, offset, max_keep, idx_out, n_pass_out,
+                            mask, bsum, stream);
 }
 
 template <int L, bool PHYS>
@@ -977,7 +1071,7 @@ __global__ __launch_bounds__(256) void k_lnl_batch(const StarDev *__restrict__ s
 {
     __shared__ unsigned s_item;
     __shared__ double2 s_tab[JK_TAB_N + 1];
-    if constexpr (!MIXED) jk_stage_tab(s_tab);
+    jk_stage_tab(s_tab);
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
         __syncthreads();
@@ -1097,7 +1191,7 @@ __global__ __launch_bounds__(256) void k_seg_flags(const double *__restrict__ ll
     const int64_t T = (int64_t)gridDim.x * 256;
     if (g - lane >= sg.n) return;
     const double mx = llmax[blockIdx.y];
-    u128 s = pcg_jump_table(sg.s_base, (uint64_t)g, sg.inc);
+    u128 s = pcg_thread_start(sg.s_base, sg.inc);
     for (int64_t base = g - lane; base < sg.n; base += T) {
         const int64_t i = base + lane;
         bool keep = false;

@@ -21,7 +21,7 @@ from . import _lib
 from . import units as u
 from .samples import NONLINEAR_INTERNAL_UNITS, NONLINEAR_PACKED_ORDER, JokerSamples
 
-__all__ = ["DeviceStar", "PriorCache", "star_arrays", "lnl", "polish", "reject",
+__all__ = ["DeviceStar", "PriorCache", "star_arrays", "lnl", "lnl_reject", "polish", "reject",
            "linear_posterior", "uniforms",
            "pcg64_state", "pcg64_advance_state", "JITTER_MODES"]
 
@@ -340,6 +340,37 @@ def lnl(star, cache, rows=None, n=None, precision="f64", out=None, start=0):
         raise ValueError("precision must be 'f64' or 'mixed'")
     _lib.check(fn(star.handle, n, P, *cols, _dptr(rows), _dptr(out), _stream(cache.device)), what)
     return out
+
+
+def lnl_reject(star, cache, rng_state, max_keep, rows=None, n=None, start=0, out=None, offset=0):
+    """Plain FP64 lnL of cache rows (as :func:`lnl`) and the rejection step over
+    them (as :func:`reject`) in one call (jk_lnl_reject_f64).  Returns
+    (device lnL tensor, idx, n_pass, llmax)."""
+    torch = _torch()
+    lib = _lib.ensure_device(cache.device)
+    if rows is not None:
+        n = int(rows.shape[0])
+    elif n is None:
+        n = len(cache) - start
+    if n <= 0:
+        raise ValueError("zero-size array to reduction operation maximum which has no identity")
+    if out is None:
+        out = torch.empty(n, dtype=torch.float64, device=cache.P.device)
+    off = 0 if rows is not None else 8 * start
+    cols = [ctypes.c_void_p(c.data_ptr() + off) for c in
+            (cache.P, cache.e, cache.omega, cache.M0, cache.s)]
+    max_keep = int(min(max_keep, n))
+    res = torch.empty(2 + max(max_keep, 1), dtype=torch.int64, device=cache.P.device)
+    p = res.data_ptr()
+    state, inc = rng_state
+    _lib.check(lib.jk_lnl_reject_f64(star.handle, n, *cols, _dptr(rows), _dptr(out), *_split(state),
+                                     *_split(inc), int(offset), max_keep, ctypes.c_void_p(p + 16),
+                                     ctypes.c_void_p(p), ctypes.c_void_p(p + 8),
+                                     _stream(cache.device)), "jk_lnl_reject_f64")
+    h = to_host(res)
+    n_pass = int(h[0])
+    k = min(n_pass, max_keep)
+    return out, h[2:2 + k].copy(), n_pass, float(h[1:2].view(np.float64)[0])
 
 
 def polish(star, cache, ll, rows=None, start=0, window=40.0, first=0):

@@ -74,6 +74,7 @@ class Request:
         self.it = None
         self.region = None      # (start, capacity) in the arena
         self.slot = None        # star-table slot
+        self.post = None        # (packed rows, a, cov) of the accepted rows
         self.result = None
         self.error = None
         self.rows_evaluated = 0
@@ -142,6 +143,7 @@ class JokerServer:
         self.profile = profile
         self.phase_s = dict.fromkeys(self.PHASES, 0.0)
         self._t_mark = None
+        self._to_finish = []      # retired requests whose samples are still to be drawn
         self._helper = TheJoker(prior, rng=np.random.default_rng(0), jitter_mode=jitter_mode,
                                 precision=precision, device=device)
 
@@ -156,7 +158,7 @@ class JokerServer:
         return r
 
     def busy(self):
-        return bool(self.pending or self.active)
+        return bool(self.pending or self.active or self._to_finish)
 
     def _mark(self, phase=None):
         """Profiling: charge the time since the last mark to `phase`."""
@@ -230,14 +232,32 @@ class JokerServer:
         r.region = (new, new_cap)
         return True
 
+    def _build_results(self):
+        """Host half of the finished requests' results (linear-parameter draws
+        and JokerSamples); returns those requests."""
+        done, self._to_finish = self._to_finish, []
+        for r in done:
+            r.result = self._helper._samples_from_posterior(
+                r.star, r.post, r.rng, self.n_linear_samples,
+                spawn_batches=None if self.in_memory else 1)
+            r.post = None
+            r.t_done = self.now()
+            self.finished.append(r)
+        return done
+
     def step(self):
         """Admit, evaluate one round of every active request, retire the
-        finished.  Returns the requests finished in this step."""
+        finished.  The host work of the requests retired in the previous step
+        (their linear-parameter draws) runs while this step's likelihood
+        launch is on the device.  Returns the requests whose results were
+        completed in this call."""
         self._mark()
         self._admit()
         self._mark("admit")
         if not self.active:
-            return []
+            done = self._build_results()
+            self._mark("finish")
+            return done
         plan = []
         for r in self.active:
             rows, start, n_now = r.it.next_batch()
@@ -262,8 +282,13 @@ class JokerServer:
         items = items[heaviest_first(items, self._table.M)]
         d_items = upload_items(items, self.device)
         self._mark("plan")
+        if self.profile:
+            finished = self._build_results()
+            self._mark("finish")
         lnl_batch(self._table, self.cache, d_items, len(items), 0, self.precision, self.shape,
                   out=self.arena, workspace=self._ws)
+        if not self.profile:
+            finished = self._build_results()      # overlaps the launch above
         M = self._table.M[slots]
         self.sample_epochs += int(np.sum(sizes * M))
         seg_n = before + sizes
@@ -277,7 +302,7 @@ class JokerServer:
                                            [pcg64_state(r.rng) for r, _, _ in plan],
                                            self.n_requested, device=self.device)
         self._mark("reject")
-        done = []
+        retired = []
         for k, (r, _, n_now) in enumerate(plan):
             r.rows_evaluated += n_now
             r.sample_epochs += int(n_now * M[k])
@@ -290,21 +315,19 @@ class JokerServer:
                 r.error = exc
                 r.it.done = True
             if r.it.done:
-                done.append(r)
-        self._mark("update")
-        for r in done:
+                retired.append(r)
+        for r in retired:
             self.active.remove(r)
             if r.error is None:
                 full_idx, _ = r.it.result()
-                r.result = self._helper._full_samples(r.star, self.cache, full_idx, r.rng,
-                                                      self.n_linear_samples,
-                                                      spawn_batches=None if self.in_memory else 1)
-            self._release(r)
-            r.t_done = self.now()
-            self.finished.append(r)
+                r.post = self._helper._posterior_inputs(r.star, self.cache, full_idx)
+                self._to_finish.append(r)
+                self._release(r)
+            else:
+                self._fail(r, r.error)
         self.steps += 1
-        self._mark("finish")
-        return done
+        self._mark("update")
+        return finished
 
 
 def run_poisson(server, arrival_times, datas, rngs, max_wall=None):

@@ -30,8 +30,8 @@ import numpy as np
 
 from . import units as u
 from .data import RVData
-from .device import (DeviceStar, PriorCache, linear_internal_units, lnl, pcg64_advance_state,
-                     pcg64_state, reject)
+from .device import (DeviceStar, PriorCache, linear_internal_units, lnl, lnl_reject,
+                     pcg64_advance_state, pcg64_state, reject, to_host)
 from .likelihood_helpers import batch_tasks, next_batch_size, validate_prepare_data
 from .prior import JokerPrior
 from .samples import NONLINEAR_PACKED_ORDER, JokerSamples
@@ -146,16 +146,29 @@ class TheJoker:
     def _full_samples(self, star, cache, rows, rng, n_linear_samples, spawn_batches=None):
         """Linear parameters of accepted rows (fast_likelihood.pyx:471-545) and
         JokerSamples.unpack (samples.py:447-478)."""
+        post = self._posterior_inputs(star, cache, rows)
+        return self._samples_from_posterior(star, post, rng, n_linear_samples, spawn_batches)
+
+    @staticmethod
+    def _posterior_inputs(star, cache, rows):
+        """The device part of _full_samples: (packed rows, a, cov) of the
+        accepted cache rows."""
         rows = np.asarray(rows, dtype=np.int64)
         a, cov, _ = _linear_posterior(star, cache, rows)
         packed = cache.host_rows(rows) if len(rows) else np.zeros((0, 5))
-        L = star.L
-        n_pars = 5 + L
-        raw = np.zeros((len(rows), n_linear_samples, n_pars))
+        return packed, a, cov
+
+    def _samples_from_posterior(self, star, post, rng, n_linear_samples, spawn_batches=None):
+        """The host part of _full_samples: linear-parameter draws in row order
+        from `rng` (or its spawned children) and the JokerSamples."""
+        packed, a, cov = post
+        n = len(packed)
+        n_pars = 5 + star.L
+        raw = np.zeros((n, n_linear_samples, n_pars))
         if spawn_batches is None:
-            gens = [(np.arange(len(rows)), rng)]
+            gens = [(np.arange(n), rng)]
         else:
-            tasks = batch_tasks(len(rows), n_batches=spawn_batches, arr=np.arange(len(rows)))
+            tasks = batch_tasks(n, n_batches=spawn_batches, arr=np.arange(n))
             from numpy.random import PCG64, Generator
             sg = rng.bit_generator._seed_seq.spawn(len(tasks))
             gens = [(tasks[i][0], Generator(PCG64(sg[i]))) for i in range(len(tasks))]
@@ -164,7 +177,7 @@ class TheJoker:
             if len(sel):
                 raw[sel, :, :5] = packed[sel][:, None, :]
                 raw[sel, :, 5:] = mvn_draws(g, a[sel], cov[sel], n_linear_samples)
-        raw = raw.reshape(len(rows) * n_linear_samples, n_pars)
+        raw = raw.reshape(n * n_linear_samples, n_pars)
         return JokerSamples.unpack(raw, star.internal_units, t_ref=star.data.t_ref,
                                    poly_trend=self.prior.poly_trend,
                                    n_offsets=self.prior.n_offsets)
@@ -216,28 +229,34 @@ class TheJoker:
         if max_posterior_samples is None:
             max_posterior_samples = n_use
 
+        d_rows = None
         if rows is not None:
             d_rows = torch.as_tensor(np.asarray(rows, dtype=np.int64), device=cache.P.device)
-            lls = self._lnl_rows(star, cache, rows=d_rows)
+        if self.precision == "f64":
+            # likelihood and rejection step in one call
+            lls, good, _, _ = lnl_reject(star, cache, pcg64_state(rng), max_posterior_samples,
+                                         rows=d_rows, start=0, n=n_use)
         else:
-            lls = self._lnl_rows(star, cache, start=0, n=n_use)
-
-        good, _, _ = reject(lls, pcg64_state(rng), max_posterior_samples, device=self.device)
+            lls = self._lnl_rows(star, cache, rows=d_rows, start=0, n=n_use)
+            good, _, _ = reject(lls, pcg64_state(rng), max_posterior_samples, device=self.device)
         pcg64_advance_state(rng, n_use)
         full_idx = rows[good] if rows is not None else good
 
         samples = self._full_samples(star, cache, full_idx, rng, n_linear_samples,
                                      spawn_batches=None if in_memory else self._n_batches(n_batches))
-        lls_h = lls.cpu().numpy()
+        ll_good = None
+        if (in_memory and ln_prior is not None and ln_prior is not False) or \
+                (not in_memory and return_logprobs):
+            ll_good = to_host(lls[torch.as_tensor(good, device=lls.device)])
         if in_memory:
             if ln_prior is not None and ln_prior is not False:
                 samples["ln_prior"] = ln_prior[good]
-                samples["ln_likelihood"] = lls_h[good]
+                samples["ln_likelihood"] = ll_good
         elif return_logprobs:
-            samples["ln_likelihood"] = lls_h[good]
+            samples["ln_likelihood"] = ll_good
             samples["ln_prior"] = cache.ln_prior[full_idx]
         if return_all_logprobs:
-            return samples, lls_h
+            return samples, to_host(lls)
         return samples
 
     def iterative_rejection_sample(self, data, prior_samples, n_requested_samples,
