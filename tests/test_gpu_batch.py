@@ -190,3 +190,18 @@ def test_slot_table_equals_single_star_launches(setup):
             assert np.array_equal(ll[off[k]:off[k] + n].cpu().numpy(), single), slot
         assign[7] = 9
         table.set(7, stars[9][1])
+
+
+def test_posterior_inputs_many_equals_single(setup):
+    """The one-block, one-copy posterior inputs of many stars equal the
+    single-star ones, including a star with no accepted rows."""
+    from thejoker_amd.thejoker import TheJoker
+    stars, cache, _ = setup
+    g = np.random.default_rng(4)
+    sel = [stars[i][1] for i in (0, 3, 7, 8)]
+    rows = [g.choice(len(cache), size=k, replace=False) for k in (5, 0, 256, 1)]
+    many = D.posterior_inputs_many(sel, cache, rows)
+    for st, r, got in zip(sel, rows, many):
+        want = TheJoker._posterior_inputs(st, cache, r)
+        for a, b in zip(got, want):
+            assert a.shape == b.shape and np.array_equal(a, b)

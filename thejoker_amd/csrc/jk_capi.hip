@@ -514,7 +514,7 @@ int jk_reject_batch(const double *ll, int n_seg, const jk_segment *segs, int64_t
     }
     if (any_keep && !idx_out) return JK_EINVAL;
     int pb = (int)((max_n + 4095) / 4096);
-    if (pb > 64) pb = 64;
+    if (pb > 1024) pb = 1024;
     if (pb < 1) pb = 1;
     hipStream_t st = (hipStream_t)stream;
     const size_t seg_bytes = sizeof(SegDev) * n_seg;
@@ -574,13 +574,14 @@ int jk_polish_batch(const jk_batch *batch, int n_seg, const jk_polish_segment *s
         if (d.nb > max_nb) max_nb = d.nb;
     }
     int pb = (int)((max_n + 4095) / 4096);
-    if (pb > 64) pb = 64;
+    if (pb > 1024) pb = 1024;
     if (pb < 1) pb = 1;
     hipStream_t st = (hipStream_t)stream;
     const size_t seg_bytes = sizeof(SegDev) * n_seg;
     const size_t seg_pad = ((seg_bytes + 255) / 256) * 256;
     const size_t bytes = seg_pad + sizeof(double) * ((size_t)pb * n_seg + n_seg) +
-                         sizeof(uint64_t) * mask_words + sizeof(int64_t) * (bsum_n + idx_n);
+                         sizeof(uint64_t) * mask_words +
+                         sizeof(int64_t) * (bsum_n + idx_n + n_seg + 1);
     char *scratch = nullptr;
     if (hipMallocAsync((void **)&scratch, bytes, st) != hipSuccess) return JK_ENOMEM;
     SegDev *d_segs = (SegDev *)scratch;
@@ -589,10 +590,11 @@ int jk_polish_batch(const jk_batch *batch, int n_seg, const jk_polish_segment *s
     uint64_t *mask = (uint64_t *)(llmax + n_seg);
     int64_t *bsum = (int64_t *)(mask + mask_words);
     int64_t *idx = bsum + bsum_n;
+    int64_t *cum = idx + idx_n;
     hipError_t err = hipMemcpyAsync(d_segs, h.data(), seg_bytes, hipMemcpyHostToDevice, st);
     if (err == hipSuccess)
         err = jk_launch_polish_batch(batch, n_seg, d_segs, max_n, max_nb, partials, pb, mask, bsum,
-                                     idx, llmax, P, e, omega, M0, s, rows, ll_inout, window,
+                                     idx, llmax, cum, P, e, omega, M0, s, rows, ll_inout, window,
                                      n_polished_out, st);
     // The host descriptor array must outlive the queued copy.
     hipError_t err3 = hipStreamSynchronize(st);

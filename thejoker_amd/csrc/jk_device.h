@@ -173,7 +173,12 @@ __device__ __forceinline__ double jk_finish_lnl(const double *G /* LxL, lower us
         const double q = (fma(g11, u[0] * u[0], g00 * u[1] * u[1]) - 2.0 * g10 * u[0] * u[1]) / det;
         return -0.5 * (chi0 - q + lnc + log(scale * det));
     } else {
+        // Cholesky G = R R^T without divisions: each pivot's 1/sqrt from the
+        // hardware v_rsq_f64 refined by two Newton steps; R[j][j] itself is
+        // never needed, only its inverse (for the column below it and for
+        // the forward substitution z = R^-1 u, chi2 correction = z^T z).
         double R[L * L];
+        double rinv[L];
         double z[L];
         double zz = 0.0;
         double detm = scale;   // product of the pivots, renormalised
@@ -182,29 +187,30 @@ __device__ __forceinline__ double jk_finish_lnl(const double *G /* LxL, lower us
         for (int j = 0; j < L; ++j) {
             double d = G[j * L + j];
 #pragma unroll
-            for (int k = 0; k < j; ++k) d -= R[j * L + k] * R[j * L + k];
+            for (int k = 0; k < j; ++k) d = fma(-R[j * L + k], R[j * L + k], d);
             if (!(d > 0.0)) return INFINITY;
             int ex;
             detm = frexp(detm * d, &ex);
             dexp += ex;
-            d = sqrt(d);
-            R[j * L + j] = d;
-            const double inv = 1.0 / d;
+            double y = __builtin_amdgcn_rsq(d);
+            y = y * fma(-0.5 * d * y, y, 1.5);
+            y = y * fma(-0.5 * d * y, y, 1.5);
+            rinv[j] = y;
 #pragma unroll
             for (int i = j + 1; i < L; ++i) {
                 double v = G[i * L + j];
 #pragma unroll
-                for (int k = 0; k < j; ++k) v -= R[i * L + k] * R[j * L + k];
-                R[i * L + j] = v * inv;
+                for (int k = 0; k < j; ++k) v = fma(-R[i * L + k], R[j * L + k], v);
+                R[i * L + j] = v * y;
             }
         }
 #pragma unroll
         for (int i = 0; i < L; ++i) {
             double v = u[i];
 #pragma unroll
-            for (int k = 0; k < i; ++k) v -= R[i * L + k] * z[k];
-            z[i] = v / R[i * L + i];
-            zz += z[i] * z[i];
+            for (int k = 0; k < i; ++k) v = fma(-R[i * L + k], z[k], v);
+            z[i] = v * rinv[i];
+            zz = fma(z[i], z[i], zz);
         }
         return -0.5 * (chi0 - zz + lnc + (log(detm) + (double)dexp * 0.69314718055994530942));
     }

@@ -88,7 +88,7 @@ void jk_host_pcg_jump(u128 state, u128 inc, uint64_t k, u128 *out);
 hipError_t jk_launch_polish_batch(const jk_batch *b, int n_seg, const SegDev *d_segs,
                                   int64_t max_n, int64_t max_nb, double *partials, int pb,
                                   uint64_t *mask, int64_t *bsum, int64_t *idx, double *llmax,
-                                  const double *P, const double *e, const double *om,
+                                  int64_t *cum, const double *P, const double *e, const double *om,
                                   const double *M0, const double *s, const int64_t *rows,
                                   double *ll, double window, int64_t *n_polished,
                                   hipStream_t stream);

@@ -1151,7 +1151,7 @@ hipError_t jk_launch_lnl_batch(const jk_batch *b, int precision, int64_t n_items
 // Segmented rejection: the steps of jk_launch_reject with blockIdx.y picking
 // the segment.  Blocks past a segment's extent exit at once.
 // ---------------------------------------------------------------------------
-#define JK_SEG_MAX_BLOCKS 64
+#define JK_SEG_MAX_BLOCKS 1024
 
 __global__ __launch_bounds__(256) void k_seg_max_partial(const double *__restrict__ ll,
                                                          const SegDev *__restrict__ segs,
@@ -1336,10 +1336,41 @@ __global__ __launch_bounds__(256) void k_seg_window_flags(const double *__restri
     }
 }
 
+// Padded exclusive prefix of the per-segment counts: cum[s] = sum over t < s
+// of count[t] rounded up to a multiple of 64, so that a wavefront of the
+// scatter below never straddles two segments.  One workgroup.
+__global__ __launch_bounds__(256) void k_seg_count_scan(const int64_t *__restrict__ count,
+                                                        int n_seg, int64_t *__restrict__ cum)
+{
+    __shared__ int64_t wsum[4];
+    const int per = (n_seg + 255) / 256;
+    const int lo = threadIdx.x * per;
+    const int hi = min(lo + per, n_seg);
+    int64_t mine = 0;
+    for (int i = lo; i < hi; ++i) mine += (count[i] + 63) & ~(int64_t)63;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int64_t incl = wave_incl_scan_i64(mine, lane);
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    int64_t run = incl - mine;
+    for (int k = 0; k < wid; ++k) run += wsum[k];
+    for (int i = lo; i < hi; ++i) {
+        cum[i] = run;
+        run += (count[i] + 63) & ~(int64_t)63;
+    }
+    if (threadIdx.x == 255) cum[n_seg] = run;
+}
+
+// FP64 re-evaluation of every segment's selected entries as one flat list
+// spread over the whole grid: wavefront w takes padded positions
+// [64 w, 64 w + 64), all in one segment (found by binary search in cum and
+// made wave-uniform, so the star's constants stay in scalar loads).
 template <int L, bool PHYS>
 __global__ __launch_bounds__(256) void k_seg_scatter_f64(const StarDev *__restrict__ stars,
                                                          const SegDev *__restrict__ segs,
+                                                         int n_seg,
                                                          const int64_t *__restrict__ count,
+                                                         const int64_t *__restrict__ cum,
                                                          const int64_t *__restrict__ idx,
                                                          const double *__restrict__ P,
                                                          const void *__restrict__ e_,
@@ -1351,32 +1382,44 @@ __global__ __launch_bounds__(256) void k_seg_scatter_f64(const StarDev *__restri
 {
     __shared__ double2 s_tab[JK_TAB_N + 1];
     jk_stage_tab(s_tab);
-    const SegDev sg = segs[blockIdx.y];
-    const int64_t cnt = count[blockIdx.y];
-    const StarDev &st = stars[sg.star];
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += stride) {
-        const int64_t p = idx[sg.idx_off + j];
-        const int64_t q = sg.row0 + p;
-        const int64_t r = rows ? rows[q] : q;
-        ll[sg.off + p] = jk_lnl_row<L, PHYS, false>(st, r, P, e_, om_, M0_, s_, s_tab);
+    const int lane = threadIdx.x & 63;
+    const int64_t total_waves = cum[n_seg] >> 6;
+    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+         w < total_waves; w += wstride) {
+        const int64_t j0 = w << 6;
+        int lo = 0, hi = n_seg;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (cum[mid] <= j0) lo = mid; else hi = mid;
+        }
+        const int sidx = __builtin_amdgcn_readfirstlane(lo);
+        const SegDev &sg = segs[sidx];
+        const int64_t local = j0 - cum[sidx] + lane;
+        if (local < count[sidx]) {
+            const int64_t p = idx[sg.idx_off + local];
+            const int64_t q = sg.row0 + p;
+            const int64_t r = rows ? rows[q] : q;
+            ll[sg.off + p] = jk_lnl_row<L, PHYS, false>(stars[sg.star], r, P, e_, om_, M0_, s_, s_tab);
+        }
     }
 }
 
 template <int L, bool PHYS>
 static void launch_seg_scatter_t(const jk_batch *b, int n_seg, const SegDev *d_segs,
-                                 const int64_t *count, const int64_t *idx, int gx, const double *P,
-                                 const void *e, const void *om, const void *M0, const void *s,
-                                 const int64_t *rows, double *ll, hipStream_t stream)
+                                 const int64_t *count, const int64_t *cum, const int64_t *idx,
+                                 int gx, const double *P, const void *e, const void *om,
+                                 const void *M0, const void *s, const int64_t *rows, double *ll,
+                                 hipStream_t stream)
 {
-    hipLaunchKernelGGL((k_seg_scatter_f64<L, PHYS>), dim3(gx, n_seg), dim3(256), 0, stream,
-                       b->d_stars, d_segs, count, idx, P, e, om, M0, s, rows, ll);
+    hipLaunchKernelGGL((k_seg_scatter_f64<L, PHYS>), dim3(gx), dim3(256), 0, stream, b->d_stars,
+                       d_segs, n_seg, count, cum, idx, P, e, om, M0, s, rows, ll);
 }
 
 hipError_t jk_launch_polish_batch(const jk_batch *b, int n_seg, const SegDev *d_segs,
                                   int64_t max_n, int64_t max_nb, double *partials, int pb,
                                   uint64_t *mask, int64_t *bsum, int64_t *idx, double *llmax,
-                                  const double *P, const double *e, const double *om,
+                                  int64_t *cum, const double *P, const double *e, const double *om,
                                   const double *M0, const double *s, const int64_t *rows,
                                   double *ll, double window, int64_t *n_polished,
                                   hipStream_t stream)
@@ -1392,17 +1435,18 @@ hipError_t jk_launch_polish_batch(const jk_batch *b, int n_seg, const SegDev *d_
                        mask, bsum);
     hipLaunchKernelGGL(k_seg_compact, dim3((unsigned)max_nb, n_seg), dim3(256), 0, stream, d_segs,
                        mask, bsum, idx, n_polished);
-    int gx = (int)((max_n + 255) / 256);
-    if (gx > 16) gx = 16;
+    hipLaunchKernelGGL(k_seg_count_scan, dim3(1), dim3(256), 0, stream, n_polished, n_seg, cum);
+    // The flagged entries of all segments form one list spread over the grid.
+    const int gx = 2048;
     const bool phys = b->jitter_mode == JK_JITTER_PHYSICAL;
 #define JK_PCASE(LL)                                                                             \
     case LL:                                                                                     \
         if (phys)                                                                                \
-            launch_seg_scatter_t<LL, true>(b, n_seg, d_segs, n_polished, idx, gx, P, e, om, M0, s, \
-                                           rows, ll, stream);                                    \
+            launch_seg_scatter_t<LL, true>(b, n_seg, d_segs, n_polished, cum, idx, gx, P, e, om,   \
+                                           M0, s, rows, ll, stream);                             \
         else                                                                                     \
-            launch_seg_scatter_t<LL, false>(b, n_seg, d_segs, n_polished, idx, gx, P, e, om, M0,   \
-                                            s, rows, ll, stream);                                \
+            launch_seg_scatter_t<LL, false>(b, n_seg, d_segs, n_polished, cum, idx, gx, P, e, om,  \
+                                            M0, s, rows, ll, stream);                            \
         break;
     switch (b->L) {
         JK_PCASE(1)

@@ -22,6 +22,7 @@ from . import units as u
 from .samples import NONLINEAR_INTERNAL_UNITS, NONLINEAR_PACKED_ORDER, JokerSamples
 
 __all__ = ["DeviceStar", "PriorCache", "star_arrays", "lnl", "lnl_reject", "polish", "reject",
+           "posterior_inputs_many", "to_host",
            "linear_posterior", "uniforms",
            "pcg64_state", "pcg64_advance_state", "JITTER_MODES"]
 
@@ -431,6 +432,53 @@ def reject(ll, rng_state, max_keep, offset=0, llmax=None, device=0):
     n_pass = int(h[0])
     k = min(n_pass, max_keep)
     return h[2:2 + k].copy(), n_pass, float(h[1:2].view(np.float64)[0])
+
+
+def posterior_inputs_many(stars, cache, rows_list):
+    """For each (star, accepted cache rows): (packed rows (k, 5), a (k, L),
+    cov (k, L, L)) -- jk_linear_posterior launched once per star into one
+    device block, the prior rows gathered into the same block, one copy back."""
+    torch = _torch()
+    lib = _lib.ensure_device(cache.device)
+    dev = cache.P.device
+    ks = [len(r) for r in rows_list]
+    K = int(sum(ks))
+    if K == 0:
+        return [(np.zeros((0, 5)), np.zeros((0, st.L)), np.zeros((0, st.L, st.L)))
+                for st in stars]
+    per = [k * (st.L + st.L * st.L + 1) for k, st in zip(ks, stars)]
+    base = int(sum(per))
+    res = torch.empty(base + 5 * K, dtype=torch.float64, device=dev)
+    rows = torch.as_tensor(np.concatenate([np.asarray(r, dtype=np.int64) for r in rows_list]),
+                           device=dev)
+    p0, r0 = res.data_ptr(), rows.data_ptr()
+    off, roff = 0, 0
+    cols = [_dptr(c) for c in (cache.P, cache.e, cache.omega, cache.M0, cache.s)]
+    for st, k, n in zip(stars, ks, per):
+        if k:
+            L = st.L
+            a_ptr = p0 + 8 * off
+            _lib.check(lib.jk_linear_posterior(st.handle, k, *cols, ctypes.c_void_p(r0 + 8 * roff),
+                                               ctypes.c_void_p(a_ptr),
+                                               ctypes.c_void_p(a_ptr + 8 * k * L),
+                                               ctypes.c_void_p(a_ptr + 8 * k * (L + L * L)),
+                                               _stream(cache.device)), "jk_linear_posterior")
+        off += n
+        roff += k
+    packed = res[base:].view(5, K)
+    for j, c in enumerate((cache.P, cache.e, cache.omega, cache.M0, cache.s)):
+        torch.index_select(c, 0, rows, out=packed[j])
+    h = to_host(res)
+    out, off, roff = [], 0, 0
+    hp = h[base:].reshape(5, K)
+    for st, k, n in zip(stars, ks, per):
+        L = st.L
+        blk = h[off:off + n]
+        out.append((hp[:, roff:roff + k].T.copy(), blk[:k * L].reshape(k, L).copy(),
+                    blk[k * L:k * (L + L * L)].reshape(k, L, L).copy()))
+        off += n
+        roff += k
+    return out
 
 
 def to_host(t):

@@ -21,7 +21,8 @@ import numpy as np
 
 from .batch import LaunchShape, StarBatch, heaviest_first, lnl_batch, polish_batch, reject_batch, \
     upload_items, work_items
-from .device import DeviceStar, _torch, linear_internal_units, pcg64_state, star_arrays
+from .device import (DeviceStar, _torch, linear_internal_units, pcg64_state, posterior_inputs_many,
+                     star_arrays)
 from .likelihood_helpers import validate_prepare_data
 from .thejoker import POLISH_WINDOW, IterativeRejection, TheJoker
 
@@ -275,9 +276,12 @@ class JokerServer:
         begins = np.array([b for _, b, _ in plan], dtype=np.int64)
         before = np.array([r.it.n_evals for r, _, _ in plan], dtype=np.int64)
         seg_off = np.array([r.region[0] for r, _, _ in plan], dtype=np.int64)
-        items, off, total = work_items(sizes, self.shape.tile, row_begin=begins, stars=slots)
+        # rows per work item: the launch shape's tile, widened so that a step
+        # never queues more than ~64K items (host ordering and upload stay cheap)
+        tile = max(self.shape.tile, 256 * -(-int(sizes.sum()) // (256 * 65536)))
+        items, off, total = work_items(sizes, tile, row_begin=begins, stars=slots)
         # write each request's rows straight into its arena region
-        counts = (sizes + self.shape.tile - 1) // self.shape.tile
+        counts = (sizes + tile - 1) // tile
         items["out_offset"] += np.repeat(seg_off + before - off, counts)
         items = items[heaviest_first(items, self._table.M)]
         d_items = upload_items(items, self.device)
@@ -316,11 +320,15 @@ class JokerServer:
                 r.it.done = True
             if r.it.done:
                 retired.append(r)
+        ok = [r for r in retired if r.error is None]
+        if ok:
+            posts = posterior_inputs_many([r.star for r in ok], self.cache,
+                                          [r.it.result()[0] for r in ok])
+            for r, post in zip(ok, posts):
+                r.post = post
         for r in retired:
             self.active.remove(r)
             if r.error is None:
-                full_idx, _ = r.it.result()
-                r.post = self._helper._posterior_inputs(r.star, self.cache, full_idx)
                 self._to_finish.append(r)
                 self._release(r)
             else:
