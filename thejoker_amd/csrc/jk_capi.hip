@@ -116,8 +116,9 @@ int jk_star_create(int n, const double *t, const double *rv, const double *ivar,
 
     // Host epoch arrays: dt, r' = y - T mu_trend, y, ivar, T (L-1 rows of n).
     const int LT = L - 1;
-    std::vector<double> h((size_t)n * (4 + LT));
-    double *h_dt = h.data(), *h_rp = h_dt + n, *h_y = h_rp + n, *h_iv = h_y + n, *h_T = h_iv + n;
+    std::vector<double> h((size_t)n * (5 + LT));
+    double *h_dt = h.data(), *h_rp = h_dt + n, *h_y = h_rp + n, *h_iv = h_y + n, *h_s2 = h_iv + n,
+           *h_T = h_s2 + n;
     d.sum_ln_ivar = 0.0;
     d.S_rr = 0.0;
     for (int k = 0; k < n; ++k) {
@@ -135,6 +136,7 @@ int jk_star_create(int n, const double *t, const double *rv, const double *ivar,
         h_rp[k] = rp;
         h_y[k] = rv[k];
         h_iv[k] = ivar[k];
+        h_s2[k] = 1.0 / ivar[k];
         d.sum_ln_ivar += log(ivar[k]);
         d.S_rr += ivar[k] * rp * rp;
         for (int j = 0; j < LT; ++j) {
@@ -146,8 +148,9 @@ int jk_star_create(int n, const double *t, const double *rv, const double *ivar,
             }
         }
     }
-    d.ln_c = (double)n * JK_LN_TWO_PI - d.sum_ln_ivar + d.ln_lam_trend +
-             (k_prior_mode == JK_KPRIOR_NORMAL ? log(lambda_diag[0]) : 0.0);
+    d.ln_c_phys = (double)n * JK_LN_TWO_PI + d.ln_lam_trend +
+                  (k_prior_mode == JK_KPRIOR_NORMAL ? log(lambda_diag[0]) : 0.0);
+    d.ln_c = d.ln_c_phys - d.sum_ln_ivar;
     for (int j = 0; j < LT; ++j)
         for (int q = 0; q < j; ++q) d.S_tt[q * JK_LMAX + j] = d.S_tt[j * JK_LMAX + q];
 
@@ -167,7 +170,8 @@ int jk_star_create(int n, const double *t, const double *rv, const double *ivar,
     d.rp = base + n;
     d.y = base + 2 * n;
     d.ivar = base + 3 * n;
-    d.T = base + 4 * n;
+    d.sig2 = base + 4 * n;
+    d.T = base + 5 * n;
     *out = st;
     return JK_OK;
 }
@@ -348,7 +352,7 @@ int jk_batch_create(const jk_star *const *stars, int n_stars, jk_batch **out)
     size_t n_epoch_doubles = 0;
     for (int i = 0; i < n_stars; ++i) {
         if (!stars[i] || stars[i]->dev.L != L || stars[i]->dev.jitter_mode != jm) return JK_EINVAL;
-        n_epoch_doubles += (size_t)stars[i]->dev.n * (4 + L - 1);
+        n_epoch_doubles += (size_t)stars[i]->dev.n * (5 + L - 1);
     }
     jk_batch *b = new (std::nothrow) jk_batch;
     if (!b) return JK_ENOMEM;
@@ -368,7 +372,7 @@ int jk_batch_create(const jk_star *const *stars, int n_stars, jk_batch **out)
     size_t off = 0;
     for (int i = 0; i < n_stars; ++i) {
         h[i] = stars[i]->dev;
-        const size_t cnt = (size_t)h[i].n * (4 + L - 1);
+        const size_t cnt = (size_t)h[i].n * (5 + L - 1);
         if (hipMemcpy(base + off, stars[i]->d_epochs, sizeof(double) * cnt,
                       hipMemcpyDeviceToDevice) != hipSuccess) {
             (void)hipFree(b->d_stars);
@@ -382,7 +386,8 @@ int jk_batch_create(const jk_star *const *stars, int n_stars, jk_batch **out)
         h[i].rp = p + n;
         h[i].y = p + 2 * n;
         h[i].ivar = p + 3 * n;
-        h[i].T = p + 4 * n;
+        h[i].sig2 = p + 4 * n;
+        h[i].T = p + 5 * n;
         off += cnt;
     }
     if (hipMemcpy(b->d_stars, h.data(), sizeof(StarDev) * n_stars, hipMemcpyHostToDevice) !=

@@ -45,8 +45,11 @@ struct StarDev {
     double ln_lam_trend;       // sum_{i>=1} ln Lambda_i
     double sum_ln_ivar;        // sum_n ln ivar_n
     // N ln 2pi - sum_n ln ivar_n + sum_{i>=1} ln Lambda_i, plus ln Lambda_0
-    // for the Normal K prior: the sample-independent part of the log-constant.
+    // for the Normal K prior: the sample-independent part of the log-constant
+    // with w = ivar; ln_c_phys the same without the ivar term (physical
+    // jitter, where sum ln w comes from the sample's product of sigma^2 + s^2).
     double ln_c;
+    double ln_c_phys;
     // Trend-only moments with w = ivar (the reference jitter mode, where they
     // do not depend on the sample): S_tt[j][k] = sum ivar T_j T_k,
     // S_tr[j] = sum ivar T_j r'_n, S_rr = sum ivar r'^2, r' = y - T mu_trend.
@@ -55,11 +58,13 @@ struct StarDev {
     double S_rr;
     // Posterior right-hand side pieces: sum ivar T_j y (j >= 1).
     double S_ty[JK_LMAX];
-    // Epoch arrays (device): dt = t - t_ref, rp = r', y, ivar, T (L-1 rows of n).
+    // Epoch arrays (device): dt = t - t_ref, rp = r', y, ivar, sig2 = 1/ivar,
+    // T (L-1 rows of n).
     const double *dt;
     const double *rp;
     const double *y;
     const double *ivar;
+    const double *sig2;
     const double *T;
 };
 

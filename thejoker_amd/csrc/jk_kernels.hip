@@ -84,43 +84,11 @@ __device__ __forceinline__ void jk_sincos_tab(float Ef, const double2 *__restric
 }
 
 // ---------------------------------------------------------------------------
-// sin and cos on [0, pi] from truncated Taylor series on [0, pi/2]:
+// sin and cos in FP32 on [0, pi] from truncated Taylor series on [0, pi/2]:
 // E <= pi/2: y = E;  E > pi/2: y = pi - E (exact subtraction by Sterbenz,
-// plus the low part of pi), sin E = sin y, cos E = -cos y.
-// FP64: sin to y^21, cos to y^22 -- truncation below 1.3e-18 on [0, pi/2].
-// FP32: sin to y^13, cos to y^14 -- truncation below 7e-10.
+// plus the low part of pi), sin E = sin y, cos E = -cos y.  sin to y^13, cos
+// to y^14: truncation below 7e-10.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void jk_sincos_0pi_f64(double E, double &s, double &c)
-{
-    const bool hi = E > JK_PI_HALF;
-    const double y = hi ? ((JK_PI_HI - E) + JK_PI_LO) : E;
-    const double y2 = y * y;
-    double ps = 1.9572941063391263e-20;
-    ps = fma(ps, y2, -8.22063524662433e-18);
-    ps = fma(ps, y2, 2.8114572543455206e-15);
-    ps = fma(ps, y2, -7.647163731819816e-13);
-    ps = fma(ps, y2, 1.6059043836821613e-10);
-    ps = fma(ps, y2, -2.505210838544172e-08);
-    ps = fma(ps, y2, 2.7557319223985893e-06);
-    ps = fma(ps, y2, -0.0001984126984126984);
-    ps = fma(ps, y2, 0.008333333333333333);
-    ps = fma(ps, y2, -0.16666666666666666);
-    s = fma(ps * y2, y, y);
-    double pc = -8.896791392450574e-22;
-    pc = fma(pc, y2, 4.110317623312165e-19);
-    pc = fma(pc, y2, -1.5619206968586225e-16);
-    pc = fma(pc, y2, 4.779477332387385e-14);
-    pc = fma(pc, y2, -1.1470745597729725e-11);
-    pc = fma(pc, y2, 2.08767569878681e-09);
-    pc = fma(pc, y2, -2.755731922398589e-07);
-    pc = fma(pc, y2, 2.48015873015873e-05);
-    pc = fma(pc, y2, -0.001388888888888889);
-    pc = fma(pc, y2, 0.041666666666666664);
-    pc = fma(pc, y2, -0.5);
-    c = fma(pc, y2, 1.0);
-    c = hi ? -c : c;
-}
-
 __device__ __forceinline__ void jk_sincos_0pi_f32(float E, float &s, float &c)
 {
     const bool hi = E > 1.57079632679f;
@@ -145,14 +113,16 @@ __device__ __forceinline__ void jk_sincos_0pi_f32(float E, float &s, float &c)
 }
 
 // Kepler in FP64 on ax = |x| in [0, pi] (see jk_kepler_f64 in jk_device.h for
-// the convergence argument); this variant evaluates sin/cos with the [0, pi]
-// polynomial pair above.
-__device__ __forceinline__ void jk_kepler_poly_f64(double x, double e, double &sinE, double &cosE)
+// the convergence argument); this variant evaluates sin/cos with the LDS
+// table (jk_sincos_tab_f64).
+__device__ __forceinline__ void jk_kepler_poly_f64(double x, double e,
+                                                   const double2 *__restrict__ tab,
+                                                   double &sinE, double &cosE)
 {
     const double ax = fabs(x);
     double E = fmin(fma(0.85, e, ax), JK_PI);
     double s, c;
-    jk_sincos_0pi_f64(E, s, c);
+    jk_sincos_tab_f64(E, tab, s, c);
 #pragma unroll 1
     for (int it = 0; it < JK_KEPLER_MAXITER; ++it) {
         const double fp = fma(-e, c, 1.0);
@@ -165,7 +135,7 @@ __device__ __forceinline__ void jk_kepler_poly_f64(double x, double e, double &s
             s = s1;
             break;
         }
-        jk_sincos_0pi_f64(E, s, c);
+        jk_sincos_tab_f64(E, tab, s, c);
     }
     sinE = x < 0.0 ? -s : s;
     cosE = c;
@@ -250,7 +220,12 @@ __device__ __forceinline__ double jk_rcp2(double x)
     return fma(r, fma(-x, r, 1.0), r);
 }
 
-__device__ __forceinline__ void jk_kepler_markley_f64(double x, double e, float ef, float omef,
+// Returns false when the correction is outside the starter's error bound
+// (|d| >= dmax), or always when dmax < 0 (e so close to 1 that its FP32 copy
+// misleads the starter): the caller then re-evaluates the whole sample with
+// the bracketed Newton iteration, outside the epoch loop, so that rare path
+// adds no registers to the loop.
+__device__ __forceinline__ bool jk_kepler_markley_f64(double x, double e, float ef, float omef,
                                                       float a1f, double dmax,
                                                       const double2 *__restrict__ tab,
                                                       double &sinE, double &cosE)
@@ -272,21 +247,13 @@ __device__ __forceinline__ void jk_kepler_markley_f64(double x, double e, float 
     const double c3 = fma(2.0, b22, -b3);
     const double c4 = b2 * fma(5.0, b3 - b22, 1.0 / 12.0);   // -b4 = b2 / 12
     const double d = t * fma(t, fma(t, fma(t, c4, c3), -b2), 1.0);
-    double s, c;
-    if (fabs(d) < dmax) {
-        const double d2 = d * d;
-        const double cd = fma(d2, fma(d2, 1.0 / 24.0, -0.5), 1.0);
-        const double sd = d * fma(d2, -1.0 / 6.0, 1.0);
-        s = fma(s1, cd, c1 * sd);
-        c = fma(c1, cd, -s1 * sd);
-    } else {
-        // Outside the starter's error bound, or e so close to 1 that its
-        // FP32 copy misleads the starter (dmax < 0): fall back to the
-        // bracketed Newton iteration.
-        jk_kepler_poly_f64(ax, e, s, c);
-    }
+    const double d2 = d * d;
+    const double cd = fma(d2, fma(d2, 1.0 / 24.0, -0.5), 1.0);
+    const double sd = d * fma(d2, -1.0 / 6.0, 1.0);
+    const double s = fma(s1, cd, c1 * sd);
+    cosE = fma(c1, cd, -s1 * sd);
     sinE = x < 0.0 ? -s : s;
-    cosE = c;
+    return fabs(d) < dmax;
 }
 
 __device__ __forceinline__ void jk_kepler_markley_f32(float x, float e, float ome, float a1,
@@ -317,7 +284,7 @@ __device__ __forceinline__ void jk_kepler_markley_f32(float x, float e, float om
 // ---------------------------------------------------------------------------
 // Fused marginal-likelihood kernel.
 // ---------------------------------------------------------------------------
-template <int L, bool PHYS, bool MIXED>
+template <int L, bool PHYS, bool MIXED, bool SAFE = false>
 __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
                                              const double *__restrict__ P,
                                              const void *__restrict__ e_,
@@ -389,6 +356,7 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
     }
 
     const int N = st.n;
+    bool bad = false;
 #pragma unroll 2
     for (int k = 0; k < N; ++k) {
         const double dtk = st.dt[k];
@@ -403,22 +371,25 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
             m = (double)fmaf(cwf, cf, fmaf(-swf, sf, ecwf));
         } else {
             double sE, cE;
-            jk_kepler_markley_f64(x, ev, ef, omef, a1f, dmax, tab, sE, cE);
+            if constexpr (SAFE)
+                jk_kepler_poly_f64(x, ev, tab, sE, cE);
+            else
+                bad |= !jk_kepler_markley_f64(x, ev, ef, omef, a1f, dmax, tab, sE, cE);
             // m = [cos w (cos E - e) - sin w sqrt(1-e^2) sin E] / (1 - e cos E) + e cos w
             const double den = jk_rcp2(fma(-ev, cE, 1.0));
             m = fma(fma(cw, cE - ev, -swsq * sE), den, ecw);
         }
-        const double iv = st.ivar[k];
         const double rp = st.rp[k];
         double w;
         if constexpr (PHYS) {
-            const double d = fma(s2, iv, 1.0);
-            w = iv * jk_rcp2(d);
+            // w = 1 / (sigma^2 + s^2); sum ln w = -ln of the running product
+            const double d = st.sig2[k] + s2;
+            w = jk_rcp2(d);
             int ex;
             prod = frexp(prod * d, &ex);
             pexp += ex;
         } else {
-            w = iv;
+            w = st.ivar[k];
         }
         const double wm = w * m;
         Smm = fma(wm, m, Smm);
@@ -439,21 +410,27 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
         if constexpr (PHYS) Srr = fma(w * rp, rp, Srr);
     }
 
+    if constexpr (!MIXED && !SAFE) {
+        if (bad) return jk_lnl_row<L, PHYS, false, true>(st, r, P, e_, om_, M0_, s_, tab);
+    }
+
     // Assemble G (lower triangle used), u, chi0, log-constant.
     double G[L * L], u[L];
     const double mu0 = st.mu0;
     const bool fcm = st.k_prior_mode == JK_KPRIOR_FIXED_COMPANION_MASS;
     G[0] = (fcm ? jk_rcp2(lam0) : st.lam_inv[0]) + Smm;
     u[0] = fma(-mu0, Smm, Smr);
-    // log-constant: st.ln_c holds everything but the sample's jitter product
-    // prod(1 + s^2 ivar_n) = prod * 2^pexp and, for FixedCompanionMass, ln lam0
-    double chi0, lnc = st.ln_c, scale = 1.0;
+    // log-constant: st.ln_c / ln_c_phys hold everything but the sample's
+    // product prod(sigma_n^2 + s^2) = prod * 2^pexp and, for FixedCompanionMass,
+    // ln lam0
+    double chi0, lnc, scale = 1.0;
     if constexpr (PHYS) {
         chi0 = Srr;
-        lnc += (double)pexp * JK_LN2;
+        lnc = st.ln_c_phys + (double)pexp * JK_LN2;
         scale = prod;
     } else {
         chi0 = st.S_rr;
+        lnc = st.ln_c;
     }
     if (fcm) scale *= lam0;
     chi0 = fma(mu0 * mu0, Smm, fma(-2.0 * mu0, Smr, chi0));
@@ -958,7 +935,7 @@ hipError_t jk_launch_reject(const double *ll, int64_t n, const double *llmax, u1
 int64_t jk_lnl_reject_scratch_bytes(int64_t n)
 {
     const int64_t blocks = (n + 255) / 256;
-    return (int64_t)sizeof(double) * (blocks + 1) +
+    return (int64_t)sizeof(double) * (blocks + 1024) +
            (int64_t)sizeof(uint64_t) * jk_reject_mask_words(n) +
            (int64_t)sizeof(int64_t) * jk_reject_bsum_count(n);
 }
@@ -971,13 +948,16 @@ hipError_t jk_launch_lnl_reject_f64(const jk_star *st, int64_t n, const double *
                                     hipStream_t stream)
 {
     const int64_t blocks = (n + 255) / 256;
-    double *partials = (double *)scratch;
-    uint64_t *mask = (uint64_t *)(partials + blocks + 1);
+    double *partials = (double *)scratch;       // one per likelihood workgroup
+    double *partials2 = partials + blocks;      // second level, at most 1024
+    uint64_t *mask = (uint64_t *)(partials2 + 1024);
     int64_t *bsum = (int64_t *)(mask + jk_reject_mask_words(n));
     hipError_t err = launch_lnl_dispatch<false>(st, n, P, e, om, M0, s, rows, ll, partials, stream);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(256), 0, stream, partials, (int)blocks,
-                       llmax_out);
+    // the per-workgroup maxima reduced like any array (two parallel levels)
+    err = jk_launch_max(partials, blocks, partials2, jk_max_partials_count(blocks), llmax_out,
+                        stream);
+    if (err != hipSuccess) return err;
     return jk_launch_reject(ll, n, llmax_out, state, inc, offset, max_keep, idx_out, n_pass_out,
                             mask, bsum, stream);
 }
