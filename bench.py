@@ -81,6 +81,36 @@ def cpu_baseline(star, rows, mu, lam, n_sample):
                       f"restated in C (-O3), {cores} threads, {dt:.2f} s wall"}
 
 
+def mixed_vs_f64(torch, np, D, dstar, cache, llf, llm, bnd, seed, max_keep=None):
+    """Raw mixed-vs-FP64 errors (all rows, and rows within 40 of the maximum),
+    the largest error as a fraction of the kernel's bound, then the product
+    path -- settle the mixed buffer, reject -- against the FP64 rejection with
+    the same uniforms: rows re-evaluated and accepted-set differences."""
+    from thejoker_amd.thejoker import TheJoker
+    n = int(llf.shape[0])
+    d = (llf - llm).abs()
+    live = llf >= llf.max() - WINDOW
+    res = {"max_abs_dlnl_all": float(d.max()), "max_abs_dlnl_live": float(d[live].max()),
+           "live_rows": int(live.sum()),
+           "max_dlnl_over_bound": float((d / bnd.double()).max())}
+    del d, live
+    max_keep = n if max_keep is None else max_keep
+    st = D.pcg64_state(np.random.default_rng(seed))
+    TheJoker._settle(dstar, cache, llm, bnd, st)
+    res["rows_settled_in_fp64"] = int((bnd == 0).sum())
+    a_f, n_f, _ = D.reject(llf, st, max_keep)
+    a_m, _, _ = D.reject(llm, st, max_keep)
+    res["accepted"] = int(min(n_f, max_keep))
+    res["accept_flips_vs_fp64"] = int(len(np.setxor1d(a_f, a_m)))
+    return res
+
+
+def fmt_mixed(r):
+    return (f"max|dlnL| all {r['max_abs_dlnl_all']:.2e} live {r['max_abs_dlnl_live']:.2e}, "
+            f"max |dlnL|/bound {r['max_dlnl_over_bound']:.3f}, settled "
+            f"{r['rows_settled_in_fp64']} rows, flips {r['accept_flips_vs_fp64']}")
+
+
 def timed(fn, reps, torch):
     """Mean ms of fn() over reps launches, HIP events on the current stream."""
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -155,22 +185,15 @@ def leg_single(args, torch, np, D, syn):
 
     # mixed precision on the same rows
     ll32 = torch.empty(n, dtype=torch.float64, device="cuda")
+    bnd = torch.empty(n, dtype=torch.float32, device="cuda")
     D.lnl(dstar, cache, precision="mixed", out=ll32)
     m_ms = timed(lambda: D.lnl(dstar, cache, precision="mixed", out=ll32), reps, torch)
+    D.lnl(dstar, cache, precision="mixed", out=ll32, bound=bnd)
     D.lnl(dstar, cache, precision="f64", out=ll)
-    ll_h, m_h = ll.cpu().numpy(), ll32.cpu().numpy()
-    d = np.abs(m_h - ll_h)
-    live = ll_h >= ll_h.max() - WINDOW
-    g1, g2 = np.random.default_rng(7), np.random.default_rng(7)
-    D.polish(dstar, cache, ll32, window=WINDOW)
-    a_f, _, _ = D.reject(ll, D.pcg64_state(g1), n)
-    a_m, _, _ = D.reject(ll32, D.pcg64_state(g2), n)
     out["mixed"] = {"kernel_ms": m_ms, "sample_epochs_per_s": n * star.M / (m_ms * 1e-3),
-                    "speedup_vs_f64_kernel": k_ms / m_ms, "max_abs_dlnl_all": float(d.max()),
-                    "max_abs_dlnl_live": float(d[live].max()), "live_rows": int(live.sum()),
-                    "accept_flips_after_fp64_window": int(len(np.setxor1d(a_f, a_m)))}
-    log(f"[bench] mixed: {m_ms:.3f} ms ({k_ms / m_ms:.2f}x), max|dlnL| all {d.max():.2e} "
-        f"live {d[live].max():.2e}, flips {out['mixed']['accept_flips_after_fp64_window']}")
+                    "speedup_vs_f64_kernel": k_ms / m_ms,
+                    **mixed_vs_f64(torch, np, D, dstar, cache, ll, ll32, bnd, seed=7)}
+    log(f"[bench] mixed: {m_ms:.3f} ms ({k_ms / m_ms:.2f}x), {fmt_mixed(out['mixed'])}")
 
     if not args.no_cpu:
         rows = np.ascontiguousarray(np.stack([cols[k] for k in ("P", "e", "omega", "M0", "s")],
@@ -211,16 +234,17 @@ def leg_batched(args, torch, np, D, syn):
     ws = torch.zeros(64, dtype=torch.int32, device="cuda")
     llm = torch.empty(total, dtype=torch.float64, device="cuda")
     llf = torch.empty(total, dtype=torch.float64, device="cuda")
+    bnd = torch.empty(total, dtype=torch.float32, device="cuda")
     run_m = lambda: B.lnl_batch(batch, cache, d_items, len(items), total, "mixed", shape,
-                                out=llm, workspace=ws)
+                                out=llm, workspace=ws, bound=bnd)
     run_f = lambda: B.lnl_batch(batch, cache, d_items, len(items), total, "f64", shape,
                                 out=llf, workspace=ws)
     run_m()
     run_f()
     m_ms = timed(run_m, 3, torch)
     f_ms = timed(run_f, 2, torch)
-    # errors before the FP64 window, per star
-    dmax_all, dmax_live = 0.0, 0.0
+    # raw mixed errors (before settling) and how they compare with the bounds
+    dmax_all, dmax_live, ratio = 0.0, 0.0, 0.0
     for s in range(n_stars):
         a = llf[off[s]:off[s] + n]
         b = llm[off[s]:off[s] + n]
@@ -228,22 +252,34 @@ def leg_batched(args, torch, np, D, syn):
         live = a >= a.max() - WINDOW
         dmax_all = max(dmax_all, float(dd.max()))
         dmax_live = max(dmax_live, float(dd[live].max()))
+        ratio = max(ratio, float((dd / bnd[off[s]:off[s] + n].double()).max()))
     sizes = np.full(n_stars, n)
     states = lambda: [D.pcg64_state(np.random.default_rng(1000 + s)) for s in range(n_stars)]
+    settled = [0]
 
     def step():
         run_m()
-        B.polish_batch(batch, cache, llm, off, sizes, window=WINDOW)
-        return B.reject_batch(llm, off, sizes, states(), 256)
+        st = states()
+        settled[0] = int(B.settle_batch(batch, cache, llm, bnd, off, sizes, st).sum())
+        return B.reject_batch(llm, off, sizes, st, 256)
+
+    def step_f():
+        run_f()
+        return B.reject_batch(llf, off, sizes, states(), 256)
 
     step()
+    step_f()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(2):
         good_m, np_m, _ = step()
     torch.cuda.synchronize()
     e2e_ms = (time.perf_counter() - t0) / 2 * 1e3
-    good_f, np_f, _ = B.reject_batch(llf, off, sizes, states(), 256)
+    t0 = time.perf_counter()
+    for _ in range(2):
+        good_f, np_f, _ = step_f()
+    torch.cuda.synchronize()
+    e2e_f_ms = (time.perf_counter() - t0) / 2 * 1e3
     flips = sum(len(np.setxor1d(x, y)) for x, y in zip(good_m, good_f))
     res = {"stars": n_stars, "prior_rows": n, "epochs_total": int(M.sum()), "linear_params": 3,
            "launch_shape": shape.as_dict(), "autotune_s": tune_s,
@@ -254,12 +290,17 @@ def leg_batched(args, torch, np, D, syn):
            "mixed_speedup_vs_f64": f_ms / m_ms,
            "mixed_end_to_end_ms": e2e_ms,
            "mixed_end_to_end_sample_epochs_per_s": work / (e2e_ms * 1e-3),
+           "f64_end_to_end_ms": e2e_f_ms,
+           "f64_end_to_end_sample_epochs_per_s": work / (e2e_f_ms * 1e-3),
            "max_abs_dlnl_all": dmax_all, "max_abs_dlnl_live": dmax_live,
-           "accept_flips_after_fp64_window": int(flips),
+           "max_dlnl_over_bound": ratio,
+           "rows_settled_in_fp64": settled[0], "rows_settled_fraction": settled[0] / total,
+           "accept_flips_vs_fp64": int(flips),
            "accepted_total": int(np.minimum(np_f, 256).sum())}
     log(f"[bench] batched: mixed {m_ms:.2f} ms ({res['mixed_sample_epochs_per_s']:.3e}/s), f64 "
-        f"{f_ms:.2f} ms, e2e {e2e_ms:.2f} ms, |dlnL| all {dmax_all:.2e} live {dmax_live:.2e}, "
-        f"flips {flips}")
+        f"{f_ms:.2f} ms; end to end mixed {e2e_ms:.2f} ms, f64 {e2e_f_ms:.2f} ms; |dlnL| all "
+        f"{dmax_all:.2e} live {dmax_live:.2e}, max |dlnL|/bound {ratio:.3f}, settled "
+        f"{settled[0] / total:.2e} of rows, flips {flips}")
     del llm, llf, cache, batch
     torch.cuda.empty_cache()
     return res
@@ -375,15 +416,12 @@ def leg_stress(args, torch, np, D, syn):
     f_ms = timed(lambda: run("f64", llf), 2, torch)
     m_ms = timed(lambda: run("mixed", llm), 2, torch)
     work = n * star.M
-    d = (llf - llm).abs()
-    mx = llf.max()
-    live = llf >= mx - WINDOW
-    dmax_all, dmax_live = float(d.max()), float(d[live].max())
-    n_live = int(live.sum())
-    del d, live
-    D.polish(dstar, cache, llm, window=WINDOW)
-    a_f, n_f, _ = D.reject(llf, D.pcg64_state(np.random.default_rng(5)), 256)
-    a_m, n_m, _ = D.reject(llm, D.pcg64_state(np.random.default_rng(5)), 256)
+    bnd = torch.empty(n, dtype=torch.float32, device="cuda")
+    for s0 in range(0, n, chunk):
+        D.lnl(dstar, cache, precision="mixed", start=s0, n=min(chunk, n - s0),
+              out=llm[s0:s0 + chunk], bound=bnd[s0:s0 + chunk])
+    cmp = mixed_vs_f64(torch, np, D, dstar, cache, llf, llm, bnd, seed=5, max_keep=256)
+    del bnd
     flops = n * (star.M * F_EPOCH_JITTER + F_SAMPLE[2])
     res = {"prior_rows": n, "epochs": star.M, "generate_s": gen_s,
            "f64_ms": f_ms, "mixed_ms": m_ms,
@@ -394,12 +432,9 @@ def leg_stress(args, torch, np, D, syn):
            "f64_frac_fp64_peak": flops / (f_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
            "f64_hbm_gbs": n * 48 / (f_ms * 1e-3) / 1e9,
            "mixed_hbm_gbs": n * 32 / (m_ms * 1e-3) / 1e9,
-           "max_abs_dlnl_all": dmax_all, "max_abs_dlnl_live": dmax_live, "live_rows": n_live,
-           "accepted": int(min(n_f, 256)),
-           "accept_flips_after_fp64_window": int(len(np.setxor1d(a_f, a_m)))}
+           **cmp}
     log(f"[bench] stress: f64 {f_ms:.1f} ms mixed {m_ms:.1f} ms ({f_ms / m_ms:.2f}x), "
-        f"|dlnL| all {dmax_all:.2e} live {dmax_live:.2e}, flips "
-        f"{res['accept_flips_after_fp64_window']}")
+        f"{fmt_mixed(cmp)}")
     del llf, llm, cache
     torch.cuda.empty_cache()
     return res

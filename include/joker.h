@@ -98,12 +98,17 @@ int jk_lnl_f64(const jk_star *star, int64_t n, const double *P, const double *e,
 
 /* Mixed precision marginal log-likelihood: FP64 phase 2 pi (t - t_ref)/P - M0
  * and its 2 pi wrap, FP32 Kepler solve and design column, FP64 moment
- * accumulation, Cholesky log-det and lnL.  Same semantics as jk_lnl_f64; the
- * angular / eccentricity / jitter columns are the FP32 copy of the cache.
+ * accumulation, jitter weights, Cholesky log-det and lnL.  Same arguments
+ * and semantics as jk_lnl_f64 (the FP64 columns: the kernel is bound by
+ * arithmetic, and an FP32 e would cost sqrt(1 - e^2) its accuracy as
+ * e -> 1).
+ * bound_out (device, n floats, optional) receives for every row a bound on
+ * |lnL - lnL of jk_lnl_f64| (first order in the FP32 design-column error,
+ * doubled), +inf where none is known; jk_settle_batch uses it.
  * No reference counterpart beyond fast_likelihood.pyx:428-469. */
-int jk_lnl_mixed(const jk_star *star, int64_t n, const double *P, const float *e,
-                 const float *omega, const float *M0, const float *s,
-                 const int64_t *rows, double *ll_out, void *stream);
+int jk_lnl_mixed(const jk_star *star, int64_t n, const double *P, const double *e,
+                 const double *omega, const double *M0, const double *s,
+                 const int64_t *rows, double *ll_out, float *bound_out, void *stream);
 
 /* FP64 re-evaluation of the live set after a mixed-precision pass: every
  * entry i >= first of ll_inout (n rows, as in jk_lnl_mixed) that is not below
@@ -226,14 +231,16 @@ int jk_batch_create_slots(int capacity, int L, int jitter_mode, jk_batch **out);
 int jk_batch_set_star(jk_batch *batch, int slot, const jk_star *star, void *stream);
 
 /* Persistent work-queue launch over device-resident work items.
- * precision 0: FP64 (e, omega, M0, s are double columns); 1: mixed (float
- * columns).  block: threads per workgroup (64, 128 or 256); grid: workgroups
- * (0 = 8 per compute unit).  workspace: NULL or 256 device bytes (the queue
- * counter; pass one to keep the call free of allocation). */
+ * precision 0: FP64; 1: mixed (as jk_lnl_mixed).  The columns are the FP64
+ * ones in both cases.  bound_out (mixed only, optional): per-row error bounds as in
+ * jk_lnl_mixed, at the same offsets as ll_out.  block: threads per workgroup
+ * (64, 128 or 256); grid: workgroups (0 = 8 per compute unit).  workspace:
+ * NULL or 256 device bytes (the queue counter; pass one to keep the call free
+ * of allocation). */
 int jk_lnl_batch(const jk_batch *batch, int precision, int64_t n_items,
                  const jk_work_item *items, const double *P, const void *e, const void *omega,
                  const void *M0, const void *s, const int64_t *rows, double *ll_out,
-                 int block, int grid, void *workspace, void *stream);
+                 float *bound_out, int block, int grid, void *workspace, void *stream);
 
 /* Segmented rejection step (jk_reject per segment, one launch sequence for
  * all): segs is a HOST array of n_seg descriptors; n_pass_out[i] and
@@ -262,6 +269,35 @@ int jk_polish_batch(const jk_batch *batch, int n_seg, const jk_polish_segment *s
                     const double *P, const double *e, const double *omega, const double *M0,
                     const double *s, const int64_t *rows, double *ll_inout, double window,
                     int64_t *n_polished_out, void *stream);
+
+/* One segment of jk_settle_batch: ll[offset .. offset + n) and
+ * bound[offset .. offset + n) of star `star` (star-table index) on cache rows
+ * row_begin .. (or rows[row_begin ..]), and the numpy PCG64 stream its
+ * rejection step will draw (state / inc / stream_offset as in jk_segment). */
+typedef struct {
+    int64_t offset;
+    int64_t n;
+    int32_t star;
+    int32_t reserved;
+    int64_t row_begin;
+    uint64_t state_hi, state_lo, inc_hi, inc_lo;
+    uint64_t stream_offset;
+} jk_settle_segment;
+
+/* Make a mixed-precision lnL buffer decide like the FP64 one.  For every
+ * segment, with bound (from jk_lnl_batch / jk_lnl_mixed; 0 for rows already
+ * FP64) bounding |lnL - lnL(FP64)| per row, the rows that could hold the
+ * FP64 maximum, and then the rows whose rejection decision
+ * exp(lnL - max) > u (u: the segment's uniform for that row) the bound leaves
+ * open, are re-evaluated in FP64 in place and their bound set to 0.  A
+ * following jk_reject_batch on the same segments and streams then accepts
+ * exactly the rows the FP64 buffer would.  n_settled_out[i] (device) counts
+ * segment i's re-evaluated rows.  segs is a HOST array; FP64 columns as in
+ * jk_lnl_f64.  No reference counterpart (the reference has one precision). */
+int jk_settle_batch(const jk_batch *batch, int n_seg, const jk_settle_segment *segs,
+                    const double *P, const double *e, const double *omega, const double *M0,
+                    const double *s, const int64_t *rows, double *ll_inout, float *bound_inout,
+                    int64_t *n_settled_out, void *stream);
 
 /* Fill rows [row0, row0 + n) of a device prior cache (arrays indexed from
  * 0 for row row0) with draws of the default nonlinear prior

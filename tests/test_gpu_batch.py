@@ -205,3 +205,72 @@ def test_posterior_inputs_many_equals_single(setup):
         want = TheJoker._posterior_inputs(st, cache, r)
         for a, b in zip(got, want):
             assert a.shape == b.shape and np.array_equal(a, b)
+
+
+def _bound_check(dstar, cache, n, cols=None, label=""):
+    mix = torch.empty(n, dtype=torch.float64, device="cuda")
+    bnd = torch.empty(n, dtype=torch.float32, device="cuda")
+    D.lnl(dstar, cache, n=n, precision="mixed", out=mix, bound=bnd)
+    f64 = D.lnl(dstar, cache, n=n, precision="f64").cpu().numpy()
+    m, b = mix.cpu().numpy(), bnd.cpu().numpy().astype(np.float64)
+    ok = np.isfinite(f64)
+    d = np.abs(m - f64)
+    r = np.where(ok, d / b, 0.0)
+    k = int(np.argmax(r))
+    info = f"{label} row {k}: ratio {r[k]:.3g} lnL64 {f64[k]!r} mixed {m[k]!r} bound {b[k]!r}"
+    if cols is not None:
+        info += " " + " ".join(f"{c}={cols[c][k]!r}" for c in ("P", "e", "omega", "M0", "s"))
+    return d[ok], b[ok], info
+
+
+def test_mixed_error_bound_holds(setup):
+    """|lnL(mixed) - lnL(FP64)| <= the kernel's per-row bound, on every row of
+    binaries and constant-velocity stars (L = 3, physical jitter)."""
+    stars, cache, cols = setup
+    ratios = []
+    for i, (_, dstar) in enumerate(stars):
+        d, b, info = _bound_check(dstar, cache, len(cache), cols, f"star {i}")
+        assert np.all(d <= b), info
+        ratios.append((d / b).max())
+    assert max(ratios) < 0.5
+
+
+def test_mixed_error_bound_holds_high_eccentricity(gpu):
+    """The same near e -> 1, where the FP32 solve degrades the most."""
+    from thejoker_amd import synthetic as syn
+    st = syn.single_star()
+    cols = syn.prior_columns(1 << 16, seed=77, jitter=True)
+    g = np.random.default_rng(2)
+    cols["e"] = 1.0 - 10.0 ** g.uniform(-6.5, -0.5, len(cols["e"]))
+    cache = D.PriorCache(cols["P"], cols["e"], cols["omega"], cols["M0"], cols["s"])
+    dstar = D.DeviceStar(st.t, st.rv, st.ivar(), st.trend(), np.zeros(2), np.array([900.0, 1e4]),
+                         st.t_ref)
+    d, b, info = _bound_check(dstar, cache, len(cache), cols, "high e")
+    assert np.all(d <= b), info
+
+
+@pytest.mark.parametrize("max_keep", [64, 1 << 20])
+def test_settle_then_reject_equals_fp64(setup, max_keep):
+    """lnl_batch (mixed, with bounds) + settle_batch + reject_batch accepts
+    exactly what the FP64 buffer accepts, for every star, re-evaluating a
+    small fraction of the rows."""
+    stars, cache, _ = setup
+    batch = B.StarBatch([d for _, d in stars])
+    n = len(cache)
+    items, off, total = B.work_items([n] * len(stars), 1024)
+    d_items = B.upload_items(items)
+    ll64 = B.lnl_batch(batch, cache, d_items, len(items), total, "f64")
+    bound = torch.empty(total, dtype=torch.float32, device="cuda")
+    llm = B.lnl_batch(batch, cache, d_items, len(items), total, "mixed", bound=bound)
+    states = [D.pcg64_state(np.random.default_rng(900 + i)) for i in range(len(stars))]
+    sizes = np.full(len(stars), n)
+    cnt = B.settle_batch(batch, cache, llm, bound, off, sizes, states).cpu().numpy()
+    want, npw, mxw = B.reject_batch(ll64, off, sizes, states, max_keep)
+    got, npg, mxg = B.reject_batch(llm, off, sizes, states, max_keep)
+    for i in range(len(stars)):
+        assert np.array_equal(got[i], want[i]), i
+    assert np.array_equal(npg, npw) and np.array_equal(mxg, mxw)
+    assert cnt.sum() < 0.2 * total
+    # settled rows carry their FP64 values and a zero bound
+    b = bound.cpu().numpy()
+    assert np.array_equal(llm.cpu().numpy()[b == 0], ll64.cpu().numpy()[b == 0])

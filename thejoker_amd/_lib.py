@@ -10,7 +10,7 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libjoker.so")
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 JK_OK = 0
 JK_JITTER_REFERENCE = 0
@@ -38,7 +38,7 @@ SIGNATURES = [
       ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.POINTER(_vp)]),
     ("jk_star_destroy", ctypes.c_int, [_vp]),
     ("jk_lnl_f64", ctypes.c_int, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
-    ("jk_lnl_mixed", ctypes.c_int, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("jk_lnl_mixed", ctypes.c_int, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("jk_max_f64", ctypes.c_int, [_vp, _i64, _vp, _vp]),
     ("jk_polish_f64", ctypes.c_int,
      [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _i64, _vp, _vp]),
@@ -56,11 +56,13 @@ SIGNATURES = [
                                               ctypes.POINTER(_vp)]),
     ("jk_batch_set_star", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
     ("jk_lnl_batch", ctypes.c_int,
-     [_vp, ctypes.c_int, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
-      _vp, _vp]),
+     [_vp, ctypes.c_int, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int,
+      ctypes.c_int, _vp, _vp]),
     ("jk_reject_batch", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]),
     ("jk_polish_batch", ctypes.c_int,
      [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _vp, _vp]),
+    ("jk_settle_batch", ctypes.c_int,
+     [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("jk_prior_fill", ctypes.c_int,
      [_i64, _i64, _u64, ctypes.c_double, ctypes.c_double, _vp, ctypes.c_int, ctypes.c_double,
       ctypes.c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

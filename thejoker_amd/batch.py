@@ -18,14 +18,18 @@ from . import _lib
 from .device import (DeviceStar, _dptr, _split, _stream, _torch, pcg64_advance_state, pcg64_state,
                      to_host)
 
-__all__ = ["WORK_ITEM_DTYPE", "SEGMENT_DTYPE", "POLISH_SEGMENT_DTYPE", "StarBatch",
-           "work_items", "lnl_batch", "reject_batch", "polish_batch", "autotune", "LaunchShape",
-           "rejection_sample_many"]
+__all__ = ["WORK_ITEM_DTYPE", "SEGMENT_DTYPE", "POLISH_SEGMENT_DTYPE", "SETTLE_SEGMENT_DTYPE",
+           "StarBatch", "work_items", "lnl_batch", "reject_batch", "polish_batch", "settle_batch",
+           "autotune", "LaunchShape", "rejection_sample_many"]
 
 WORK_ITEM_DTYPE = np.dtype([("star", "<i4"), ("n_rows", "<i4"), ("row_begin", "<i8"),
                             ("out_offset", "<i8")])
 POLISH_SEGMENT_DTYPE = np.dtype([("offset", "<i8"), ("n", "<i8"), ("star", "<i4"),
                                  ("reserved", "<i4"), ("row_begin", "<i8"), ("first", "<i8")])
+SETTLE_SEGMENT_DTYPE = np.dtype([("offset", "<i8"), ("n", "<i8"), ("star", "<i4"),
+                                 ("reserved", "<i4"), ("row_begin", "<i8"), ("state_hi", "<u8"),
+                                 ("state_lo", "<u8"), ("inc_hi", "<u8"), ("inc_lo", "<u8"),
+                                 ("stream_offset", "<u8")])
 SEGMENT_DTYPE = np.dtype([("offset", "<i8"), ("n", "<i8"), ("max_keep", "<i8"),
                           ("idx_offset", "<i8"), ("state_hi", "<u8"), ("state_lo", "<u8"),
                           ("inc_hi", "<u8"), ("inc_lo", "<u8"), ("stream_offset", "<u8")])
@@ -141,21 +145,24 @@ def upload_items(items, device=0):
 
 
 def lnl_batch(batch, cache, d_items, n_items, n_out, precision="mixed", shape=None, out=None,
-              rows=None, workspace=None):
-    """lnL of every work item into a device float64 tensor of n_out entries."""
+              rows=None, workspace=None, bound=None):
+    """lnL of every work item into a device float64 tensor of n_out entries;
+    with `bound` (mixed only, a device float32 tensor indexed like `out`) each
+    row's bound on |lnL(mixed) - lnL(FP64)| as well."""
     torch = _torch()
     lib = _lib.ensure_device(cache.device)
     shape = shape or LaunchShape()
     if out is None:
         out = torch.empty(n_out, dtype=torch.float64, device=cache.P.device)
+    cols = (cache.e, cache.omega, cache.M0, cache.s)
     if precision == "f64":
-        cols, prec = (cache.e, cache.omega, cache.M0, cache.s), 0
+        prec = 0
     elif precision == "mixed":
-        cols, prec = cache.fp32, 1
+        prec = 1
     else:
         raise ValueError("precision must be 'f64' or 'mixed'")
     _lib.check(lib.jk_lnl_batch(batch.handle, prec, int(n_items), _dptr(d_items), _dptr(cache.P),
-                                *[_dptr(c) for c in cols], _dptr(rows), _dptr(out),
+                                *[_dptr(c) for c in cols], _dptr(rows), _dptr(out), _dptr(bound),
                                 shape.block, shape.grid, _dptr(workspace),
                                 _stream(cache.device)), "jk_lnl_batch")
     return out
@@ -220,6 +227,35 @@ def polish_batch(batch, cache, ll, offsets, sizes, stars=None, row_begins=None, 
     return count
 
 
+def settle_batch(batch, cache, ll, bound, offsets, sizes, rng_states, stars=None,
+                 row_begins=None, rows=None, stream_offsets=None, device=0):
+    """jk_settle_batch: re-evaluate in FP64 the rows of each mixed-precision
+    segment whose rejection decision (with that segment's numpy PCG64 state,
+    as passed to :func:`reject_batch` next) the per-row bounds leave open, and
+    the rows that could hold the FP64 maximum.  Returns a device int64 tensor
+    of per-segment counts."""
+    torch = _torch()
+    lib = _lib.ensure_device(device)
+    n_seg = len(sizes)
+    segs = np.zeros(n_seg, dtype=SETTLE_SEGMENT_DTYPE)
+    segs["offset"] = offsets
+    segs["n"] = sizes
+    segs["star"] = np.arange(n_seg) if stars is None else stars
+    segs["row_begin"] = 0 if row_begins is None else row_begins
+    for i, (st, inc) in enumerate(rng_states):
+        segs["state_hi"][i], segs["state_lo"][i] = _split(int(st))
+        segs["inc_hi"][i], segs["inc_lo"][i] = _split(int(inc))
+    if stream_offsets is not None:
+        segs["stream_offset"] = stream_offsets
+    count = torch.zeros(n_seg, dtype=torch.int64, device=ll.device)
+    _lib.check(lib.jk_settle_batch(batch.handle, n_seg, segs.ctypes.data_as(ctypes.c_void_p),
+                                   _dptr(cache.P), _dptr(cache.e), _dptr(cache.omega),
+                                   _dptr(cache.M0), _dptr(cache.s), _dptr(rows), _dptr(ll),
+                                   _dptr(bound), _dptr(count), _stream(device)),
+               "jk_settle_batch")
+    return count
+
+
 def autotune(batch, cache, n_rows, precision="mixed", candidates=None, sample_stars=64,
              reps=3, device=0):
     """Time candidate launch shapes on a subset of the workload (the first
@@ -266,7 +302,7 @@ def rejection_sample_many(datas, prior, prior_samples, rngs, max_posterior_sampl
     (and the per-star lnL arrays with return_all_logprobs)."""
     from .device import PriorCache, linear_internal_units, star_arrays
     from .likelihood_helpers import validate_prepare_data
-    from .thejoker import POLISH_WINDOW, TheJoker
+    from .thejoker import TheJoker
 
     torch = _torch()
     stars, units, all_data = [], [], []
@@ -295,26 +331,29 @@ def rejection_sample_many(datas, prior, prior_samples, rngs, max_posterior_sampl
     shape = shape or LaunchShape()
     items, off, total = work_items([n] * len(stars), shape.tile)
     items = items[heaviest_first(items, batch.M)]
-    ll = lnl_batch(batch, cache, upload_items(items, device), len(items), total, precision, shape)
-    if precision == "mixed":
-        polish_batch(batch, cache, ll, off, np.full(len(stars), n), window=POLISH_WINDOW,
-                     device=device)
+    bound = (torch.empty(total, dtype=torch.float32, device=cache.P.device)
+             if precision == "mixed" else None)
+    ll = lnl_batch(batch, cache, upload_items(items, device), len(items), total, precision, shape,
+                   bound=bound)
     states = [pcg64_state(g) for g in rngs]
+    if precision == "mixed":
+        # FP64 re-evaluation of exactly the rows whose decision the mixed
+        # likelihood's error bound leaves open
+        settle_batch(batch, cache, ll, bound, off, np.full(len(stars), n), states, device=device)
     good, _, _ = reject_batch(ll, off, np.full(len(stars), n), states, max_posterior_samples,
                               device=device)
     for g in rngs:
         pcg64_advance_state(g, n)
-    ll_h = ll.cpu().numpy() if (return_logprobs or return_all_logprobs) else None
+    ll_h = ll.cpu().numpy() if return_all_logprobs else None
     helper = TheJoker(prior, rng=np.random.default_rng(0), jitter_mode=jitter_mode,
                       precision=precision, device=device)
     out = []
     for s, st in enumerate(stars):
         samp = helper._full_samples(st, cache, good[s], rngs[s], n_linear_samples)
         if return_logprobs:
-            seg = ll_h[off[s]:off[s] + n]
             if cache.ln_prior is not None:
                 samp["ln_prior"] = cache.ln_prior[good[s]]
-            samp["ln_likelihood"] = seg[good[s]]
+            samp["ln_likelihood"] = TheJoker._fp64_at(st, cache, good[s])
         if return_all_logprobs:
             out.append((samp, ll_h[off[s]:off[s] + n].copy()))
         else:

@@ -10,7 +10,7 @@
 
 #include "jk_internal.h"
 
-#define JK_ABI_VERSION 2
+#define JK_ABI_VERSION 3
 
 namespace {
 
@@ -199,14 +199,14 @@ int jk_lnl_f64(const jk_star *star, int64_t n, const double *P, const double *e,
                                        (hipStream_t)stream));
 }
 
-int jk_lnl_mixed(const jk_star *star, int64_t n, const double *P, const float *e,
-                 const float *omega, const float *M0, const float *s, const int64_t *rows,
-                 double *ll_out, void *stream)
+int jk_lnl_mixed(const jk_star *star, int64_t n, const double *P, const double *e,
+                 const double *omega, const double *M0, const double *s, const int64_t *rows,
+                 double *ll_out, float *bound_out, void *stream)
 {
     if (!star || n < 0 || (n > 0 && (!P || !e || !omega || !M0 || !ll_out))) return JK_EINVAL;
     if (star->dev.jitter_mode == JK_JITTER_PHYSICAL && n > 0 && !s) return JK_EINVAL;
     if (current_device_inited() < 0) return JK_ENODEV;
-    return check_hip(jk_launch_lnl_mixed(star, n, P, e, omega, M0, s, rows, ll_out,
+    return check_hip(jk_launch_lnl_mixed(star, n, P, e, omega, M0, s, rows, ll_out, bound_out,
                                          (hipStream_t)stream));
 }
 
@@ -461,10 +461,11 @@ int jk_batch_destroy(jk_batch *b)
 
 int jk_lnl_batch(const jk_batch *batch, int precision, int64_t n_items, const jk_work_item *items,
                  const double *P, const void *e, const void *omega, const void *M0, const void *s,
-                 const int64_t *rows, double *ll_out, int block, int grid, void *workspace,
-                 void *stream)
+                 const int64_t *rows, double *ll_out, float *bound_out, int block, int grid,
+                 void *workspace, void *stream)
 {
     if (!batch || n_items < 0 || (precision != 0 && precision != 1)) return JK_EINVAL;
+    if (bound_out && precision != 1) return JK_EINVAL;
     if (n_items == 0) return JK_OK;
     if (!items || !P || !e || !omega || !M0 || !ll_out) return JK_EINVAL;
     if (batch->jitter_mode == JK_JITTER_PHYSICAL && !s) return JK_EINVAL;
@@ -482,7 +483,7 @@ int jk_lnl_batch(const jk_batch *batch, int precision, int64_t n_items, const jk
     void *ws = workspace;
     if (!ws && hipMallocAsync(&ws, 256, st) != hipSuccess) return JK_ENOMEM;
     hipError_t err = jk_launch_lnl_batch(batch, precision, n_items, items, P, e, omega, M0, s, rows,
-                                         ll_out, block, grid, (unsigned *)ws, st);
+                                         ll_out, bound_out, block, grid, (unsigned *)ws, st);
     hipError_t err2 = workspace ? hipSuccess : hipFreeAsync(ws, st);
     if (err != hipSuccess) return JK_EHIP;
     return check_hip(err2);
@@ -601,6 +602,74 @@ int jk_polish_batch(const jk_batch *batch, int n_seg, const jk_polish_segment *s
         err = jk_launch_polish_batch(batch, n_seg, d_segs, max_n, max_nb, partials, pb, mask, bsum,
                                      idx, llmax, cum, P, e, omega, M0, s, rows, ll_inout, window,
                                      n_polished_out, st);
+    // The host descriptor array must outlive the queued copy.
+    hipError_t err3 = hipStreamSynchronize(st);
+    hipError_t err2 = hipFreeAsync(scratch, st);
+    if (err != hipSuccess || err3 != hipSuccess) return JK_EHIP;
+    return check_hip(err2);
+}
+
+int jk_settle_batch(const jk_batch *batch, int n_seg, const jk_settle_segment *segs,
+                    const double *P, const double *e, const double *omega, const double *M0,
+                    const double *s, const int64_t *rows, double *ll_inout, float *bound_inout,
+                    int64_t *n_settled_out, void *stream)
+{
+    if (!batch || n_seg <= 0 || !segs || !ll_inout || !bound_inout || !n_settled_out)
+        return JK_EINVAL;
+    if (!P || !e || !omega || !M0) return JK_EINVAL;
+    if (batch->jitter_mode == JK_JITTER_PHYSICAL && !s) return JK_EINVAL;
+    if (current_device_inited() < 0) return JK_ENODEV;
+    std::vector<SegDev> h(n_seg);
+    int64_t max_n = 1, max_nb = 1, mask_words = 0, bsum_n = 0, idx_n = 0;
+    for (int i = 0; i < n_seg; ++i) {
+        const jk_settle_segment &g = segs[i];
+        if (g.n <= 0 || g.offset < 0 || g.row_begin < 0 || g.star < 0 || g.star >= batch->n_stars)
+            return JK_EINVAL;
+        SegDev &d = h[i];
+        memset(&d, 0, sizeof(d));
+        d.off = g.offset;
+        d.n = g.n;
+        d.max_keep = g.n;
+        d.idx_off = idx_n;
+        d.mask_off = mask_words;
+        d.nb = jk_reject_bsum_count(g.n);
+        d.bsum_off = bsum_n;
+        d.star = g.star;
+        d.row0 = g.row_begin;
+        u128 state = make_u128(g.state_hi, g.state_lo);
+        d.inc = make_u128(g.inc_hi, g.inc_lo);
+        jk_host_pcg_jump(state, d.inc, g.stream_offset + 1, &d.s_base);
+        idx_n += g.n;
+        mask_words += jk_reject_mask_words(g.n);
+        bsum_n += d.nb;
+        if (g.n > max_n) max_n = g.n;
+        if (d.nb > max_nb) max_nb = d.nb;
+    }
+    int pb = (int)((max_n + 4095) / 4096);
+    if (pb > 1024) pb = 1024;
+    if (pb < 1) pb = 1;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t seg_bytes = sizeof(SegDev) * n_seg;
+    const size_t seg_pad = ((seg_bytes + 255) / 256) * 256;
+    const size_t bytes = seg_pad + sizeof(double) * ((size_t)pb * n_seg + 2 * (size_t)n_seg) +
+                         sizeof(uint64_t) * mask_words +
+                         sizeof(int64_t) * (bsum_n + idx_n + 2 * (size_t)n_seg + 1);
+    char *scratch = nullptr;
+    if (hipMallocAsync((void **)&scratch, bytes, st) != hipSuccess) return JK_ENOMEM;
+    SegDev *d_segs = (SegDev *)scratch;
+    double *partials = (double *)(scratch + seg_pad);
+    double *lo = partials + (size_t)pb * n_seg;
+    double *llmax = lo + n_seg;
+    uint64_t *mask = (uint64_t *)(llmax + n_seg);
+    int64_t *bsum = (int64_t *)(mask + mask_words);
+    int64_t *idx = bsum + bsum_n;
+    int64_t *cum = idx + idx_n;
+    int64_t *count1 = cum + n_seg + 1;
+    hipError_t err = hipMemcpyAsync(d_segs, h.data(), seg_bytes, hipMemcpyHostToDevice, st);
+    if (err == hipSuccess)
+        err = jk_launch_settle_batch(batch, n_seg, d_segs, max_n, max_nb, partials, pb, mask, bsum,
+                                     idx, lo, llmax, cum, count1, P, e, omega, M0, s, rows,
+                                     ll_inout, bound_inout, n_settled_out, st);
     // The host descriptor array must outlive the queued copy.
     hipError_t err3 = hipStreamSynchronize(st);
     hipError_t err2 = hipFreeAsync(scratch, st);

@@ -135,24 +135,6 @@ __device__ __forceinline__ double jk_k_variance(const StarDev *__restrict__ st, 
     return v;
 }
 
-// ---------------------------------------------------------------------------
-// Per-sample accumulators.  The Keplerian column enters through
-//   Smm = sum w m^2,  Smr = sum w m r',  Smt[j] = sum w m T_j.
-// In the physical jitter mode w depends on the sample, so the trend-only
-// moments are accumulated too; in the reference mode they come from StarDev.
-// ---------------------------------------------------------------------------
-template <int L>
-struct Moments {
-    double Smm, Smr;
-    double Smt[L > 1 ? L - 1 : 1];
-    // physical jitter only
-    double Stt[L > 1 ? (L - 1) * (L - 1) : 1];
-    double Str[L > 1 ? L - 1 : 1];
-    double Srr;
-    double prod_w;   // running product of (1 + s^2 ivar_n), renormalised
-    int exp_w;       // binary exponent removed from prod_w
-};
-
 // Solve the L x L system and return lnL.
 //   G = [[1/lam0 + Smm, Smt], [Smt, diag(lam_inv) + Stt]],
 //   u = [Smr - mu0 Smm, Str - mu0 Smt],  chi0 = Srr - 2 mu0 Smr + mu0^2 Smm,
@@ -160,22 +142,36 @@ struct Moments {
 // where lnc + ln(scale) = N ln 2pi - sum ln w + sum ln Lambda: the sample-
 // dependent factors (the jitter product, a per-sample Lambda_0) arrive as
 // `scale` and share the one logarithm of the determinant.
+// With POST, post[0] = (G^-1)_00, post[1] = (G^-1 u)_0 (the posterior mean of
+// the K column minus its prior mean) and post[2] = chi0 - u^T G^-1 u (the
+// minimised chi^2) are also returned, for the mixed-precision error bound.
 // Returns +inf if the Cholesky factorisation breaks down, as the reference
 // returns INF when its LU of Ainv fails (fast_likelihood.pyx:283-291,381-382).
-template <int L>
+template <int L, bool POST = false>
 __device__ __forceinline__ double jk_finish_lnl(const double *G /* LxL, lower used */,
                                                 const double *u, double chi0, double lnc,
-                                                double scale)
+                                                double scale, double *post = nullptr)
 {
     if constexpr (L == 1) {
         const double g = G[0];
         if (!(g > 0.0)) return INFINITY;
-        return -0.5 * (chi0 - u[0] * u[0] / g + lnc + log(scale * g));
+        const double chi2 = chi0 - u[0] * u[0] / g;
+        if constexpr (POST) {
+            post[0] = 1.0 / g;
+            post[1] = u[0] / g;
+            post[2] = chi2;
+        }
+        return -0.5 * (chi2 + lnc + log(scale * g));
     } else if constexpr (L == 2) {
         const double g00 = G[0], g10 = G[2], g11 = G[3];
         const double det = fma(g00, g11, -g10 * g10);
         if (!(g00 > 0.0) || !(det > 0.0)) return INFINITY;
         const double q = (fma(g11, u[0] * u[0], g00 * u[1] * u[1]) - 2.0 * g10 * u[0] * u[1]) / det;
+        if constexpr (POST) {
+            post[0] = g11 / det;
+            post[1] = fma(g11, u[0], -g10 * u[1]) / det;
+            post[2] = chi0 - q;
+        }
         return -0.5 * (chi0 - q + lnc + log(scale * det));
     } else {
         // Cholesky G = R R^T without divisions: each pivot's 1/sqrt from the
@@ -216,6 +212,23 @@ __device__ __forceinline__ double jk_finish_lnl(const double *G /* LxL, lower us
             for (int k = 0; k < i; ++k) v = fma(-R[i * L + k], z[k], v);
             z[i] = v * rinv[i];
             zz = fma(z[i], z[i], zz);
+        }
+        if constexpr (POST) {
+            // ze = R^-1 e_0: (G^-1)_00 = |ze|^2 and (G^-1 u)_0 = ze . z
+            double ze[L];
+            double g00 = 0.0, a0 = 0.0;
+#pragma unroll
+            for (int i = 0; i < L; ++i) {
+                double v = i == 0 ? 1.0 : 0.0;
+#pragma unroll
+                for (int k = 0; k < i; ++k) v = fma(-R[i * L + k], ze[k], v);
+                ze[i] = v * rinv[i];
+                g00 = fma(ze[i], ze[i], g00);
+                a0 = fma(ze[i], z[i], a0);
+            }
+            post[0] = g00;
+            post[1] = a0;
+            post[2] = chi0 - zz;
         }
         return -0.5 * (chi0 - zz + lnc + (log(detm) + (double)dexp * 0.69314718055994530942));
     }

@@ -47,9 +47,9 @@ void jk_pcg_jump_coeffs(uint64_t k, u128 *A, u128 *D);
 hipError_t jk_launch_lnl_f64(const jk_star *st, int64_t n, const double *P, const double *e,
                              const double *om, const double *M0, const double *s,
                              const int64_t *rows, double *ll, hipStream_t stream);
-hipError_t jk_launch_lnl_mixed(const jk_star *st, int64_t n, const double *P, const float *e,
-                               const float *om, const float *M0, const float *s,
-                               const int64_t *rows, double *ll, hipStream_t stream);
+hipError_t jk_launch_lnl_mixed(const jk_star *st, int64_t n, const double *P, const double *e,
+                               const double *om, const double *M0, const double *s,
+                               const int64_t *rows, double *ll, float *bound, hipStream_t stream);
 hipError_t jk_launch_max(const double *ll, int64_t n, double *partials, int n_partials,
                          double *max_out, hipStream_t stream);
 int jk_max_partials_count(int64_t n);
@@ -78,7 +78,7 @@ int64_t jk_polish_scratch_bytes(int64_t n);
 hipError_t jk_launch_lnl_batch(const jk_batch *b, int precision, int64_t n_items,
                                const jk_work_item *items, const double *P, const void *e,
                                const void *om, const void *M0, const void *s,
-                               const int64_t *rows, double *ll, int block, int grid,
+                               const int64_t *rows, double *ll, float *bound, int block, int grid,
                                unsigned *counter, hipStream_t stream);
 hipError_t jk_launch_reject_batch(const double *ll, int n_seg, const SegDev *d_segs,
                                   int64_t max_n, int64_t max_nb, double *partials, int pb,
@@ -98,6 +98,13 @@ hipError_t jk_launch_prior_fill(int64_t n, int64_t row0, uint64_t seed, double P
                                 float *e32, float *om32, float *M032, float *s32,
                                 hipStream_t stream);
 int jk_seg_flag_blocks(int64_t max_n);
+hipError_t jk_launch_settle_batch(const jk_batch *b, int n_seg, const SegDev *d_segs,
+                                  int64_t max_n, int64_t max_nb, double *partials, int pb,
+                                  uint64_t *mask, int64_t *bsum, int64_t *idx, double *lo,
+                                  double *llmax, int64_t *cum, int64_t *count1, const double *P,
+                                  const double *e, const double *om, const double *M0,
+                                  const double *s, const int64_t *rows, double *ll, float *bound,
+                                  int64_t *n_settled, hipStream_t stream);
 hipError_t jk_launch_polish(const jk_star *st, int64_t n, const double *P, const double *e,
                             const double *om, const double *M0, const double *s,
                             const int64_t *rows, double *ll, double window, int64_t first,

@@ -284,26 +284,45 @@ __device__ __forceinline__ void jk_kepler_markley_f32(float x, float e, float om
 // ---------------------------------------------------------------------------
 // Fused marginal-likelihood kernel.
 // ---------------------------------------------------------------------------
-template <int L, bool PHYS, bool MIXED, bool SAFE = false>
+// With BOUND (mixed precision only), *bnd receives a bound on
+// |lnL(mixed) - lnL(FP64)| for the row.  First order in the design-column
+// error dm_n (the only FP32 quantity that reaches the FP64 moments):
+//   dlnL = a_K sum_n w_n r_n dm_n - sum_n w_n dm_n (G^-1 x_n)_0,
+// with a_K the posterior mean of K and r_n the residuals at the optimum, so by
+// Cauchy-Schwarz, with |dm_n| <= eps_n,
+//   |dlnL| <= sqrt(sum_n w_n eps_n^2) (|a_K| sqrt(chi2_min) + sqrt((G^-1)_00)).
+// eps_n = 2^-22 (1 + d_n)^2 (1 + d_n / 1000), d_n = 1 / (1 - e cos E_n), covers
+// the FP32 solve and design column with a factor of two to spare (emulated
+// over the Beta prior and e up to 1 - 1e-6); the bound is doubled again for
+// the terms of second order and given a relative floor of 1e-9.
+#define JK_BOUND_EPS 2.384185791015625e-07   // 2^-22
+
+template <int L, bool PHYS, bool MIXED, bool SAFE = false, bool BOUND = false>
 __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
                                              const double *__restrict__ P,
                                              const void *__restrict__ e_,
                                              const void *__restrict__ om_,
                                              const void *__restrict__ M0_,
                                              const void *__restrict__ s_,
-                                             const double2 *__restrict__ tab)
+                                             const double2 *__restrict__ tab,
+                                             float *bnd = nullptr)
 {
-    using TF = typename std::conditional<MIXED, float, double>::type;
-    const TF *__restrict__ ecol = static_cast<const TF *>(e_);
-    const TF *__restrict__ omcol = static_cast<const TF *>(om_);
-    const TF *__restrict__ M0col = static_cast<const TF *>(M0_);
-    const TF *__restrict__ scol = static_cast<const TF *>(s_);
+    static_assert(!BOUND || MIXED, "the error bound belongs to the mixed-precision row");
+    if constexpr (BOUND) *bnd = INFINITY;
+    // Both precisions read the FP64 columns: the kernels are bound by
+    // arithmetic, not by the prior stream, and an FP32 copy of e would carry
+    // a relative error of up to 2^-24 / (1 - e) into sqrt(1 - e^2), the scale
+    // of the whole design column as e -> 1.
+    const double *__restrict__ ecol = static_cast<const double *>(e_);
+    const double *__restrict__ omcol = static_cast<const double *>(om_);
+    const double *__restrict__ M0col = static_cast<const double *>(M0_);
+    const double *__restrict__ scol = static_cast<const double *>(s_);
 
     const double Pv = P[r];
-    const double ev = (double)ecol[r];
-    const double omv = (double)omcol[r];
-    const double M0v = (double)M0col[r];
-    const double sv = PHYS ? (double)scol[r] : 0.0;
+    const double ev = ecol[r];
+    const double omv = omcol[r];
+    const double M0v = M0col[r];
+    const double sv = PHYS ? scol[r] : 0.0;
 
     bool valid = isfinite(Pv) && Pv != 0.0 && ev >= 0.0 && ev < 1.0 && isfinite(omv) &&
                  isfinite(M0v);
@@ -322,17 +341,18 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
     // Per-sample orbit constants.
     double cw, ecw, swsq;
     float cwf, swf, ecwf, sqf;
-    // FP32 copies for the Kepler starter (e capped below 1 in FP32).
+    // FP32 copies for the Kepler starter (e capped below 1 in FP32; 1 - e
+    // rounded from its FP64 value, so it keeps its relative accuracy).
     const float ef = fminf((float)ev, 0.99999994f);
-    const float omef = 1.0f - ef;
+    const float omef = fmaxf((float)(1.0 - ev), 5.9604645e-08f);
     const float a1f = JK_MARKLEY_A1N * __builtin_amdgcn_rcpf(1.0f + ef);
     double sw;
     jk_sincos_tab_f64(omv, tab, sw, cw);
     if constexpr (MIXED) {
         swf = (float)sw;
         cwf = (float)cw;
-        ecwf = ef * cwf;
-        sqf = sqrtf(omef * (1.0f + ef));
+        ecwf = (float)(ev * cw);
+        sqf = (float)sqrt((1.0 - ev) * (1.0 + ev));
     } else {
         ecw = ev * cw;
         swsq = sw * sqrt((1.0 - ev) * (1.0 + ev));
@@ -357,11 +377,13 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
 
     const int N = st.n;
     bool bad = false;
+    float Sb = 0.0f;       // sum w eps^2 / JK_BOUND_EPS^2 (BOUND only)
 #pragma unroll 2
     for (int k = 0; k < N; ++k) {
         const double dtk = st.dt[k];
         const double x = jk_reduce_2pi(fma(nP, dtk, -M0v));
         double m;
+        float denb = 0.0f;
         if constexpr (MIXED) {
             float sE, cE;
             jk_kepler_markley_f32((float)x, ef, omef, a1f, sE, cE);
@@ -369,6 +391,7 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
             const float cf = (cE - ef) * den;
             const float sf = sqf * sE * den;
             m = (double)fmaf(cwf, cf, fmaf(-swf, sf, ecwf));
+            denb = den;
         } else {
             double sE, cE;
             if constexpr (SAFE)
@@ -390,6 +413,11 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
             pexp += ex;
         } else {
             w = st.ivar[k];
+        }
+        if constexpr (BOUND) {
+            const float t1 = 1.0f + denb;
+            const float ek = t1 * t1 * fmaf(1e-3f, denb, 1.0f);
+            Sb = fmaf((float)w, ek * ek, Sb);
         }
         const double wm = w * m;
         Smm = fma(wm, m, Smm);
@@ -450,7 +478,18 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
             }
         }
     }
-    return jk_finish_lnl<L>(G, u, chi0, lnc, scale);
+    if constexpr (BOUND) {
+        double post[3];
+        const double ll = jk_finish_lnl<L, true>(G, u, chi0, lnc, scale, post);
+        const double aK = fabs(mu0 + post[1]);
+        double b = 2.0 * JK_BOUND_EPS * sqrt((double)Sb) *
+                   fma(aK, sqrt(fmax(post[2], 0.0)), sqrt(fmax(post[0], 0.0)));
+        b = fma(1e-9, 1.0 + fabs(ll), b) * 1.000001;
+        *bnd = (isfinite(b) && isfinite(ll)) ? (float)b : INFINITY;
+        return ll;
+    } else {
+        return jk_finish_lnl<L>(G, u, chi0, lnc, scale);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -484,8 +523,9 @@ __device__ __forceinline__ int64_t wave_incl_scan_i64(int64_t v, int lane)
 // Dense launch: ll[i] = lnL(row rows[i] or i), one lane per row.  With
 // blockmax != nullptr each workgroup also writes the NaN-propagating maximum
 // of its rows to blockmax[blockIdx.x], so the rejection step need not read
-// the lnL array a second time to find the maximum.
-template <int L, bool PHYS, bool MIXED>
+// the lnL array a second time to find the maximum.  With BOUND (mixed
+// precision), bound[i] receives the row's error bound (jk_lnl_row).
+template <int L, bool PHYS, bool MIXED, bool BOUND>
 __global__ __launch_bounds__(256) void k_lnl(const StarDev st, int64_t n,
                                              const double *__restrict__ P,
                                              const void *__restrict__ e_,
@@ -494,7 +534,8 @@ __global__ __launch_bounds__(256) void k_lnl(const StarDev st, int64_t n,
                                              const void *__restrict__ s_,
                                              const int64_t *__restrict__ rows,
                                              double *__restrict__ ll,
-                                             double *__restrict__ blockmax)
+                                             double *__restrict__ blockmax,
+                                             float *__restrict__ bound)
 {
     __shared__ double2 s_tab[JK_TAB_N + 1];
     __shared__ double red[4];
@@ -503,7 +544,13 @@ __global__ __launch_bounds__(256) void k_lnl(const StarDev st, int64_t n,
     double v = -INFINITY;
     if (i < n) {
         const int64_t r = rows ? rows[i] : i;
-        v = jk_lnl_row<L, PHYS, MIXED>(st, r, P, e_, om_, M0_, s_, s_tab);
+        if constexpr (BOUND) {
+            float b;
+            v = jk_lnl_row<L, PHYS, MIXED, false, true>(st, r, P, e_, om_, M0_, s_, s_tab, &b);
+            bound[i] = b;
+        } else {
+            v = jk_lnl_row<L, PHYS, MIXED>(st, r, P, e_, om_, M0_, s_, s_tab);
+        }
         ll[i] = v;
     }
     if (blockmax) {
@@ -544,26 +591,35 @@ __global__ __launch_bounds__(256) void k_lnl_scatter_f64(const StarDev st,
 template <int L, bool PHYS, bool MIXED>
 static hipError_t launch_lnl_t(const jk_star *st, int64_t n, const double *P, const void *e,
                                const void *om, const void *M0, const void *s,
-                               const int64_t *rows, double *ll, double *blockmax,
+                               const int64_t *rows, double *ll, double *blockmax, float *bound,
                                hipStream_t stream)
 {
     const int64_t blocks = (n + 255) / 256;
-    hipLaunchKernelGGL((k_lnl<L, PHYS, MIXED>), dim3((unsigned)blocks), dim3(256), 0, stream,
-                       st->dev, n, P, e, om, M0, s, rows, ll, blockmax);
+    if constexpr (MIXED) {
+        if (bound) {
+            hipLaunchKernelGGL((k_lnl<L, PHYS, true, true>), dim3((unsigned)blocks), dim3(256), 0,
+                               stream, st->dev, n, P, e, om, M0, s, rows, ll, blockmax, bound);
+            return hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL((k_lnl<L, PHYS, MIXED, false>), dim3((unsigned)blocks), dim3(256), 0,
+                       stream, st->dev, n, P, e, om, M0, s, rows, ll, blockmax, nullptr);
     return hipGetLastError();
 }
 
 template <bool MIXED>
 static hipError_t launch_lnl_dispatch(const jk_star *st, int64_t n, const double *P, const void *e,
                                       const void *om, const void *M0, const void *s,
-                                      const int64_t *rows, double *ll, double *bm,
+                                      const int64_t *rows, double *ll, double *bm, float *bd,
                                       hipStream_t stream)
 {
     const bool phys = st->dev.jitter_mode == JK_JITTER_PHYSICAL;
-#define JK_CASE(LL)                                                                               \
-    case LL:                                                                                      \
-        return phys ? launch_lnl_t<LL, true, MIXED>(st, n, P, e, om, M0, s, rows, ll, bm, stream)  \
-                    : launch_lnl_t<LL, false, MIXED>(st, n, P, e, om, M0, s, rows, ll, bm, stream);
+#define JK_CASE(LL)                                                                                \
+    case LL:                                                                                       \
+        return phys                                                                                \
+                   ? launch_lnl_t<LL, true, MIXED>(st, n, P, e, om, M0, s, rows, ll, bm, bd, stream) \
+                   : launch_lnl_t<LL, false, MIXED>(st, n, P, e, om, M0, s, rows, ll, bm, bd,         \
+                                                    stream);
     switch (st->dev.L) {
         JK_CASE(1)
         JK_CASE(2)
@@ -584,15 +640,15 @@ hipError_t jk_launch_lnl_f64(const jk_star *st, int64_t n, const double *P, cons
                              const int64_t *rows, double *ll, hipStream_t stream)
 {
     if (n <= 0) return hipSuccess;
-    return launch_lnl_dispatch<false>(st, n, P, e, om, M0, s, rows, ll, nullptr, stream);
+    return launch_lnl_dispatch<false>(st, n, P, e, om, M0, s, rows, ll, nullptr, nullptr, stream);
 }
 
-hipError_t jk_launch_lnl_mixed(const jk_star *st, int64_t n, const double *P, const float *e,
-                               const float *om, const float *M0, const float *s,
-                               const int64_t *rows, double *ll, hipStream_t stream)
+hipError_t jk_launch_lnl_mixed(const jk_star *st, int64_t n, const double *P, const double *e,
+                               const double *om, const double *M0, const double *s,
+                               const int64_t *rows, double *ll, float *bound, hipStream_t stream)
 {
     if (n <= 0) return hipSuccess;
-    return launch_lnl_dispatch<true>(st, n, P, e, om, M0, s, rows, ll, nullptr, stream);
+    return launch_lnl_dispatch<true>(st, n, P, e, om, M0, s, rows, ll, nullptr, bound, stream);
 }
 
 
@@ -952,7 +1008,8 @@ hipError_t jk_launch_lnl_reject_f64(const jk_star *st, int64_t n, const double *
     double *partials2 = partials + blocks;      // second level, at most 1024
     uint64_t *mask = (uint64_t *)(partials2 + 1024);
     int64_t *bsum = (int64_t *)(mask + jk_reject_mask_words(n));
-    hipError_t err = launch_lnl_dispatch<false>(st, n, P, e, om, M0, s, rows, ll, partials, stream);
+    hipError_t err =
+        launch_lnl_dispatch<false>(st, n, P, e, om, M0, s, rows, ll, partials, nullptr, stream);
     if (err != hipSuccess) return err;
     // the per-workgroup maxima reduced like any array (two parallel levels)
     err = jk_launch_max(partials, blocks, partials2, jk_max_partials_count(blocks), llmax_out,
@@ -1037,7 +1094,7 @@ hipError_t jk_launch_polish(const jk_star *st, int64_t n, const double *P, const
 // ---------------------------------------------------------------------------
 __global__ void k_zero_u32(unsigned *p) { *p = 0u; }
 
-template <int L, bool PHYS, bool MIXED>
+template <int L, bool PHYS, bool MIXED, bool BOUND>
 __global__ __launch_bounds__(256) void k_lnl_batch(const StarDev *__restrict__ stars,
                                                    const jk_work_item *__restrict__ items,
                                                    int64_t n_items, unsigned *__restrict__ counter,
@@ -1047,7 +1104,8 @@ __global__ __launch_bounds__(256) void k_lnl_batch(const StarDev *__restrict__ s
                                                    const void *__restrict__ M0_,
                                                    const void *__restrict__ s_,
                                                    const int64_t *__restrict__ rows,
-                                                   double *__restrict__ ll)
+                                                   double *__restrict__ ll,
+                                                   float *__restrict__ bound)
 {
     __shared__ unsigned s_item;
     __shared__ double2 s_tab[JK_TAB_N + 1];
@@ -1063,7 +1121,14 @@ __global__ __launch_bounds__(256) void k_lnl_batch(const StarDev *__restrict__ s
         for (int j = threadIdx.x; j < w.n_rows; j += blockDim.x) {
             const int64_t q = w.row_begin + j;
             const int64_t r = rows ? rows[q] : q;
-            ll[w.out_offset + j] = jk_lnl_row<L, PHYS, MIXED>(st, r, P, e_, om_, M0_, s_, s_tab);
+            if constexpr (BOUND) {
+                float b;
+                ll[w.out_offset + j] =
+                    jk_lnl_row<L, PHYS, MIXED, false, true>(st, r, P, e_, om_, M0_, s_, s_tab, &b);
+                bound[w.out_offset + j] = b;
+            } else {
+                ll[w.out_offset + j] = jk_lnl_row<L, PHYS, MIXED>(st, r, P, e_, om_, M0_, s_, s_tab);
+            }
         }
     }
 }
@@ -1071,12 +1136,20 @@ __global__ __launch_bounds__(256) void k_lnl_batch(const StarDev *__restrict__ s
 template <int L, bool PHYS, bool MIXED>
 static void launch_batch_t(const jk_batch *b, int64_t n_items, const jk_work_item *items,
                            const double *P, const void *e, const void *om, const void *M0,
-                           const void *s, const int64_t *rows, double *ll, int block, int grid,
-                           unsigned *counter, hipStream_t stream)
+                           const void *s, const int64_t *rows, double *ll, float *bound,
+                           int block, int grid, unsigned *counter, hipStream_t stream)
 {
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, stream, counter);
-    hipLaunchKernelGGL((k_lnl_batch<L, PHYS, MIXED>), dim3(grid), dim3(block), 0, stream,
-                       b->d_stars, items, n_items, counter, P, e, om, M0, s, rows, ll);
+    if constexpr (MIXED) {
+        if (bound) {
+            hipLaunchKernelGGL((k_lnl_batch<L, PHYS, true, true>), dim3(grid), dim3(block), 0,
+                               stream, b->d_stars, items, n_items, counter, P, e, om, M0, s, rows,
+                               ll, bound);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_lnl_batch<L, PHYS, MIXED, false>), dim3(grid), dim3(block), 0, stream,
+                       b->d_stars, items, n_items, counter, P, e, om, M0, s, rows, ll, nullptr);
 }
 
 template <bool MIXED>
@@ -1084,18 +1157,18 @@ static hipError_t launch_batch_dispatch(const jk_batch *b, int64_t n_items,
                                         const jk_work_item *items, const double *P,
                                         const void *e, const void *om, const void *M0,
                                         const void *s, const int64_t *rows, double *ll,
-                                        int block, int grid, unsigned *counter,
+                                        float *bd, int block, int grid, unsigned *counter,
                                         hipStream_t stream)
 {
     const bool phys = b->jitter_mode == JK_JITTER_PHYSICAL;
 #define JK_BCASE(LL)                                                                              \
     case LL:                                                                                      \
         if (phys)                                                                                 \
-            launch_batch_t<LL, true, MIXED>(b, n_items, items, P, e, om, M0, s, rows, ll, block,   \
-                                            grid, counter, stream);                               \
+            launch_batch_t<LL, true, MIXED>(b, n_items, items, P, e, om, M0, s, rows, ll, bd,      \
+                                            block, grid, counter, stream);                        \
         else                                                                                      \
-            launch_batch_t<LL, false, MIXED>(b, n_items, items, P, e, om, M0, s, rows, ll, block,  \
-                                             grid, counter, stream);                              \
+            launch_batch_t<LL, false, MIXED>(b, n_items, items, P, e, om, M0, s, rows, ll, bd,     \
+                                             block, grid, counter, stream);                       \
         break;
     switch (b->L) {
         JK_BCASE(1)
@@ -1116,15 +1189,15 @@ static hipError_t launch_batch_dispatch(const jk_batch *b, int64_t n_items,
 hipError_t jk_launch_lnl_batch(const jk_batch *b, int precision, int64_t n_items,
                                const jk_work_item *items, const double *P, const void *e,
                                const void *om, const void *M0, const void *s,
-                               const int64_t *rows, double *ll, int block, int grid,
+                               const int64_t *rows, double *ll, float *bound, int block, int grid,
                                unsigned *counter, hipStream_t stream)
 {
     if (n_items <= 0) return hipSuccess;
     if (precision == 0)
-        return launch_batch_dispatch<false>(b, n_items, items, P, e, om, M0, s, rows, ll, block,
-                                            grid, counter, stream);
-    return launch_batch_dispatch<true>(b, n_items, items, P, e, om, M0, s, rows, ll, block, grid,
-                                       counter, stream);
+        return launch_batch_dispatch<false>(b, n_items, items, P, e, om, M0, s, rows, ll, nullptr,
+                                            block, grid, counter, stream);
+    return launch_batch_dispatch<true>(b, n_items, items, P, e, om, M0, s, rows, ll, bound, block,
+                                       grid, counter, stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -1358,7 +1431,8 @@ __global__ __launch_bounds__(256) void k_seg_scatter_f64(const StarDev *__restri
                                                          const void *__restrict__ M0_,
                                                          const void *__restrict__ s_,
                                                          const int64_t *__restrict__ rows,
-                                                         double *__restrict__ ll)
+                                                         double *__restrict__ ll,
+                                                         float *__restrict__ bound)
 {
     __shared__ double2 s_tab[JK_TAB_N + 1];
     jk_stage_tab(s_tab);
@@ -1381,6 +1455,7 @@ __global__ __launch_bounds__(256) void k_seg_scatter_f64(const StarDev *__restri
             const int64_t q = sg.row0 + p;
             const int64_t r = rows ? rows[q] : q;
             ll[sg.off + p] = jk_lnl_row<L, PHYS, false>(stars[sg.star], r, P, e_, om_, M0_, s_, s_tab);
+            if (bound) bound[sg.off + p] = 0.0f;
         }
     }
 }
@@ -1390,10 +1465,46 @@ static void launch_seg_scatter_t(const jk_batch *b, int n_seg, const SegDev *d_s
                                  const int64_t *count, const int64_t *cum, const int64_t *idx,
                                  int gx, const double *P, const void *e, const void *om,
                                  const void *M0, const void *s, const int64_t *rows, double *ll,
-                                 hipStream_t stream)
+                                 float *bound, hipStream_t stream)
 {
     hipLaunchKernelGGL((k_seg_scatter_f64<L, PHYS>), dim3(gx), dim3(256), 0, stream, b->d_stars,
-                       d_segs, n_seg, count, cum, idx, P, e, om, M0, s, rows, ll);
+                       d_segs, n_seg, count, cum, idx, P, e, om, M0, s, rows, ll, bound);
+}
+
+// FP64 re-evaluation of every segment's compacted list idx (counts in count)
+// as one flat launch; bound entries of re-evaluated rows are set to zero.
+static hipError_t launch_seg_scatter(const jk_batch *b, int n_seg, const SegDev *d_segs,
+                                     const int64_t *count, int64_t *cum, const int64_t *idx,
+                                     const double *P, const double *e, const double *om,
+                                     const double *M0, const double *s, const int64_t *rows,
+                                     double *ll, float *bound, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_seg_count_scan, dim3(1), dim3(256), 0, stream, count, n_seg, cum);
+    const int gx = 2048;
+    const bool phys = b->jitter_mode == JK_JITTER_PHYSICAL;
+#define JK_SCASE2(LL)                                                                             \
+    case LL:                                                                                      \
+        if (phys)                                                                                 \
+            launch_seg_scatter_t<LL, true>(b, n_seg, d_segs, count, cum, idx, gx, P, e, om, M0, s, \
+                                           rows, ll, bound, stream);                              \
+        else                                                                                      \
+            launch_seg_scatter_t<LL, false>(b, n_seg, d_segs, count, cum, idx, gx, P, e, om, M0,   \
+                                            s, rows, ll, bound, stream);                          \
+        break;
+    switch (b->L) {
+        JK_SCASE2(1)
+        JK_SCASE2(2)
+        JK_SCASE2(3)
+        JK_SCASE2(4)
+        JK_SCASE2(5)
+        JK_SCASE2(6)
+        JK_SCASE2(7)
+        JK_SCASE2(8)
+    default:
+        return hipErrorInvalidValue;
+    }
+#undef JK_SCASE2
+    return hipGetLastError();
 }
 
 hipError_t jk_launch_polish_batch(const jk_batch *b, int n_seg, const SegDev *d_segs,
@@ -1415,32 +1526,147 @@ hipError_t jk_launch_polish_batch(const jk_batch *b, int n_seg, const SegDev *d_
                        mask, bsum);
     hipLaunchKernelGGL(k_seg_compact, dim3((unsigned)max_nb, n_seg), dim3(256), 0, stream, d_segs,
                        mask, bsum, idx, n_polished);
-    hipLaunchKernelGGL(k_seg_count_scan, dim3(1), dim3(256), 0, stream, n_polished, n_seg, cum);
-    // The flagged entries of all segments form one list spread over the grid.
-    const int gx = 2048;
-    const bool phys = b->jitter_mode == JK_JITTER_PHYSICAL;
-#define JK_PCASE(LL)                                                                             \
-    case LL:                                                                                     \
-        if (phys)                                                                                \
-            launch_seg_scatter_t<LL, true>(b, n_seg, d_segs, n_polished, cum, idx, gx, P, e, om,   \
-                                           M0, s, rows, ll, stream);                             \
-        else                                                                                     \
-            launch_seg_scatter_t<LL, false>(b, n_seg, d_segs, n_polished, cum, idx, gx, P, e, om,  \
-                                            M0, s, rows, ll, stream);                            \
-        break;
-    switch (b->L) {
-        JK_PCASE(1)
-        JK_PCASE(2)
-        JK_PCASE(3)
-        JK_PCASE(4)
-        JK_PCASE(5)
-        JK_PCASE(6)
-        JK_PCASE(7)
-        JK_PCASE(8)
-    default:
-        return hipErrorInvalidValue;
+    return launch_seg_scatter(b, n_seg, d_segs, n_polished, cum, idx, P, e, om, M0, s, rows, ll,
+                              nullptr, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Settling a mixed-precision lnL buffer for the rejection step
+// (jk_settle_batch).  Per segment, with B the per-row bounds on
+// |lnL(mixed) - lnL(FP64)| (0 for rows already FP64):
+//   1. lo = max(ll - B) is a lower bound on the FP64 maximum; a row with
+//      ll + B >= lo could be the maximum: re-evaluate it in FP64 (B := 0).
+//      A NaN anywhere makes lo NaN and every unsettled row is re-evaluated.
+//   2. mx = max(ll) is then the FP64 maximum.
+//   3. u = the row's numpy uniform (the stream the rejection step draws); a
+//      row whose decision exp(x - mx) > u differs between x = ll - B and
+//      x = ll + B is re-evaluated in FP64 (B := 0).
+// Afterwards exp(ll - mx) > u holds for exactly the rows for which it holds
+// with every row's FP64 likelihood.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_seg_lo_partial(const double *__restrict__ ll,
+                                                        const float *__restrict__ bound,
+                                                        const SegDev *__restrict__ segs,
+                                                        double *__restrict__ partials, int pb)
+{
+    __shared__ double red[4];
+    const SegDev sg = segs[blockIdx.y];
+    double m = -INFINITY;
+    const int64_t stride = (int64_t)pb * 256;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < sg.n; i += stride)
+        m = jk_nanmax(m, ll[sg.off + i] - (double)bound[sg.off + i]);
+    m = wave_nanmax(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        partials[(int64_t)blockIdx.y * pb + blockIdx.x] =
+            jk_nanmax(jk_nanmax(red[0], red[1]), jk_nanmax(red[2], red[3]));
+}
+
+__global__ __launch_bounds__(256) void k_seg_shell_flags(const double *__restrict__ ll,
+                                                         const float *__restrict__ bound,
+                                                         const SegDev *__restrict__ segs,
+                                                         const double *__restrict__ lo,
+                                                         uint64_t *__restrict__ mask)
+{
+    const SegDev sg = segs[blockIdx.y];
+    const int lane = threadIdx.x & 63;
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t T = (int64_t)gridDim.x * 256;
+    if (g - lane >= sg.n) return;
+    const double l0 = lo[blockIdx.y];
+    for (int64_t base = g - lane; base < sg.n; base += T) {
+        const int64_t i = base + lane;
+        bool keep = false;
+        if (i < sg.n) {
+            const float b = bound[sg.off + i];
+            keep = b > 0.0f && !(ll[sg.off + i] + (double)b < l0);
+        }
+        const uint64_t word = __ballot(keep);
+        if (lane == 0) mask[sg.mask_off + (base >> 6)] = word;
     }
-#undef JK_PCASE
+}
+
+__global__ __launch_bounds__(256) void k_seg_uncertain_flags(const double *__restrict__ ll,
+                                                             const float *__restrict__ bound,
+                                                             const SegDev *__restrict__ segs,
+                                                             const double *__restrict__ llmax,
+                                                             u128 AT, u128 DT,
+                                                             uint64_t *__restrict__ mask)
+{
+    const SegDev sg = segs[blockIdx.y];
+    const int lane = threadIdx.x & 63;
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t T = (int64_t)gridDim.x * 256;
+    if (g - lane >= sg.n) return;
+    const double mx = llmax[blockIdx.y];
+    const bool live = !(mx != mx);     // a NaN maximum rejects every row anyway
+    u128 s = pcg_thread_start(sg.s_base, sg.inc);
+    for (int64_t base = g - lane; base < sg.n; base += T) {
+        const int64_t i = base + lane;
+        bool keep = false;
+        if (i < sg.n) {
+            const double uu = pcg_to_double(pcg_output(s));
+            const float bf = bound[sg.off + i];
+            if (live && bf > 0.0f) {
+                const double x = ll[sg.off + i];
+                const double b = fma((double)bf, 1e-6, (double)bf) + 1e-12;
+                keep = (exp(x - b - mx) > uu) != (exp(x + b - mx) > uu);
+            }
+        }
+        const uint64_t word = __ballot(keep);
+        if (lane == 0) mask[sg.mask_off + (base >> 6)] = word;
+        s = pcg_affine(s, AT, DT, sg.inc);
+    }
+}
+
+__global__ void k_seg_add_counts(const int64_t *__restrict__ a, int64_t *__restrict__ b, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] += a[i];
+}
+
+hipError_t jk_launch_settle_batch(const jk_batch *b, int n_seg, const SegDev *d_segs,
+                                  int64_t max_n, int64_t max_nb, double *partials, int pb,
+                                  uint64_t *mask, int64_t *bsum, int64_t *idx, double *lo,
+                                  double *llmax, int64_t *cum, int64_t *count1, const double *P,
+                                  const double *e, const double *om, const double *M0,
+                                  const double *s, const int64_t *rows, double *ll, float *bound,
+                                  int64_t *n_settled, hipStream_t stream)
+{
+    if (n_seg <= 0) return hipSuccess;
+    const int fb = jk_seg_flag_blocks(max_n);
+    // 1. the rows that could hold the FP64 maximum
+    hipLaunchKernelGGL(k_seg_lo_partial, dim3(pb, n_seg), dim3(256), 0, stream, ll, bound, d_segs,
+                       partials, pb);
+    hipLaunchKernelGGL(k_seg_max_final, dim3(n_seg), dim3(64), 0, stream, partials, pb, lo);
+    hipLaunchKernelGGL(k_seg_shell_flags, dim3(fb, n_seg), dim3(256), 0, stream, ll, bound, d_segs,
+                       lo, mask);
+    hipLaunchKernelGGL(k_seg_bsum, dim3((unsigned)max_nb, n_seg), dim3(256), 0, stream, d_segs,
+                       mask, bsum);
+    hipLaunchKernelGGL(k_seg_compact, dim3((unsigned)max_nb, n_seg), dim3(256), 0, stream, d_segs,
+                       mask, bsum, idx, count1);
+    hipError_t err = launch_seg_scatter(b, n_seg, d_segs, count1, cum, idx, P, e, om, M0, s, rows,
+                                        ll, bound, stream);
+    if (err != hipSuccess) return err;
+    // 2. the FP64 maximum
+    hipLaunchKernelGGL(k_seg_max_partial, dim3(pb, n_seg), dim3(256), 0, stream, ll, d_segs,
+                       partials, pb);
+    hipLaunchKernelGGL(k_seg_max_final, dim3(n_seg), dim3(64), 0, stream, partials, pb, llmax);
+    // 3. the rows whose decision the bound leaves open
+    u128 AT, DT;
+    jk_pcg_jump_coeffs((uint64_t)fb * 256, &AT, &DT);
+    hipLaunchKernelGGL(k_seg_uncertain_flags, dim3(fb, n_seg), dim3(256), 0, stream, ll, bound,
+                       d_segs, llmax, AT, DT, mask);
+    hipLaunchKernelGGL(k_seg_bsum, dim3((unsigned)max_nb, n_seg), dim3(256), 0, stream, d_segs,
+                       mask, bsum);
+    hipLaunchKernelGGL(k_seg_compact, dim3((unsigned)max_nb, n_seg), dim3(256), 0, stream, d_segs,
+                       mask, bsum, idx, n_settled);
+    err = launch_seg_scatter(b, n_seg, d_segs, n_settled, cum, idx, P, e, om, M0, s, rows, ll,
+                             bound, stream);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(k_seg_add_counts, dim3((n_seg + 255) / 256), dim3(256), 0, stream, count1,
+                       n_settled, n_seg);
     return hipGetLastError();
 }
 

@@ -311,9 +311,11 @@ class PriorCache:
 
 
 # ---------------------------------------------------------------------------
-def lnl(star, cache, rows=None, n=None, precision="f64", out=None, start=0):
+def lnl(star, cache, rows=None, n=None, precision="f64", out=None, start=0, bound=None):
     """Marginal lnL of cache rows (all, `rows` (device int64 tensor) or
-    [start, start + n)) into a device float64 tensor."""
+    [start, start + n)) into a device float64 tensor.  `bound` (mixed only): a
+    device float32 tensor of n entries that receives each row's bound on
+    |lnL(mixed) - lnL(FP64)|."""
     torch = _torch()
     lib = _lib.ensure_device(cache.device)
     if rows is not None:
@@ -325,20 +327,22 @@ def lnl(star, cache, rows=None, n=None, precision="f64", out=None, start=0):
     if n == 0:
         return out
     off64 = 8 * start
-    off32 = 4 * start
     if rows is not None:
-        off64 = off32 = 0
+        off64 = 0
     P = ctypes.c_void_p(cache.P.data_ptr() + off64)
     if precision == "f64":
         cols = [ctypes.c_void_p(c.data_ptr() + off64) for c in (cache.e, cache.omega, cache.M0, cache.s)]
         fn, what = lib.jk_lnl_f64, "jk_lnl_f64"
     elif precision == "mixed":
-        if cache.fp32 is None:
-            raise ValueError("this PriorCache has no FP32 copy")
-        cols = [ctypes.c_void_p(c.data_ptr() + off32) for c in cache.fp32]
-        fn, what = lib.jk_lnl_mixed, "jk_lnl_mixed"
+        # the FP64 columns too: the FP32 arithmetic starts inside the kernel
+        cols = [ctypes.c_void_p(c.data_ptr() + off64) for c in (cache.e, cache.omega, cache.M0, cache.s)]
+        _lib.check(lib.jk_lnl_mixed(star.handle, n, P, *cols, _dptr(rows), _dptr(out),
+                                    _dptr(bound), _stream(cache.device)), "jk_lnl_mixed")
+        return out
     else:
         raise ValueError("precision must be 'f64' or 'mixed'")
+    if bound is not None:
+        raise ValueError("the error bound belongs to the mixed-precision likelihood")
     _lib.check(fn(star.handle, n, P, *cols, _dptr(rows), _dptr(out), _stream(cache.device)), what)
     return out
 

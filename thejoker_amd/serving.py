@@ -7,9 +7,10 @@ a slot of one device star table (``jk_batch_create_slots``) until it
 finishes.  Every step evaluates the next batch of rows of *every* active
 request in one persistent ``jk_lnl_batch`` launch (stars of different epoch
 counts side by side) writing straight into each request's region of one HBM
-lnL arena, then re-evaluates the new rows of every request's live window in
-FP64 (``jk_polish_batch``) and runs every request's rejection step with its
-own numpy PCG64 stream (``jk_reject_batch``).  The iteration state of each
+lnL arena (mixed precision: with each row's error bound in a parallel
+arena), then re-evaluates in FP64 the rows whose rejection decision the bound
+leaves open (``jk_settle_batch``) and runs every request's rejection step with
+its own numpy PCG64 stream (``jk_reject_batch``).  The iteration state of each
 request is :class:`thejoker_amd.thejoker.IterativeRejection` -- the code the
 single-star API runs -- and the batched lnL is bit-identical to the
 single-star kernel, so a request's result is exactly what
@@ -19,12 +20,12 @@ import time
 
 import numpy as np
 
-from .batch import LaunchShape, StarBatch, heaviest_first, lnl_batch, polish_batch, reject_batch, \
+from .batch import LaunchShape, StarBatch, heaviest_first, lnl_batch, reject_batch, settle_batch, \
     upload_items, work_items
 from .device import (DeviceStar, _torch, linear_internal_units, pcg64_state, posterior_inputs_many,
                      star_arrays)
 from .likelihood_helpers import validate_prepare_data
-from .thejoker import POLISH_WINDOW, IterativeRejection, TheJoker
+from .thejoker import IterativeRejection, TheJoker
 
 __all__ = ["Request", "RangeAllocator", "JokerServer", "run_poisson"]
 
@@ -134,8 +135,12 @@ class JokerServer:
         self.finished = []
         self.steps = 0
         self.sample_epochs = 0
-        self.arena = torch.empty(int(hbm_budget_bytes) // 8, dtype=torch.float64,
+        # lnL (8 B) and, in mixed precision, its error bound (4 B) per row
+        per_row = 12 if precision == "mixed" else 8
+        self.arena = torch.empty(int(hbm_budget_bytes) // per_row, dtype=torch.float64,
                                  device=cache.P.device)
+        self.arena_b = (torch.empty(self.arena.shape[0], dtype=torch.float32,
+                                    device=cache.P.device) if precision == "mixed" else None)
         self.alloc = RangeAllocator(self.arena.shape[0])
         self._table = StarBatch.slots(self.max_active, 1 + prior.poly_trend + prior.n_offsets,
                                       jitter_mode, device=device)
@@ -229,6 +234,8 @@ class JokerServer:
         n = r.it.n_evals
         if n:
             self.arena[new:new + n] = self.arena[start:start + n]
+            if self.arena_b is not None:
+                self.arena_b[new:new + n] = self.arena_b[start:start + n]
         self.alloc.release(start, cap)
         r.region = (new, new_cap)
         return True
@@ -290,21 +297,22 @@ class JokerServer:
             finished = self._build_results()
             self._mark("finish")
         lnl_batch(self._table, self.cache, d_items, len(items), 0, self.precision, self.shape,
-                  out=self.arena, workspace=self._ws)
+                  out=self.arena, workspace=self._ws, bound=self.arena_b)
         if not self.profile:
             finished = self._build_results()      # overlaps the launch above
         M = self._table.M[slots]
         self.sample_epochs += int(np.sum(sizes * M))
         seg_n = before + sizes
         self._mark("lnl")
+        states = [pcg64_state(r.rng) for r, _, _ in plan]
         if self.precision == "mixed":
-            # rows evaluated in earlier steps are FP64 where it matters already
-            polish_batch(self._table, self.cache, self.arena, seg_off, seg_n, stars=slots,
-                         window=POLISH_WINDOW, first=before, device=self.device)
+            # FP64 for the rows whose decision (under this round's uniforms)
+            # or claim to the maximum the mixed error bound leaves open
+            settle_batch(self._table, self.cache, self.arena, self.arena_b, seg_off, seg_n, states,
+                         stars=slots, device=self.device)
         self._mark("polish")
-        good, n_pass, llmax = reject_batch(self.arena, seg_off, seg_n,
-                                           [pcg64_state(r.rng) for r, _, _ in plan],
-                                           self.n_requested, device=self.device)
+        good, n_pass, llmax = reject_batch(self.arena, seg_off, seg_n, states, self.n_requested,
+                                           device=self.device)
         self._mark("reject")
         retired = []
         for k, (r, _, n_now) in enumerate(plan):

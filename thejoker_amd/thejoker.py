@@ -59,9 +59,11 @@ class TheJoker:
         They agree whenever s = 0 (the default prior).
     precision : {"f64", "mixed"}
         "f64": the plain FP64 kernel.  "mixed": FP32 Kepler / design column
-        with FP64 phase and accumulation; every row within 40 of the maximum
-        lnL is then re-evaluated in FP64, so the accepted set equals the FP64
-        one.
+        with FP64 phase and accumulation, plus a per-row bound on the
+        difference to FP64; the rows whose rejection decision (or claim to the
+        maximum) the bound leaves open are re-evaluated in FP64 before the
+        rejection step, so the accepted set equals the FP64 one.  Reported
+        ln_likelihood values of accepted samples are FP64.
     device : int
         HIP device ordinal.
     """
@@ -129,13 +131,34 @@ class TheJoker:
                         "containing prior samples, or must be a JokerSamples instance, not: "
                         f"{type(prior_samples)}")
 
-    def _lnl_rows(self, star, cache, rows=None, start=0, n=None):
-        """Device lnL at the current precision; mixed runs are re-evaluated in
-        FP64 on every row within 40 of the maximum."""
-        ll = lnl(star, cache, rows=rows, start=start, n=n, precision=self.precision)
-        if self.precision == "mixed":
-            ll = _polish(star, cache, ll, rows=rows, start=start)
-        return ll
+    def _lnl_rows(self, star, cache, rows=None, start=0, n=None, bound=None):
+        """Device lnL at the current precision (mixed: with each row's error
+        bound written to `bound`)."""
+        return lnl(star, cache, rows=rows, start=start, n=n, precision=self.precision,
+                   bound=bound)
+
+    @staticmethod
+    def _settle(star, cache, ll, bound, rng_state, rows=None, device=0):
+        """Mixed precision: re-evaluate in FP64 the rows of ll (cache rows
+        `rows`, or 0 ..) whose rejection decision under rng_state, or whose
+        claim to the maximum, the error bound leaves open (jk_settle_batch on
+        a one-star table)."""
+        from .batch import StarBatch, settle_batch
+        table = getattr(star, "_settle_table", None)
+        if table is None:
+            table = star._settle_table = StarBatch([star], device=device)
+        settle_batch(table, cache, ll, bound, [0], [int(ll.shape[0])], [rng_state], rows=rows,
+                     device=device)
+
+    @staticmethod
+    def _fp64_at(star, cache, cache_rows):
+        """FP64 lnL of the given cache rows (the values reported for accepted
+        samples, whatever the precision of the search)."""
+        import torch
+        if len(cache_rows) == 0:
+            return np.zeros(0)
+        r = torch.as_tensor(np.asarray(cache_rows, dtype=np.int64), device=cache.P.device)
+        return to_host(lnl(star, cache, rows=r, precision="f64"))
 
     def _n_batches(self, n_batches):
         if n_batches is not None:
@@ -237,7 +260,10 @@ class TheJoker:
             lls, good, _, _ = lnl_reject(star, cache, pcg64_state(rng), max_posterior_samples,
                                          rows=d_rows, start=0, n=n_use)
         else:
-            lls = self._lnl_rows(star, cache, rows=d_rows, start=0, n=n_use)
+            bound = torch.empty(n_use, dtype=torch.float32, device=cache.P.device)
+            lls = self._lnl_rows(star, cache, rows=d_rows, start=0, n=n_use, bound=bound)
+            self._settle(star, cache, lls, bound, pcg64_state(rng), rows=d_rows,
+                         device=self.device)
             good, _, _ = reject(lls, pcg64_state(rng), max_posterior_samples, device=self.device)
         pcg64_advance_state(rng, n_use)
         full_idx = rows[good] if rows is not None else good
@@ -247,7 +273,7 @@ class TheJoker:
         ll_good = None
         if (in_memory and ln_prior is not None and ln_prior is not False) or \
                 (not in_memory and return_logprobs):
-            ll_good = to_host(lls[torch.as_tensor(good, device=lls.device)])
+            ll_good = self._fp64_at(star, cache, full_idx)
         if in_memory:
             if ln_prior is not None and ln_prior is not False:
                 samples["ln_prior"] = ln_prior[good]
@@ -279,26 +305,35 @@ class TheJoker:
                                 randomize_prior_order=randomize_prior_order,
                                 init_batch_size=init_batch_size, growth_factor=growth_factor,
                                 device=cache.P.device)
-        all_ll = torch.empty(it.max_prior, dtype=torch.float64, device=cache.P.device)
+        dev = cache.P.device
+        all_ll = torch.empty(it.max_prior, dtype=torch.float64, device=dev)
+        mixed = self.precision == "mixed"
+        all_b = torch.empty(it.max_prior, dtype=torch.float32, device=dev) if mixed else None
         while not it.done:
             rows, start, n_now = it.next_batch()
-            part = all_ll[it.n_evals:it.n_evals + n_now]
+            a, b = it.n_evals, it.n_evals + n_now
+            part_b = all_b[a:b] if mixed else None
             if rows is not None:
-                part.copy_(self._lnl_rows(star, cache, rows=rows))
+                lnl(star, cache, rows=rows, precision=self.precision, out=all_ll[a:b],
+                    bound=part_b)
             else:
-                part.copy_(self._lnl_rows(star, cache, start=start, n=n_now))
-            it.absorb(all_ll[:it.n_evals + n_now], self.device)
+                lnl(star, cache, start=start, n=n_now, precision=self.precision,
+                    out=all_ll[a:b], bound=part_b)
+            if mixed:
+                self._settle(star, cache, all_ll[:b], all_b[:b], pcg64_state(self.rng),
+                             rows=it._d_all_idx, device=self.device)
+            it.absorb(all_ll[:b], self.device)
         full_idx, good = it.result()
         samples = self._full_samples(star, cache, full_idx, self.rng, n_linear_samples,
                                      spawn_batches=None if in_memory else self._n_batches(n_batches))
         if return_logprobs:
-            ll_h = all_ll[:it.n_evals].cpu().numpy()
+            ll_good = self._fp64_at(star, cache, full_idx)
             if in_memory:
                 if cache.ln_prior is not None:
                     samples["ln_prior"] = cache.ln_prior[full_idx]
-                    samples["ln_likelihood"] = ll_h[good]
+                    samples["ln_likelihood"] = ll_good
             else:
-                samples["ln_likelihood"] = ll_h[good]
+                samples["ln_likelihood"] = ll_good
                 samples["ln_prior"] = cache.ln_prior[full_idx]
         return samples
 
@@ -423,16 +458,3 @@ def mvn_draws(rng, means, covs, size):
     factor = u * np.sqrt(s)[:, None, :]
     z = rng.standard_normal((n, size, L))
     return means[:, None, :] + np.matmul(z, np.swapaxes(factor, 1, 2))
-
-
-# Re-evaluation window of the mixed path: rows whose mixed lnL is within this
-# of the mixed maximum are recomputed in FP64.  exp(-40) = 4e-18 is below the
-# smallest nonzero uniform (2^-53 = 1.1e-16), and the mixed-vs-FP64 error is
-# orders of magnitude below the margin.
-POLISH_WINDOW = 40.0
-
-
-def _polish(star, cache, ll, rows=None, start=0):
-    from .device import polish
-    polish(star, cache, ll, rows=rows, start=start, window=POLISH_WINDOW)
-    return ll
