@@ -281,6 +281,21 @@ def leg_batched(args, torch, np, D, syn):
     torch.cuda.synchronize()
     e2e_f_ms = (time.perf_counter() - t0) / 2 * 1e3
     flips = sum(len(np.setxor1d(x, y)) for x, y in zip(good_m, good_f))
+    # the same mixed step captured once in a HIP graph and replayed
+    t0 = time.perf_counter()
+    graph = B.StepGraph(batch, cache, d_items, len(items), total, off, sizes, states(), 256,
+                        "mixed", shape)
+    capture_s = time.perf_counter() - t0
+    graph.run(states())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(2):
+        good_g, _, _, _ = graph.run(states())
+    torch.cuda.synchronize()
+    graph_ms = (time.perf_counter() - t0) / 2 * 1e3
+    graph_flips = sum(len(np.setxor1d(x, y)) for x, y in zip(good_g, good_f))
+    graph.plan.close()
+    del graph
     res = {"stars": n_stars, "prior_rows": n, "epochs_total": int(M.sum()), "linear_params": 3,
            "launch_shape": shape.as_dict(), "autotune_s": tune_s,
            "autotune_ms": {k: round(v, 3) for k, v in table.items()},
@@ -290,6 +305,8 @@ def leg_batched(args, torch, np, D, syn):
            "mixed_speedup_vs_f64": f_ms / m_ms,
            "mixed_end_to_end_ms": e2e_ms,
            "mixed_end_to_end_sample_epochs_per_s": work / (e2e_ms * 1e-3),
+           "graph_step_ms": graph_ms, "graph_capture_s": capture_s,
+           "graph_accept_flips_vs_fp64": int(graph_flips),
            "f64_end_to_end_ms": e2e_f_ms,
            "f64_end_to_end_sample_epochs_per_s": work / (e2e_f_ms * 1e-3),
            "max_abs_dlnl_all": dmax_all, "max_abs_dlnl_live": dmax_live,
@@ -298,7 +315,8 @@ def leg_batched(args, torch, np, D, syn):
            "accept_flips_vs_fp64": int(flips),
            "accepted_total": int(np.minimum(np_f, 256).sum())}
     log(f"[bench] batched: mixed {m_ms:.2f} ms ({res['mixed_sample_epochs_per_s']:.3e}/s), f64 "
-        f"{f_ms:.2f} ms; end to end mixed {e2e_ms:.2f} ms, f64 {e2e_f_ms:.2f} ms; |dlnL| all "
+        f"{f_ms:.2f} ms; end to end mixed {e2e_ms:.2f} ms (graph replay {graph_ms:.2f} ms), "
+        f"f64 {e2e_f_ms:.2f} ms; |dlnL| all "
         f"{dmax_all:.2e} live {dmax_live:.2e}, max |dlnL|/bound {ratio:.3f}, settled "
         f"{settled[0] / total:.2e} of rows, flips {flips}")
     del llm, llf, cache, batch

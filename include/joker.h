@@ -299,6 +299,35 @@ int jk_settle_batch(const jk_batch *batch, int n_seg, const jk_settle_segment *s
                     const double *s, const int64_t *rows, double *ll_inout, float *bound_inout,
                     int64_t *n_settled_out, void *stream);
 
+/* A segment plan: the descriptors and scratch of jk_settle_batch and
+ * jk_reject_batch for a fixed layout of segments (jk_settle_segment: offsets,
+ * sizes, stars, first rows, first PCG64 streams), kept on the device, so that
+ * the launches of jk_seg_plan_settle / jk_seg_plan_reject allocate nothing,
+ * copy nothing from the host and never wait -- the form a HIP graph captures,
+ * to be replayed step after step.  max_keep: the rejection step's limit per
+ * segment (idx_out of jk_seg_plan_reject holds the segments' kept indices one
+ * after the other, min(n, max_keep) places each).  The batch must outlive the
+ * plan.  No reference counterpart. */
+typedef struct jk_seg_plan jk_seg_plan;
+int jk_seg_plan_create(const jk_batch *batch, int n_seg, const jk_settle_segment *segs,
+                       int64_t max_keep, jk_seg_plan **out);
+int jk_seg_plan_destroy(jk_seg_plan *plan);
+
+/* New numpy PCG64 streams for the plan's segments: streams holds n_seg groups
+ * of {state_hi, state_lo, inc_hi, inc_lo, stream_offset}.  Staged in pinned
+ * memory and copied on `stream`, ordered before the work queued there later
+ * (e.g. the next replay of a graph that holds the plan's launches); waits only
+ * for the previous update's copy. */
+int jk_seg_plan_set_streams(jk_seg_plan *plan, const uint64_t *streams, void *stream);
+
+/* jk_settle_batch and jk_reject_batch on the plan's segments. */
+int jk_seg_plan_settle(const jk_seg_plan *plan, const double *P, const double *e,
+                       const double *omega, const double *M0, const double *s,
+                       const int64_t *rows, double *ll_inout, float *bound_inout,
+                       int64_t *n_settled_out, void *stream);
+int jk_seg_plan_reject(const jk_seg_plan *plan, const double *ll, int64_t *idx_out,
+                       int64_t *n_pass_out, double *llmax_out, void *stream);
+
 /* Fill rows [row0, row0 + n) of a device prior cache (arrays indexed from
  * 0 for row row0) with draws of the default nonlinear prior
  * (prior.py:410-493): P log-uniform on [P_min, P_max] (distributions.py:

@@ -274,3 +274,36 @@ def test_settle_then_reject_equals_fp64(setup, max_keep):
     # settled rows carry their FP64 values and a zero bound
     b = bound.cpu().numpy()
     assert np.array_equal(llm.cpu().numpy()[b == 0], ll64.cpu().numpy()[b == 0])
+
+
+@pytest.mark.parametrize("precision", ["mixed", "f64"])
+def test_step_graph_replays_equal_eager(setup, precision):
+    """The graph-captured step (persistent lnL launch, settling, rejection on a
+    device-resident segment plan) replayed with new streams, with eager work
+    and waits between replays, gives the eager path's results every time."""
+    stars, cache, _ = setup
+    batch = B.StarBatch([d for _, d in stars])
+    n = len(cache)
+    shape = B.LaunchShape(256, 512, 1024)
+    items, off, total = B.work_items([n] * len(stars), shape.tile)
+    items = items[B.heaviest_first(items, batch.M)]
+    d_items = B.upload_items(items)
+    sizes = np.full(len(stars), n)
+
+    def states(k):
+        return [D.pcg64_state(np.random.default_rng(1000 * k + i)) for i in range(len(stars))]
+
+    graph = B.StepGraph(batch, cache, d_items, len(items), total, off, sizes, states(0), 128,
+                        precision, shape)
+    for k in range(1, 4):
+        got, npg, mxg, _ = graph.run(states(k))
+        bound = torch.empty(total, dtype=torch.float32, device="cuda") if precision == "mixed" else None
+        ll = B.lnl_batch(batch, cache, d_items, len(items), total, precision, shape, bound=bound)
+        if precision == "mixed":
+            B.settle_batch(batch, cache, ll, bound, off, sizes, states(k))
+        want, npw, mxw = B.reject_batch(ll, off, sizes, states(k), 128)
+        torch.cuda.synchronize()
+        for i in range(len(stars)):
+            assert np.array_equal(got[i], want[i]), (k, i)
+        assert np.array_equal(npg, npw) and np.array_equal(mxg, mxw)
+    graph.plan.close()

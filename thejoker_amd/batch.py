@@ -20,7 +20,7 @@ from .device import (DeviceStar, _dptr, _split, _stream, _torch, pcg64_advance_s
 
 __all__ = ["WORK_ITEM_DTYPE", "SEGMENT_DTYPE", "POLISH_SEGMENT_DTYPE", "SETTLE_SEGMENT_DTYPE",
            "StarBatch", "work_items", "lnl_batch", "reject_batch", "polish_batch", "settle_batch",
-           "autotune", "LaunchShape", "rejection_sample_many"]
+           "autotune", "LaunchShape", "rejection_sample_many", "SegPlan", "StepGraph"]
 
 WORK_ITEM_DTYPE = np.dtype([("star", "<i4"), ("n_rows", "<i4"), ("row_begin", "<i8"),
                             ("out_offset", "<i8")])
@@ -359,3 +359,134 @@ def rejection_sample_many(datas, prior, prior_samples, rngs, max_posterior_sampl
         else:
             out.append(samp)
     return out
+
+
+class SegPlan:
+    """jk_seg_plan: the settle and rejection steps of a fixed set of segments
+    with their descriptors and scratch on the device, so that their launches
+    can be captured in a HIP graph.  Results of a step: :meth:`results`."""
+
+    def __init__(self, batch, offsets, sizes, rng_states, max_keep, stars=None, row_begins=None,
+                 device=0):
+        torch = _torch()
+        lib = _lib.ensure_device(device)
+        self._lib, self.device, self.batch = lib, device, batch
+        n_seg = len(sizes)
+        self.n_seg = n_seg
+        self.sizes = np.asarray(sizes, dtype=np.int64)
+        segs = np.zeros(n_seg, dtype=SETTLE_SEGMENT_DTYPE)
+        segs["offset"] = offsets
+        segs["n"] = sizes
+        segs["star"] = np.arange(n_seg) if stars is None else stars
+        segs["row_begin"] = 0 if row_begins is None else row_begins
+        self._fill(segs, rng_states)
+        h = ctypes.c_void_p()
+        _lib.check(lib.jk_seg_plan_create(batch.handle, n_seg, segs.ctypes.data_as(ctypes.c_void_p),
+                                          int(max_keep), ctypes.byref(h)), "jk_seg_plan_create")
+        self._h = h
+        self.keep = np.minimum(self.sizes, int(max_keep))
+        self.keep_off = np.concatenate([[0], np.cumsum(self.keep)[:-1]]).astype(np.int64)
+        # [n_pass | llmax | n_settled | kept indices]
+        self.res = torch.zeros(3 * n_seg + max(int(self.keep.sum()), 1), dtype=torch.int64,
+                               device=torch.device("cuda", device))
+
+    @staticmethod
+    def _fill(segs, rng_states, stream_offsets=None):
+        for i, (st, inc) in enumerate(rng_states):
+            segs["state_hi"][i], segs["state_lo"][i] = _split(int(st))
+            segs["inc_hi"][i], segs["inc_lo"][i] = _split(int(inc))
+        if stream_offsets is not None:
+            segs["stream_offset"] = stream_offsets
+
+    def set_streams(self, rng_states, stream_offsets=None):
+        """New PCG64 streams, ordered on the current stream before later work."""
+        q = np.zeros((self.n_seg, 5), dtype=np.uint64)
+        for i, (st, inc) in enumerate(rng_states):
+            q[i, 0], q[i, 1] = _split(int(st))
+            q[i, 2], q[i, 3] = _split(int(inc))
+        if stream_offsets is not None:
+            q[:, 4] = stream_offsets
+        _lib.check(self._lib.jk_seg_plan_set_streams(self._h, q.ctypes.data_as(ctypes.c_void_p),
+                                                     _stream(self.device)),
+                   "jk_seg_plan_set_streams")
+
+    def settle(self, cache, ll, bound, rows=None):
+        p = self.res.data_ptr()
+        _lib.check(self._lib.jk_seg_plan_settle(
+            self._h, _dptr(cache.P), _dptr(cache.e), _dptr(cache.omega), _dptr(cache.M0),
+            _dptr(cache.s), _dptr(rows), _dptr(ll), _dptr(bound),
+            ctypes.c_void_p(p + 16 * self.n_seg), _stream(self.device)), "jk_seg_plan_settle")
+
+    def reject(self, ll):
+        p = self.res.data_ptr()
+        _lib.check(self._lib.jk_seg_plan_reject(
+            self._h, _dptr(ll), ctypes.c_void_p(p + 24 * self.n_seg), ctypes.c_void_p(p),
+            ctypes.c_void_p(p + 8 * self.n_seg), _stream(self.device)), "jk_seg_plan_reject")
+
+    def results(self):
+        """(kept index arrays, n_pass, llmax, rows settled in FP64) of the last
+        step, after waiting for it."""
+        h = to_host(self.res)
+        n = self.n_seg
+        n_pass = h[:n].copy()
+        mx = h[n:2 * n].view(np.float64).copy()
+        settled = h[2 * n:3 * n].copy()
+        idx = h[3 * n:]
+        out = [idx[o:o + min(int(k), int(npass))].copy()
+               for o, k, npass in zip(self.keep_off, self.keep, n_pass)]
+        return out, n_pass, mx, settled
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.jk_seg_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class StepGraph:
+    """One rejection-sampling step of many stars -- the persistent lnL launch
+    (with error bounds in mixed precision), the FP64 settling and the
+    segmented rejection step -- captured once in a HIP graph (through
+    torch.cuda.CUDAGraph) and replayed with new numpy PCG64 streams.  Nothing
+    in the step allocates, copies from the host or waits, so a replay is a
+    single graph launch."""
+
+    def __init__(self, batch, cache, d_items, n_items, total, offsets, sizes, rng_states,
+                 max_keep, precision="mixed", shape=None, device=0):
+        torch = _torch()
+        dev = torch.device("cuda", device)
+        self.precision, self.cache = precision, cache
+        self.ll = torch.empty(total, dtype=torch.float64, device=dev)
+        self.bound = (torch.empty(total, dtype=torch.float32, device=dev)
+                      if precision == "mixed" else None)
+        self.ws = torch.zeros(64, dtype=torch.int32, device=dev)
+        self.plan = SegPlan(batch, offsets, sizes, rng_states, max_keep, device=device)
+        shape = shape or LaunchShape()
+        if shape.grid <= 0:
+            props = torch.cuda.get_device_properties(device)
+            shape = LaunchShape(shape.block, 8 * props.multi_processor_count, shape.tile)
+
+        def body():
+            lnl_batch(batch, cache, d_items, n_items, total, precision, shape, out=self.ll,
+                      workspace=self.ws, bound=self.bound)
+            if precision == "mixed":
+                self.plan.settle(cache, self.ll, self.bound)
+            self.plan.reject(self.ll)
+
+        body()                      # warm up every launch path once, eagerly
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            body()
+        torch.cuda.synchronize()
+
+    def run(self, rng_states, stream_offsets=None):
+        """Replay the step with these streams; returns SegPlan.results()."""
+        self.plan.set_streams(rng_states, stream_offsets)
+        self.graph.replay()
+        return self.plan.results()

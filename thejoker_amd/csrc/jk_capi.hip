@@ -677,6 +677,190 @@ int jk_settle_batch(const jk_batch *batch, int n_seg, const jk_settle_segment *s
     return check_hip(err2);
 }
 
+// ---------------------------------------------------------------------------
+// Segment plans: the descriptors and scratch of jk_settle_batch and
+// jk_reject_batch for a fixed segment layout, kept on the device, so that
+// their launches allocate nothing, copy nothing from the host and never wait.
+// ---------------------------------------------------------------------------
+struct jk_seg_plan {
+    const jk_batch *batch;
+    int n_seg;
+    int device;
+    int64_t max_n, max_nb;
+    int pb;
+    std::vector<SegDev> h_settle, h_reject;   // host images (streams rewritten in place)
+    SegDev *pinned;                            // staging for the stream updates
+    hipEvent_t staged;                         // the last staging copy has been read
+    char *mem;                                 // device: descriptors + scratch
+    SegDev *d_settle, *d_reject;
+    double *partials, *lo, *llmax;
+    uint64_t *mask;
+    int64_t *bsum, *idx, *cum, *count1;
+};
+
+static void plan_fill_streams(jk_seg_plan *p, const uint64_t *streams)
+{
+    for (int i = 0; i < p->n_seg; ++i) {
+        const uint64_t *q = streams + 5 * (size_t)i;
+        u128 state = make_u128(q[0], q[1]);
+        u128 inc = make_u128(q[2], q[3]);
+        u128 base;
+        jk_host_pcg_jump(state, inc, q[4] + 1, &base);
+        p->h_settle[i].s_base = p->h_reject[i].s_base = base;
+        p->h_settle[i].inc = p->h_reject[i].inc = inc;
+    }
+}
+
+int jk_seg_plan_destroy(jk_seg_plan *p)
+{
+    if (!p) return JK_EINVAL;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipSetDevice(p->device);
+    hipError_t e1 = p->staged ? hipEventSynchronize(p->staged) : hipSuccess;
+    hipError_t e2 = p->mem ? hipFree(p->mem) : hipSuccess;
+    hipError_t e3 = p->pinned ? hipHostFree(p->pinned) : hipSuccess;
+    if (p->staged) (void)hipEventDestroy(p->staged);
+    (void)hipSetDevice(dev);
+    delete p;
+    return (e1 == hipSuccess && e2 == hipSuccess && e3 == hipSuccess) ? JK_OK : JK_EHIP;
+}
+
+int jk_seg_plan_create(const jk_batch *batch, int n_seg, const jk_settle_segment *segs,
+                       int64_t max_keep, jk_seg_plan **out)
+{
+    if (!batch || n_seg <= 0 || !segs || !out || max_keep < 0) return JK_EINVAL;
+    *out = nullptr;
+    const int dev = current_device_inited();
+    if (dev < 0) return JK_ENODEV;
+    jk_seg_plan *p = new (std::nothrow) jk_seg_plan();
+    if (!p) return JK_ENOMEM;
+    p->batch = batch;
+    p->n_seg = n_seg;
+    p->device = dev;
+    p->h_settle.resize(n_seg);
+    p->h_reject.resize(n_seg);
+    int64_t max_n = 1, max_nb = 1, mask_words = 0, bsum_n = 0, idx_n = 0, keep_n = 0;
+    std::vector<uint64_t> streams(5 * (size_t)n_seg);
+    for (int i = 0; i < n_seg; ++i) {
+        const jk_settle_segment &g = segs[i];
+        if (g.n <= 0 || g.offset < 0 || g.row_begin < 0 || g.star < 0 ||
+            g.star >= batch->n_stars) {
+            delete p;
+            return JK_EINVAL;
+        }
+        SegDev d;
+        memset(&d, 0, sizeof(d));
+        d.off = g.offset;
+        d.n = g.n;
+        d.mask_off = mask_words;
+        d.nb = jk_reject_bsum_count(g.n);
+        d.bsum_off = bsum_n;
+        d.star = g.star;
+        d.row0 = g.row_begin;
+        SegDev r = d;
+        d.max_keep = g.n;
+        d.idx_off = idx_n;
+        r.max_keep = g.n < max_keep ? g.n : max_keep;
+        r.idx_off = keep_n;
+        p->h_settle[i] = d;
+        p->h_reject[i] = r;
+        idx_n += g.n;
+        keep_n += r.max_keep;
+        mask_words += jk_reject_mask_words(g.n);
+        bsum_n += d.nb;
+        if (g.n > max_n) max_n = g.n;
+        if (d.nb > max_nb) max_nb = d.nb;
+        uint64_t *q = streams.data() + 5 * (size_t)i;
+        q[0] = g.state_hi; q[1] = g.state_lo; q[2] = g.inc_hi; q[3] = g.inc_lo;
+        q[4] = g.stream_offset;
+    }
+    plan_fill_streams(p, streams.data());
+    int pb = (int)((max_n + 4095) / 4096);
+    if (pb > 1024) pb = 1024;
+    if (pb < 1) pb = 1;
+    p->max_n = max_n;
+    p->max_nb = max_nb;
+    p->pb = pb;
+    const size_t seg_bytes = sizeof(SegDev) * n_seg;
+    const size_t seg_pad = ((seg_bytes + 255) / 256) * 256;
+    const size_t bytes = 2 * seg_pad +
+                         sizeof(double) * ((size_t)pb * n_seg + 2 * (size_t)n_seg) +
+                         sizeof(uint64_t) * mask_words +
+                         sizeof(int64_t) * (bsum_n + idx_n + 2 * (size_t)n_seg + 1);
+    if (hipMalloc((void **)&p->mem, bytes) != hipSuccess ||
+        hipHostMalloc((void **)&p->pinned, 2 * seg_bytes, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&p->staged, hipEventDisableTiming) != hipSuccess) {
+        jk_seg_plan_destroy(p);
+        return JK_ENOMEM;
+    }
+    p->d_settle = (SegDev *)p->mem;
+    p->d_reject = (SegDev *)(p->mem + seg_pad);
+    p->partials = (double *)(p->mem + 2 * seg_pad);
+    p->lo = p->partials + (size_t)pb * n_seg;
+    p->llmax = p->lo + n_seg;
+    p->mask = (uint64_t *)(p->llmax + n_seg);
+    p->bsum = (int64_t *)(p->mask + mask_words);
+    p->idx = p->bsum + bsum_n;
+    p->cum = p->idx + idx_n;
+    p->count1 = p->cum + n_seg + 1;
+    if (hipMemcpy(p->d_settle, p->h_settle.data(), seg_bytes, hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(p->d_reject, p->h_reject.data(), seg_bytes, hipMemcpyHostToDevice) !=
+            hipSuccess) {
+        jk_seg_plan_destroy(p);
+        return JK_EHIP;
+    }
+    *out = p;
+    return JK_OK;
+}
+
+int jk_seg_plan_set_streams(jk_seg_plan *p, const uint64_t *streams, void *stream)
+{
+    if (!p || !streams) return JK_EINVAL;
+    if (current_device_inited() < 0) return JK_ENODEV;
+    hipStream_t st = (hipStream_t)stream;
+    // the staging buffer may still be read by the previous update's copy
+    if (hipEventSynchronize(p->staged) != hipSuccess) return JK_EHIP;
+    plan_fill_streams(p, streams);
+    const size_t seg_bytes = sizeof(SegDev) * p->n_seg;
+    memcpy(p->pinned, p->h_settle.data(), seg_bytes);
+    memcpy(p->pinned + p->n_seg, p->h_reject.data(), seg_bytes);
+    if (hipMemcpyAsync(p->d_settle, p->pinned, seg_bytes, hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
+        hipMemcpyAsync(p->d_reject, p->pinned + p->n_seg, seg_bytes, hipMemcpyHostToDevice,
+                       st) != hipSuccess ||
+        hipEventRecord(p->staged, st) != hipSuccess)
+        return JK_EHIP;
+    return JK_OK;
+}
+
+int jk_seg_plan_settle(const jk_seg_plan *p, const double *P, const double *e,
+                       const double *omega, const double *M0, const double *s,
+                       const int64_t *rows, double *ll_inout, float *bound_inout,
+                       int64_t *n_settled_out, void *stream)
+{
+    if (!p || !P || !e || !omega || !M0 || !ll_inout || !bound_inout || !n_settled_out)
+        return JK_EINVAL;
+    if (p->batch->jitter_mode == JK_JITTER_PHYSICAL && !s) return JK_EINVAL;
+    if (current_device_inited() < 0) return JK_ENODEV;
+    return check_hip(jk_launch_settle_batch(p->batch, p->n_seg, p->d_settle, p->max_n, p->max_nb,
+                                            p->partials, p->pb, p->mask, p->bsum, p->idx, p->lo,
+                                            p->llmax, p->cum, p->count1, P, e, omega, M0, s, rows,
+                                            ll_inout, bound_inout, n_settled_out,
+                                            (hipStream_t)stream));
+}
+
+int jk_seg_plan_reject(const jk_seg_plan *p, const double *ll, int64_t *idx_out,
+                       int64_t *n_pass_out, double *llmax_out, void *stream)
+{
+    if (!p || !ll || !n_pass_out || !llmax_out) return JK_EINVAL;
+    if (current_device_inited() < 0) return JK_ENODEV;
+    return check_hip(jk_launch_reject_batch(ll, p->n_seg, p->d_reject, p->max_n, p->max_nb,
+                                            p->partials, p->pb, p->mask, p->bsum, idx_out,
+                                            n_pass_out, llmax_out, (hipStream_t)stream));
+}
+
 int jk_prior_fill(int64_t n, int64_t row0, uint64_t seed, double P_min, double P_max,
                   const double *e_quantiles, int n_tab, double ln_s_mu, double ln_s_sigma,
                   double *P, double *e, double *omega, double *M0, double *s, float *e32,

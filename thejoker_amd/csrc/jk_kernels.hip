@@ -340,7 +340,6 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
 
     // Per-sample orbit constants.
     double cw, ecw, swsq;
-    float cwf, swf, ecwf, sqf;
     // FP32 copies for the Kepler starter (e capped below 1 in FP32; 1 - e
     // rounded from its FP64 value, so it keeps its relative accuracy).
     const float ef = fminf((float)ev, 0.99999994f);
@@ -348,15 +347,8 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
     const float a1f = JK_MARKLEY_A1N * __builtin_amdgcn_rcpf(1.0f + ef);
     double sw;
     jk_sincos_tab_f64(omv, tab, sw, cw);
-    if constexpr (MIXED) {
-        swf = (float)sw;
-        cwf = (float)cw;
-        ecwf = (float)(ev * cw);
-        sqf = (float)sqrt((1.0 - ev) * (1.0 + ev));
-    } else {
-        ecw = ev * cw;
-        swsq = sw * sqrt((1.0 - ev) * (1.0 + ev));
-    }
+    ecw = ev * cw;
+    swsq = sw * sqrt((1.0 - ev) * (1.0 + ev));
     // The FP32 starter sees e rounded to FP32; within 1e-7 of 1 that can put
     // it outside the correction's reach, so such samples take the fallback.
     const double dmax = ev > 0.9999999 ? -1.0 : 1e-3;
@@ -388,9 +380,10 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
             float sE, cE;
             jk_kepler_markley_f32((float)x, ef, omef, a1f, sE, cE);
             const float den = __builtin_amdgcn_rcpf(fmaf(-ef, cE, 1.0f));
-            const float cf = (cE - ef) * den;
-            const float sf = sqf * sE * den;
-            m = (double)fmaf(cwf, cf, fmaf(-swf, sf, ecwf));
+            // the FP32 (sin E, cos E) combined into the design value in FP64
+            // (same issue rate as FP32 here): no rounding beyond their own
+            const double cEd = (double)cE;
+            m = fma(fma(cw, cEd - ev, -swsq * (double)sE), (double)den, ecw);
             denb = den;
         } else {
             double sE, cE;
