@@ -31,6 +31,7 @@ cpu_baseline: the oracle's restatement of the reference algorithm (LU +
 Woodbury, fast_likelihood.pyx:255-469) on the host cores, bounded sample.
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -166,7 +167,7 @@ def leg_single(args, torch, np, D, syn):
             traffic = tr["hbm_bytes_per_launch"]
     roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
-                "kernel": "k_lnl<2,true,false> (FP64)", "kernel_ms": k_ms,
+                "kernel": "k_lnl<2,true,false,false> (FP64)", "kernel_ms": k_ms,
                 "flops_per_launch": flops,
                 "hbm_gbs_prior_stream": n * BYTES_PER_SAMPLE_F64 / (k_ms * 1e-3) / 1e9}
     log(f"[bench] single: lnL kernel {k_ms:.3f} ms -> {achieved:.2f} TFLOP/s algorithmic "
@@ -344,8 +345,9 @@ def leg_serving(args, torch, np, D, syn):
     wsrv = S.JokerServer(prior, cache, precision="mixed", max_active=8)
     S.run_poisson(wsrv, np.zeros(4), datas[:4], [np.random.default_rng(s) for s in range(4)])
     del wsrv
+    arena = int(args.serving_arena_gib * (1 << 30))
     srv = S.JokerServer(prior, cache, precision="mixed", max_active=args.serving_active,
-                        hbm_budget_bytes=8 << 30)
+                        hbm_budget_bytes=arena)
     rngs = [np.random.default_rng(10_000 + i) for i in range(n_req)]
     t0 = time.perf_counter()
     done = S.run_poisson(srv, times, datas, rngs)
@@ -361,21 +363,63 @@ def leg_serving(args, torch, np, D, syn):
            "max_active": args.serving_active, "prior_rows": len(cache)}
     log(f"[bench] serving: p50 {res['p50_latency_s'] * 1e3:.1f} ms p99 "
         f"{res['p99_latency_s'] * 1e3:.1f} ms, {res['stars_per_s']:.1f} stars/s, "
-        f"{res['sample_epochs_per_s']:.3e} sample-epochs/s, {srv.steps} steps")
-    # capacity: the whole stream submitted at once; phases timed with a
-    # device synchronisation between them
+        f"{res['sample_epochs_per_s']:.3e} sample-epochs/s, {srv.steps} steps, "
+        f"{srv.preemptions} restarts")
+    res["preemptions"] = srv.preemptions
+    res["arena_gib"] = args.serving_arena_gib
+    del srv
+    torch.cuda.empty_cache()
+    # capacity: the whole stream submitted at once
     srv2 = S.JokerServer(prior, cache, precision="mixed", max_active=args.serving_active,
-                         hbm_budget_bytes=8 << 30, profile=True)
+                         hbm_budget_bytes=arena)
     rngs2 = [np.random.default_rng(10_000 + i) for i in range(n_req)]
+    step_log = []
+    step0 = srv2.step
+
+    def logged_step():
+        t = time.perf_counter()
+        n_act = len(srv2.active)
+        out = step0()
+        step_log.append((time.perf_counter() - t, n_act, srv2._waiting))
+        return out
+
+    srv2.step = logged_step
     t0 = time.perf_counter()
-    S.run_poisson(srv2, np.zeros(n_req), datas, rngs2)
+    done2 = S.run_poisson(srv2, np.zeros(n_req), datas, rngs2)
     wall2 = time.perf_counter() - t0
-    res["saturated"] = {"wall_s": wall2, "stars_per_s": n_req / wall2,
-                        "sample_epochs_per_s": srv2.sample_epochs / wall2,
-                        "steps": srv2.steps,
-                        "phase_s": {k: round(v, 4) for k, v in srv2.phase_s.items()}}
-    log(f"[bench] serving: saturated {res['saturated']['stars_per_s']:.1f} stars/s, phases "
-        f"{res['saturated']['phase_s']}")
+    sl = np.array(step_log)
+    ok2 = [r for r in done2 if r.error is None]
+    same = sum(int(np.array_equal(a.result["P"].value, b.result["P"].value))
+               for a, b in zip(done, done2) if a.error is None and b.error is None)
+    sat = {"preemptions": srv2.preemptions, "sample_epochs_per_s": srv2.sample_epochs / wall2,
+           "steps": srv2.steps, "region_moves": dict(srv2.region_moves)}
+    del srv2, logged_step, step0
+    gc.collect()
+    torch.cuda.empty_cache()
+    # where a step's time goes: the same stream again, the device
+    # synchronised between the phases of every step
+    srv3 = S.JokerServer(prior, cache, precision="mixed", max_active=args.serving_active,
+                         hbm_budget_bytes=arena, profile=True)
+    rngs3 = [np.random.default_rng(10_000 + i) for i in range(n_req)]
+    t0 = time.perf_counter()
+    S.run_poisson(srv3, np.zeros(n_req), datas, rngs3)
+    wall3 = time.perf_counter() - t0
+    res["saturated"] = {"wall_s": wall2, "stars_per_s": len(ok2) / wall2,
+                        "completed": len(ok2), "failed": n_req - len(ok2), **sat,
+                        "results_equal_latency_run": same,
+                        "step_s_quantiles": [float(q) for q in
+                                             np.percentile(sl[:, 0], [10, 50, 90, 100])],
+                        "step_s_total": float(sl[:, 0].sum()),
+                        "active_per_step_mean": float(sl[:, 1].mean()),
+                        "waiting_per_step_mean": float(sl[:, 2].mean()),
+                        "profiled_wall_s": wall3,
+                        "phase_s_profiled": {k: round(v, 4) for k, v in srv3.phase_s.items()}}
+    del srv3
+    torch.cuda.empty_cache()
+    log(f"[bench] serving: saturated {res['saturated']['stars_per_s']:.1f} stars/s "
+        f"({sat['sample_epochs_per_s']:.3e} sample-epochs/s, {sat['steps']} steps, "
+        f"{sat['preemptions']} restarts, {n_req - len(ok2)} failed, {same} results equal the "
+        f"latency run's); profiled phases {res['saturated']['phase_s_profiled']}")
     # The plain path: the same stars one at a time through
     # TheJoker(precision="f64").iterative_rejection_sample.  Same rows, so
     # the same sample-epochs as in the server; also checks for flips.
@@ -449,7 +493,7 @@ def leg_stress(args, torch, np, D, syn):
            "f64_tflops_algorithmic": flops / (f_ms * 1e-3) / 1e12,
            "f64_frac_fp64_peak": flops / (f_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
            "f64_hbm_gbs": n * 48 / (f_ms * 1e-3) / 1e9,
-           "mixed_hbm_gbs": n * 32 / (m_ms * 1e-3) / 1e9,
+           "mixed_hbm_gbs": n * 48 / (m_ms * 1e-3) / 1e9,
            **cmp}
     log(f"[bench] stress: f64 {f_ms:.1f} ms mixed {m_ms:.1f} ms ({f_ms / m_ms:.2f}x), "
         f"{fmt_mixed(cmp)}")
@@ -468,6 +512,8 @@ def main():
                     choices=["all", "single", "batched", "serving", "stress"])
     ap.add_argument("--serving-requests", type=int, default=1000)
     ap.add_argument("--serving-active", type=int, default=64)
+    ap.add_argument("--serving-arena-gib", type=float, default=64.0,
+                    help="HBM for the server's lnL arena (12 B per evaluated row)")
     ap.add_argument("--serving-plain", type=int, default=64,
                     help="stars of the serving stream also run one at a time in plain FP64")
     ap.add_argument("--stress-log2-rows", type=int, default=29)

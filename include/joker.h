@@ -299,6 +299,39 @@ int jk_settle_batch(const jk_batch *batch, int n_seg, const jk_settle_segment *s
                     const double *s, const int64_t *rows, double *ll_inout, float *bound_inout,
                     int64_t *n_settled_out, void *stream);
 
+/* One segment of jk_settle_live_batch: entries [0, n) of the lnL buffer at
+ * offset (cache rows row_begin + i, or rows[row_begin + i]) of table star;
+ * entries [first, n) are new since the segment's previous call, and floor is
+ * the FP64 maximum of entries [0, first) (the maximum the rejection step
+ * found over them; -inf when first = 0). */
+typedef struct {
+    int64_t offset;
+    int64_t n;
+    int64_t first;
+    int32_t star;
+    int32_t reserved;
+    int64_t row_begin;
+    double floor;
+} jk_live_segment;
+
+/* Settle a growing mixed-precision lnL buffer for EVERY future uniform, as
+ * the iterative rejection loop (likelihood_helpers.py:164-201) needs: it
+ * draws new uniforms for all entries each round.  Of each segment's new
+ * entries, the ones with lnL + bound >= lo - margin are re-evaluated in FP64
+ * in place (bound := 0), where lo = max(floor, max over the new entries of
+ * lnL - bound) is a lower bound on the FP64 maximum.  An entry left in mixed
+ * precision then lies more than margin below the maximum in both precisions,
+ * so exp(lnL - max) is 0 and the entry is rejected whatever the uniform;
+ * after a call on every new entry, jk_reject_batch over [0, n) accepts
+ * exactly the FP64 set under any stream.  margin >= 746 (exp underflows to
+ * 0 below -745.14); 750 is the value the library uses.  Work is proportional
+ * to the new entries only.  n_settled_out[i] (device, int64) receives segment
+ * i's count.  segs is a HOST array.  No reference counterpart. */
+int jk_settle_live_batch(const jk_batch *batch, int n_seg, const jk_live_segment *segs,
+                         const double *P, const double *e, const double *omega, const double *M0,
+                         const double *s, const int64_t *rows, double *ll_inout,
+                         float *bound_inout, double margin, int64_t *n_settled_out, void *stream);
+
 /* A segment plan: the descriptors and scratch of jk_settle_batch and
  * jk_reject_batch for a fixed layout of segments (jk_settle_segment: offsets,
  * sizes, stars, first rows, first PCG64 streams), kept on the device, so that

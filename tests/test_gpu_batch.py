@@ -276,6 +276,45 @@ def test_settle_then_reject_equals_fp64(setup, max_keep):
     assert np.array_equal(llm.cpu().numpy()[b == 0], ll64.cpu().numpy()[b == 0])
 
 
+def test_settle_live_grows_and_decides_like_fp64_for_any_stream(setup):
+    """Segments grown in rounds of different sizes, each round's new rows
+    settled with jk_settle_live_batch (floor: the maximum the previous
+    round's rejection step found): after every round the rejection step
+    accepts exactly the FP64 set under several unrelated streams, and the
+    maximum is the FP64 maximum."""
+    stars, cache, _ = setup
+    batch = B.StarBatch([d for _, d in stars])
+    n = len(cache)
+    items, off, total = B.work_items([n] * len(stars), 1024)
+    d_items = B.upload_items(items)
+    ll64 = B.lnl_batch(batch, cache, d_items, len(items), total, "f64")
+    bound = torch.empty(total, dtype=torch.float32, device="cuda")
+    llm = B.lnl_batch(batch, cache, d_items, len(items), total, "mixed", bound=bound)
+    g = np.random.default_rng(5)
+    # per star: round ends 0 < c1 < c2 < n (c1 may be tiny)
+    ends = np.sort(g.integers(1, n, size=(len(stars), 2)), axis=1)
+    ends = np.concatenate([ends, np.full((len(stars), 1), n)], axis=1)
+    first = np.zeros(len(stars), dtype=np.int64)
+    floors = np.full(len(stars), -np.inf)
+    settled = 0
+    for k in range(3):
+        sizes = ends[:, k].astype(np.int64)
+        cnt = B.settle_live_batch(batch, cache, llm, bound, off, sizes, first, floors)
+        settled += int(cnt.sum())
+        for seed in (11 * k + 1, 11 * k + 2, 11 * k + 3):
+            states = [D.pcg64_state(np.random.default_rng(seed * 100 + i))
+                      for i in range(len(stars))]
+            want, npw, mxw = B.reject_batch(ll64, off, sizes, states, 128)
+            got, npg, mxg = B.reject_batch(llm, off, sizes, states, 128)
+            for i in range(len(stars)):
+                assert np.array_equal(got[i], want[i]), (k, seed, i)
+            assert np.array_equal(npg, npw) and np.array_equal(mxg, mxw), (k, seed)
+        first, floors = sizes, mxg
+    b = bound.cpu().numpy()
+    assert np.array_equal(llm.cpu().numpy()[b == 0], ll64.cpu().numpy()[b == 0])
+    print(f"settled {settled} of {total} rows", flush=True)
+
+
 @pytest.mark.parametrize("precision", ["mixed", "f64"])
 def test_step_graph_replays_equal_eager(setup, precision):
     """The graph-captured step (persistent lnL launch, settling, rejection on a

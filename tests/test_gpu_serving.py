@@ -87,3 +87,33 @@ def test_rejection_sample_many_equals_single(cache):
             assert np.array_equal(many[i][1], lls)
             assert np.array_equal(many[i][0]["P"].value, one["P"].value)
             assert np.array_equal(many[i][0]["K"].value, one["K"].value)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "f64"])
+def test_server_restarts_under_memory_pressure(cache, precision):
+    """An lnL arena of one full cache range plus 4096 values: requests that
+    cannot grow wait, the youngest is restarted when all wait, regions move
+    or extend in place -- and every result still equals the star run alone."""
+    prior = _prior()
+    _, stars = syn.arrivals(8, seed=17, M_range=(40, 120))
+    datas = [s.to_rvdata() for s in stars]
+    per_row = 12 if precision == "mixed" else 8
+    srv = S.JokerServer(prior, cache, n_requested=16, precision=precision, max_active=4,
+                        growth_factor=8, hbm_budget_bytes=per_row * (len(cache) + 4096),
+                        shape=B.LaunchShape(128, 0, 1024))
+    for i, d in enumerate(datas):
+        srv.submit(d, np.random.default_rng(700 + i), rid=i)
+    while srv.busy():
+        srv.step()
+    got = sorted(srv.finished, key=lambda r: r.id)
+    assert len(got) == len(datas)
+    print(f"preemptions {srv.preemptions}, steps {srv.steps}", flush=True)
+    assert srv.preemptions > 0
+    for i, r in enumerate(got):
+        assert r.error is None, r.error
+        joker = tj.TheJoker(prior, rng=np.random.default_rng(700 + i), precision=precision)
+        alone = joker.iterative_rejection_sample(datas[i], cache, n_requested_samples=16,
+                                                 growth_factor=8)
+        assert np.array_equal(r.result["P"].value, alone["P"].value), i
+        assert np.array_equal(r.result["K"].value, alone["K"].value), i
+    assert srv.alloc.used() == 0

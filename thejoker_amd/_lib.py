@@ -10,7 +10,7 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libjoker.so")
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 JK_OK = 0
 JK_JITTER_REFERENCE = 0
@@ -63,6 +63,8 @@ SIGNATURES = [
      [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _vp, _vp]),
     ("jk_settle_batch", ctypes.c_int,
      [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("jk_settle_live_batch", ctypes.c_int,
+     [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _vp, _vp]),
     ("jk_seg_plan_create", ctypes.c_int,
      [_vp, ctypes.c_int, _vp, _i64, ctypes.POINTER(_vp)]),
     ("jk_seg_plan_destroy", ctypes.c_int, [_vp]),

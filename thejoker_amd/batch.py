@@ -19,13 +19,17 @@ from .device import (DeviceStar, _dptr, _split, _stream, _torch, pcg64_advance_s
                      to_host)
 
 __all__ = ["WORK_ITEM_DTYPE", "SEGMENT_DTYPE", "POLISH_SEGMENT_DTYPE", "SETTLE_SEGMENT_DTYPE",
-           "StarBatch", "work_items", "lnl_batch", "reject_batch", "polish_batch", "settle_batch",
+           "LIVE_SEGMENT_DTYPE", "StarBatch", "work_items", "lnl_batch", "reject_batch",
+           "polish_batch", "settle_batch", "settle_live_batch",
            "autotune", "LaunchShape", "rejection_sample_many", "SegPlan", "StepGraph"]
 
 WORK_ITEM_DTYPE = np.dtype([("star", "<i4"), ("n_rows", "<i4"), ("row_begin", "<i8"),
                             ("out_offset", "<i8")])
 POLISH_SEGMENT_DTYPE = np.dtype([("offset", "<i8"), ("n", "<i8"), ("star", "<i4"),
                                  ("reserved", "<i4"), ("row_begin", "<i8"), ("first", "<i8")])
+LIVE_SEGMENT_DTYPE = np.dtype([("offset", "<i8"), ("n", "<i8"), ("first", "<i8"), ("star", "<i4"),
+                               ("reserved", "<i4"), ("row_begin", "<i8"), ("floor", "<f8")])
+LIVE_MARGIN = 750.0
 SETTLE_SEGMENT_DTYPE = np.dtype([("offset", "<i8"), ("n", "<i8"), ("star", "<i4"),
                                  ("reserved", "<i4"), ("row_begin", "<i8"), ("state_hi", "<u8"),
                                  ("state_lo", "<u8"), ("inc_hi", "<u8"), ("inc_lo", "<u8"),
@@ -253,6 +257,33 @@ def settle_batch(batch, cache, ll, bound, offsets, sizes, rng_states, stars=None
                                    _dptr(cache.M0), _dptr(cache.s), _dptr(rows), _dptr(ll),
                                    _dptr(bound), _dptr(count), _stream(device)),
                "jk_settle_batch")
+    return count
+
+
+def settle_live_batch(batch, cache, ll, bound, offsets, sizes, firsts, floors, stars=None,
+                      row_begins=None, rows=None, device=0):
+    """jk_settle_live_batch: settle the new entries [firsts[i], sizes[i]) of
+    each growing mixed-precision segment for every future uniform (floors[i]:
+    the FP64 maximum of the entries before, -inf for none), so that the
+    rejection step over the whole segment accepts the FP64 set under any
+    stream.  Returns a device int64 tensor of per-segment counts."""
+    torch = _torch()
+    lib = _lib.ensure_device(device)
+    n_seg = len(sizes)
+    segs = np.zeros(n_seg, dtype=LIVE_SEGMENT_DTYPE)
+    segs["offset"] = offsets
+    segs["n"] = sizes
+    segs["first"] = firsts
+    segs["star"] = np.arange(n_seg) if stars is None else stars
+    segs["row_begin"] = 0 if row_begins is None else row_begins
+    segs["floor"] = floors
+    count = torch.empty(n_seg, dtype=torch.int64, device=ll.device)
+    _lib.check(lib.jk_settle_live_batch(batch.handle, n_seg, segs.ctypes.data_as(ctypes.c_void_p),
+                                        _dptr(cache.P), _dptr(cache.e), _dptr(cache.omega),
+                                        _dptr(cache.M0), _dptr(cache.s), _dptr(rows), _dptr(ll),
+                                        _dptr(bound), LIVE_MARGIN, _dptr(count),
+                                        _stream(device)),
+               "jk_settle_live_batch")
     return count
 
 

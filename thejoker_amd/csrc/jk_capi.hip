@@ -10,7 +10,7 @@
 
 #include "jk_internal.h"
 
-#define JK_ABI_VERSION 3
+#define JK_ABI_VERSION 4
 
 namespace {
 
@@ -670,6 +670,74 @@ int jk_settle_batch(const jk_batch *batch, int n_seg, const jk_settle_segment *s
         err = jk_launch_settle_batch(batch, n_seg, d_segs, max_n, max_nb, partials, pb, mask, bsum,
                                      idx, lo, llmax, cum, count1, P, e, omega, M0, s, rows,
                                      ll_inout, bound_inout, n_settled_out, st);
+    // The host descriptor array must outlive the queued copy.
+    hipError_t err3 = hipStreamSynchronize(st);
+    hipError_t err2 = hipFreeAsync(scratch, st);
+    if (err != hipSuccess || err3 != hipSuccess) return JK_EHIP;
+    return check_hip(err2);
+}
+
+int jk_settle_live_batch(const jk_batch *batch, int n_seg, const jk_live_segment *segs,
+                         const double *P, const double *e, const double *omega, const double *M0,
+                         const double *s, const int64_t *rows, double *ll_inout,
+                         float *bound_inout, double margin, int64_t *n_settled_out, void *stream)
+{
+    if (!batch || n_seg <= 0 || !segs || !ll_inout || !bound_inout || !n_settled_out)
+        return JK_EINVAL;
+    if (!P || !e || !omega || !M0) return JK_EINVAL;
+    if (batch->jitter_mode == JK_JITTER_PHYSICAL && !s) return JK_EINVAL;
+    if (!(margin >= 746.0)) return JK_EINVAL;
+    if (current_device_inited() < 0) return JK_ENODEV;
+    std::vector<SegDev> h(n_seg);
+    int64_t max_n = 1, max_nb = 1, mask_words = 0, bsum_n = 0, idx_n = 0;
+    for (int i = 0; i < n_seg; ++i) {
+        const jk_live_segment &g = segs[i];
+        if (g.n < 0 || g.first < 0 || g.first > g.n || g.offset < 0 || g.row_begin < 0 ||
+            g.star < 0 || g.star >= batch->n_stars)
+            return JK_EINVAL;
+        // the segment's new rows [first, n) as a segment of their own
+        const int64_t n = g.n - g.first;
+        SegDev &d = h[i];
+        memset(&d, 0, sizeof(d));
+        d.off = g.offset + g.first;
+        d.n = n;
+        d.max_keep = n;
+        d.idx_off = idx_n;
+        d.mask_off = mask_words;
+        d.nb = jk_reject_bsum_count(n) > 0 ? jk_reject_bsum_count(n) : 1;  // block 0 writes the count
+        d.bsum_off = bsum_n;
+        d.star = g.star;
+        d.row0 = g.row_begin + g.first;
+        d.floor = g.floor;
+        idx_n += n;
+        mask_words += jk_reject_mask_words(n);
+        bsum_n += d.nb;
+        if (n > max_n) max_n = n;
+        if (d.nb > max_nb) max_nb = d.nb;
+    }
+    int pb = (int)((max_n + 4095) / 4096);
+    if (pb > 1024) pb = 1024;
+    if (pb < 1) pb = 1;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t seg_bytes = sizeof(SegDev) * n_seg;
+    const size_t seg_pad = ((seg_bytes + 255) / 256) * 256;
+    const size_t bytes = seg_pad + sizeof(double) * ((size_t)pb * n_seg + (size_t)n_seg) +
+                         sizeof(uint64_t) * mask_words +
+                         sizeof(int64_t) * (bsum_n + idx_n + (size_t)n_seg + 1);
+    char *scratch = nullptr;
+    if (hipMallocAsync((void **)&scratch, bytes, st) != hipSuccess) return JK_ENOMEM;
+    SegDev *d_segs = (SegDev *)scratch;
+    double *partials = (double *)(scratch + seg_pad);
+    double *lo = partials + (size_t)pb * n_seg;
+    uint64_t *mask = (uint64_t *)(lo + n_seg);
+    int64_t *bsum = (int64_t *)(mask + mask_words);
+    int64_t *idx = bsum + bsum_n;
+    int64_t *cum = idx + idx_n;
+    hipError_t err = hipMemcpyAsync(d_segs, h.data(), seg_bytes, hipMemcpyHostToDevice, st);
+    if (err == hipSuccess)
+        err = jk_launch_settle_live_batch(batch, n_seg, d_segs, max_n, max_nb, partials, pb, mask,
+                                          bsum, idx, lo, cum, P, e, omega, M0, s, rows, ll_inout,
+                                          bound_inout, margin, n_settled_out, st);
     // The host descriptor array must outlive the queued copy.
     hipError_t err3 = hipStreamSynchronize(st);
     hipError_t err2 = hipFreeAsync(scratch, st);

@@ -32,6 +32,7 @@ struct SegDev {
     u128 s_base, inc;
     int64_t star, row0;   // segmented polish: star-table index, first cache row
     int64_t first;        // segmented polish: first entry still in mixed precision
+    double floor;         // live settling: FP64 maximum of the rows settled before
 };
 
 // Jump table of numpy's PCG64 LCG: (A^(2^i), D_(2^i)) for i = 0..63, where
@@ -105,6 +106,13 @@ hipError_t jk_launch_settle_batch(const jk_batch *b, int n_seg, const SegDev *d_
                                   const double *e, const double *om, const double *M0,
                                   const double *s, const int64_t *rows, double *ll, float *bound,
                                   int64_t *n_settled, hipStream_t stream);
+hipError_t jk_launch_settle_live_batch(const jk_batch *b, int n_seg, const SegDev *d_segs,
+                                       int64_t max_n, int64_t max_nb, double *partials, int pb,
+                                       uint64_t *mask, int64_t *bsum, int64_t *idx, double *lo,
+                                       int64_t *cum, const double *P, const double *e,
+                                       const double *om, const double *M0, const double *s,
+                                       const int64_t *rows, double *ll, float *bound,
+                                       double margin, int64_t *n_settled, hipStream_t stream);
 hipError_t jk_launch_polish(const jk_star *st, int64_t n, const double *P, const double *e,
                             const double *om, const double *M0, const double *s,
                             const int64_t *rows, double *ll, double window, int64_t first,

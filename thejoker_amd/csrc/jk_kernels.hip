@@ -1663,6 +1663,66 @@ hipError_t jk_launch_settle_batch(const jk_batch *b, int n_seg, const SegDev *d_
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Settling for every future uniform (the iterative rejection loop redraws
+// the uniforms of all rows each round).  For each segment's new rows:
+//   1. lo = max(floor, max(ll - B)) is a lower bound on the FP64 maximum of
+//      the segment (floor: the FP64 maximum of the rows settled before);
+//   2. every new row with ll + B >= lo - margin is re-evaluated in FP64.
+// A row left in mixed precision then has lnL(FP64) <= ll + B < max - margin,
+// and so does ll itself: with margin >= 746, exp(x - max) is 0 for both, so
+// the row is rejected whatever its uniform, and it cannot be the maximum.
+// The maximum row itself is always re-evaluated.  A NaN in lo re-evaluates
+// every new row still in mixed precision.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_seg_live_flags(const double *__restrict__ ll,
+                                                        const float *__restrict__ bound,
+                                                        const SegDev *__restrict__ segs,
+                                                        const double *__restrict__ lo,
+                                                        double margin,
+                                                        uint64_t *__restrict__ mask)
+{
+    const SegDev sg = segs[blockIdx.y];
+    const int lane = threadIdx.x & 63;
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t T = (int64_t)gridDim.x * 256;
+    if (g - lane >= sg.n) return;
+    const double thr = jk_nanmax(lo[blockIdx.y], sg.floor) - margin;
+    for (int64_t base = g - lane; base < sg.n; base += T) {
+        const int64_t i = base + lane;
+        bool keep = false;
+        if (i < sg.n) {
+            const float b = bound[sg.off + i];
+            keep = b > 0.0f && !(ll[sg.off + i] + (double)b < thr);
+        }
+        const uint64_t word = __ballot(keep);
+        if (lane == 0) mask[sg.mask_off + (base >> 6)] = word;
+    }
+}
+
+hipError_t jk_launch_settle_live_batch(const jk_batch *b, int n_seg, const SegDev *d_segs,
+                                       int64_t max_n, int64_t max_nb, double *partials, int pb,
+                                       uint64_t *mask, int64_t *bsum, int64_t *idx, double *lo,
+                                       int64_t *cum, const double *P, const double *e,
+                                       const double *om, const double *M0, const double *s,
+                                       const int64_t *rows, double *ll, float *bound,
+                                       double margin, int64_t *n_settled, hipStream_t stream)
+{
+    if (n_seg <= 0) return hipSuccess;
+    const int fb = jk_seg_flag_blocks(max_n);
+    hipLaunchKernelGGL(k_seg_lo_partial, dim3(pb, n_seg), dim3(256), 0, stream, ll, bound, d_segs,
+                       partials, pb);
+    hipLaunchKernelGGL(k_seg_max_final, dim3(n_seg), dim3(64), 0, stream, partials, pb, lo);
+    hipLaunchKernelGGL(k_seg_live_flags, dim3(fb, n_seg), dim3(256), 0, stream, ll, bound, d_segs,
+                       lo, margin, mask);
+    hipLaunchKernelGGL(k_seg_bsum, dim3((unsigned)max_nb, n_seg), dim3(256), 0, stream, d_segs,
+                       mask, bsum);
+    hipLaunchKernelGGL(k_seg_compact, dim3((unsigned)max_nb, n_seg), dim3(256), 0, stream, d_segs,
+                       mask, bsum, idx, n_settled);
+    return launch_seg_scatter(b, n_seg, d_segs, n_settled, cum, idx, P, e, om, M0, s, rows, ll,
+                              bound, stream);
+}
+
 void jk_host_pcg_jump(u128 state, u128 inc, uint64_t k, u128 *out) { *out = h_jump(state, inc, k); }
 
 // ---------------------------------------------------------------------------

@@ -8,20 +8,27 @@ finishes.  Every step evaluates the next batch of rows of *every* active
 request in one persistent ``jk_lnl_batch`` launch (stars of different epoch
 counts side by side) writing straight into each request's region of one HBM
 lnL arena (mixed precision: with each row's error bound in a parallel
-arena), then re-evaluates in FP64 the rows whose rejection decision the bound
-leaves open (``jk_settle_batch``) and runs every request's rejection step with
-its own numpy PCG64 stream (``jk_reject_batch``).  The iteration state of each
-request is :class:`thejoker_amd.thejoker.IterativeRejection` -- the code the
-single-star API runs -- and the batched lnL is bit-identical to the
-single-star kernel, so a request's result is exactly what
+arena), then re-evaluates in FP64 the new rows that the bound leaves within
+reach of the maximum, which makes every row decide like FP64 in this and all
+later rounds (``jk_settle_live_batch``), and runs every request's rejection
+step with its own numpy PCG64 stream (``jk_reject_batch``).  The iteration
+state of each request is :class:`thejoker_amd.thejoker.IterativeRejection`
+-- the code the single-star API runs -- and the batched lnL is bit-identical
+to the single-star kernel, so a request's result is exactly what
 ``TheJoker.iterative_rejection_sample`` returns for that star alone.
+
+When every active request waits for arena memory to grow, the most recently
+admitted one is restarted: its region and slot are freed and it is queued
+again with its random stream as it was at admission, so it repeats the same
+rounds and its result is unchanged.  No request is admitted while another
+waits for memory.
 """
 import time
 
 import numpy as np
 
-from .batch import LaunchShape, StarBatch, heaviest_first, lnl_batch, reject_batch, settle_batch, \
-    upload_items, work_items
+from .batch import LaunchShape, StarBatch, heaviest_first, lnl_batch, reject_batch, \
+    settle_live_batch, upload_items, work_items
 from .device import (DeviceStar, _torch, linear_internal_units, pcg64_state, posterior_inputs_many,
                      star_arrays)
 from .likelihood_helpers import validate_prepare_data
@@ -47,6 +54,29 @@ class RangeAllocator:
                     self.free[k] = (a + n, b)
                 return a
         return None
+
+    def extend(self, start, n, new_n):
+        """Grow the range [start, start + n) in place to new_n elements if the
+        free range right after it is large enough; True on success."""
+        end, grow = int(start) + int(n), int(new_n) - int(n)
+        for k, (a, b) in enumerate(self.free):
+            if a == end and b - a >= grow:
+                if b - a == grow:
+                    self.free.pop(k)
+                else:
+                    self.free[k] = (a + grow, b)
+                return True
+        return False
+
+    def take(self, start, n):
+        """Reserve exactly [start, start + n), which must be free."""
+        a0, b0 = int(start), int(start) + int(n)
+        for k, (a, b) in enumerate(self.free):
+            if a <= a0 and b0 <= b:
+                parts = [(x, y) for x, y in ((a, a0), (b0, b)) if y > x]
+                self.free[k:k + 1] = parts
+                return
+        raise ValueError(f"range [{a0}, {b0}) is not free")
 
     def release(self, start, n):
         a, b = int(start), int(start) + int(n)
@@ -81,6 +111,9 @@ class Request:
         self.error = None
         self.rows_evaluated = 0
         self.sample_epochs = 0
+        self.rng_state0 = None  # the stream at admission, for a restart
+        self.preempted = 0      # times restarted to free arena memory
+        self.need_hint = 0      # arena values held when last preempted
 
     @property
     def latency(self):
@@ -150,6 +183,9 @@ class JokerServer:
         self.phase_s = dict.fromkeys(self.PHASES, 0.0)
         self._t_mark = None
         self._to_finish = []      # retired requests whose samples are still to be drawn
+        self._waiting = 0         # active requests the last step could not grow
+        self.preemptions = 0
+        self.region_moves = {"extend": 0, "move": 0, "overlap": 0}
         self._helper = TheJoker(prior, rng=np.random.default_rng(0), jitter_mode=jitter_mode,
                                 precision=precision, device=device)
 
@@ -192,15 +228,20 @@ class JokerServer:
         self.finished.append(r)
 
     def _admit(self):
-        while self.pending and len(self.active) < self.max_active:
+        # no admissions while an active request waits for arena memory
+        while self.pending and len(self.active) < self.max_active and not self._waiting:
             r = self.pending[0]
             first = self.growth_factor * self.n_requested
+            if r.need_hint and self.alloc.capacity - self.alloc.used() < r.need_hint:
+                break                      # a restarted request waits for its room
             start = self.alloc.alloc(first)
             if start is None:
                 break                      # wait for memory
             self.pending.pop(0)
             r.region = (start, first)
             try:
+                if r.rng_state0 is None:
+                    r.rng_state0 = r.rng.bit_generator.state
                 dd, ids, trend = validate_prepare_data(r.data, self.prior.poly_trend,
                                                        self.prior.n_offsets)
                 a = star_arrays(dd, self.prior, trend)
@@ -221,23 +262,55 @@ class JokerServer:
             r.t_admit = self.now()
             self.active.append(r)
 
+    def _preempt(self, r):
+        """Free r's arena region and slot and queue r again at the front,
+        with its random stream as it was at admission: the restart repeats
+        the same rounds, so its result does not change."""
+        self.active.remove(r)
+        r.need_hint = r.region[1]
+        self._release(r)
+        r.it = None
+        r.rng.bit_generator.state = r.rng_state0
+        r.preempted += 1
+        self.preemptions += 1
+        self.pending.insert(0, r)
+
     def _ensure_region(self, r, need):
         """Grow r's arena region to hold `need` values; False if the arena is
         full for now."""
         start, cap = r.region
         if need <= cap:
             return True
-        new_cap = min(max(need, 2 * cap), r.it.max_prior)
-        new = self.alloc.alloc(new_cap)
+        caps = [min(max(need, 2 * cap), r.it.max_prior)]
+        if caps[0] > need:
+            caps.append(need)
+        # in place, when the range after the region is free
+        for c in caps:
+            if self.alloc.extend(start, cap, c):
+                r.region = (start, c)
+                self.region_moves["extend"] += 1
+                return True
+        # elsewhere, the old region counted as free (the new range may
+        # overlap it: the values then move through a temporary copy)
+        self.alloc.release(start, cap)
+        new = None
+        for c in caps:
+            new = self.alloc.alloc(c)
+            if new is not None:
+                break
         if new is None:
+            self.alloc.take(start, cap)
             return False
         n = r.it.n_evals
-        if n:
-            self.arena[new:new + n] = self.arena[start:start + n]
-            if self.arena_b is not None:
-                self.arena_b[new:new + n] = self.arena_b[start:start + n]
-        self.alloc.release(start, cap)
-        r.region = (new, new_cap)
+        self.region_moves["move"] += 1
+        if n and new != start:
+            overlap = new < start + n and start < new + n
+            self.region_moves["overlap"] += int(overlap)
+            for buf in (self.arena, self.arena_b):
+                if buf is not None:
+                    src = buf[start:start + n]
+                    buf[new:new + n] = src.clone() if overlap else src
+        r.region = (new, c)
         return True
 
     def _build_results(self):
@@ -266,18 +339,24 @@ class JokerServer:
             done = self._build_results()
             self._mark("finish")
             return done
-        plan = []
-        for r in self.active:
-            rows, start, n_now = r.it.next_batch()
-            if self._ensure_region(r, r.it.n_evals + n_now):
-                plan.append((r, start, n_now))
-        if not plan:
-            # Every active request is waiting to grow and the arena is full:
-            # fail the one holding the largest region so the rest can go on.
-            r = max(self.active, key=lambda q: q.region[1])
-            self.active.remove(r)
-            self._fail(r, MemoryError("lnL arena too small for this request's next batch"))
-            return [r]
+        while True:
+            plan = []
+            for r in self.active:
+                rows, start, n_now = r.it.next_batch()
+                if self._ensure_region(r, r.it.n_evals + n_now):
+                    plan.append((r, start, n_now))
+            self._waiting = len(self.active) - len(plan)
+            if plan:
+                break
+            # Every active request waits to grow and the arena is full.  With
+            # others beside it, restart the most recently admitted one; alone,
+            # the arena cannot hold the request at all.
+            if len(self.active) == 1:
+                r = self.active.pop()
+                self._fail(r, MemoryError("lnL arena too small for this request's next batch"))
+                self._waiting = 0
+                return [r]
+            self._preempt(max(self.active, key=lambda q: q.t_admit))
         slots = np.array([r.slot for r, _, _ in plan], dtype=np.int64)
         sizes = np.array([n for _, _, n in plan], dtype=np.int64)
         begins = np.array([b for _, b, _ in plan], dtype=np.int64)
@@ -306,10 +385,11 @@ class JokerServer:
         self._mark("lnl")
         states = [pcg64_state(r.rng) for r, _, _ in plan]
         if self.precision == "mixed":
-            # FP64 for the rows whose decision (under this round's uniforms)
-            # or claim to the maximum the mixed error bound leaves open
-            settle_batch(self._table, self.cache, self.arena, self.arena_b, seg_off, seg_n, states,
-                         stars=slots, device=self.device)
+            # FP64 for the new rows within reach of the maximum: with them,
+            # every row decides like FP64 in this and all later rounds
+            floors = np.array([r.it.floor for r, _, _ in plan], dtype=np.float64)
+            settle_live_batch(self._table, self.cache, self.arena, self.arena_b, seg_off, seg_n,
+                              before, floors, stars=slots, device=self.device)
         self._mark("polish")
         good, n_pass, llmax = reject_batch(self.arena, seg_off, seg_n, states, self.n_requested,
                                            device=self.device)
@@ -322,6 +402,7 @@ class JokerServer:
                 if self.in_memory and not np.isfinite(llmax[k]):
                     raise RuntimeError("There are NaN or Inf likelihood values in iteration step "
                                        f"{r.it.iteration}!")
+                r.it.floor = float(llmax[k])
                 r.it.update(seg_n[k], good[k], n_pass[k])
             except Exception as exc:
                 r.error = exc

@@ -151,6 +151,20 @@ class TheJoker:
                      device=device)
 
     @staticmethod
+    def _settle_live(star, cache, ll, bound, first, floor, rows=None, device=0):
+        """Mixed precision, iterative loop: re-evaluate in FP64 the new rows
+        ll[first:] that lie within a 750 margin of the maximum, so that the
+        rejection step over ll decides like FP64 for every later uniform
+        (jk_settle_live_batch on a one-star table; floor: the maximum of
+        ll[:first])."""
+        from .batch import StarBatch, settle_live_batch
+        table = getattr(star, "_settle_table", None)
+        if table is None:
+            table = star._settle_table = StarBatch([star], device=device)
+        settle_live_batch(table, cache, ll, bound, [0], [int(ll.shape[0])], [int(first)],
+                          [float(floor)], rows=rows, device=device)
+
+    @staticmethod
     def _fp64_at(star, cache, cache_rows):
         """FP64 lnL of the given cache rows (the values reported for accepted
         samples, whatever the precision of the search)."""
@@ -320,8 +334,9 @@ class TheJoker:
                 lnl(star, cache, start=start, n=n_now, precision=self.precision,
                     out=all_ll[a:b], bound=part_b)
             if mixed:
-                self._settle(star, cache, all_ll[:b], all_b[:b], pcg64_state(self.rng),
-                             rows=it._d_all_idx, device=self.device)
+                # the new rows, for this and every later round's uniforms
+                self._settle_live(star, cache, all_ll[:b], all_b[:b], a, it.floor,
+                                  rows=it._d_all_idx, device=self.device)
             it.absorb(all_ll[:b], self.device)
         full_idx, good = it.result()
         samples = self._full_samples(star, cache, full_idx, self.rng, n_linear_samples,
@@ -386,6 +401,7 @@ class IterativeRejection:
         self.good = None
         self.n_good = 0
         self.done = False
+        self.floor = -np.inf     # lnL maximum over the rows evaluated so far
 
     def next_batch(self):
         """(device rows or None, first cache row, count) of the next round."""
@@ -403,7 +419,9 @@ class IterativeRejection:
         if self.in_memory and not bool(torch.isfinite(ll_so_far).all()):
             raise RuntimeError(f"There are NaN or Inf likelihood values in iteration step "
                                f"{self.iteration}!")
-        good, n_good, _ = reject(ll_so_far, pcg64_state(self.rng), self.n_requested, device=device)
+        good, n_good, mx = reject(ll_so_far, pcg64_state(self.rng), self.n_requested,
+                                  device=device)
+        self.floor = mx
         self.update(n_evals, good, n_good)
 
     def update(self, n_evals, good, n_good):
