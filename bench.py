@@ -346,8 +346,9 @@ def leg_serving(args, torch, np, D, syn):
     S.run_poisson(wsrv, np.zeros(4), datas[:4], [np.random.default_rng(s) for s in range(4)])
     del wsrv
     arena = int(args.serving_arena_gib * (1 << 30))
+    step_work = int(args.serving_step_work) if args.serving_step_work > 0 else None
     srv = S.JokerServer(prior, cache, precision="mixed", max_active=args.serving_active,
-                        hbm_budget_bytes=arena)
+                        hbm_budget_bytes=arena, max_step_work=step_work)
     rngs = [np.random.default_rng(10_000 + i) for i in range(n_req)]
     t0 = time.perf_counter()
     done = S.run_poisson(srv, times, datas, rngs)
@@ -367,13 +368,14 @@ def leg_serving(args, torch, np, D, syn):
         f"{srv.preemptions} restarts")
     res["preemptions"] = srv.preemptions
     res["arena_gib"] = args.serving_arena_gib
+    res["max_step_work"] = step_work
     del srv
     torch.cuda.empty_cache()
     # capacity: the whole stream submitted at once
     srv2 = S.JokerServer(prior, cache, precision="mixed", max_active=args.serving_active,
-                         hbm_budget_bytes=arena)
+                         hbm_budget_bytes=arena, max_step_work=step_work)
     rngs2 = [np.random.default_rng(10_000 + i) for i in range(n_req)]
-    step_log = []
+    step_log, step_info = [], []
     step0 = srv2.step
 
     def logged_step():
@@ -381,6 +383,7 @@ def leg_serving(args, torch, np, D, syn):
         n_act = len(srv2.active)
         out = step0()
         step_log.append((time.perf_counter() - t, n_act, srv2._waiting))
+        step_info.append(dict(srv2.last_step, step_s=round(step_log[-1][0], 4)))
         return out
 
     srv2.step = logged_step
@@ -399,7 +402,7 @@ def leg_serving(args, torch, np, D, syn):
     # where a step's time goes: the same stream again, the device
     # synchronised between the phases of every step
     srv3 = S.JokerServer(prior, cache, precision="mixed", max_active=args.serving_active,
-                         hbm_budget_bytes=arena, profile=True)
+                         hbm_budget_bytes=arena, profile=True, max_step_work=step_work)
     rngs3 = [np.random.default_rng(10_000 + i) for i in range(n_req)]
     t0 = time.perf_counter()
     S.run_poisson(srv3, np.zeros(n_req), datas, rngs3)
@@ -412,8 +415,13 @@ def leg_serving(args, torch, np, D, syn):
                         "step_s_total": float(sl[:, 0].sum()),
                         "active_per_step_mean": float(sl[:, 1].mean()),
                         "waiting_per_step_mean": float(sl[:, 2].mean()),
+                        "slowest_steps": sorted(step_info, key=lambda d: -d["step_s"])[:4],
                         "profiled_wall_s": wall3,
                         "phase_s_profiled": {k: round(v, 4) for k, v in srv3.phase_s.items()}}
+    slow = sorted(srv3.step_profile, key=lambda d: -sum(v for k, v in d.items()
+                                                        if k in S.JokerServer.PHASES))[:3]
+    res["saturated"]["slowest_steps_profiled"] = [
+        {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()} for d in slow]
     del srv3
     torch.cuda.empty_cache()
     log(f"[bench] serving: saturated {res['saturated']['stars_per_s']:.1f} stars/s "
@@ -512,6 +520,8 @@ def main():
                     choices=["all", "single", "batched", "serving", "stress"])
     ap.add_argument("--serving-requests", type=int, default=1000)
     ap.add_argument("--serving-active", type=int, default=64)
+    ap.add_argument("--serving-step-work", type=float, default=0.0,
+                    help="sample-epochs one serving step may evaluate (0: whole rounds)")
     ap.add_argument("--serving-arena-gib", type=float, default=64.0,
                     help="HBM for the server's lnL arena (12 B per evaluated row)")
     ap.add_argument("--serving-plain", type=int, default=64,

@@ -64,7 +64,10 @@ const char *jk_status_string(int status);
 int jk_abi_version(void);
 
 /* Select and initialise the device; idempotent.  Replaces nothing in the
- * reference (CPU only); required before any other call on `device`. */
+ * reference (CPU only); required before any other call on `device`.  Sets the
+ * release threshold of the device's default stream-ordered memory pool to
+ * 32 GiB, so that the scratch the settling and rejection calls take with
+ * hipMallocAsync is reused from call to call. */
 int jk_init(int device);
 
 /* Build the per-star constants.  Replaces CJokerHelper.__init__
@@ -248,6 +251,14 @@ int jk_lnl_batch(const jk_batch *batch, int precision, int64_t n_items,
 int jk_reject_batch(const double *ll, int n_seg, const jk_segment *segs, int64_t *idx_out,
                     int64_t *n_pass_out, double *llmax_out, void *stream);
 
+/* jk_reject_batch that also sets zero_u_out[i] (device int64, cleared first)
+ * to 1 when one of segment i's uniforms is exactly 0 (probability 2^-53 per
+ * row): the one case that settling with a margin below 746 leaves open
+ * (jk_settle_live_batch).  No reference counterpart. */
+int jk_reject_batch_zero_check(const double *ll, int n_seg, const jk_segment *segs,
+                               int64_t *idx_out, int64_t *n_pass_out, double *llmax_out,
+                               int64_t *zero_u_out, void *stream);
+
 /* One segment of jk_polish_batch: ll[offset .. offset + n) holds the lnL of
  * star `star` (star-table index) on cache rows row_begin .. (or
  * rows[row_begin ..] when a rows array is given); entries from `first` on are
@@ -323,9 +334,12 @@ typedef struct {
  * precision then lies more than margin below the maximum in both precisions,
  * so exp(lnL - max) is 0 and the entry is rejected whatever the uniform;
  * after a call on every new entry, jk_reject_batch over [0, n) accepts
- * exactly the FP64 set under any stream.  margin >= 746 (exp underflows to
- * 0 below -745.14); 750 is the value the library uses.  Work is proportional
- * to the new entries only.  n_settled_out[i] (device, int64) receives segment
+ * exactly the FP64 set under any stream.  That needs margin >= 746 (exp
+ * underflows to 0 below -745.14; the library uses 750).  A margin from 40 up
+ * covers every uniform but exactly 0 (exp(-40) < 2^-53, the smallest
+ * positive uniform): jk_reject_batch_zero_check reports that case, and the
+ * caller then settles the segment with margin 750 from first = 0 and repeats
+ * the rejection step.  Work is proportional to the new entries only.  n_settled_out[i] (device, int64) receives segment
  * i's count.  segs is a HOST array.  No reference counterpart. */
 int jk_settle_live_batch(const jk_batch *batch, int n_seg, const jk_live_segment *segs,
                          const double *P, const double *e, const double *omega, const double *M0,

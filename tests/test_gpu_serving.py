@@ -36,12 +36,16 @@ def _stars(n, seed):
 
 
 @pytest.mark.parametrize("precision", ["mixed", "f64"])
-@pytest.mark.parametrize("in_memory", [False, True])
-def test_server_results_equal_single_star_runs(cache, precision, in_memory):
+@pytest.mark.parametrize("in_memory,budget", [(False, None), (True, None), (False, 60000),
+                                              (True, 60000)])
+def test_server_results_equal_single_star_runs(cache, precision, in_memory, budget):
+    """Whole rounds per step, or rounds spread over steps under a work
+    budget: every result equals the star run alone."""
     prior = _prior()
     datas = _stars(10, seed=5)
     srv = S.JokerServer(prior, cache, n_requested=16, precision=precision, max_active=4,
-                        in_memory=in_memory, growth_factor=8, shape=B.LaunchShape(128, 0, 1024))
+                        in_memory=in_memory, growth_factor=8, shape=B.LaunchShape(128, 0, 1024),
+                        max_step_work=budget, min_chunk_rows=256)
     for i, d in enumerate(datas):
         srv.submit(d, np.random.default_rng(1000 + i), rid=i)
     while srv.busy():
@@ -117,3 +121,29 @@ def test_server_restarts_under_memory_pressure(cache, precision):
         assert np.array_equal(r.result["P"].value, alone["P"].value), i
         assert np.array_equal(r.result["K"].value, alone["K"].value), i
     assert srv.alloc.used() == 0
+
+
+def test_zero_uniform_takes_the_exact_path(cache):
+    """A request whose first rejection step draws a uniform of exactly 0: the
+    server notices it, settles that request for every uniform and repeats its
+    rejection step; the result equals the star run alone."""
+    from tests.test_serving_host import zero_uniform_rng
+    prior = _prior()
+    datas = _stars(3, seed=23)
+    srv = S.JokerServer(prior, cache, n_requested=16, precision="mixed", max_active=3,
+                        growth_factor=8, shape=B.LaunchShape(128, 0, 1024))
+    rngs = [zero_uniform_rng(37), np.random.default_rng(2), zero_uniform_rng(5, inc_seed=9)]
+    for i, d in enumerate(datas):
+        srv.submit(d, rngs[i], rid=i)
+    while srv.busy():
+        srv.step()
+    assert srv.zero_uniform_events >= 2
+    got = sorted(srv.finished, key=lambda r: r.id)
+    alone_rngs = [zero_uniform_rng(37), np.random.default_rng(2), zero_uniform_rng(5, inc_seed=9)]
+    for i, r in enumerate(got):
+        assert r.error is None, r.error
+        joker = tj.TheJoker(prior, rng=alone_rngs[i], precision="f64")
+        alone = joker.iterative_rejection_sample(datas[i], cache, n_requested_samples=16,
+                                                 growth_factor=8)
+        assert np.array_equal(r.result["P"].value, alone["P"].value), i
+        assert np.array_equal(r.result["K"].value, alone["K"].value), i

@@ -81,3 +81,58 @@ def test_random_operations_keep_the_tiling():
             else:
                 held[k] = (t, m)
         _check(al, held)
+
+
+def test_share_budget():
+    from thejoker_amd.serving import share_budget
+    # no budget pressure: everyone gets its whole remainder
+    assert list(share_budget([10, 20, 30], [1, 2, 3], 10_000, 1)) == [10, 20, 30]
+    # small requests finish, the large ones split what is left equally (in work)
+    out = share_budget([5, 1000, 1000], [10, 10, 20], 6050, 1)
+    assert out[0] == 5
+    assert out[1] * 10 == 3000 and out[2] * 20 == 3000
+    # the minimum chunk wins over a tiny share
+    assert list(share_budget([1000, 1000], [100, 100], 10, 64)) == [64, 64]
+    # random cases: never more than a remainder, at least min(rem, min_rows),
+    # and within budget whenever the minimum chunk does not force more
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        n = int(rng.integers(1, 20))
+        rem = rng.integers(1, 10**6, n)
+        M = rng.integers(3, 200, n)
+        budget = int(rng.integers(10**4, 10**9))
+        out = share_budget(rem, M, budget, 256)
+        assert (out <= rem).all() and (out >= np.minimum(rem, 256)).all()
+        forced = np.minimum(rem, 256) * M
+        assert (out * M).sum() <= max(budget, 0) + forced.sum()
+
+
+def zero_uniform_rng(k, inc_seed=5):
+    """A numpy Generator (PCG64) whose k-th uniform (0-based) is exactly 0:
+    the state after k + 1 LCG steps is 0, whose XSL-RR output is 0."""
+    inc = np.random.default_rng(inc_seed).bit_generator.state["state"]["inc"]
+    mult = 0x2360ED051FC65DA44385DF649FCCF645
+    inv = pow(mult, -1, 1 << 128)
+    s = 0
+    for _ in range(k + 1):
+        s = ((s - inc) * inv) % (1 << 128)
+    # a fixed seed sequence: spawned child generators must be reproducible too
+    bg = np.random.PCG64(inc_seed)
+    bg.state = {"bit_generator": "PCG64", "state": {"state": s, "inc": inc},
+                "has_uint32": 0, "uinteger": 0}
+    return np.random.Generator(bg)
+
+
+def test_zero_uniform_rng_construction():
+    for k in (0, 1, 37, 1000):
+        u = zero_uniform_rng(k).uniform(size=k + 2)
+        assert u[k] == 0.0 and (np.delete(u, k) > 0).all()
+
+
+def test_margins_cover_the_uniforms():
+    """The settling margins: exp(-40) is below the smallest positive uniform
+    2^-53, and exp underflows to exactly 0 below -745.14 (so -746 and -750)."""
+    from thejoker_amd.batch import LIVE_MARGIN, LIVE_MARGIN_NONZERO_U
+    assert np.exp(-LIVE_MARGIN_NONZERO_U) < 2.0 ** -53
+    assert np.exp(-746.0) == 0.0 and np.exp(-LIVE_MARGIN) == 0.0
+    assert np.exp(-745.0) > 0.0

@@ -59,6 +59,7 @@ SIGNATURES = [
      [_vp, ctypes.c_int, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int,
       ctypes.c_int, _vp, _vp]),
     ("jk_reject_batch", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]),
+    ("jk_reject_batch_zero_check", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("jk_polish_batch", ctypes.c_int,
      [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _vp, _vp]),
     ("jk_settle_batch", ctypes.c_int,

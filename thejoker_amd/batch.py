@@ -20,7 +20,8 @@ from .device import (DeviceStar, _dptr, _split, _stream, _torch, pcg64_advance_s
 
 __all__ = ["WORK_ITEM_DTYPE", "SEGMENT_DTYPE", "POLISH_SEGMENT_DTYPE", "SETTLE_SEGMENT_DTYPE",
            "LIVE_SEGMENT_DTYPE", "StarBatch", "work_items", "lnl_batch", "reject_batch",
-           "polish_batch", "settle_batch", "settle_live_batch",
+           "polish_batch", "settle_batch", "settle_live_batch", "LIVE_MARGIN",
+           "LIVE_MARGIN_NONZERO_U",
            "autotune", "LaunchShape", "rejection_sample_many", "SegPlan", "StepGraph"]
 
 WORK_ITEM_DTYPE = np.dtype([("star", "<i4"), ("n_rows", "<i4"), ("row_begin", "<i8"),
@@ -29,7 +30,10 @@ POLISH_SEGMENT_DTYPE = np.dtype([("offset", "<i8"), ("n", "<i8"), ("star", "<i4"
                                  ("reserved", "<i4"), ("row_begin", "<i8"), ("first", "<i8")])
 LIVE_SEGMENT_DTYPE = np.dtype([("offset", "<i8"), ("n", "<i8"), ("first", "<i8"), ("star", "<i4"),
                                ("reserved", "<i4"), ("row_begin", "<i8"), ("floor", "<f8")])
+# Settling margins of jk_settle_live_batch: every uniform (exp underflows to
+# 0 below -745.14), and every uniform but exactly 0 (exp(-40) < 2^-53).
 LIVE_MARGIN = 750.0
+LIVE_MARGIN_NONZERO_U = 40.0
 SETTLE_SEGMENT_DTYPE = np.dtype([("offset", "<i8"), ("n", "<i8"), ("star", "<i4"),
                                  ("reserved", "<i4"), ("row_begin", "<i8"), ("state_hi", "<u8"),
                                  ("state_lo", "<u8"), ("inc_hi", "<u8"), ("inc_lo", "<u8"),
@@ -172,9 +176,11 @@ def lnl_batch(batch, cache, d_items, n_items, n_out, precision="mixed", shape=No
     return out
 
 
-def reject_batch(ll, offsets, sizes, rng_states, max_keep, stream_offsets=None, device=0):
+def reject_batch(ll, offsets, sizes, rng_states, max_keep, stream_offsets=None, device=0,
+                 zero_check=False):
     """Segmented rejection step.  rng_states: list of (state, inc).  Returns
-    (list of index arrays, n_pass array, llmax array)."""
+    (list of index arrays, n_pass array, llmax array), and with zero_check a
+    fourth array: 1 for the segments one of whose uniforms is exactly 0."""
     torch = _torch()
     lib = _lib.ensure_device(device)
     n_seg = len(sizes)
@@ -189,21 +195,27 @@ def reject_batch(ll, offsets, sizes, rng_states, max_keep, stream_offsets=None, 
         segs["inc_hi"][i], segs["inc_lo"][i] = _split(int(inc))
     if stream_offsets is not None:
         segs["stream_offset"] = stream_offsets
-    # one device block [n_pass (n_seg), max (n_seg), idx...], one copy back
-    res = torch.empty(2 * n_seg + max(int(mk.sum()), 1), dtype=torch.int64, device=ll.device)
+    # one device block [n_pass (n_seg), max (n_seg), zero (n_seg), idx...],
+    # one copy back
+    res = torch.empty(3 * n_seg + max(int(mk.sum()), 1), dtype=torch.int64, device=ll.device)
     p = res.data_ptr()
-    _lib.check(lib.jk_reject_batch(_dptr(ll), n_seg, segs.ctypes.data_as(ctypes.c_void_p),
-                                   ctypes.c_void_p(p + 16 * n_seg), ctypes.c_void_p(p),
-                                   ctypes.c_void_p(p + 8 * n_seg), _stream(device)),
-               "jk_reject_batch")
+    args = [_dptr(ll), n_seg, segs.ctypes.data_as(ctypes.c_void_p),
+            ctypes.c_void_p(p + 24 * n_seg), ctypes.c_void_p(p), ctypes.c_void_p(p + 8 * n_seg)]
+    if zero_check:
+        _lib.check(lib.jk_reject_batch_zero_check(*args, ctypes.c_void_p(p + 16 * n_seg),
+                                                  _stream(device)), "jk_reject_batch_zero_check")
+    else:
+        _lib.check(lib.jk_reject_batch(*args, _stream(device)), "jk_reject_batch")
     h = to_host(res)
     n_pass = h[:n_seg].copy()
     mx = h[n_seg:2 * n_seg].view(np.float64).copy()
-    idx_h = h[2 * n_seg:]
+    idx_h = h[3 * n_seg:]
     out = []
     for i in range(n_seg):
         k = int(min(n_pass[i], mk[i]))
         out.append(idx_h[segs["idx_offset"][i]:segs["idx_offset"][i] + k].copy())
+    if zero_check:
+        return out, n_pass, mx, h[2 * n_seg:3 * n_seg].copy()
     return out, n_pass, mx
 
 
@@ -261,12 +273,14 @@ def settle_batch(batch, cache, ll, bound, offsets, sizes, rng_states, stars=None
 
 
 def settle_live_batch(batch, cache, ll, bound, offsets, sizes, firsts, floors, stars=None,
-                      row_begins=None, rows=None, device=0):
+                      row_begins=None, rows=None, device=0, margin=LIVE_MARGIN):
     """jk_settle_live_batch: settle the new entries [firsts[i], sizes[i]) of
     each growing mixed-precision segment for every future uniform (floors[i]:
     the FP64 maximum of the entries before, -inf for none), so that the
     rejection step over the whole segment accepts the FP64 set under any
-    stream.  Returns a device int64 tensor of per-segment counts."""
+    stream -- with margin LIVE_MARGIN_NONZERO_U, under any stream none of whose
+    uniforms is exactly 0 (``reject_batch(..., zero_check=True)`` tells).
+    Returns a device int64 tensor of per-segment counts."""
     torch = _torch()
     lib = _lib.ensure_device(device)
     n_seg = len(sizes)
@@ -281,7 +295,7 @@ def settle_live_batch(batch, cache, ll, bound, offsets, sizes, firsts, floors, s
     _lib.check(lib.jk_settle_live_batch(batch.handle, n_seg, segs.ctypes.data_as(ctypes.c_void_p),
                                         _dptr(cache.P), _dptr(cache.e), _dptr(cache.omega),
                                         _dptr(cache.M0), _dptr(cache.s), _dptr(rows), _dptr(ll),
-                                        _dptr(bound), LIVE_MARGIN, _dptr(count),
+                                        _dptr(bound), float(margin), _dptr(count),
                                         _stream(device)),
                "jk_settle_live_batch")
     return count

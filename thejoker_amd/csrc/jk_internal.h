@@ -84,7 +84,8 @@ hipError_t jk_launch_lnl_batch(const jk_batch *b, int precision, int64_t n_items
 hipError_t jk_launch_reject_batch(const double *ll, int n_seg, const SegDev *d_segs,
                                   int64_t max_n, int64_t max_nb, double *partials, int pb,
                                   uint64_t *mask, int64_t *bsum, int64_t *idx_out,
-                                  int64_t *n_pass_out, double *llmax_out, hipStream_t stream);
+                                  int64_t *n_pass_out, double *llmax_out, hipStream_t stream,
+                                  int64_t *zero_u = nullptr);
 void jk_host_pcg_jump(u128 state, u128 inc, uint64_t k, u128 *out);
 hipError_t jk_launch_polish_batch(const jk_batch *b, int n_seg, const SegDev *d_segs,
                                   int64_t max_n, int64_t max_nb, double *partials, int pb,

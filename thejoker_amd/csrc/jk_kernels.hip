@@ -1226,10 +1226,14 @@ __global__ __launch_bounds__(64) void k_seg_max_final(const double *__restrict__
     if (threadIdx.x == 0) llmax[blockIdx.x] = m;
 }
 
+// zero_u (optional): zero_u[segment] is set to 1 when one of the segment's
+// uniforms is exactly 0 (probability 2^-53 per row), the one case settling
+// with a margin below 746 leaves open.
 __global__ __launch_bounds__(256) void k_seg_flags(const double *__restrict__ ll,
                                                    const SegDev *__restrict__ segs,
                                                    const double *__restrict__ llmax, u128 AT,
-                                                   u128 DT, uint64_t *__restrict__ mask)
+                                                   u128 DT, uint64_t *__restrict__ mask,
+                                                   int64_t *__restrict__ zero_u)
 {
     const SegDev sg = segs[blockIdx.y];
     const int lane = threadIdx.x & 63;
@@ -1244,6 +1248,7 @@ __global__ __launch_bounds__(256) void k_seg_flags(const double *__restrict__ ll
         if (i < sg.n) {
             const double uu = pcg_to_double(pcg_output(s));
             keep = exp(ll[sg.off + i] - mx) > uu;
+            if (zero_u != nullptr && uu == 0.0) zero_u[blockIdx.y] = 1;
         }
         const uint64_t word = __ballot(keep);
         if (lane == 0) mask[sg.mask_off + (base >> 6)] = word;
@@ -1336,20 +1341,30 @@ int jk_seg_flag_blocks(int64_t max_n)
     return (int)b;
 }
 
+__global__ __launch_bounds__(256) void k_fill_i64(int64_t *__restrict__ p, int n, int64_t v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 hipError_t jk_launch_reject_batch(const double *ll, int n_seg, const SegDev *d_segs,
                                   int64_t max_n, int64_t max_nb, double *partials, int pb,
                                   uint64_t *mask, int64_t *bsum, int64_t *idx_out,
-                                  int64_t *n_pass_out, double *llmax_out, hipStream_t stream)
+                                  int64_t *n_pass_out, double *llmax_out, hipStream_t stream,
+                                  int64_t *zero_u)
 {
     if (n_seg <= 0) return hipSuccess;
     const int fb = jk_seg_flag_blocks(max_n);
     u128 AT, DT;
     jk_pcg_jump_coeffs((uint64_t)fb * 256, &AT, &DT);
+    if (zero_u)
+        hipLaunchKernelGGL(k_fill_i64, dim3((n_seg + 255) / 256), dim3(256), 0, stream, zero_u,
+                           n_seg, (int64_t)0);
     hipLaunchKernelGGL(k_seg_max_partial, dim3(pb, n_seg), dim3(256), 0, stream, ll, d_segs,
                        partials, pb);
     hipLaunchKernelGGL(k_seg_max_final, dim3(n_seg), dim3(64), 0, stream, partials, pb, llmax_out);
     hipLaunchKernelGGL(k_seg_flags, dim3(fb, n_seg), dim3(256), 0, stream, ll, d_segs, llmax_out,
-                       AT, DT, mask);
+                       AT, DT, mask, zero_u);
     hipLaunchKernelGGL(k_seg_bsum, dim3((unsigned)max_nb, n_seg), dim3(256), 0, stream, d_segs,
                        mask, bsum);
     hipLaunchKernelGGL(k_seg_compact, dim3((unsigned)max_nb, n_seg), dim3(256), 0, stream, d_segs,

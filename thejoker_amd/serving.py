@@ -9,13 +9,16 @@ request in one persistent ``jk_lnl_batch`` launch (stars of different epoch
 counts side by side) writing straight into each request's region of one HBM
 lnL arena (mixed precision: with each row's error bound in a parallel
 arena), then re-evaluates in FP64 the new rows that the bound leaves within
-reach of the maximum, which makes every row decide like FP64 in this and all
-later rounds (``jk_settle_live_batch``), and runs every request's rejection
-step with its own numpy PCG64 stream (``jk_reject_batch``).  The iteration
-state of each request is :class:`thejoker_amd.thejoker.IterativeRejection`
--- the code the single-star API runs -- and the batched lnL is bit-identical
-to the single-star kernel, so a request's result is exactly what
-``TheJoker.iterative_rejection_sample`` returns for that star alone.
+40 of the maximum, which makes every row decide like FP64 in this and all
+later rounds under every uniform but exactly 0 (``jk_settle_live_batch``),
+and runs every request's rejection step with its own numpy PCG64 stream
+(``jk_reject_batch_zero_check``; a zero uniform, probability 2^-53 per row,
+settles that request's rows for every uniform and repeats its rejection
+step).  The iteration state of each request is
+:class:`thejoker_amd.thejoker.IterativeRejection` -- the code the single-star
+API runs -- and the batched lnL is bit-identical to the single-star kernel,
+so a request's result is exactly what ``TheJoker.iterative_rejection_sample``
+returns for that star alone.
 
 When every active request waits for arena memory to grow, the most recently
 admitted one is restarted: its region and slot are freed and it is queued
@@ -27,14 +30,14 @@ import time
 
 import numpy as np
 
-from .batch import LaunchShape, StarBatch, heaviest_first, lnl_batch, reject_batch, \
-    settle_live_batch, upload_items, work_items
+from .batch import LIVE_MARGIN, LIVE_MARGIN_NONZERO_U, LaunchShape, StarBatch, heaviest_first, \
+    lnl_batch, reject_batch, settle_live_batch, upload_items, work_items
 from .device import (DeviceStar, _torch, linear_internal_units, pcg64_state, posterior_inputs_many,
                      star_arrays)
 from .likelihood_helpers import validate_prepare_data
 from .thejoker import IterativeRejection, TheJoker
 
-__all__ = ["Request", "RangeAllocator", "JokerServer", "run_poisson"]
+__all__ = ["Request", "RangeAllocator", "JokerServer", "run_poisson", "share_budget"]
 
 
 class RangeAllocator:
@@ -94,6 +97,23 @@ class RangeAllocator:
         return self.capacity - sum(b - a for a, b in self.free)
 
 
+def share_budget(rem, M, budget, min_rows):
+    """Rows for each request from a budget of sample-epochs: in increasing
+    order of remaining work rem[i] * M[i], each takes its remainder or an
+    equal share of the budget left (at least min_rows), whichever is less."""
+    rem = np.asarray(rem, dtype=np.int64)
+    M = np.asarray(M, dtype=np.int64)
+    left = float(budget)
+    out = np.zeros(len(rem), dtype=np.int64)
+    order = np.argsort(rem * M, kind="stable")
+    for k, i in enumerate(order):
+        share = left / (len(order) - k)
+        rows = min(int(rem[i]), max(int(min_rows), int(share // M[i])))
+        out[i] = rows
+        left -= rows * M[i]
+    return out
+
+
 class Request:
     def __init__(self, rid, data, rng, arrival):
         self.id = rid
@@ -114,6 +134,7 @@ class Request:
         self.rng_state0 = None  # the stream at admission, for a restart
         self.preempted = 0      # times restarted to free arena memory
         self.need_hint = 0      # arena values held when last preempted
+        self.round_done = 0     # rows of the current round already evaluated
 
     @property
     def latency(self):
@@ -141,6 +162,14 @@ class JokerServer:
         Synchronise the device between the phases of every step and add each
         phase's wall time to ``self.phase_s`` (admit, plan, lnl, polish,
         reject, update, finish).
+    max_step_work : int or None
+        Sample-epochs (rows x epochs) one step may evaluate; None evaluates
+        every active request's whole round in each step.  A round larger than
+        a request's share is spread over several steps (its rejection step
+        runs when the round is complete), so that short requests are not held
+        behind long ones.
+    min_chunk_rows : int
+        Smallest part of a round a step evaluates under a budget.
     """
 
     PHASES = ("admit", "plan", "lnl", "polish", "reject", "update", "finish")
@@ -148,8 +177,10 @@ class JokerServer:
     def __init__(self, prior, cache, n_requested=256, precision="mixed", jitter_mode="physical",
                  max_active=64, hbm_budget_bytes=4 << 30, in_memory=False, growth_factor=128,
                  n_linear_samples=1, shape=None, device=0, clock=time.perf_counter,
-                 profile=False):
+                 profile=False, max_step_work=None, min_chunk_rows=1 << 16):
         torch = _torch()
+        self.max_step_work = None if max_step_work is None else int(max_step_work)
+        self.min_chunk_rows = int(min_chunk_rows)
         self.prior = prior
         self.cache = cache
         self.n_requested = int(n_requested)
@@ -182,9 +213,13 @@ class JokerServer:
         self.profile = profile
         self.phase_s = dict.fromkeys(self.PHASES, 0.0)
         self._t_mark = None
+        self._cur = {}
+        self.step_profile = []    # profile: per step, phase times and sizes
+        self.last_step = {}
         self._to_finish = []      # retired requests whose samples are still to be drawn
         self._waiting = 0         # active requests the last step could not grow
         self.preemptions = 0
+        self.zero_uniform_events = 0
         self.region_moves = {"extend": 0, "move": 0, "overlap": 0}
         self._helper = TheJoker(prior, rng=np.random.default_rng(0), jitter_mode=jitter_mode,
                                 precision=precision, device=device)
@@ -210,6 +245,7 @@ class JokerServer:
         t = time.perf_counter()
         if phase is not None and self._t_mark is not None:
             self.phase_s[phase] += t - self._t_mark
+            self._cur[phase] = self._cur.get(phase, 0.0) + t - self._t_mark
         self._t_mark = t
 
     # ------------------------------------------------------------------
@@ -270,6 +306,7 @@ class JokerServer:
         r.need_hint = r.region[1]
         self._release(r)
         r.it = None
+        r.round_done = 0
         r.rng.bit_generator.state = r.rng_state0
         r.preempted += 1
         self.preemptions += 1
@@ -326,14 +363,38 @@ class JokerServer:
             self.finished.append(r)
         return done
 
+    def _chunks(self, plan):
+        """Rows of each planned round to evaluate in this step.  Without a
+        step budget: all that remain.  With one (max_step_work sample-epochs),
+        the budget is shared like a processor: requests in increasing order of
+        remaining work each take up to an equal share of what is left (at
+        least min_chunk_rows), so short requests finish in one short step."""
+        rem = np.array([n - r.round_done for r, _, n in plan], dtype=np.int64)
+        if self.max_step_work is None:
+            return rem
+        M = self._table.M[[r.slot for r, _, _ in plan]]
+        return share_budget(rem, M, self.max_step_work, self.min_chunk_rows)
+
     def step(self):
-        """Admit, evaluate one round of every active request, retire the
-        finished.  The host work of the requests retired in the previous step
-        (their linear-parameter draws) runs while this step's likelihood
-        launch is on the device.  Returns the requests whose results were
-        completed in this call."""
+        """Admit, evaluate the next rows of every active request (a whole
+        round, or a share of the step budget), and run the rejection step of
+        the requests whose round is complete; retire the finished.  The host
+        work of the requests retired in the previous step (their
+        linear-parameter draws) runs while this step's likelihood launch is
+        on the device.  Returns the requests whose results were completed in
+        this call."""
+        self._cur = {}
+        tick = [time.perf_counter()]
+        host = {}
+
+        def lap(name):
+            t = time.perf_counter()
+            host[name] = host.get(name, 0.0) + t - tick[0]
+            tick[0] = t
+
         self._mark()
         self._admit()
+        lap("admit")
         self._mark("admit")
         if not self.active:
             done = self._build_results()
@@ -343,7 +404,8 @@ class JokerServer:
             plan = []
             for r in self.active:
                 rows, start, n_now = r.it.next_batch()
-                if self._ensure_region(r, r.it.n_evals + n_now):
+                # a round's region is sized when the round starts
+                if r.round_done or self._ensure_region(r, r.it.n_evals + n_now):
                     plan.append((r, start, n_now))
             self._waiting = len(self.active) - len(plan)
             if plan:
@@ -358,8 +420,10 @@ class JokerServer:
                 return [r]
             self._preempt(max(self.active, key=lambda q: q.t_admit))
         slots = np.array([r.slot for r, _, _ in plan], dtype=np.int64)
-        sizes = np.array([n for _, _, n in plan], dtype=np.int64)
-        begins = np.array([b for _, b, _ in plan], dtype=np.int64)
+        n_round = np.array([n for _, _, n in plan], dtype=np.int64)
+        done0 = np.array([r.round_done for r, _, _ in plan], dtype=np.int64)
+        sizes = self._chunks(plan)
+        begins = np.array([b for _, b, _ in plan], dtype=np.int64) + done0
         before = np.array([r.it.n_evals for r, _, _ in plan], dtype=np.int64)
         seg_off = np.array([r.region[0] for r, _, _ in plan], dtype=np.int64)
         # rows per work item: the launch shape's tile, widened so that a step
@@ -368,47 +432,81 @@ class JokerServer:
         items, off, total = work_items(sizes, tile, row_begin=begins, stars=slots)
         # write each request's rows straight into its arena region
         counts = (sizes + tile - 1) // tile
-        items["out_offset"] += np.repeat(seg_off + before - off, counts)
+        items["out_offset"] += np.repeat(seg_off + before + done0 - off, counts)
         items = items[heaviest_first(items, self._table.M)]
         d_items = upload_items(items, self.device)
+        lap("plan")
         self._mark("plan")
         if self.profile:
             finished = self._build_results()
             self._mark("finish")
         lnl_batch(self._table, self.cache, d_items, len(items), 0, self.precision, self.shape,
                   out=self.arena, workspace=self._ws, bound=self.arena_b)
+        lap("lnl_launch")
         if not self.profile:
             finished = self._build_results()      # overlaps the launch above
+            lap("finish")
         M = self._table.M[slots]
         self.sample_epochs += int(np.sum(sizes * M))
-        seg_n = before + sizes
         self._mark("lnl")
-        states = [pcg64_state(r.rng) for r, _, _ in plan]
         if self.precision == "mixed":
             # FP64 for the new rows within reach of the maximum: with them,
             # every row decides like FP64 in this and all later rounds
             floors = np.array([r.it.floor for r, _, _ in plan], dtype=np.float64)
-            settle_live_batch(self._table, self.cache, self.arena, self.arena_b, seg_off, seg_n,
-                              before, floors, stars=slots, device=self.device)
+            cnt = settle_live_batch(self._table, self.cache, self.arena, self.arena_b, seg_off,
+                                    before + done0 + sizes, before + done0, floors, stars=slots,
+                                    device=self.device, margin=LIVE_MARGIN_NONZERO_U)
+            lap("settle")
+            if self.profile:
+                self._cur["settled"] = int(cnt.sum())
         self._mark("polish")
-        good, n_pass, llmax = reject_batch(self.arena, seg_off, seg_n, states, self.n_requested,
-                                           device=self.device)
-        self._mark("reject")
+        for k, (r, _, _) in enumerate(plan):
+            r.round_done += int(sizes[k])
+            r.rows_evaluated += int(sizes[k])
+            r.sample_epochs += int(sizes[k] * M[k])
+        ends = [k for k, (r, _, n) in enumerate(plan) if r.round_done == n]
         retired = []
-        for k, (r, _, n_now) in enumerate(plan):
-            r.rows_evaluated += n_now
-            r.sample_epochs += int(n_now * M[k])
-            try:
-                if self.in_memory and not np.isfinite(llmax[k]):
-                    raise RuntimeError("There are NaN or Inf likelihood values in iteration step "
-                                       f"{r.it.iteration}!")
-                r.it.floor = float(llmax[k])
-                r.it.update(seg_n[k], good[k], n_pass[k])
-            except Exception as exc:
-                r.error = exc
-                r.it.done = True
-            if r.it.done:
-                retired.append(r)
+        if ends:
+            seg_n = before[ends] + n_round[ends]
+            states = [pcg64_state(plan[k][0].rng) for k in ends]
+            mixed = self.precision == "mixed"
+            got = reject_batch(self.arena, seg_off[ends], seg_n, states, self.n_requested,
+                               device=self.device, zero_check=mixed)
+            good, n_pass, llmax = got[:3]
+            if mixed and got[3].any():
+                # A uniform of exactly 0 accepts any row within 745 of the
+                # maximum, which the 40 margin did not settle: settle those
+                # segments for every uniform and repeat their rejection step.
+                z = np.flatnonzero(got[3])
+                ks = np.asarray(ends)[z]
+                settle_live_batch(self._table, self.cache, self.arena, self.arena_b, seg_off[ks],
+                                  seg_n[z], np.zeros(len(z), dtype=np.int64),
+                                  np.full(len(z), -np.inf), stars=slots[ks], device=self.device,
+                                  margin=LIVE_MARGIN)
+                g2, p2, m2 = reject_batch(self.arena, seg_off[ks], seg_n[z],
+                                          [states[j] for j in z], self.n_requested,
+                                          device=self.device)
+                for t, j in enumerate(z):
+                    good[j], n_pass[j], llmax[j] = g2[t], p2[t], m2[t]
+                self.zero_uniform_events += len(z)
+            lap("reject")
+            self._mark("reject")
+            for j, k in enumerate(ends):
+                r = plan[k][0]
+                r.round_done = 0
+                try:
+                    if self.in_memory and not np.isfinite(llmax[j]):
+                        raise RuntimeError("There are NaN or Inf likelihood values in iteration "
+                                           f"step {r.it.iteration}!")
+                    r.it.floor = float(llmax[j])
+                    r.it.update(seg_n[j], good[j], n_pass[j])
+                except Exception as exc:
+                    r.error = exc
+                    r.it.done = True
+                if r.it.done:
+                    retired.append(r)
+        else:
+            self._mark("reject")
         ok = [r for r in retired if r.error is None]
         if ok:
             posts = posterior_inputs_many([r.star for r in ok], self.cache,
@@ -423,7 +521,15 @@ class JokerServer:
             else:
                 self._fail(r, r.error)
         self.steps += 1
+        lap("update")
         self._mark("update")
+        self.last_step = dict(requests=len(plan), rows=int(sizes.sum()),
+                              sample_epochs=int(np.sum(sizes * M)), rounds_done=len(ends),
+                              finished=len(finished), retired=len(retired),
+                              host_s={k: round(v, 4) for k, v in host.items()})
+        if self.profile:
+            self._cur.update(self.last_step)
+            self.step_profile.append(self._cur)
         return finished
 
 
