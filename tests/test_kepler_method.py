@@ -117,3 +117,70 @@ def test_truncated_series_would_fail():
     T = newton_ld(M, e, E)
     cond = np.finfo(float).eps * (np.abs(T) + e * np.abs(np.sin(T)) + M) / (1 - e * np.cos(T))
     assert (np.abs((E - T).astype(float)) / cond).max() > 10
+
+
+def _sincos_0pi_f32(x):
+    """The mixed kernel's FP32 sin/cos on [0, pi] (Taylor on [0, pi/2] after
+    reflection), in numpy float32."""
+    y = x.astype(f32)
+    hi = y > f32(1.57079632679)
+    y = np.where(hi, (PIF - y) + f32(-8.742278e-08), y).astype(f32)
+    y2 = y * y
+    ps = f32(1.6059043836821613e-10)
+    for c in (-2.505210838544172e-08, 2.7557319223985893e-06, -0.0001984126984126984,
+              0.008333333333333333, -0.16666666666666666):
+        ps = (ps * y2 + f32(c)).astype(f32)
+    s = (ps * y2 * y + y).astype(f32)
+    pc = f32(-1.1470745597729725e-11)
+    for c in (2.08767569878681e-09, -2.755731922398589e-07, 2.48015873015873e-05,
+              -0.001388888888888889, 0.041666666666666664, -0.5):
+        pc = (pc * y2 + f32(c)).astype(f32)
+    c = (pc * y2 + f32(1)).astype(f32)
+    return s, np.where(hi, -c, c).astype(f32)
+
+
+def _mixed_design(M, e, om, newton=True):
+    """The mixed row's design value m: FP32 starter, FP32 sin/cos, one
+    first-order Newton step in FP32 (fused multiply-adds rounded once),
+    1 / (1 - e cos E) in FP32, the combination in FP64."""
+    ef = np.minimum(e.astype(f32), f32(0.99999994))
+    E1 = starter(M.astype(f32), ef)
+    s1, c1 = _sincos_0pi_f32(E1)
+    if newton:
+        f0 = (E1.astype(np.float64) - M.astype(f32)).astype(f32)
+        f0 = (f0 - ef.astype(np.float64) * s1).astype(f32)
+        f1 = (1.0 - ef.astype(np.float64) * c1).astype(f32)
+        t = (-f0 * (f32(1) / f1)).astype(f32)
+        sE = (c1.astype(np.float64) * t + s1).astype(f32)
+        cE = (-s1.astype(np.float64) * t + c1).astype(f32)
+    else:
+        sE, cE = s1, c1
+    den = (f32(1) / (f32(1) - ef * cE)).astype(f32)
+    m = (np.cos(om) * (cE.astype(np.float64) - e) - np.sin(om) * np.sqrt((1 - e) * (1 + e)) * sE) \
+        * den + e * np.cos(om)
+    return m, den.astype(np.float64)
+
+
+@pytest.mark.parametrize("dist", ["beta", 0.9, 0.999, 0.99999, 1 - 1e-6])
+def test_mixed_design_error_within_bound_model(dist):
+    """The per-epoch error model of the mixed error bound,
+    eps = 2^-22 (1 + d)^2 (1 + d / 1000) with d = 1 / (1 - e cos E), covers
+    the FP32 chain's design-column error with a factor 2 to spare; without
+    the Newton step the starter's own error breaks it for e up to 0.999 (the
+    check has teeth; nearer 1 the model's (1 + d)^2 grows faster than the
+    starter's error)."""
+    rng = np.random.default_rng(11)
+    n = 300_000
+    e = rng.beta(0.867, 3.03, n) if dist == "beta" else np.full(n, dist)
+    M = np.concatenate([rng.uniform(0, np.pi, n // 2), np.exp(rng.uniform(np.log(1e-9), 0, n - n // 2))])
+    om = rng.uniform(0, 2 * np.pi, n)
+    _, s64, c64, _ = solve(M, e)
+    den64 = 1 / (1 - e * c64)
+    m64 = (np.cos(om) * (c64 - e) - np.sin(om) * np.sqrt(1 - e * e) * s64) * den64 + e * np.cos(om)
+    m, den = _mixed_design(M, e, om)
+    model = 2.0 ** -22 * (1 + den) ** 2 * (1 + den / 1000)
+    assert np.max(np.abs(m - m64) / model) < 0.5
+    if dist in ("beta", 0.9, 0.999):
+        m0, den0 = _mixed_design(M, e, om, newton=False)
+        model0 = 2.0 ** -22 * (1 + den0) ** 2 * (1 + den0 / 1000)
+        assert np.max(np.abs(m0 - m64) / model0) > 1.0
