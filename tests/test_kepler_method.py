@@ -119,33 +119,43 @@ def test_truncated_series_would_fail():
     assert (np.abs((E - T).astype(float)) / cond).max() > 10
 
 
-def _sincos_0pi_f32(x):
-    """The mixed kernel's FP32 sin/cos on [0, pi] (Taylor on [0, pi/2] after
-    reflection), in numpy float32."""
-    y = x.astype(f32)
-    hi = y > f32(1.57079632679)
-    y = np.where(hi, (PIF - y) + f32(-8.742278e-08), y).astype(f32)
-    y2 = y * y
-    ps = f32(1.6059043836821613e-10)
-    for c in (-2.505210838544172e-08, 2.7557319223985893e-06, -0.0001984126984126984,
-              0.008333333333333333, -0.16666666666666666):
-        ps = (ps * y2 + f32(c)).astype(f32)
-    s = (ps * y2 * y + y).astype(f32)
-    pc = f32(-1.1470745597729725e-11)
-    for c in (2.08767569878681e-09, -2.755731922398589e-07, 2.48015873015873e-05,
-              -0.001388888888888889, 0.041666666666666664, -0.5):
-        pc = (pc * y2 + f32(c)).astype(f32)
-    c = (pc * y2 + f32(1)).astype(f32)
-    return s, np.where(hi, -c, c).astype(f32)
+_H32 = np.float64(np.float32(np.pi / 256))
+_G32 = np.arange(NT + 1, dtype=LD) * LD(_H32)
+_TS32 = np.sin(_G32).astype(f32)
+_TC32 = np.cos(_G32).astype(f32)
+
+
+def _sincos_tab_f32(x):
+    """The mixed kernel's FP32 sin/cos on [0, pi]: the FP32 grid i * H32
+    (H32 = pi / 256 rounded to FP32, values rounded once from long double),
+    d = x - i H32 rounded once (fma), sin d to d^3, cos d to d^2."""
+    E = x.astype(f32)
+    idx = np.clip((E * f32(256 / np.pi)).astype(np.int32), 0, NT)
+    d = (E.astype(np.float64) - idx * _H32).astype(f32)
+    d2 = d * d
+    sd = (d.astype(np.float64) * d2 * (-1 / 6) + d).astype(f32)
+    cd = (d2.astype(np.float64) * -0.5 + 1).astype(f32)
+    ts, tc = _TS32[idx], _TC32[idx]
+    s = (ts.astype(np.float64) * cd + (tc * sd)).astype(f32)
+    c = (tc.astype(np.float64) * cd - (ts * sd)).astype(f32)
+    return s, c
+
+
+def test_fp32_table_sincos_accuracy():
+    E = np.random.default_rng(3).uniform(0, np.pi, 300_000).astype(f32)
+    s, c = _sincos_tab_f32(E)
+    El = E.astype(LD)
+    assert np.abs((s - np.sin(El)).astype(float)).max() < 1.2e-7
+    assert np.abs((c - np.cos(El)).astype(float)).max() < 1.2e-7
 
 
 def _mixed_design(M, e, om, newton=True):
-    """The mixed row's design value m: FP32 starter, FP32 sin/cos, one
+    """The mixed row's design value m: FP32 starter, FP32 table sin/cos, one
     first-order Newton step in FP32 (fused multiply-adds rounded once),
     1 / (1 - e cos E) in FP32, the combination in FP64."""
     ef = np.minimum(e.astype(f32), f32(0.99999994))
     E1 = starter(M.astype(f32), ef)
-    s1, c1 = _sincos_0pi_f32(E1)
+    s1, c1 = _sincos_tab_f32(E1)
     if newton:
         f0 = (E1.astype(np.float64) - M.astype(f32)).astype(f32)
         f0 = (f0 - ef.astype(np.float64) * s1).astype(f32)

@@ -36,10 +36,20 @@ __constant__ u128 c_tid_D[256];
 #define JK_TAB_H 0x1.921fb54442e00p-7
 #define JK_TAB_INV_H_F 81.48733f
 __device__ double2 g_sincos_tab[JK_TAB_N + 1];
+// The FP32 grid for the mixed-precision Kepler step: g_i = i * JK_TAB_H32
+// (pi / 256 rounded to FP32), (sin g_i, cos g_i) rounded once to FP32.
+#define JK_TAB_H32 0x1.921fb6p-7f
+__device__ float2 g_sincos_tab32[JK_TAB_N + 1];
+// LDS image: the FP64 table, then the FP32 table packed two per double2 slot.
+#define JK_TAB_LDS (JK_TAB_N + 1 + (JK_TAB_N + 2) / 2)
 
 __device__ __forceinline__ void jk_stage_tab(double2 *s_tab)
 {
-    for (int i = threadIdx.x; i <= JK_TAB_N; i += blockDim.x) s_tab[i] = g_sincos_tab[i];
+    float2 *t32 = reinterpret_cast<float2 *>(s_tab + JK_TAB_N + 1);
+    for (int i = threadIdx.x; i <= JK_TAB_N; i += blockDim.x) {
+        s_tab[i] = g_sincos_tab[i];
+        t32[i] = g_sincos_tab32[i];
+    }
     __syncthreads();
 }
 
@@ -81,6 +91,25 @@ __device__ __forceinline__ void jk_sincos_tab(float Ef, const double2 *__restric
     const double cd = fma(d2, fma(d2, fma(d2, -1.0 / 720.0, 1.0 / 24.0), -0.5), 1.0);
     s = fma(t.x, cd, t.y * sd);
     c = fma(t.y, cd, -(t.x * sd));
+}
+
+// sin and cos in FP32 of E in [0, pi] (a little outside is fine) from the FP32
+// grid staged after the FP64 table: E = g_i + d, d = E - i H32 rounded once
+// (fma), |d| <= 1.01 H; sin d to d^3 and cos d to d^2 (truncation below
+// 1e-9, under FP32 rounding).  The index is clamped into the table.
+__device__ __forceinline__ void jk_sincos_tab_f32(float E, const double2 *__restrict__ tab,
+                                                  float &s, float &c)
+{
+    const float2 *__restrict__ t32 = reinterpret_cast<const float2 *>(tab + JK_TAB_N + 1);
+    int idx = (int)(E * JK_TAB_INV_H_F);
+    idx = min(max(idx, 0), JK_TAB_N);
+    const float2 t = t32[idx];
+    const float d = fmaf(-(float)idx, JK_TAB_H32, E);
+    const float d2 = d * d;
+    const float sd = fmaf(d * d2, -1.0f / 6.0f, d);
+    const float cd = fmaf(d2, -0.5f, 1.0f);
+    s = fmaf(t.x, cd, t.y * sd);
+    c = fmaf(t.y, cd, -(t.x * sd));
 }
 
 // ---------------------------------------------------------------------------
@@ -264,12 +293,13 @@ __device__ __forceinline__ bool jk_kepler_markley_f64(double x, double e, float 
 // (0.2) of the per-epoch error model of the bound, which FP32 rounding of
 // sin E, cos E and 1 / (1 - e cos E) sets, not the truncation (t^2 ~ 2e-7).
 __device__ __forceinline__ void jk_kepler_markley_f32(float x, float e, float ome, float a1,
+                                                      const double2 *__restrict__ tab,
                                                       float &sinE, float &cosE)
 {
     const float ax = fabsf(x);
     const float E1 = jk_markley_starter(ax, e, ome, a1);
     float s1, c1;
-    jk_sincos_0pi_f32(E1, s1, c1);
+    jk_sincos_tab_f32(E1, tab, s1, c1);
     const float f0 = fmaf(-e, s1, E1 - ax);
     const float t = -f0 * __builtin_amdgcn_rcpf(fmaf(-e, c1, 1.0f));
     const float s = fmaf(c1, t, s1);
@@ -374,7 +404,7 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
         float denb = 0.0f;
         if constexpr (MIXED) {
             float sE, cE;
-            jk_kepler_markley_f32((float)x, ef, omef, a1f, sE, cE);
+            jk_kepler_markley_f32((float)x, ef, omef, a1f, tab, sE, cE);
             const float den = __builtin_amdgcn_rcpf(fmaf(-ef, cE, 1.0f));
             // the FP32 (sin E, cos E) combined into the design value in FP64
             // (same issue rate as FP32 here): no rounding beyond their own
@@ -526,7 +556,7 @@ __global__ __launch_bounds__(256) void k_lnl(const StarDev st, int64_t n,
                                              double *__restrict__ blockmax,
                                              float *__restrict__ bound)
 {
-    __shared__ double2 s_tab[JK_TAB_N + 1];
+    __shared__ double2 s_tab[JK_TAB_LDS];
     __shared__ double red[4];
     jk_stage_tab(s_tab);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -566,7 +596,7 @@ __global__ __launch_bounds__(256) void k_lnl_scatter_f64(const StarDev st,
                                                          const int64_t *__restrict__ rows,
                                                          double *__restrict__ ll)
 {
-    __shared__ double2 s_tab[JK_TAB_N + 1];
+    __shared__ double2 s_tab[JK_TAB_LDS];
     jk_stage_tab(s_tab);
     const int64_t cnt = *count;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -931,16 +961,22 @@ hipError_t jk_upload_pcg_table(int device)
     return hipMemcpyToSymbol(HIP_SYMBOL(c_tid_D), tD, sizeof(tD), 0, hipMemcpyHostToDevice);
 }
 
-// The sin / cos grid of jk_sincos_tab, from 80-bit long double on the host
-// rounded once to FP64.
+// The sin / cos grids of jk_sincos_tab and jk_sincos_tab_f32, from 80-bit
+// long double on the host rounded once to FP64 / FP32.
 hipError_t jk_upload_sincos_table(int device)
 {
     double2 tab[JK_TAB_N + 1];
+    float2 tab32[JK_TAB_N + 1];
     for (int i = 0; i <= JK_TAB_N; ++i) {
         const long double g = (long double)i * (long double)JK_TAB_H;
         tab[i] = make_double2((double)sinl(g), (double)cosl(g));
+        const long double g32 = (long double)i * (long double)JK_TAB_H32;
+        tab32[i] = make_float2((float)sinl(g32), (float)cosl(g32));
     }
-    return hipMemcpyToSymbol(HIP_SYMBOL(g_sincos_tab), tab, sizeof(tab), 0,
+    hipError_t err = hipMemcpyToSymbol(HIP_SYMBOL(g_sincos_tab), tab, sizeof(tab), 0,
+                                       hipMemcpyHostToDevice);
+    if (err != hipSuccess) return err;
+    return hipMemcpyToSymbol(HIP_SYMBOL(g_sincos_tab32), tab32, sizeof(tab32), 0,
                              hipMemcpyHostToDevice);
 }
 
@@ -1097,7 +1133,7 @@ __global__ __launch_bounds__(256) void k_lnl_batch(const StarDev *__restrict__ s
                                                    float *__restrict__ bound)
 {
     __shared__ unsigned s_item;
-    __shared__ double2 s_tab[JK_TAB_N + 1];
+    __shared__ double2 s_tab[JK_TAB_LDS];
     jk_stage_tab(s_tab);
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
@@ -1438,7 +1474,7 @@ __global__ __launch_bounds__(256) void k_seg_scatter_f64(const StarDev *__restri
                                                          double *__restrict__ ll,
                                                          float *__restrict__ bound)
 {
-    __shared__ double2 s_tab[JK_TAB_N + 1];
+    __shared__ double2 s_tab[JK_TAB_LDS];
     jk_stage_tab(s_tab);
     const int lane = threadIdx.x & 63;
     const int64_t total_waves = cum[n_seg] >> 6;
