@@ -7,7 +7,9 @@ K ~ N(0, 30), v0 ~ N(0, 100), physical jitter mode.  One step = the FP64
 marginal-likelihood kernel over all 2^24 device-resident rows, the on-device
 rejection step (max, numpy-PCG64 uniforms, ordered compaction of at most 256
 indices) and the read-back of the accepted indices; value = prior-samples x
-epochs / s of the step.
+epochs / s of the step.  Steps are pipelined: the next step's stream depends
+only on the row count, so it is queued before the host collects the previous
+step's indices.
 
 Further legs (each an object of the same line; --workload picks one):
   mixed     -- the mixed-precision kernel on the same rows, its error
@@ -137,18 +139,32 @@ def leg_single(args, torch, np, D, syn):
     rng = np.random.default_rng(42)
     ll = torch.empty(n, dtype=torch.float64, device="cuda")
 
+    pending = []
+
     def step():
-        # lnL of every row and the rejection step (max reduced in the lnL launch)
-        _, idx, n_pass, mx = D.lnl_reject(dstar, cache, D.pcg64_state(rng), 256, out=ll)
+        # lnL of every row and the rejection step (max reduced in the lnL
+        # launch), queued; the stream of the next step depends only on n, so
+        # the previous step's accepted indices are collected while this one
+        # runs
+        pending.append(D.lnl_reject_async(dstar, cache, D.pcg64_state(rng), 256, out=ll))
         D.pcg64_advance_state(rng, n)
+        if len(pending) > 1:
+            _, idx, n_pass, _ = pending.pop(0).result()
+            return idx, n_pass
+        return None, None
+
+    def drain():
+        _, idx, n_pass, _ = pending.pop(0).result()
         return idx, n_pass
 
     for _ in range(args.warmup):
         step()
+    drain()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        idx, n_pass = step()
+        step()
+    idx, n_pass = drain()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     ms_step = dt / args.steps * 1e3

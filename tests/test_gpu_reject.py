@@ -125,3 +125,32 @@ def test_lnl_reject_equals_separate_calls(gpu, n):
                                        rows=rows)
     assert np.array_equal(ll.cpu().numpy(), want_ll.cpu().numpy(), equal_nan=True)
     assert np.array_equal(idx, want[0]) and n_pass == want[1]
+
+
+def test_lnl_reject_async_pipelined_equals_sequential(gpu):
+    """Calls queued back to back (each collected after the next is queued,
+    the stream advanced by n in between) give the sequential results."""
+    from thejoker_amd import synthetic as syn
+    n = 1 << 18
+    st = syn.single_star()
+    cols = syn.prior_columns(n, seed=4, jitter=True)
+    star = D.DeviceStar(st.t, st.rv, st.ivar(), st.trend(), np.zeros(2), np.array([900.0, 1e4]),
+                        st.t_ref)
+    cache = D.PriorCache(cols["P"], cols["e"], cols["omega"], cols["M0"], cols["s"])
+    g = np.random.default_rng(12)
+    want = []
+    for _ in range(4):
+        _, idx, n_pass, mx = D.lnl_reject(star, cache, D.pcg64_state(g), 64)
+        D.pcg64_advance_state(g, n)
+        want.append((idx, n_pass, mx))
+    g = np.random.default_rng(12)
+    ll = torch.empty(n, dtype=torch.float64, device="cuda")
+    pending, got = [], []
+    for _ in range(4):
+        pending.append(D.lnl_reject_async(star, cache, D.pcg64_state(g), 64, out=ll))
+        D.pcg64_advance_state(g, n)
+        if len(pending) > 1:
+            got.append(pending.pop(0).result()[1:])
+    got.append(pending.pop(0).result()[1:])
+    for (a, b, c), (x, y, z) in zip(want, got):
+        assert np.array_equal(a, x) and b == y and c == z

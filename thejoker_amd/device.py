@@ -21,7 +21,8 @@ from . import _lib
 from . import units as u
 from .samples import NONLINEAR_INTERNAL_UNITS, NONLINEAR_PACKED_ORDER, JokerSamples
 
-__all__ = ["DeviceStar", "PriorCache", "star_arrays", "lnl", "lnl_reject", "polish", "reject",
+__all__ = ["DeviceStar", "PriorCache", "star_arrays", "lnl", "lnl_reject", "lnl_reject_async",
+           "PendingReject", "polish", "reject",
            "posterior_inputs_many", "to_host",
            "linear_posterior", "uniforms",
            "pcg64_state", "pcg64_advance_state", "JITTER_MODES"]
@@ -347,10 +348,36 @@ def lnl(star, cache, rows=None, n=None, precision="f64", out=None, start=0, boun
     return out
 
 
+class PendingReject:
+    """Result of :func:`lnl_reject_async`: the copy of the accepted indices to
+    pinned host memory is queued behind the launches; :meth:`result` waits
+    for it alone, so work queued after it on the stream keeps running."""
+
+    def __init__(self, out, host, event, max_keep):
+        self._out, self._host, self._event, self._max_keep = out, host, event, max_keep
+
+    def result(self):
+        """(device lnL tensor, idx, n_pass, llmax)."""
+        self._event.synchronize()
+        h = self._host.numpy()
+        n_pass = int(h[0])
+        k = min(n_pass, self._max_keep)
+        return self._out, h[2:2 + k].copy(), n_pass, float(h[1:2].view(np.float64)[0])
+
+
 def lnl_reject(star, cache, rng_state, max_keep, rows=None, n=None, start=0, out=None, offset=0):
     """Plain FP64 lnL of cache rows (as :func:`lnl`) and the rejection step over
     them (as :func:`reject`) in one call (jk_lnl_reject_f64).  Returns
     (device lnL tensor, idx, n_pass, llmax)."""
+    return lnl_reject_async(star, cache, rng_state, max_keep, rows=rows, n=n, start=start,
+                            out=out, offset=offset).result()
+
+
+def lnl_reject_async(star, cache, rng_state, max_keep, rows=None, n=None, start=0, out=None,
+                     offset=0):
+    """:func:`lnl_reject` without the wait: returns a :class:`PendingReject`.
+    The host may queue the next call (with the stream advanced by n, which
+    does not depend on this result) before collecting this one."""
     torch = _torch()
     lib = _lib.ensure_device(cache.device)
     if rows is not None:
@@ -372,10 +399,11 @@ def lnl_reject(star, cache, rng_state, max_keep, rows=None, n=None, start=0, out
                                      *_split(inc), int(offset), max_keep, ctypes.c_void_p(p + 16),
                                      ctypes.c_void_p(p), ctypes.c_void_p(p + 8),
                                      _stream(cache.device)), "jk_lnl_reject_f64")
-    h = to_host(res)
-    n_pass = int(h[0])
-    k = min(n_pass, max_keep)
-    return out, h[2:2 + k].copy(), n_pass, float(h[1:2].view(np.float64)[0])
+    host = torch.empty(res.shape, dtype=res.dtype, pin_memory=True)
+    host.copy_(res, non_blocking=True)
+    ev = torch.cuda.Event()
+    ev.record(torch.cuda.current_stream(cache.device))
+    return PendingReject(out, host, ev, max_keep)
 
 
 def polish(star, cache, ll, rows=None, start=0, window=40.0, first=0):
