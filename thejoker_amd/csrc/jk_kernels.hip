@@ -112,35 +112,6 @@ __device__ __forceinline__ void jk_sincos_tab_f32(float E, const double2 *__rest
     c = fmaf(t.y, cd, -(t.x * sd));
 }
 
-// ---------------------------------------------------------------------------
-// sin and cos in FP32 on [0, pi] from truncated Taylor series on [0, pi/2]:
-// E <= pi/2: y = E;  E > pi/2: y = pi - E (exact subtraction by Sterbenz,
-// plus the low part of pi), sin E = sin y, cos E = -cos y.  sin to y^13, cos
-// to y^14: truncation below 7e-10.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void jk_sincos_0pi_f32(float E, float &s, float &c)
-{
-    const bool hi = E > 1.57079632679f;
-    const float y = hi ? ((JK_PI_F_HI - E) + JK_PI_F_LO) : E;
-    const float y2 = y * y;
-    float ps = 1.6059043836821613e-10f;
-    ps = fmaf(ps, y2, -2.505210838544172e-08f);
-    ps = fmaf(ps, y2, 2.7557319223985893e-06f);
-    ps = fmaf(ps, y2, -0.0001984126984126984f);
-    ps = fmaf(ps, y2, 0.008333333333333333f);
-    ps = fmaf(ps, y2, -0.16666666666666666f);
-    s = fmaf(ps * y2, y, y);
-    float pc = -1.1470745597729725e-11f;
-    pc = fmaf(pc, y2, 2.08767569878681e-09f);
-    pc = fmaf(pc, y2, -2.755731922398589e-07f);
-    pc = fmaf(pc, y2, 2.48015873015873e-05f);
-    pc = fmaf(pc, y2, -0.001388888888888889f);
-    pc = fmaf(pc, y2, 0.041666666666666664f);
-    pc = fmaf(pc, y2, -0.5f);
-    c = fmaf(pc, y2, 1.0f);
-    c = hi ? -c : c;
-}
-
 // Kepler in FP64 on ax = |x| in [0, pi] (see jk_kepler_f64 in jk_device.h for
 // the convergence argument); this variant evaluates sin/cos with the LDS
 // table (jk_sincos_tab_f64).
@@ -167,32 +138,6 @@ __device__ __forceinline__ void jk_kepler_poly_f64(double x, double e,
         jk_sincos_tab_f64(E, tab, s, c);
     }
     sinE = x < 0.0 ? -s : s;
-    cosE = c;
-}
-
-// Kepler in FP32, same start and monotone Newton; stops when the step is
-// within 2^-22 of E (the FP32 iterate has stopped moving) or after 32 steps.
-__device__ __forceinline__ void jk_kepler_f32(float x, float e, float &sinE, float &cosE)
-{
-    const float ax = fabsf(x);
-    float E = fminf(fmaf(0.85f, e, ax), JK_PI_F_HI);
-    float s, c;
-    jk_sincos_0pi_f32(E, s, c);
-#pragma unroll 1
-    for (int it = 0; it < 32; ++it) {
-        const float fp = fmaf(-e, c, 1.0f);
-        const float f = fmaf(-e, s, E - ax);
-        const float d = f * __builtin_amdgcn_rcpf(fp);
-        E -= d;
-        if (fabsf(d) <= 2.4e-7f * E) {
-            const float s1 = fmaf(-d, c, s);
-            c = fmaf(d, s, c);
-            s = s1;
-            break;
-        }
-        jk_sincos_0pi_f32(E, s, c);
-    }
-    sinE = x < 0.0f ? -s : s;
     cosE = c;
 }
 
