@@ -346,3 +346,32 @@ def test_step_graph_replays_equal_eager(setup, precision):
             assert np.array_equal(got[i], want[i]), (k, i)
         assert np.array_equal(npg, npw) and np.array_equal(mxg, mxw)
     graph.plan.close()
+
+
+def test_live_settle_and_zero_check_reject_bad_arguments(setup):
+    """jk_settle_live_batch refuses a margin below 40 and first > n;
+    jk_reject_batch_zero_check refuses a missing flag array."""
+    import ctypes
+    from thejoker_amd import _lib
+    stars, cache, _ = setup
+    batch = B.StarBatch([d for _, d in stars[:2]])
+    n = 1024
+    ll = torch.zeros(2 * n, dtype=torch.float64, device="cuda")
+    bound = torch.ones(2 * n, dtype=torch.float32, device="cuda")
+    with pytest.raises(_lib.JokerLibError):
+        B.settle_live_batch(batch, cache, ll, bound, [0, n], [n, n], [0, 0], [-np.inf] * 2,
+                            margin=30.0)
+    with pytest.raises(_lib.JokerLibError):
+        B.settle_live_batch(batch, cache, ll, bound, [0, n], [n, n], [0, n + 1], [-np.inf] * 2)
+    lib = _lib.ensure_device(0)
+    segs = np.zeros(1, dtype=B.SEGMENT_DTYPE)
+    segs["n"] = n
+    out = torch.zeros(4, dtype=torch.int64, device="cuda")
+    p = out.data_ptr()
+    rc = lib.jk_reject_batch_zero_check(ctypes.c_void_p(ll.data_ptr()), 1,
+                                        segs.ctypes.data_as(ctypes.c_void_p), None,
+                                        ctypes.c_void_p(p), ctypes.c_void_p(p + 8), None, None)
+    assert rc != 0
+    # a first = n segment has no new rows: nothing settled, count 0
+    cnt = B.settle_live_batch(batch, cache, ll, bound, [0, n], [n, n], [n, n], [-np.inf] * 2)
+    assert cnt.cpu().tolist() == [0, 0]
