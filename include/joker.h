@@ -88,6 +88,24 @@ int jk_star_create(int n, const double *t, const double *rv, const double *ivar,
                    double t_ref, int k_prior_mode, double sigma_K0, double P0, double max_K,
                    int jitter_mode, jk_star **out);
 
+/* A star whose data have a full covariance matrix: cov[n*n] row-major, in
+ * (RV unit)^2, symmetric positive definite (its symmetric part is used);
+ * n <= 64.  Other arguments as jk_star_create.  The reference's fast path
+ * reads a 1-D ivar (fast_likelihood.pyx:165-166, :317) and rejects 2-D data
+ * (data.py:146-150 makes ivar a matrix); its slow test twin
+ * (tests/py_likelihood.py:74-78) is diagonal too, so this entry point has no
+ * counterpart: the likelihood is ln N(y | M mu, C + s^2 I + M Lambda M^T)
+ * (reference jitter mode: C), computed in the eigenbasis of C, where
+ * C + s^2 I is diagonal.  The eigen-decomposition (cyclic Jacobi) runs here
+ * on the host.  Such a star is accepted by jk_lnl_f64, jk_lnl_reject_f64 and
+ * jk_linear_posterior; the mixed-precision, polishing and many-star calls
+ * return JK_EINVAL for it.  Not positive definite: JK_EINVAL; n > 64:
+ * JK_ERANGE. */
+int jk_star_create_cov(int n, const double *t, const double *rv, const double *cov, int L,
+                       const double *trend, const double *mu, const double *lambda_diag,
+                       double t_ref, int k_prior_mode, double sigma_K0, double P0, double max_K,
+                       int jitter_mode, jk_star **out);
+
 int jk_star_destroy(jk_star *star);
 
 /* Plain FP64 marginal log-likelihood.  Replaces

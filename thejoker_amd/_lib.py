@@ -36,6 +36,9 @@ SIGNATURES = [
     ("jk_star_create", ctypes.c_int,
      [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int,
       ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.POINTER(_vp)]),
+    ("jk_star_create_cov", ctypes.c_int,
+     [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int,
+      ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.POINTER(_vp)]),
     ("jk_star_destroy", ctypes.c_int, [_vp]),
     ("jk_lnl_f64", ctypes.c_int, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("jk_lnl_mixed", ctypes.c_int, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -359,6 +359,9 @@ def rejection_sample_many(datas, prior, prior_samples, rngs, max_posterior_sampl
         elif not dd.rv.unit.is_equivalent(rv_unit) or dd.rv.unit != rv_unit:
             raise ValueError("all stars of a batch must share one RV unit")
         a = star_arrays(dd, prior, trend)
+        if a["cov"] is not None:
+            raise NotImplementedError("data with a full covariance are evaluated one star at a "
+                                      "time (TheJoker, precision='f64')")
         st = DeviceStar(a["t"], a["rv"], a["ivar"], a["trend"], a["mu"], a["lam"], a["t_ref"],
                         a["k_prior_mode"], a["sigma_K0"], a["P0"], a["max_K"],
                         jitter_mode=jitter_mode, device=device)

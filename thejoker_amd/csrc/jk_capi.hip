@@ -5,6 +5,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
 #include <vector>
 
@@ -82,10 +83,13 @@ int jk_init(int device)
     return JK_OK;
 }
 
-int jk_star_create(int n, const double *t, const double *rv, const double *ivar, int L,
-                   const double *trend, const double *mu, const double *lambda_diag,
-                   double t_ref, int k_prior_mode, double sigma_K0, double P0, double max_K,
-                   int jitter_mode, jk_star **out)
+// The star from epoch times t (epoch order) and rv, ivar, trend (N x (L-1)
+// row-major) either in epoch order (Qt == nullptr) or rotated into the
+// eigenbasis of a full covariance whose eigenvectors are the rows of Qt.
+static int star_build(int n, const double *t, const double *rv, const double *ivar, int L,
+                      const double *trend, const double *mu, const double *lambda_diag,
+                      double t_ref, int k_prior_mode, double sigma_K0, double P0, double max_K,
+                      int jitter_mode, const double *Qt, jk_star **out)
 {
     if (!out) return JK_EINVAL;
     *out = nullptr;
@@ -126,7 +130,8 @@ int jk_star_create(int n, const double *t, const double *rv, const double *ivar,
 
     // Host epoch arrays: dt, r' = y - T mu_trend, y, ivar, T (L-1 rows of n).
     const int LT = L - 1;
-    std::vector<double> h((size_t)n * (5 + LT));
+    const size_t n_q = Qt ? (size_t)n * n : 0;
+    std::vector<double> h((size_t)n * (5 + LT) + n_q);
     double *h_dt = h.data(), *h_rp = h_dt + n, *h_y = h_rp + n, *h_iv = h_y + n, *h_s2 = h_iv + n,
            *h_T = h_s2 + n;
     d.sum_ln_ivar = 0.0;
@@ -163,6 +168,7 @@ int jk_star_create(int n, const double *t, const double *rv, const double *ivar,
     d.ln_c = d.ln_c_phys - d.sum_ln_ivar;
     for (int j = 0; j < LT; ++j)
         for (int q = 0; q < j; ++q) d.S_tt[q * JK_LMAX + j] = d.S_tt[j * JK_LMAX + q];
+    if (Qt) std::copy(Qt, Qt + n_q, h.begin() + (size_t)n * (5 + LT));
 
     void *dmem = nullptr;
     if (hipMalloc(&dmem, h.size() * sizeof(double)) != hipSuccess) {
@@ -182,9 +188,107 @@ int jk_star_create(int n, const double *t, const double *rv, const double *ivar,
     d.ivar = base + 3 * n;
     d.sig2 = base + 4 * n;
     d.T = base + 5 * n;
+    d.Qt = Qt ? base + (size_t)n * (5 + LT) : nullptr;
     *out = st;
     return JK_OK;
 }
+
+int jk_star_create(int n, const double *t, const double *rv, const double *ivar, int L,
+                   const double *trend, const double *mu, const double *lambda_diag,
+                   double t_ref, int k_prior_mode, double sigma_K0, double P0, double max_K,
+                   int jitter_mode, jk_star **out)
+{
+    return star_build(n, t, rv, ivar, L, trend, mu, lambda_diag, t_ref, k_prior_mode, sigma_K0,
+                      P0, max_K, jitter_mode, nullptr, out);
+}
+
+// Eigen-decomposition of a symmetric n x n matrix A (row-major, overwritten)
+// by cyclic Jacobi rotations (Golub & Van Loan, Matrix Computations, 4th ed.,
+// sec. 8.5): on return the diagonal of A holds the eigenvalues and the rows
+// of V the eigenvectors (V A_in V^T = diag).  Sweeps stop when the
+// off-diagonal mass falls below 1e-30 of the total, or after 100 sweeps.
+static bool jacobi_eigen(int n, std::vector<double> &A, std::vector<double> &V)
+{
+    V.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) V[(size_t)i * n + i] = 1.0;
+    double total = 0.0;
+    for (double a : A) total += a * a;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) off += 2.0 * A[(size_t)p * n + q] * A[(size_t)p * n + q];
+        if (off <= 1e-30 * total) return true;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = A[(size_t)p * n + q];
+                if (apq == 0.0) continue;
+                const double app = A[(size_t)p * n + p], aqq = A[(size_t)q * n + q];
+                // the rotation angle that zeroes A[p][q]: t = tan(theta)
+                const double th = (aqq - app) / (2.0 * apq);
+                const double tn = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                const double c = 1.0 / sqrt(tn * tn + 1.0), sn = tn * c;
+                for (int k = 0; k < n; ++k) {       // columns p, q
+                    const double akp = A[(size_t)k * n + p], akq = A[(size_t)k * n + q];
+                    A[(size_t)k * n + p] = c * akp - sn * akq;
+                    A[(size_t)k * n + q] = sn * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {       // rows p, q
+                    const double apk = A[(size_t)p * n + k], aqk = A[(size_t)q * n + k];
+                    A[(size_t)p * n + k] = c * apk - sn * aqk;
+                    A[(size_t)q * n + k] = sn * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {       // eigenvector rows p, q
+                    const double vpk = V[(size_t)p * n + k], vqk = V[(size_t)q * n + k];
+                    V[(size_t)p * n + k] = c * vpk - sn * vqk;
+                    V[(size_t)q * n + k] = sn * vpk + c * vqk;
+                }
+            }
+    }
+    return false;
+}
+
+int jk_star_create_cov(int n, const double *t, const double *rv, const double *cov, int L,
+                       const double *trend, const double *mu, const double *lambda_diag,
+                       double t_ref, int k_prior_mode, double sigma_K0, double P0, double max_K,
+                       int jitter_mode, jk_star **out)
+{
+    if (!out) return JK_EINVAL;
+    *out = nullptr;
+    if (n <= 0 || !cov || !rv || !t) return JK_EINVAL;
+    if (n > JK_COV_NMAX) return JK_ERANGE;
+    if (L > 1 && !trend) return JK_EINVAL;
+    if (L < 1 || L > JK_LMAX) return JK_EINVAL;
+    // the symmetric part of cov, checked finite
+    std::vector<double> A((size_t)n * n), V;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            const double a = 0.5 * (cov[(size_t)i * n + j] + cov[(size_t)j * n + i]);
+            if (!isfinite(a)) return JK_EINVAL;
+            A[(size_t)i * n + j] = a;
+        }
+    if (!jacobi_eigen(n, A, V)) return JK_EINVAL;
+    // rotated data: y' = Q^T y, T' = Q^T T, ivar' = 1 / lambda (C positive definite)
+    const int LT = L - 1;
+    std::vector<double> yr(n), ivr(n), Tr((size_t)n * (LT > 0 ? LT : 1));
+    for (int i = 0; i < n; ++i) {
+        const double lam = A[(size_t)i * n + i];
+        if (!(lam > 0.0)) return JK_EINVAL;
+        ivr[i] = 1.0 / lam;
+        const double *q = V.data() + (size_t)i * n;
+        double y = 0.0;
+        for (int k = 0; k < n; ++k) y += q[k] * rv[k];
+        yr[i] = y;
+        for (int j = 0; j < LT; ++j) {
+            double tv = 0.0;
+            for (int k = 0; k < n; ++k) tv += q[k] * trend[(size_t)k * LT + j];
+            Tr[(size_t)i * LT + j] = tv;
+        }
+    }
+    return star_build(n, t, yr.data(), ivr.data(), L, LT > 0 ? Tr.data() : trend, mu,
+                      lambda_diag, t_ref, k_prior_mode, sigma_K0, P0, max_K, jitter_mode,
+                      V.data(), out);
+}
+
 
 int jk_star_destroy(jk_star *star)
 {
@@ -213,6 +317,7 @@ int jk_lnl_mixed(const jk_star *star, int64_t n, const double *P, const double *
                  const double *omega, const double *M0, const double *s, const int64_t *rows,
                  double *ll_out, float *bound_out, void *stream)
 {
+    if (star && star->dev.Qt) return JK_EINVAL;   // full covariance: FP64 only
     if (!star || n < 0 || (n > 0 && (!P || !e || !omega || !M0 || !ll_out))) return JK_EINVAL;
     if (star->dev.jitter_mode == JK_JITTER_PHYSICAL && n > 0 && !s) return JK_EINVAL;
     if (current_device_inited() < 0) return JK_ENODEV;
@@ -335,6 +440,7 @@ int jk_polish_f64(const jk_star *star, int64_t n, const double *P, const double 
                   double *ll_inout, double window, int64_t first, int64_t *n_polished_out,
                   void *stream)
 {
+    if (star && star->dev.Qt) return JK_EINVAL;   // full covariance: FP64 only
     if (!star || n < 0 || !(window >= 0.0) || first < 0) return JK_EINVAL;
     if (n == 0) return JK_OK;
     if (!P || !e || !omega || !M0 || !ll_inout || !n_polished_out) return JK_EINVAL;
@@ -362,6 +468,7 @@ int jk_batch_create(const jk_star *const *stars, int n_stars, jk_batch **out)
     size_t n_epoch_doubles = 0;
     for (int i = 0; i < n_stars; ++i) {
         if (!stars[i] || stars[i]->dev.L != L || stars[i]->dev.jitter_mode != jm) return JK_EINVAL;
+        if (stars[i]->dev.Qt) return JK_EINVAL;   // full covariance: single-star calls only
         n_epoch_doubles += (size_t)stars[i]->dev.n * (5 + L - 1);
     }
     jk_batch *b = new (std::nothrow) jk_batch;
@@ -450,6 +557,7 @@ int jk_batch_set_star(jk_batch *b, int slot, const jk_star *star, void *stream)
 {
     if (!b || !star || slot < 0 || slot >= b->n_stars || b->d_epochs) return JK_EINVAL;
     if (star->dev.L != b->L || star->dev.jitter_mode != b->jitter_mode) return JK_EINVAL;
+    if (star->dev.Qt) return JK_EINVAL;   // full covariance: single-star calls only
     if (star->device != b->device) return JK_EINVAL;
     if (current_device_inited() < 0) return JK_ENODEV;
     return check_hip(hipMemcpyAsync(b->d_stars + slot, &star->dev, sizeof(StarDev),

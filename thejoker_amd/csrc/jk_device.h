@@ -66,6 +66,12 @@ struct StarDev {
     const double *ivar;
     const double *sig2;
     const double *T;
+    // Full-covariance stars (nullptr otherwise): Q^T, n x n row-major, whose
+    // rows are the eigenvectors of the data covariance C.  rp, y, T are then
+    // rotated into that eigenbasis (Q^T r', Q^T y, Q^T T), ivar and sig2 hold
+    // 1 / lambda_i and lambda_i, and only dt stays in epoch order: a row's
+    // design column is computed per epoch and rotated, m' = Q^T m.
+    const double *Qt;
 };
 
 // ---------------------------------------------------------------------------

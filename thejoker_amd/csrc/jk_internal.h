@@ -7,6 +7,10 @@
 
 #include "jk_device.h"
 
+// Largest epoch count of a full-covariance star (jk_star_create_cov): its
+// kernels keep a row's design column, one double per epoch and lane, in LDS.
+#define JK_COV_NMAX 64
+
 // Host mirror of one star: the StarDev block passed by value to kernels,
 // plus the device allocation that holds the epoch arrays it points into.
 struct jk_star {

@@ -457,6 +457,150 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
 }
 
 // ---------------------------------------------------------------------------
+// Full-covariance row, FP64.  With C = Q diag(lambda) Q^T, C + s^2 I is
+// diagonal in the same eigenbasis (lambda + s^2), and the marginal
+// likelihood ln N(y | M mu, C + s^2 I + M Lambda M^T) does not change under
+// the orthogonal change of basis (|det Q| = 1).  The star's data are stored
+// rotated (StarDev::Qt); the row's design column is computed per epoch as in
+// jk_lnl_row, kept in this lane's column of an LDS buffer (epoch k at
+// s_m[k * JK_COV_LANES]), rotated, m'_i = sum_k Qt[i][k] m_k, and fed to the
+// same moments and finish with the eigenvalues as the variances.  Each lane
+// reads back only its own writes, so no barrier is needed.
+// ---------------------------------------------------------------------------
+#define JK_COV_LANES 64
+
+template <int L, bool PHYS, bool SAFE = false>
+__device__ double jk_lnl_row_cov(const StarDev &st, int64_t r, const double *__restrict__ P,
+                                 const void *__restrict__ e_, const void *__restrict__ om_,
+                                 const void *__restrict__ M0_, const void *__restrict__ s_,
+                                 const double2 *__restrict__ tab, double *s_m)
+{
+    const double *__restrict__ ecol = static_cast<const double *>(e_);
+    const double *__restrict__ omcol = static_cast<const double *>(om_);
+    const double *__restrict__ M0col = static_cast<const double *>(M0_);
+    const double *__restrict__ scol = static_cast<const double *>(s_);
+    const double Pv = P[r], ev = ecol[r], omv = omcol[r], M0v = M0col[r];
+    const double sv = PHYS ? scol[r] : 0.0;
+    bool valid = isfinite(Pv) && Pv != 0.0 && ev >= 0.0 && ev < 1.0 && isfinite(omv) &&
+                 isfinite(M0v);
+    if (PHYS) valid = valid && isfinite(sv);
+    if (!valid) return __builtin_nan("");
+    const double lam0 = st.k_prior_mode == JK_KPRIOR_FIXED_COMPANION_MASS
+                            ? jk_k_variance(&st, Pv, ev, true)
+                            : st.lam0;
+    const double nP = JK_TWO_PI * jk_rcp2(Pv);
+    const double s2 = sv * sv;
+    const float ef = fminf((float)ev, 0.99999994f);
+    const float omef = fmaxf((float)(1.0 - ev), 5.9604645e-08f);
+    const float a1f = JK_MARKLEY_A1N * __builtin_amdgcn_rcpf(1.0f + ef);
+    double sw, cw;
+    jk_sincos_tab_f64(omv, tab, sw, cw);
+    const double ecw = ev * cw;
+    const double swsq = sw * sqrt((1.0 - ev) * (1.0 + ev));
+    const double dmax = ev > 0.9999999 ? -1.0 : 1e-3;
+    const int N = st.n;
+
+    // the design column in epoch order
+    bool bad = false;
+    for (int k = 0; k < N; ++k) {
+        const double x = jk_reduce_2pi(fma(nP, st.dt[k], -M0v));
+        double sE, cE;
+        if constexpr (SAFE)
+            jk_kepler_poly_f64(x, ev, tab, sE, cE);
+        else
+            bad |= !jk_kepler_markley_f64(x, ev, ef, omef, a1f, dmax, tab, sE, cE);
+        const double den = jk_rcp2(fma(-ev, cE, 1.0));
+        s_m[k * JK_COV_LANES] = fma(fma(cw, cE - ev, -swsq * sE), den, ecw);
+    }
+    if constexpr (!SAFE) {
+        if (bad) return jk_lnl_row_cov<L, PHYS, true>(st, r, P, e_, om_, M0_, s_, tab, s_m);
+    }
+
+    // moments in the eigenbasis
+    constexpr int LT = L > 1 ? L - 1 : 1;
+    double Smm = 0.0, Smr = 0.0, Srr = 0.0;
+    double Smt[LT], Stt[LT * LT], Str[LT];
+    double prod = 1.0;
+    int pexp = 0;
+#pragma unroll
+    for (int j = 0; j < LT; ++j) {
+        Smt[j] = 0.0;
+        Str[j] = 0.0;
+#pragma unroll
+        for (int q = 0; q < LT; ++q) Stt[j * LT + q] = 0.0;
+    }
+    for (int i = 0; i < N; ++i) {
+        const double *__restrict__ q = st.Qt + (size_t)i * N;
+        double m = 0.0;
+        for (int k = 0; k < N; ++k) m = fma(q[k], s_m[k * JK_COV_LANES], m);
+        const double rp = st.rp[i];
+        double w;
+        if constexpr (PHYS) {
+            const double d = st.sig2[i] + s2;
+            w = jk_rcp2(d);
+            int ex;
+            prod = frexp(prod * d, &ex);
+            pexp += ex;
+        } else {
+            w = st.ivar[i];
+        }
+        const double wm = w * m;
+        Smm = fma(wm, m, Smm);
+        Smr = fma(wm, rp, Smr);
+        if constexpr (L > 1) {
+#pragma unroll
+            for (int j = 0; j < L - 1; ++j) {
+                const double Tj = st.T[j * N + i];
+                Smt[j] = fma(wm, Tj, Smt[j]);
+                if constexpr (PHYS) {
+                    const double wT = w * Tj;
+                    Str[j] = fma(wT, rp, Str[j]);
+#pragma unroll
+                    for (int qq = 0; qq <= j; ++qq)
+                        Stt[j * LT + qq] = fma(wT, st.T[qq * N + i], Stt[j * LT + qq]);
+                }
+            }
+        }
+        if constexpr (PHYS) Srr = fma(w * rp, rp, Srr);
+    }
+
+    // G, u, chi0 and the log-constant as in jk_lnl_row
+    double G[L * L], u[L];
+    const double mu0 = st.mu0;
+    const bool fcm = st.k_prior_mode == JK_KPRIOR_FIXED_COMPANION_MASS;
+    G[0] = (fcm ? jk_rcp2(lam0) : st.lam_inv[0]) + Smm;
+    u[0] = fma(-mu0, Smm, Smr);
+    double chi0, lnc, scale = 1.0;
+    if constexpr (PHYS) {
+        chi0 = Srr;
+        lnc = st.ln_c_phys + (double)pexp * JK_LN2;
+        scale = prod;
+    } else {
+        chi0 = st.S_rr;
+        lnc = st.ln_c;
+    }
+    if (fcm) scale *= lam0;
+    chi0 = fma(mu0 * mu0, Smm, fma(-2.0 * mu0, Smr, chi0));
+    if constexpr (L > 1) {
+#pragma unroll
+        for (int j = 1; j < L; ++j) {
+            G[j * L] = Smt[j - 1];
+            G[j] = Smt[j - 1];
+            const double trj = PHYS ? Str[j - 1] : st.S_tr[j - 1];
+            u[j] = fma(-mu0, Smt[j - 1], trj);
+#pragma unroll
+            for (int q = 1; q <= j; ++q) {
+                const double tt = PHYS ? Stt[(j - 1) * LT + (q - 1)] : st.S_tt[(j - 1) * JK_LMAX + (q - 1)];
+                const double g = (q == j ? st.lam_inv[j] : 0.0) + tt;
+                G[j * L + q] = g;
+                G[q * L + j] = g;
+            }
+        }
+    }
+    return jk_finish_lnl<L>(G, u, chi0, lnc, scale);
+}
+
+// ---------------------------------------------------------------------------
 // Block-level helpers (256 threads = 4 wavefronts of 64).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double wave_nanmax(double v)
@@ -526,7 +670,67 @@ __global__ __launch_bounds__(256) void k_lnl(const StarDev st, int64_t n,
     }
 }
 
-// Scatter launch over a device-resident index list: for j < *count,
+// Full-covariance stars, FP64: one lane per row, 64 lanes per workgroup, the
+// design columns in dynamic LDS (N x 64 doubles).
+template <int L, bool PHYS>
+__global__ __launch_bounds__(JK_COV_LANES) void k_lnl_cov(const StarDev st, int64_t n,
+                                                          const double *__restrict__ P,
+                                                          const void *__restrict__ e_,
+                                                          const void *__restrict__ om_,
+                                                          const void *__restrict__ M0_,
+                                                          const void *__restrict__ s_,
+                                                          const int64_t *__restrict__ rows,
+                                                          double *__restrict__ ll)
+{
+    __shared__ double2 s_tab[JK_TAB_LDS];
+    extern __shared__ double s_m[];
+    jk_stage_tab(s_tab);
+    const int64_t i = (int64_t)blockIdx.x * JK_COV_LANES + threadIdx.x;
+    if (i < n) {
+        const int64_t r = rows ? rows[i] : i;
+        ll[i] = jk_lnl_row_cov<L, PHYS>(st, r, P, e_, om_, M0_, s_, s_tab, s_m + threadIdx.x);
+    }
+}
+
+template <int L, bool PHYS>
+static hipError_t launch_lnl_cov_t(const jk_star *st, int64_t n, const double *P, const void *e,
+                                   const void *om, const void *M0, const void *s,
+                                   const int64_t *rows, double *ll, hipStream_t stream)
+{
+    const int64_t blocks = (n + JK_COV_LANES - 1) / JK_COV_LANES;
+    const size_t lds = sizeof(double) * (size_t)st->dev.n * JK_COV_LANES;
+    hipLaunchKernelGGL((k_lnl_cov<L, PHYS>), dim3((unsigned)blocks), dim3(JK_COV_LANES), lds,
+                       stream, st->dev, n, P, e, om, M0, s, rows, ll);
+    return hipGetLastError();
+}
+
+static hipError_t launch_lnl_cov(const jk_star *st, int64_t n, const double *P, const void *e,
+                                 const void *om, const void *M0, const void *s,
+                                 const int64_t *rows, double *ll, hipStream_t stream)
+{
+    if (st->dev.n > JK_COV_NMAX || (n + JK_COV_LANES - 1) / JK_COV_LANES > 0x7fffffff)
+        return hipErrorInvalidValue;
+    const bool phys = st->dev.jitter_mode == JK_JITTER_PHYSICAL;
+#define JK_CCASE(LL)                                                                           \
+    case LL:                                                                                   \
+        return phys ? launch_lnl_cov_t<LL, true>(st, n, P, e, om, M0, s, rows, ll, stream)     \
+                    : launch_lnl_cov_t<LL, false>(st, n, P, e, om, M0, s, rows, ll, stream);
+    switch (st->dev.L) {
+        JK_CCASE(1)
+        JK_CCASE(2)
+        JK_CCASE(3)
+        JK_CCASE(4)
+        JK_CCASE(5)
+        JK_CCASE(6)
+        JK_CCASE(7)
+        JK_CCASE(8)
+    default:
+        return hipErrorInvalidValue;
+    }
+#undef JK_CCASE
+}
+
+// Scatter launch over a device-resident index list: for j < *count,// Scatter launch over a device-resident index list: for j < *count,
 // i = idx[j], ll[i] = lnL(row rows[i] or i).  Grid-stride, so the host does
 // not need the count.
 template <int L, bool PHYS>
@@ -604,6 +808,7 @@ hipError_t jk_launch_lnl_f64(const jk_star *st, int64_t n, const double *P, cons
                              const int64_t *rows, double *ll, hipStream_t stream)
 {
     if (n <= 0) return hipSuccess;
+    if (st->dev.Qt) return launch_lnl_cov(st, n, P, e, om, M0, s, rows, ll, stream);
     return launch_lnl_dispatch<false>(st, n, P, e, om, M0, s, rows, ll, nullptr, nullptr, stream);
 }
 
@@ -978,12 +1183,20 @@ hipError_t jk_launch_lnl_reject_f64(const jk_star *st, int64_t n, const double *
     double *partials2 = partials + blocks;      // second level, at most 1024
     uint64_t *mask = (uint64_t *)(partials2 + 1024);
     int64_t *bsum = (int64_t *)(mask + jk_reject_mask_words(n));
-    hipError_t err =
-        launch_lnl_dispatch<false>(st, n, P, e, om, M0, s, rows, ll, partials, nullptr, stream);
-    if (err != hipSuccess) return err;
-    // the per-workgroup maxima reduced like any array (two parallel levels)
-    err = jk_launch_max(partials, blocks, partials2, jk_max_partials_count(blocks), llmax_out,
-                        stream);
+    hipError_t err;
+    if (st->dev.Qt) {
+        // full covariance: the maximum in its own two-level pass over ll
+        err = launch_lnl_cov(st, n, P, e, om, M0, s, rows, ll, stream);
+        if (err != hipSuccess) return err;
+        err = jk_launch_max(ll, n, partials2, jk_max_partials_count(n), llmax_out, stream);
+    } else {
+        err = launch_lnl_dispatch<false>(st, n, P, e, om, M0, s, rows, ll, partials, nullptr,
+                                         stream);
+        if (err != hipSuccess) return err;
+        // the per-workgroup maxima reduced like any array (two parallel levels)
+        err = jk_launch_max(partials, blocks, partials2, jk_max_partials_count(blocks), llmax_out,
+                            stream);
+    }
     if (err != hipSuccess) return err;
     return jk_launch_reject(ll, n, llmax_out, state, inc, offset, max_keep, idx_out, n_pass_out,
                             mask, bsum, stream);
@@ -1866,10 +2079,27 @@ __global__ __launch_bounds__(64) void k_linear_posterior(const StarDev st, int64
     const double nP = JK_TWO_PI / Pv;
     double chi0 = 0.0, sum_ln_w = 0.0;
     double x[JK_LMAX];
+    // full covariance: the design column in epoch order, kept in this lane's
+    // LDS column, then rotated into the eigenbasis the star's data are in
+    extern __shared__ double s_m[];
+    double *m_col = s_m + threadIdx.x;
+    if (st.Qt) {
+        for (int k = 0; k < N; ++k) {
+            double sE, cE;
+            jk_kepler_f64(jk_reduce_2pi(fma(nP, st.dt[k], -M0v)), ev, sE, cE);
+            m_col[k * JK_COV_LANES] = jk_design_f64(sE, cE, ev, sq, cw, sw, ecw);
+        }
+    }
     for (int k = 0; k < N; ++k) {
-        double sE, cE;
-        jk_kepler_f64(jk_reduce_2pi(fma(nP, st.dt[k], -M0v)), ev, sE, cE);
-        x[0] = jk_design_f64(sE, cE, ev, sq, cw, sw, ecw);
+        if (st.Qt) {
+            double m = 0.0;
+            for (int q = 0; q < N; ++q) m = fma(st.Qt[(size_t)k * N + q], m_col[q * JK_COV_LANES], m);
+            x[0] = m;
+        } else {
+            double sE, cE;
+            jk_kepler_f64(jk_reduce_2pi(fma(nP, st.dt[k], -M0v)), ev, sE, cE);
+            x[0] = jk_design_f64(sE, cE, ev, sq, cw, sw, ecw);
+        }
         for (int j = 1; j < L; ++j) x[j] = st.T[(j - 1) * N + k];
         const double iv = st.ivar[k];
         const double w = st.jitter_mode == JK_JITTER_PHYSICAL ? iv / fma(sv * sv, iv, 1.0) : iv;
@@ -1950,7 +2180,8 @@ hipError_t jk_launch_linear_posterior(const jk_star *st, int64_t n, const double
 {
     if (n <= 0) return hipSuccess;
     const int64_t blocks = (n + 63) / 64;
-    hipLaunchKernelGGL(k_linear_posterior, dim3((unsigned)blocks), dim3(64), 0, stream, st->dev, n,
-                       P, e, om, M0, s, rows, a_out, cov_out, ll_out);
+    const size_t lds = st->dev.Qt ? sizeof(double) * (size_t)st->dev.n * JK_COV_LANES : 0;
+    hipLaunchKernelGGL(k_linear_posterior, dim3((unsigned)blocks), dim3(64), lds, stream, st->dev,
+                       n, P, e, om, M0, s, rows, a_out, cov_out, ll_out);
     return hipGetLastError();
 }

@@ -104,10 +104,20 @@ class RVData:
         """1 / sigma^2 in 1 / (rv unit)^2, as CJokerHelper reads it
         (fast_likelihood.pyx:165-166)."""
         if self._has_cov:
-            raise NotImplementedError("full-covariance data are not supported by the likelihood "
-                                      "kernel (the reference's fast path is diagonal-only)")
+            raise ValueError("this RVData has a full covariance: use cov_value()")
         err = self.rv_err.to_value(self.rv.unit)
         return 1.0 / np.asarray(err) ** 2
+
+    def cov_value(self):
+        """The full covariance matrix in (rv unit)^2.  A matrix given in the
+        RV unit itself is taken as covariance values in that unit squared (the
+        reference's ivar reads it so, data.py:146-150)."""
+        if not self._has_cov:
+            return np.diag(1.0 / self.ivar_value())
+        try:
+            return np.asarray(self.rv_err.to_value(self.rv.unit ** 2), dtype=np.float64)
+        except Exception:
+            return np.asarray(self.rv_err.to_value(self.rv.unit), dtype=np.float64)
 
     def phase(self, P, t_ref=None):
         if t_ref is None:

@@ -103,17 +103,26 @@ class DeviceStar:
     t, rv, ivar (n,), trend (n, L-1), mu and lam (L,) in M_T row order
     (K, v0, offsets..., v1, ...), t_ref; k_prior_mode/sigma_K0/P0/max_K for
     the FixedCompanionMass K prior; jitter_mode "physical" or "reference".
+    cov: instead of ivar, a full (n, n) data covariance (jk_star_create_cov;
+    n <= 64, FP64 single-star calls only).
     """
 
     def __init__(self, t, rv, ivar, trend, mu, lam, t_ref, k_prior_mode=_lib.JK_KPRIOR_NORMAL,
-                 sigma_K0=0.0, P0=1.0, max_K=np.inf, jitter_mode="physical", device=0):
+                 sigma_K0=0.0, P0=1.0, max_K=np.inf, jitter_mode="physical", device=0, cov=None):
         self.device = device
         lib = _lib.ensure_device(device)
         self._lib = lib
         self.t = np.ascontiguousarray(t, dtype=np.float64)
         self.rv = np.ascontiguousarray(rv, dtype=np.float64)
-        self.ivar = np.ascontiguousarray(ivar, dtype=np.float64)
         n = self.t.size
+        self.cov = None
+        if cov is not None:
+            self.cov = np.ascontiguousarray(cov, dtype=np.float64)
+            if self.cov.shape != (n, n):
+                raise ValueError(f"Invalid covariance shape {self.cov.shape}, expected {(n, n)}")
+            self.ivar = 1.0 / np.diag(self.cov)
+        else:
+            self.ivar = np.ascontiguousarray(ivar, dtype=np.float64)
         self.mu = np.ascontiguousarray(mu, dtype=np.float64)
         self.lam = np.ascontiguousarray(lam, dtype=np.float64)
         L = self.mu.size
@@ -128,11 +137,18 @@ class DeviceStar:
         self.t_ref = float(t_ref)
         self.sigma_K0, self.P0, self.max_K = float(sigma_K0), float(P0), float(max_K)
         h = ctypes.c_void_p()
-        _lib.check(lib.jk_star_create(n, _hp(self.t), _hp(self.rv), _hp(self.ivar), L,
-                                      _hp(self.trend), _hp(self.mu), _hp(self.lam), self.t_ref,
-                                      self.k_prior_mode, self.sigma_K0, self.P0, self.max_K,
-                                      JITTER_MODES[jitter_mode], ctypes.byref(h)),
-                   "jk_star_create")
+        if self.cov is None:
+            _lib.check(lib.jk_star_create(n, _hp(self.t), _hp(self.rv), _hp(self.ivar), L,
+                                          _hp(self.trend), _hp(self.mu), _hp(self.lam),
+                                          self.t_ref, self.k_prior_mode, self.sigma_K0, self.P0,
+                                          self.max_K, JITTER_MODES[jitter_mode],
+                                          ctypes.byref(h)), "jk_star_create")
+        else:
+            _lib.check(lib.jk_star_create_cov(n, _hp(self.t), _hp(self.rv), _hp(self.cov), L,
+                                              _hp(self.trend), _hp(self.mu), _hp(self.lam),
+                                              self.t_ref, self.k_prior_mode, self.sigma_K0,
+                                              self.P0, self.max_K, JITTER_MODES[jitter_mode],
+                                              ctypes.byref(h)), "jk_star_create_cov")
         self._h = h
 
     @property
@@ -165,12 +181,13 @@ class DeviceStar:
         a = star_arrays(data, prior, trend_M)
         return cls(a["t"], a["rv"], a["ivar"], a["trend"], a["mu"], a["lam"], a["t_ref"],
                    a["k_prior_mode"], a["sigma_K0"], a["P0"], a["max_K"],
-                   jitter_mode=jitter_mode, device=device)
+                   jitter_mode=jitter_mode, device=device, cov=a["cov"])
 
 
 def star_arrays(data, prior, trend_M):
     """The arrays CJokerHelper.__init__ derives from (data, prior, trend_M)
-    (fast_likelihood.pyx:134-252): times, RVs and 1/sigma^2 in the data's RV
+    (fast_likelihood.pyx:134-252): times, RVs and 1/sigma^2 (or, for data with
+    a full covariance, that covariance; cov is None otherwise) in the data's RV
     unit, and the prior mean / variance of the linear parameters in M_T row
     order -- K at 0, v0 at 1, the v0 offsets at 2.., v_i (i >= 1) after them
     -- in internal units.  For a FixedCompanionMass K prior, sigma_K0, P0
@@ -202,8 +219,10 @@ def star_arrays(data, prior, trend_M):
     for i in range(n_off):
         m_, s_ = prior.pars[f"dv0_{i + 1}"].mean_std(rv_unit)
         mu[2 + i], lam[2 + i] = m_, s_ ** 2
+    cov = data.cov_value() if data._has_cov else None
     return dict(t=np.asarray(data._t_bmjd, dtype=np.float64),
-                rv=np.asarray(data.rv.value, dtype=np.float64), ivar=data.ivar_value(),
+                rv=np.asarray(data.rv.value, dtype=np.float64),
+                ivar=None if cov is not None else data.ivar_value(), cov=cov,
                 trend=np.asarray(trend_M, dtype=np.float64), mu=mu, lam=lam,
                 t_ref=float(data._t_ref_bmjd), k_prior_mode=kmode, sigma_K0=sK0, P0=P0,
                 max_K=maxK)

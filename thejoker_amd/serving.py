@@ -281,6 +281,9 @@ class JokerServer:
                 dd, ids, trend = validate_prepare_data(r.data, self.prior.poly_trend,
                                                        self.prior.n_offsets)
                 a = star_arrays(dd, self.prior, trend)
+                if a["cov"] is not None:
+                    raise NotImplementedError("data with a full covariance are evaluated one "
+                                              "star at a time (TheJoker, precision='f64')")
                 r.star = DeviceStar(a["t"], a["rv"], a["ivar"], a["trend"], a["mu"], a["lam"],
                                     a["t_ref"], a["k_prior_mode"], a["sigma_K0"], a["P0"],
                                     a["max_K"], jitter_mode=self.jitter_mode, device=self.device)

@@ -107,6 +107,9 @@ class TheJoker:
         """thejoker.py:87-91: validate data, build the star on the device."""
         all_data, ids, trend_M = validate_prepare_data(data, self.prior.poly_trend,
                                                        self.prior.n_offsets)
+        if all_data._has_cov and self.precision != "f64":
+            raise NotImplementedError("data with a full covariance need precision='f64' (the "
+                                      "mixed kernels and their error bound are diagonal)")
         star = DeviceStar.from_data_prior(all_data, self.prior, trend_M,
                                           jitter_mode=self.jitter_mode, device=self.device)
         star.data = all_data
