@@ -264,8 +264,11 @@ __device__ __forceinline__ void jk_kepler_markley_f32(float x, float e, float om
 //   |dlnL| <= sqrt(sum_n w_n eps_n^2) (|a_K| sqrt(chi2_min) + sqrt((G^-1)_00)).
 // eps_n = 2^-22 (1 + d_n)^2 (1 + d_n / 1000), d_n = 1 / (1 - e cos E_n), covers
 // the FP32 solve and design column with a factor of two to spare (emulated
-// over the Beta prior and e up to 1 - 1e-6); the bound is doubled again for
-// the terms of second order and given a relative floor of 1e-9.
+// over the Beta prior and e up to 1 - 1e-6).  eps_n grows with d_n, so
+// sum_n w_n eps_n^2 <= eps(max_n d_n)^2 sum_n w_n: the epoch loop keeps only
+// sum w and the largest d_n (two instructions per epoch).  The bound is
+// doubled again for the terms of second order and given a relative floor of
+// 1e-9.
 #define JK_BOUND_EPS 2.384185791015625e-07   // 2^-22
 
 template <int L, bool PHYS, bool MIXED, bool SAFE = false, bool BOUND = false>
@@ -340,7 +343,8 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
 
     const int N = st.n;
     bool bad = false;
-    float Sb = 0.0f;       // sum w eps^2 / JK_BOUND_EPS^2 (BOUND only)
+    double Sw = 0.0;       // sum w (BOUND only)
+    float dmx = 0.0f;      // largest 1 / (1 - e cos E) (BOUND only)
 #pragma unroll 2
     for (int k = 0; k < N; ++k) {
         const double dtk = st.dt[k];
@@ -379,9 +383,8 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
             w = st.ivar[k];
         }
         if constexpr (BOUND) {
-            const float t1 = 1.0f + denb;
-            const float ek = t1 * t1 * fmaf(1e-3f, denb, 1.0f);
-            Sb = fmaf((float)w, ek * ek, Sb);
+            Sw += w;
+            dmx = fmaxf(dmx, denb);
         }
         const double wm = w * m;
         Smm = fma(wm, m, Smm);
@@ -446,7 +449,9 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
         double post[3];
         const double ll = jk_finish_lnl<L, true>(G, u, chi0, lnc, scale, post);
         const double aK = fabs(mu0 + post[1]);
-        double b = 2.0 * JK_BOUND_EPS * sqrt((double)Sb) *
+        const double t1 = 1.0 + (double)dmx;
+        const double ek = t1 * t1 * fma(1e-3, (double)dmx, 1.0);
+        double b = 2.0 * JK_BOUND_EPS * ek * sqrt(Sw) *
                    fma(aK, sqrt(fmax(post[2], 0.0)), sqrt(fmax(post[0], 0.0)));
         b = fma(1e-9, 1.0 + fabs(ll), b) * 1.000001;
         *bnd = (isfinite(b) && isfinite(ll)) ? (float)b : INFINITY;
