@@ -367,15 +367,17 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
             else
                 bad |= !jk_kepler_markley_f64(x, ev, ef, omef, a1f, dmax, tab, sE, cE);
             // m = [cos w (cos E - e) - sin w sqrt(1-e^2) sin E] / (1 - e cos E) + e cos w
-            const double den = jk_rcp2(fma(-ev, cE, 1.0));
+            // (one Newton step on the hardware reciprocal: relative 2e-15)
+            const double den = jk_rcp1(fma(-ev, cE, 1.0));
             m = fma(fma(cw, cE - ev, -swsq * sE), den, ecw);
         }
         const double rp = st.rp[k];
         double w;
         if constexpr (PHYS) {
-            // w = 1 / (sigma^2 + s^2); sum ln w = -ln of the running product
+            // w = 1 / (sigma^2 + s^2), relative error below 2.4e-15 (one Newton
+            // step); sum ln w = -ln of the running product
             const double d = st.sig2[k] + s2;
-            w = jk_rcp2(d);
+            w = jk_rcp1(d);
             int ex;
             prod = frexp(prod * d, &ex);
             pexp += ex;
@@ -453,6 +455,10 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
         const double ek = t1 * t1 * fma(1e-3, (double)dmx, 1.0);
         double b = 2.0 * JK_BOUND_EPS * ek * sqrt(Sw) *
                    fma(aK, sqrt(fmax(post[2], 0.0)), sqrt(fmax(post[0], 0.0)));
+        // the weights' relative error e_w < 2.4e-15 moves lnL by at most
+        // e_w (chi2_min + L) / 2 (envelope theorem on chi2_min; the log-det
+        // term is tr(G^-1 X^T W X) e_w <= L e_w)
+        b += 2e-15 * (fmax(post[2], 0.0) + (double)L);
         b = fma(1e-9, 1.0 + fabs(ll), b) * 1.000001;
         *bnd = (isfinite(b) && isfinite(ll)) ? (float)b : INFINITY;
         return ll;
