@@ -134,8 +134,7 @@ def leg_single(args, torch, np, D, syn):
     mu, lam = np.zeros(2), np.array([900.0, 1e4])
     dstar = D.DeviceStar(star.t, star.rv, star.ivar(), star.trend(), mu, lam, star.t_ref,
                          jitter_mode="physical", device=0)
-    cache = D.PriorCache(cols["P"], cols["e"], cols["omega"], cols["M0"], cols["s"], device=0,
-                         fp32_copy=True)
+    cache = D.PriorCache(cols["P"], cols["e"], cols["omega"], cols["M0"], cols["s"], device=0)
     rng = np.random.default_rng(42)
     ll = torch.empty(n, dtype=torch.float64, device="cuda")
 

@@ -114,7 +114,8 @@ def test_polish_makes_live_set_fp64(setup):
 
 def test_prior_fill_distribution_and_determinism(gpu):
     from scipy import stats
-    c = D.PriorCache.generate(1 << 20, seed=7, P_min=1.0, P_max=16384.0, ln_s=(-3.0, 1.0))
+    c = D.PriorCache.generate(1 << 20, seed=7, P_min=1.0, P_max=16384.0, ln_s=(-3.0, 1.0),
+                              fp32_copy=True)
     c2 = D.PriorCache.generate(1 << 16, seed=7, P_min=1.0, P_max=16384.0, ln_s=(-3.0, 1.0))
     P = c.P.cpu().numpy()
     e = c.e.cpu().numpy()

@@ -372,8 +372,7 @@ def rejection_sample_many(datas, prior, prior_samples, rngs, max_posterior_sampl
     if isinstance(prior_samples, PriorCache):
         cache = prior_samples
     else:
-        cache = PriorCache.from_samples(prior_samples, rv_unit=rv_unit, device=device,
-                                        fp32_copy=precision == "mixed")
+        cache = PriorCache.from_samples(prior_samples, rv_unit=rv_unit, device=device)
     n = len(cache)
     batch = StarBatch(stars, device=device)
     shape = shape or LaunchShape()

@@ -246,12 +246,13 @@ def linear_internal_units(prior, rv_unit):
 class PriorCache:
     """Prior samples resident in HBM as SoA columns.
 
-    FP64 master: P [day], e, omega [rad], M0 [rad], s [rv unit] (40 B/row);
-    FP32 copy of e, omega, M0, s (16 B/row) for the mixed-precision kernel.
+    FP64 columns: P [day], e, omega [rad], M0 [rad], s [rv unit] (40 B/row),
+    read by both precisions.  fp32_copy adds FP32 copies of e, omega, M0, s
+    (16 B/row) for callers of their own; no kernel here reads them.
     `ln_prior` (optional) stays on the host.
     """
 
-    def __init__(self, P, e, omega, M0, s=None, ln_prior=None, device=0, fp32_copy=True):
+    def __init__(self, P, e, omega, M0, s=None, ln_prior=None, device=0, fp32_copy=False):
         torch = _torch()
         _lib.ensure_device(device)
         self.device = device
@@ -269,13 +270,13 @@ class PriorCache:
         return int(self.P.shape[0])
 
     @classmethod
-    def from_packed(cls, packed, ln_prior=None, device=0, fp32_copy=True):
+    def from_packed(cls, packed, ln_prior=None, device=0, fp32_copy=False):
         packed = np.asarray(packed, dtype=np.float64)
         return cls(packed[:, 0], packed[:, 1], packed[:, 2], packed[:, 3], packed[:, 4],
                    ln_prior=ln_prior, device=device, fp32_copy=fp32_copy)
 
     @classmethod
-    def from_samples(cls, samples, rv_unit=u.km / u.s, device=0, fp32_copy=True):
+    def from_samples(cls, samples, rv_unit=u.km / u.s, device=0, fp32_copy=False):
         units = dict(NONLINEAR_INTERNAL_UNITS)
         units["s"] = rv_unit
         packed, _ = samples.pack(units=units, names=NONLINEAR_PACKED_ORDER)
@@ -284,7 +285,7 @@ class PriorCache:
 
     @classmethod
     def generate(cls, n, seed=0, P_min=2.0, P_max=2048.0, e_ab=(0.867, 3.03), ln_s=None,
-                 device=0, fp32_copy=True, n_tab=1 << 16):
+                 device=0, fp32_copy=False, n_tab=1 << 16):
         """n prior rows drawn on the device (jk_prior_fill): P log-uniform on
         [P_min, P_max] days, e ~ Beta(e_ab) (inverse CDF table), omega and M0
         uniform on [0, 2 pi), ln s ~ N(ln_s) km/s or s = 0.  Row r depends

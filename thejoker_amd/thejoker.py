@@ -125,11 +125,9 @@ class TheJoker:
             prior_samples = JokerSamples.read(prior_samples)
         if isinstance(prior_samples, JokerSamples):
             return PriorCache.from_samples(prior_samples, rv_unit=star.data.rv.unit,
-                                           device=self.device,
-                                           fp32_copy=self.precision == "mixed")
+                                           device=self.device)
         if isinstance(prior_samples, np.ndarray):
-            return PriorCache.from_packed(prior_samples, device=self.device,
-                                          fp32_copy=self.precision == "mixed")
+            return PriorCache.from_packed(prior_samples, device=self.device)
         raise TypeError("prior_samples must either be a string filename specifying a cache file "
                         "containing prior samples, or must be a JokerSamples instance, not: "
                         f"{type(prior_samples)}")
