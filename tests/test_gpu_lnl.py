@@ -125,7 +125,8 @@ def test_invalid_rows_do_not_poison(gpu):
     assert np.array_equal(got[keep], good[keep])
 
 
-@pytest.mark.parametrize("M,L", [(3, 1), (3, 2), (64, 3), (200, 2), (200, 4)])
+@pytest.mark.parametrize("M,L", [(1, 1), (1, 2), (2, 3), (3, 1), (3, 2), (64, 3), (200, 2),
+                                 (200, 4)])
 def test_shapes_vs_oracle(gpu, M, L):
     rng = np.random.default_rng(M * 10 + L)
     t = np.sort(rng.uniform(0, 1500, M))
