@@ -274,10 +274,12 @@ def leg_batched(args, torch, np, D, syn):
     settled = [0]
 
     def step():
+        # lnL with bounds, then settling and rejection in one launch sequence
         run_m()
-        st = states()
-        settled[0] = int(B.settle_batch(batch, cache, llm, bnd, off, sizes, st).sum())
-        return B.reject_batch(llm, off, sizes, st, 256)
+        good, n_pass, mx, sett = B.settle_reject_batch(batch, cache, llm, bnd, off, sizes,
+                                                       states(), 256)
+        settled[0] = int(sett.sum())
+        return good, n_pass, mx
 
     def step_f():
         run_f()
@@ -304,9 +306,14 @@ def leg_batched(args, torch, np, D, syn):
     capture_s = time.perf_counter() - t0
     graph.run(states())
     torch.cuda.synchronize()
+    # the next step's streams are drawn on the host while the replay runs,
+    # as the eager step draws them while its lnL launch runs
+    st = states()
     t0 = time.perf_counter()
     for _ in range(2):
-        good_g, _, _, _ = graph.run(states())
+        graph.launch(st)
+        st = states()
+        good_g, _, _, _ = graph.results()
     torch.cuda.synchronize()
     graph_ms = (time.perf_counter() - t0) / 2 * 1e3
     graph_flips = sum(len(np.setxor1d(x, y)) for x, y in zip(good_g, good_f))

@@ -328,6 +328,22 @@ int jk_settle_batch(const jk_batch *batch, int n_seg, const jk_settle_segment *s
                     const double *s, const int64_t *rows, double *ll_inout, float *bound_inout,
                     int64_t *n_settled_out, void *stream);
 
+/* jk_settle_batch followed by jk_reject_batch (max_keep for every segment;
+ * idx_out holds the segments' kept indices one after the other, min(n,
+ * max_keep) places each), in one launch sequence that draws each row's
+ * uniform once: after the rows that could hold the maximum are settled, one
+ * pass decides every row the bound leaves no doubt about and flags the
+ * others, which are re-evaluated in FP64 and decided from their own uniform.
+ * Same accepted set, n_pass and maximum as the two calls; n_settled_out[i]
+ * counts segment i's re-evaluated rows.  segs is a HOST array.  No reference
+ * counterpart. */
+int jk_settle_reject_batch(const jk_batch *batch, int n_seg, const jk_settle_segment *segs,
+                           int64_t max_keep, const double *P, const double *e,
+                           const double *omega, const double *M0, const double *s,
+                           const int64_t *rows, double *ll_inout, float *bound_inout,
+                           int64_t *idx_out, int64_t *n_pass_out, double *llmax_out,
+                           int64_t *n_settled_out, void *stream);
+
 /* One segment of jk_settle_live_batch: entries [0, n) of the lnL buffer at
  * offset (cache rows row_begin + i, or rows[row_begin + i]) of table star;
  * entries [first, n) are new since the segment's previous call, and floor is
@@ -392,6 +408,12 @@ int jk_seg_plan_settle(const jk_seg_plan *plan, const double *P, const double *e
                        int64_t *n_settled_out, void *stream);
 int jk_seg_plan_reject(const jk_seg_plan *plan, const double *ll, int64_t *idx_out,
                        int64_t *n_pass_out, double *llmax_out, void *stream);
+/* jk_settle_reject_batch on the plan's segments (max_keep of the plan). */
+int jk_seg_plan_settle_reject(const jk_seg_plan *plan, const double *P, const double *e,
+                              const double *omega, const double *M0, const double *s,
+                              const int64_t *rows, double *ll_inout, float *bound_inout,
+                              int64_t *idx_out, int64_t *n_pass_out, double *llmax_out,
+                              int64_t *n_settled_out, void *stream);
 
 /* Fill rows [row0, row0 + n) of a device prior cache (arrays indexed from
  * 0 for row row0) with draws of the default nonlinear prior

@@ -376,3 +376,32 @@ def test_live_settle_and_zero_check_reject_bad_arguments(setup):
     # a first = n segment has no new rows: nothing settled, count 0
     cnt = B.settle_live_batch(batch, cache, ll, bound, [0, n], [n, n], [n, n], [-np.inf] * 2)
     assert cnt.cpu().tolist() == [0, 0]
+
+
+@pytest.mark.parametrize("max_keep", [1, 64, 1 << 20])
+def test_settle_reject_equals_settle_then_reject(setup, max_keep):
+    """jk_settle_reject_batch (one uniform per row, open rows decided after
+    their FP64 re-evaluation) gives the accepted sets, pass counts, maxima,
+    settled counts and settled buffer of jk_settle_batch + jk_reject_batch,
+    and so the FP64 sets."""
+    stars, cache, _ = setup
+    batch = B.StarBatch([d for _, d in stars])
+    n = len(cache)
+    items, off, total = B.work_items([n] * len(stars), 1024)
+    d_items = B.upload_items(items)
+    ll64 = B.lnl_batch(batch, cache, d_items, len(items), total, "f64")
+    b1 = torch.empty(total, dtype=torch.float32, device="cuda")
+    l1 = B.lnl_batch(batch, cache, d_items, len(items), total, "mixed", bound=b1)
+    b2, l2 = b1.clone(), l1.clone()
+    sizes = np.full(len(stars), n)
+    states = [D.pcg64_state(np.random.default_rng(77 + i)) for i in range(len(stars))]
+    cnt = B.settle_batch(batch, cache, l1, b1, off, sizes, states).cpu().numpy()
+    want, npw, mxw = B.reject_batch(l1, off, sizes, states, max_keep)
+    got, npg, mxg, sett = B.settle_reject_batch(batch, cache, l2, b2, off, sizes, states, max_keep)
+    f64, np64, _ = B.reject_batch(ll64, off, sizes, states, max_keep)
+    for i in range(len(stars)):
+        assert np.array_equal(got[i], want[i]), i
+        assert np.array_equal(got[i], f64[i]), i
+    assert np.array_equal(npg, npw) and np.array_equal(npg, np64)
+    assert np.array_equal(mxg, mxw) and np.array_equal(sett, cnt)
+    assert torch.equal(l1, l2) and torch.equal(b1, b2)

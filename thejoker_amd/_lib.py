@@ -67,6 +67,9 @@ SIGNATURES = [
      [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _vp, _vp]),
     ("jk_settle_batch", ctypes.c_int,
      [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("jk_settle_reject_batch", ctypes.c_int,
+     [_vp, ctypes.c_int, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+      _vp]),
     ("jk_settle_live_batch", ctypes.c_int,
      [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _vp, _vp]),
     ("jk_seg_plan_create", ctypes.c_int,
@@ -75,6 +78,8 @@ SIGNATURES = [
     ("jk_seg_plan_set_streams", ctypes.c_int, [_vp, _vp, _vp]),
     ("jk_seg_plan_settle", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("jk_seg_plan_reject", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    ("jk_seg_plan_settle_reject", ctypes.c_int,
+     [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("jk_prior_fill", ctypes.c_int,
      [_i64, _i64, _u64, ctypes.c_double, ctypes.c_double, _vp, ctypes.c_int, ctypes.c_double,
       ctypes.c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -20,7 +20,8 @@ from .device import (DeviceStar, _dptr, _split, _stream, _torch, pcg64_advance_s
 
 __all__ = ["WORK_ITEM_DTYPE", "SEGMENT_DTYPE", "POLISH_SEGMENT_DTYPE", "SETTLE_SEGMENT_DTYPE",
            "LIVE_SEGMENT_DTYPE", "StarBatch", "work_items", "lnl_batch", "reject_batch",
-           "polish_batch", "settle_batch", "settle_live_batch", "LIVE_MARGIN",
+           "polish_batch", "settle_batch", "settle_reject_batch", "settle_live_batch",
+           "LIVE_MARGIN",
            "LIVE_MARGIN_NONZERO_U",
            "autotune", "LaunchShape", "rejection_sample_many", "SegPlan", "StepGraph"]
 
@@ -272,6 +273,45 @@ def settle_batch(batch, cache, ll, bound, offsets, sizes, rng_states, stars=None
     return count
 
 
+def settle_reject_batch(batch, cache, ll, bound, offsets, sizes, rng_states, max_keep,
+                        stars=None, row_begins=None, rows=None, stream_offsets=None, device=0):
+    """jk_settle_reject_batch: :func:`settle_batch` then :func:`reject_batch`
+    in one launch sequence (one uniform per row).  Returns (list of index
+    arrays, n_pass array, llmax array, rows settled per segment)."""
+    torch = _torch()
+    lib = _lib.ensure_device(device)
+    n_seg = len(sizes)
+    sizes = np.asarray(sizes, dtype=np.int64)
+    segs = np.zeros(n_seg, dtype=SETTLE_SEGMENT_DTYPE)
+    segs["offset"] = offsets
+    segs["n"] = sizes
+    segs["star"] = np.arange(n_seg) if stars is None else stars
+    segs["row_begin"] = 0 if row_begins is None else row_begins
+    for i, (st, inc) in enumerate(rng_states):
+        segs["state_hi"][i], segs["state_lo"][i] = _split(int(st))
+        segs["inc_hi"][i], segs["inc_lo"][i] = _split(int(inc))
+    if stream_offsets is not None:
+        segs["stream_offset"] = stream_offsets
+    keep = np.minimum(sizes, int(max_keep))
+    keep_off = np.concatenate([[0], np.cumsum(keep)[:-1]]).astype(np.int64)
+    # one device block [n_pass | llmax | n_settled | kept indices], one copy back
+    res = torch.empty(3 * n_seg + max(int(keep.sum()), 1), dtype=torch.int64, device=ll.device)
+    p = res.data_ptr()
+    _lib.check(lib.jk_settle_reject_batch(
+        batch.handle, n_seg, segs.ctypes.data_as(ctypes.c_void_p), int(max_keep), _dptr(cache.P),
+        _dptr(cache.e), _dptr(cache.omega), _dptr(cache.M0), _dptr(cache.s), _dptr(rows),
+        _dptr(ll), _dptr(bound), ctypes.c_void_p(p + 24 * n_seg), ctypes.c_void_p(p),
+        ctypes.c_void_p(p + 8 * n_seg), ctypes.c_void_p(p + 16 * n_seg), _stream(device)),
+        "jk_settle_reject_batch")
+    h = to_host(res)
+    n_pass = h[:n_seg].copy()
+    mx = h[n_seg:2 * n_seg].view(np.float64).copy()
+    settled = h[2 * n_seg:3 * n_seg].copy()
+    idx = h[3 * n_seg:]
+    out = [idx[o:o + min(int(k), int(npass))].copy() for o, k, npass in zip(keep_off, keep, n_pass)]
+    return out, n_pass, mx, settled
+
+
 def settle_live_batch(batch, cache, ll, bound, offsets, sizes, firsts, floors, stars=None,
                       row_begins=None, rows=None, device=0, margin=LIVE_MARGIN):
     """jk_settle_live_batch: settle the new entries [firsts[i], sizes[i]) of
@@ -385,10 +425,12 @@ def rejection_sample_many(datas, prior, prior_samples, rngs, max_posterior_sampl
     states = [pcg64_state(g) for g in rngs]
     if precision == "mixed":
         # FP64 re-evaluation of exactly the rows whose decision the mixed
-        # likelihood's error bound leaves open
-        settle_batch(batch, cache, ll, bound, off, np.full(len(stars), n), states, device=device)
-    good, _, _ = reject_batch(ll, off, np.full(len(stars), n), states, max_posterior_samples,
-                              device=device)
+        # likelihood's error bound leaves open, and the rejection step
+        good = settle_reject_batch(batch, cache, ll, bound, off, np.full(len(stars), n), states,
+                                   max_posterior_samples, device=device)[0]
+    else:
+        good = reject_batch(ll, off, np.full(len(stars), n), states, max_posterior_samples,
+                            device=device)[0]
     for g in rngs:
         pcg64_advance_state(g, n)
     ll_h = ll.cpu().numpy() if return_all_logprobs else None
@@ -464,6 +506,16 @@ class SegPlan:
             _dptr(cache.s), _dptr(rows), _dptr(ll), _dptr(bound),
             ctypes.c_void_p(p + 16 * self.n_seg), _stream(self.device)), "jk_seg_plan_settle")
 
+    def settle_reject(self, cache, ll, bound, rows=None):
+        """Settling and rejection in one launch sequence (jk_settle_reject_batch)."""
+        p = self.res.data_ptr()
+        _lib.check(self._lib.jk_seg_plan_settle_reject(
+            self._h, _dptr(cache.P), _dptr(cache.e), _dptr(cache.omega), _dptr(cache.M0),
+            _dptr(cache.s), _dptr(rows), _dptr(ll), _dptr(bound),
+            ctypes.c_void_p(p + 24 * self.n_seg), ctypes.c_void_p(p),
+            ctypes.c_void_p(p + 8 * self.n_seg), ctypes.c_void_p(p + 16 * self.n_seg),
+            _stream(self.device)), "jk_seg_plan_settle_reject")
+
     def reject(self, ll):
         p = self.res.data_ptr()
         _lib.check(self._lib.jk_seg_plan_reject(
@@ -522,8 +574,9 @@ class StepGraph:
             lnl_batch(batch, cache, d_items, n_items, total, precision, shape, out=self.ll,
                       workspace=self.ws, bound=self.bound)
             if precision == "mixed":
-                self.plan.settle(cache, self.ll, self.bound)
-            self.plan.reject(self.ll)
+                self.plan.settle_reject(cache, self.ll, self.bound)
+            else:
+                self.plan.reject(self.ll)
 
         body()                      # warm up every launch path once, eagerly
         torch.cuda.synchronize()
@@ -532,8 +585,17 @@ class StepGraph:
             body()
         torch.cuda.synchronize()
 
-    def run(self, rng_states, stream_offsets=None):
-        """Replay the step with these streams; returns SegPlan.results()."""
+    def launch(self, rng_states, stream_offsets=None):
+        """Queue a replay of the step with these streams and return at once;
+        :meth:`results` collects it."""
         self.plan.set_streams(rng_states, stream_offsets)
         self.graph.replay()
+
+    def results(self):
+        """SegPlan.results() of the last launched step, after waiting for it."""
         return self.plan.results()
+
+    def run(self, rng_states, stream_offsets=None):
+        """Replay the step with these streams; returns SegPlan.results()."""
+        self.launch(rng_states, stream_offsets)
+        return self.results()

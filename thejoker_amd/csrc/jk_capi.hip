@@ -810,6 +810,88 @@ int jk_settle_batch(const jk_batch *batch, int n_seg, const jk_settle_segment *s
     return check_hip(err2);
 }
 
+int jk_settle_reject_batch(const jk_batch *batch, int n_seg, const jk_settle_segment *segs,
+                           int64_t max_keep, const double *P, const double *e,
+                           const double *omega, const double *M0, const double *s,
+                           const int64_t *rows, double *ll_inout, float *bound_inout,
+                           int64_t *idx_out, int64_t *n_pass_out, double *llmax_out,
+                           int64_t *n_settled_out, void *stream)
+{
+    if (!batch || n_seg <= 0 || !segs || !ll_inout || !bound_inout || !n_pass_out ||
+        !llmax_out || !n_settled_out || max_keep < 0 || (max_keep > 0 && !idx_out))
+        return JK_EINVAL;
+    if (!P || !e || !omega || !M0) return JK_EINVAL;
+    if (batch->jitter_mode == JK_JITTER_PHYSICAL && !s) return JK_EINVAL;
+    if (current_device_inited() < 0) return JK_ENODEV;
+    std::vector<SegDev> hs(n_seg), hr(n_seg);
+    int64_t max_n = 1, max_nb = 1, mask_words = 0, bsum_n = 0, idx_n = 0, keep_n = 0;
+    for (int i = 0; i < n_seg; ++i) {
+        const jk_settle_segment &g = segs[i];
+        if (g.n <= 0 || g.offset < 0 || g.row_begin < 0 || g.star < 0 || g.star >= batch->n_stars)
+            return JK_EINVAL;
+        SegDev &d = hs[i];
+        memset(&d, 0, sizeof(d));
+        d.off = g.offset;
+        d.n = g.n;
+        d.max_keep = g.n;
+        d.idx_off = idx_n;
+        d.mask_off = mask_words;
+        d.nb = jk_reject_bsum_count(g.n);
+        d.bsum_off = bsum_n;
+        d.star = g.star;
+        d.row0 = g.row_begin;
+        u128 state = make_u128(g.state_hi, g.state_lo);
+        d.inc = make_u128(g.inc_hi, g.inc_lo);
+        jk_host_pcg_jump(state, d.inc, g.stream_offset + 1, &d.s_base);
+        // the final compaction: the caller's max_keep and packed output
+        SegDev &r = hr[i];
+        r = d;
+        r.max_keep = g.n < max_keep ? g.n : max_keep;
+        r.idx_off = keep_n;
+        keep_n += r.max_keep;
+        idx_n += g.n;
+        mask_words += jk_reject_mask_words(g.n);
+        bsum_n += d.nb;
+        if (g.n > max_n) max_n = g.n;
+        if (d.nb > max_nb) max_nb = d.nb;
+    }
+    int pb = (int)((max_n + 4095) / 4096);
+    if (pb > 1024) pb = 1024;
+    if (pb < 1) pb = 1;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t seg_bytes = sizeof(SegDev) * n_seg;
+    const size_t seg_pad = ((seg_bytes + 255) / 256) * 256;
+    const size_t bytes = 2 * seg_pad + sizeof(double) * ((size_t)pb * n_seg + (size_t)n_seg) +
+                         2 * sizeof(uint64_t) * mask_words +
+                         sizeof(int64_t) * (bsum_n + idx_n + 2 * (size_t)n_seg + 1);
+    char *scratch = nullptr;
+    if (hipMallocAsync((void **)&scratch, bytes, st) != hipSuccess) return JK_ENOMEM;
+    SegDev *d_settle = (SegDev *)scratch;
+    SegDev *d_reject = (SegDev *)(scratch + seg_pad);
+    double *partials = (double *)(scratch + 2 * seg_pad);
+    double *lo = partials + (size_t)pb * n_seg;
+    uint64_t *mask = (uint64_t *)(lo + n_seg);
+    uint64_t *open_mask = mask + mask_words;
+    int64_t *bsum = (int64_t *)(open_mask + mask_words);
+    int64_t *idx = bsum + bsum_n;
+    int64_t *cum = idx + idx_n;
+    int64_t *count1 = cum + n_seg + 1;
+    hipError_t err = hipMemcpyAsync(d_settle, hs.data(), seg_bytes, hipMemcpyHostToDevice, st);
+    if (err == hipSuccess)
+        err = hipMemcpyAsync(d_reject, hr.data(), seg_bytes, hipMemcpyHostToDevice, st);
+    if (err == hipSuccess)
+        err = jk_launch_settle_reject_batch(batch, n_seg, d_settle, d_reject, max_n, max_nb,
+                                            partials, pb, mask, open_mask, bsum, idx, lo, cum,
+                                            count1, P, e, omega, M0, s, rows, ll_inout,
+                                            bound_inout, idx_out, n_pass_out, llmax_out,
+                                            n_settled_out, st);
+    // The host descriptor arrays must outlive the queued copies.
+    hipError_t err3 = hipStreamSynchronize(st);
+    hipError_t err2 = hipFreeAsync(scratch, st);
+    if (err != hipSuccess || err3 != hipSuccess) return JK_EHIP;
+    return check_hip(err2);
+}
+
 int jk_settle_live_batch(const jk_batch *batch, int n_seg, const jk_live_segment *segs,
                          const double *P, const double *e, const double *omega, const double *M0,
                          const double *s, const int64_t *rows, double *ll_inout,
@@ -895,7 +977,7 @@ struct jk_seg_plan {
     char *mem;                                 // device: descriptors + scratch
     SegDev *d_settle, *d_reject;
     double *partials, *lo, *llmax;
-    uint64_t *mask;
+    uint64_t *mask, *open_mask;
     int64_t *bsum, *idx, *cum, *count1;
 };
 
@@ -987,7 +1069,7 @@ int jk_seg_plan_create(const jk_batch *batch, int n_seg, const jk_settle_segment
     const size_t seg_pad = ((seg_bytes + 255) / 256) * 256;
     const size_t bytes = 2 * seg_pad +
                          sizeof(double) * ((size_t)pb * n_seg + 2 * (size_t)n_seg) +
-                         sizeof(uint64_t) * mask_words +
+                         2 * sizeof(uint64_t) * mask_words +
                          sizeof(int64_t) * (bsum_n + idx_n + 2 * (size_t)n_seg + 1);
     if (hipMalloc((void **)&p->mem, bytes) != hipSuccess ||
         hipHostMalloc((void **)&p->pinned, 2 * seg_bytes, hipHostMallocDefault) != hipSuccess ||
@@ -1001,7 +1083,8 @@ int jk_seg_plan_create(const jk_batch *batch, int n_seg, const jk_settle_segment
     p->lo = p->partials + (size_t)pb * n_seg;
     p->llmax = p->lo + n_seg;
     p->mask = (uint64_t *)(p->llmax + n_seg);
-    p->bsum = (int64_t *)(p->mask + mask_words);
+    p->open_mask = p->mask + mask_words;
+    p->bsum = (int64_t *)(p->open_mask + mask_words);
     p->idx = p->bsum + bsum_n;
     p->cum = p->idx + idx_n;
     p->count1 = p->cum + n_seg + 1;
@@ -1050,6 +1133,24 @@ int jk_seg_plan_settle(const jk_seg_plan *p, const double *P, const double *e,
                                             p->llmax, p->cum, p->count1, P, e, omega, M0, s, rows,
                                             ll_inout, bound_inout, n_settled_out,
                                             (hipStream_t)stream));
+}
+
+int jk_seg_plan_settle_reject(const jk_seg_plan *p, const double *P, const double *e,
+                              const double *omega, const double *M0, const double *s,
+                              const int64_t *rows, double *ll_inout, float *bound_inout,
+                              int64_t *idx_out, int64_t *n_pass_out, double *llmax_out,
+                              int64_t *n_settled_out, void *stream)
+{
+    if (!p || !P || !e || !omega || !M0 || !ll_inout || !bound_inout || !n_pass_out ||
+        !llmax_out || !n_settled_out)
+        return JK_EINVAL;
+    if (p->batch->jitter_mode == JK_JITTER_PHYSICAL && !s) return JK_EINVAL;
+    if (current_device_inited() < 0) return JK_ENODEV;
+    return check_hip(jk_launch_settle_reject_batch(
+        p->batch, p->n_seg, p->d_settle, p->d_reject, p->max_n, p->max_nb, p->partials, p->pb,
+        p->mask, p->open_mask, p->bsum, p->idx, p->lo, p->cum, p->count1, P, e, omega, M0, s, rows,
+        ll_inout, bound_inout, idx_out, n_pass_out, llmax_out, n_settled_out,
+        (hipStream_t)stream));
 }
 
 int jk_seg_plan_reject(const jk_seg_plan *p, const double *ll, int64_t *idx_out,

@@ -118,6 +118,16 @@ hipError_t jk_launch_settle_live_batch(const jk_batch *b, int n_seg, const SegDe
                                        const double *om, const double *M0, const double *s,
                                        const int64_t *rows, double *ll, float *bound,
                                        double margin, int64_t *n_settled, hipStream_t stream);
+hipError_t jk_launch_settle_reject_batch(const jk_batch *b, int n_seg, const SegDev *d_settle,
+                                         const SegDev *d_reject, int64_t max_n, int64_t max_nb,
+                                         double *partials, int pb, uint64_t *mask,
+                                         uint64_t *open_mask, int64_t *bsum, int64_t *idx,
+                                         double *lo, int64_t *cum, int64_t *count1,
+                                         const double *P, const double *e, const double *om,
+                                         const double *M0, const double *s, const int64_t *rows,
+                                         double *ll, float *bound, int64_t *idx_out,
+                                         int64_t *n_pass_out, double *llmax_out,
+                                         int64_t *n_settled, hipStream_t stream);
 hipError_t jk_launch_polish(const jk_star *st, int64_t n, const double *P, const double *e,
                             const double *om, const double *M0, const double *s,
                             const int64_t *rows, double *ll, double window, int64_t first,

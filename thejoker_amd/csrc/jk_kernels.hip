@@ -1916,6 +1916,135 @@ __global__ __launch_bounds__(256) void k_seg_live_flags(const double *__restrict
     }
 }
 
+// ---------------------------------------------------------------------------
+// Settling and rejection in one sequence (jk_settle_reject_batch): after the
+// rows that could hold the maximum are settled and the FP64 maximum found
+// (steps 1-2 of jk_settle_batch), one pass draws every row's uniform once
+// and decides it -- directly where the bound leaves no doubt, else flags it
+// open.  The open rows (a few per million) are re-evaluated in FP64 and
+// their decisions set from a uniform jumped to their index.  The accepted
+// set is the one jk_settle_batch + jk_reject_batch give, with one pass over
+// the rows and one PCG64 stream fewer.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_seg_decide(const double *__restrict__ ll,
+                                                    const float *__restrict__ bound,
+                                                    const SegDev *__restrict__ segs,
+                                                    const double *__restrict__ llmax,
+                                                    u128 AT, u128 DT,
+                                                    uint64_t *__restrict__ mask,
+                                                    uint64_t *__restrict__ open_mask)
+{
+    const SegDev sg = segs[blockIdx.y];
+    const int lane = threadIdx.x & 63;
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t T = (int64_t)gridDim.x * 256;
+    if (g - lane >= sg.n) return;
+    const double mx = llmax[blockIdx.y];
+    u128 s = pcg_thread_start(sg.s_base, sg.inc);
+    for (int64_t base = g - lane; base < sg.n; base += T) {
+        const int64_t i = base + lane;
+        bool acc = false, open = false;
+        if (i < sg.n) {
+            const double uu = pcg_to_double(pcg_output(s));
+            const double x = ll[sg.off + i];
+            const float bf = bound[sg.off + i];
+            if (bf > 0.0f) {
+                // a NaN maximum makes both sides false: nothing accepted, nothing open
+                const double b = fma((double)bf, 1e-6, (double)bf) + 1e-12;
+                const bool lo = exp(x - b - mx) > uu, hi = exp(x + b - mx) > uu;
+                open = lo != hi;
+                acc = lo && hi;
+            } else {
+                acc = exp(x - mx) > uu;
+            }
+        }
+        const uint64_t wa = __ballot(acc), wo = __ballot(open);
+        if (lane == 0) {
+            mask[sg.mask_off + (base >> 6)] = wa;
+            open_mask[sg.mask_off + (base >> 6)] = wo;
+        }
+        s = pcg_affine(s, AT, DT, sg.inc);
+    }
+}
+
+// The decisions of the open rows, now in FP64: each row's uniform from the
+// segment's stream jumped to its index; accepted rows set their bit.
+__global__ __launch_bounds__(256) void k_seg_patch(const SegDev *__restrict__ segs,
+                                                   const int64_t *__restrict__ count,
+                                                   const int64_t *__restrict__ idx,
+                                                   const double *__restrict__ ll,
+                                                   const double *__restrict__ llmax,
+                                                   uint64_t *__restrict__ mask)
+{
+    const SegDev sg = segs[blockIdx.y];
+    const int64_t cnt = count[blockIdx.y];
+    const double mx = llmax[blockIdx.y];
+    for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < cnt;
+         j += (int64_t)gridDim.x * 256) {
+        const int64_t i = idx[sg.idx_off + j];
+        const u128 st = pcg_jump_table(sg.s_base, (uint64_t)i, sg.inc);
+        const double uu = pcg_to_double(pcg_output(st));
+        if (exp(ll[sg.off + i] - mx) > uu)
+            atomicOr((unsigned long long *)&mask[sg.mask_off + (i >> 6)], 1ull << (i & 63));
+    }
+}
+
+hipError_t jk_launch_settle_reject_batch(const jk_batch *b, int n_seg, const SegDev *d_settle,
+                                         const SegDev *d_reject, int64_t max_n, int64_t max_nb,
+                                         double *partials, int pb, uint64_t *mask,
+                                         uint64_t *open_mask, int64_t *bsum, int64_t *idx,
+                                         double *lo, int64_t *cum, int64_t *count1,
+                                         const double *P, const double *e, const double *om,
+                                         const double *M0, const double *s, const int64_t *rows,
+                                         double *ll, float *bound, int64_t *idx_out,
+                                         int64_t *n_pass_out, double *llmax_out,
+                                         int64_t *n_settled, hipStream_t stream)
+{
+    if (n_seg <= 0) return hipSuccess;
+    const int fb = jk_seg_flag_blocks(max_n);
+    // 1. the rows that could hold the FP64 maximum, settled
+    hipLaunchKernelGGL(k_seg_lo_partial, dim3(pb, n_seg), dim3(256), 0, stream, ll, bound,
+                       d_settle, partials, pb);
+    hipLaunchKernelGGL(k_seg_max_final, dim3(n_seg), dim3(64), 0, stream, partials, pb, lo);
+    hipLaunchKernelGGL(k_seg_shell_flags, dim3(fb, n_seg), dim3(256), 0, stream, ll, bound,
+                       d_settle, lo, mask);
+    hipLaunchKernelGGL(k_seg_bsum, dim3((unsigned)max_nb, n_seg), dim3(256), 0, stream, d_settle,
+                       mask, bsum);
+    hipLaunchKernelGGL(k_seg_compact, dim3((unsigned)max_nb, n_seg), dim3(256), 0, stream,
+                       d_settle, mask, bsum, idx, count1);
+    hipError_t err = launch_seg_scatter(b, n_seg, d_settle, count1, cum, idx, P, e, om, M0, s,
+                                        rows, ll, bound, stream);
+    if (err != hipSuccess) return err;
+    // 2. the FP64 maximum
+    hipLaunchKernelGGL(k_seg_max_partial, dim3(pb, n_seg), dim3(256), 0, stream, ll, d_settle,
+                       partials, pb);
+    hipLaunchKernelGGL(k_seg_max_final, dim3(n_seg), dim3(64), 0, stream, partials, pb,
+                       llmax_out);
+    // 3. every row decided or flagged open, one uniform each
+    u128 AT, DT;
+    jk_pcg_jump_coeffs((uint64_t)fb * 256, &AT, &DT);
+    hipLaunchKernelGGL(k_seg_decide, dim3(fb, n_seg), dim3(256), 0, stream, ll, bound, d_settle,
+                       llmax_out, AT, DT, mask, open_mask);
+    // 4. the open rows in FP64, then their decisions
+    hipLaunchKernelGGL(k_seg_bsum, dim3((unsigned)max_nb, n_seg), dim3(256), 0, stream, d_settle,
+                       open_mask, bsum);
+    hipLaunchKernelGGL(k_seg_compact, dim3((unsigned)max_nb, n_seg), dim3(256), 0, stream,
+                       d_settle, open_mask, bsum, idx, n_settled);
+    err = launch_seg_scatter(b, n_seg, d_settle, n_settled, cum, idx, P, e, om, M0, s, rows, ll,
+                             bound, stream);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(k_seg_patch, dim3(4, n_seg), dim3(256), 0, stream, d_settle, n_settled,
+                       idx, ll, llmax_out, mask);
+    // 5. the ordered compaction of the accepted rows
+    hipLaunchKernelGGL(k_seg_bsum, dim3((unsigned)max_nb, n_seg), dim3(256), 0, stream, d_reject,
+                       mask, bsum);
+    hipLaunchKernelGGL(k_seg_compact, dim3((unsigned)max_nb, n_seg), dim3(256), 0, stream,
+                       d_reject, mask, bsum, idx_out, n_pass_out);
+    hipLaunchKernelGGL(k_seg_add_counts, dim3((n_seg + 255) / 256), dim3(256), 0, stream, count1,
+                       n_settled, n_seg);
+    return hipGetLastError();
+}
+
 hipError_t jk_launch_settle_live_batch(const jk_batch *b, int n_seg, const SegDev *d_segs,
                                        int64_t max_n, int64_t max_nb, double *partials, int pb,
                                        uint64_t *mask, int64_t *bsum, int64_t *idx, double *lo,
