@@ -152,6 +152,17 @@ class TheJoker:
                      device=device)
 
     @staticmethod
+    def _settle_reject(star, cache, ll, bound, rng_state, max_keep, rows=None, device=0):
+        """Mixed precision, one rejection step: _settle then the rejection step
+        (jk_settle_reject_batch on a one-star table); the accepted indices."""
+        from .batch import StarBatch, settle_reject_batch
+        table = getattr(star, "_settle_table", None)
+        if table is None:
+            table = star._settle_table = StarBatch([star], device=device)
+        return settle_reject_batch(table, cache, ll, bound, [0], [int(ll.shape[0])], [rng_state],
+                                   max_keep, rows=rows, device=device)[0][0]
+
+    @staticmethod
     def _settle_live(star, cache, ll, bound, first, floor, rows=None, device=0):
         """Mixed precision, iterative loop: re-evaluate in FP64 the new rows
         ll[first:] that lie within a 750 margin of the maximum, so that the
@@ -277,9 +288,10 @@ class TheJoker:
         else:
             bound = torch.empty(n_use, dtype=torch.float32, device=cache.P.device)
             lls = self._lnl_rows(star, cache, rows=d_rows, start=0, n=n_use, bound=bound)
-            self._settle(star, cache, lls, bound, pcg64_state(rng), rows=d_rows,
-                         device=self.device)
-            good, _, _ = reject(lls, pcg64_state(rng), max_posterior_samples, device=self.device)
+            # the rows the bound leaves open re-evaluated in FP64, and the
+            # rejection step, in one launch sequence
+            good = self._settle_reject(star, cache, lls, bound, pcg64_state(rng),
+                                       max_posterior_samples, rows=d_rows, device=self.device)
         pcg64_advance_state(rng, n_use)
         full_idx = rows[good] if rows is not None else good
 
