@@ -202,6 +202,7 @@ int jk_linear_posterior(const jk_star *star, int64_t n, const double *P, const d
                         const int64_t *rows, double *a_out, double *cov_out, double *ll_out,
                         void *stream);
 
+
 /* ------------------------------------------------------------------------
  * Many stars in one launch.  No reference counterpart: the reference handles
  * one star per CJokerHelper (fast_likelihood.pyx:70) and parallelises only
@@ -237,6 +238,16 @@ typedef struct {
  * star table; stars[i] is addressed as star index i by work items.  The
  * jk_star objects may be destroyed afterwards. */
 int jk_batch_create(const jk_star *const *stars, int n_stars, jk_batch **out);
+
+/* jk_linear_posterior for the accepted rows of many stars of a table in one
+ * launch: row i (cache row rows[i]) belongs to table entry star_idx[i]
+ * (device int32); outputs laid out as jk_linear_posterior's with the
+ * table's L.  star_idx entries must be valid table indices (slots in use).
+ * No reference counterpart (the reference draws per star). */
+int jk_linear_posterior_batch(const jk_batch *batch, int64_t n, const int32_t *star_idx,
+                              const double *P, const double *e, const double *omega,
+                              const double *M0, const double *s, const int64_t *rows,
+                              double *a_out, double *cov_out, double *ll_out, void *stream);
 int jk_batch_destroy(jk_batch *batch);
 
 /* A star table of `capacity` empty slots for stars with L linear parameters

@@ -405,3 +405,17 @@ def test_settle_reject_equals_settle_then_reject(setup, max_keep):
     assert np.array_equal(npg, npw) and np.array_equal(npg, np64)
     assert np.array_equal(mxg, mxw) and np.array_equal(sett, cnt)
     assert torch.equal(l1, l2) and torch.equal(b1, b2)
+
+
+def test_posterior_table_equals_per_star(setup):
+    """jk_linear_posterior_batch (one launch over a star table) gives the
+    per-star jk_linear_posterior results bit for bit, in any slot order."""
+    stars, cache, _ = setup
+    batch = B.StarBatch([d for _, d in stars])
+    g = np.random.default_rng(4)
+    pick = [3, 0, 17, 9]
+    rows_list = [np.sort(g.choice(len(cache), k, replace=False)) for k in (5, 64, 1, 200)]
+    want = D.posterior_inputs_many([stars[i][1] for i in pick], cache, rows_list)
+    got = D.posterior_inputs_table(batch, pick, cache, rows_list)
+    for (pw, aw, cw), (pg, ag, cg) in zip(want, got):
+        assert np.array_equal(pw, pg) and np.array_equal(aw, ag) and np.array_equal(cw, cg)

@@ -83,6 +83,8 @@ SIGNATURES = [
     ("jk_prior_fill", ctypes.c_int,
      [_i64, _i64, _u64, ctypes.c_double, ctypes.c_double, _vp, ctypes.c_int, ctypes.c_double,
       ctypes.c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("jk_linear_posterior_batch", ctypes.c_int,
+     [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("jk_linear_posterior", ctypes.c_int,
      [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
 ]

@@ -422,6 +422,23 @@ int jk_pcg64_uniform(uint64_t state_hi, uint64_t state_lo, uint64_t inc_hi, uint
                                        offset, n, u_out, (hipStream_t)stream));
 }
 
+int jk_linear_posterior_batch(const jk_batch *batch, int64_t n, const int32_t *star_idx,
+                              const double *P, const double *e, const double *omega,
+                              const double *M0, const double *s, const int64_t *rows,
+                              double *a_out, double *cov_out, double *ll_out, void *stream)
+{
+    if (!batch || n < 0) return JK_EINVAL;
+    if (n > 0 && (!star_idx || !rows || !P || !e || !omega || !M0 || !a_out || !cov_out ||
+                  !ll_out))
+        return JK_EINVAL;
+    if (batch->jitter_mode == JK_JITTER_PHYSICAL && n > 0 && !s) return JK_EINVAL;
+    if ((n + 63) / 64 > (int64_t)0x7fffffff) return JK_ERANGE;
+    if (current_device_inited() < 0) return JK_ENODEV;
+    return check_hip(jk_launch_linear_posterior_batch(batch, n, star_idx, P, e, omega, M0, s, rows,
+                                                      a_out, cov_out, ll_out,
+                                                      (hipStream_t)stream));
+}
+
 int jk_linear_posterior(const jk_star *star, int64_t n, const double *P, const double *e,
                         const double *omega, const double *M0, const double *s,
                         const int64_t *rows, double *a_out, double *cov_out, double *ll_out,

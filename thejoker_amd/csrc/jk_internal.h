@@ -70,6 +70,11 @@ hipError_t jk_launch_linear_posterior(const jk_star *st, int64_t n, const double
                                       const double *e, const double *om, const double *M0,
                                       const double *s, const int64_t *rows, double *a_out,
                                       double *cov_out, double *ll_out, hipStream_t stream);
+hipError_t jk_launch_linear_posterior_batch(const jk_batch *b, int64_t n, const int32_t *star_idx,
+                                            const double *P, const double *e, const double *om,
+                                            const double *M0, const double *s,
+                                            const int64_t *rows, double *a_out, double *cov_out,
+                                            double *ll_out, hipStream_t stream);
 hipError_t jk_upload_pcg_table(int device);
 hipError_t jk_upload_sincos_table(int device);
 int64_t jk_lnl_reject_scratch_bytes(int64_t n);

@@ -2181,20 +2181,18 @@ hipError_t jk_launch_prior_fill(int64_t n, int64_t row0, uint64_t seed, double P
 // lane per row, runtime L <= JK_LMAX.  Lambda[0] is not clipped here
 // (fast_likelihood.pyx:519-521).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_linear_posterior(const StarDev st, int64_t n,
-                                                         const double *__restrict__ P,
-                                                         const double *__restrict__ E_,
-                                                         const double *__restrict__ OM,
-                                                         const double *__restrict__ MM,
-                                                         const double *__restrict__ S,
-                                                         const int64_t *__restrict__ rows,
-                                                         double *__restrict__ a_out,
-                                                         double *__restrict__ cov_out,
-                                                         double *__restrict__ ll_out)
+// One accepted row of star st: a, cov and lnL into slot i of the outputs.
+// m_col: this lane's LDS column for a full-covariance star (else unused).
+__device__ __forceinline__ void jk_posterior_row(const StarDev &st, int64_t i, int64_t r,
+                                                 const double *__restrict__ P,
+                                                 const double *__restrict__ E_,
+                                                 const double *__restrict__ OM,
+                                                 const double *__restrict__ MM,
+                                                 const double *__restrict__ S,
+                                                 double *__restrict__ a_out,
+                                                 double *__restrict__ cov_out,
+                                                 double *__restrict__ ll_out, double *m_col)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int64_t r = rows ? rows[i] : i;
     const int L = st.L, N = st.n;
     const double Pv = P[r], ev = E_[r], omv = OM[r], M0v = MM[r];
     const double sv = st.jitter_mode == JK_JITTER_PHYSICAL ? S[r] : 0.0;
@@ -2220,9 +2218,7 @@ __global__ __launch_bounds__(64) void k_linear_posterior(const StarDev st, int64
     double chi0 = 0.0, sum_ln_w = 0.0;
     double x[JK_LMAX];
     // full covariance: the design column in epoch order, kept in this lane's
-    // LDS column, then rotated into the eigenbasis the star's data are in
-    extern __shared__ double s_m[];
-    double *m_col = s_m + threadIdx.x;
+    // LDS column (m_col), then rotated into the eigenbasis the star's data are in
     if (st.Qt) {
         for (int k = 0; k < N; ++k) {
             double sE, cE;
@@ -2311,6 +2307,58 @@ __global__ __launch_bounds__(64) void k_linear_posterior(const StarDev st, int64
         a_out[i * L + a] = v;
     }
     ll_out[i] = -0.5 * (chi0 - zz + (double)N * JK_LN_TWO_PI - sum_ln_w + lnlam + lndet);
+}
+
+__global__ __launch_bounds__(64) void k_linear_posterior(const StarDev st, int64_t n,
+                                                         const double *__restrict__ P,
+                                                         const double *__restrict__ E_,
+                                                         const double *__restrict__ OM,
+                                                         const double *__restrict__ MM,
+                                                         const double *__restrict__ S,
+                                                         const int64_t *__restrict__ rows,
+                                                         double *__restrict__ a_out,
+                                                         double *__restrict__ cov_out,
+                                                         double *__restrict__ ll_out)
+{
+    extern __shared__ double s_m[];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    jk_posterior_row(st, i, rows ? rows[i] : i, P, E_, OM, MM, S, a_out, cov_out, ll_out,
+                     s_m + threadIdx.x);
+}
+
+// Accepted rows of many stars of a table in one launch: row i belongs to
+// table entry star_idx[i] (diagonal-covariance stars only).
+__global__ __launch_bounds__(64) void k_linear_posterior_batch(const StarDev *__restrict__ stars,
+                                                               const int32_t *__restrict__ star_idx,
+                                                               int64_t n,
+                                                               const double *__restrict__ P,
+                                                               const double *__restrict__ E_,
+                                                               const double *__restrict__ OM,
+                                                               const double *__restrict__ MM,
+                                                               const double *__restrict__ S,
+                                                               const int64_t *__restrict__ rows,
+                                                               double *__restrict__ a_out,
+                                                               double *__restrict__ cov_out,
+                                                               double *__restrict__ ll_out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    jk_posterior_row(stars[star_idx[i]], i, rows[i], P, E_, OM, MM, S, a_out, cov_out, ll_out,
+                     nullptr);
+}
+
+hipError_t jk_launch_linear_posterior_batch(const jk_batch *b, int64_t n, const int32_t *star_idx,
+                                            const double *P, const double *e, const double *om,
+                                            const double *M0, const double *s,
+                                            const int64_t *rows, double *a_out, double *cov_out,
+                                            double *ll_out, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = (n + 63) / 64;
+    hipLaunchKernelGGL(k_linear_posterior_batch, dim3((unsigned)blocks), dim3(64), 0, stream,
+                       b->d_stars, star_idx, n, P, e, om, M0, s, rows, a_out, cov_out, ll_out);
+    return hipGetLastError();
 }
 
 hipError_t jk_launch_linear_posterior(const jk_star *st, int64_t n, const double *P,

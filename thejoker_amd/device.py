@@ -23,7 +23,7 @@ from .samples import NONLINEAR_INTERNAL_UNITS, NONLINEAR_PACKED_ORDER, JokerSamp
 
 __all__ = ["DeviceStar", "PriorCache", "star_arrays", "lnl", "lnl_reject", "lnl_reject_async",
            "PendingReject", "polish", "reject",
-           "posterior_inputs_many", "to_host",
+           "posterior_inputs_many", "posterior_inputs_table", "to_host",
            "linear_posterior", "uniforms",
            "pcg64_state", "pcg64_advance_state", "JITTER_MODES"]
 
@@ -530,6 +530,44 @@ def posterior_inputs_many(stars, cache, rows_list):
                     blk[k * L:k * (L + L * L)].reshape(k, L, L).copy()))
         off += n
         roff += k
+    return out
+
+
+def posterior_inputs_table(table, slots, cache, rows_list):
+    """:func:`posterior_inputs_many` for stars held in a device star table
+    (`table`, a batch.StarBatch; `slots` their entries): one
+    jk_linear_posterior_batch launch for every star's accepted rows, the
+    prior rows gathered into the same block, one copy back."""
+    torch = _torch()
+    lib = _lib.ensure_device(cache.device)
+    dev = cache.P.device
+    L = table.L
+    ks = [len(r) for r in rows_list]
+    K = int(sum(ks))
+    if K == 0:
+        return [(np.zeros((0, 5)), np.zeros((0, L)), np.zeros((0, L, L))) for _ in ks]
+    per = L + L * L + 1
+    res = torch.empty(K * per + 5 * K, dtype=torch.float64, device=dev)
+    d_rows = torch.as_tensor(np.concatenate([np.asarray(r, dtype=np.int64) for r in rows_list]),
+                             device=dev)
+    d_sidx = torch.as_tensor(np.repeat(np.asarray(slots, dtype=np.int32), ks), device=dev)
+    p0 = res.data_ptr()
+    _lib.check(lib.jk_linear_posterior_batch(
+        table.handle, K, _dptr(d_sidx), _dptr(cache.P), _dptr(cache.e), _dptr(cache.omega),
+        _dptr(cache.M0), _dptr(cache.s), _dptr(d_rows), ctypes.c_void_p(p0),
+        ctypes.c_void_p(p0 + 8 * K * L), ctypes.c_void_p(p0 + 8 * K * (L + L * L)),
+        _stream(cache.device)), "jk_linear_posterior_batch")
+    packed = res[K * per:].view(5, K)
+    for j, c in enumerate((cache.P, cache.e, cache.omega, cache.M0, cache.s)):
+        torch.index_select(c, 0, d_rows, out=packed[j])
+    h = to_host(res)
+    a = h[:K * L].reshape(K, L)
+    cov = h[K * L:K * (L + L * L)].reshape(K, L, L)
+    hp = h[K * per:].reshape(5, K)
+    out, o = [], 0
+    for k in ks:
+        out.append((hp[:, o:o + k].T.copy(), a[o:o + k].copy(), cov[o:o + k].copy()))
+        o += k
     return out
 
 

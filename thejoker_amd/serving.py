@@ -32,8 +32,8 @@ import numpy as np
 
 from .batch import LIVE_MARGIN, LIVE_MARGIN_NONZERO_U, LaunchShape, StarBatch, heaviest_first, \
     lnl_batch, reject_batch, settle_live_batch, upload_items, work_items
-from .device import (DeviceStar, _torch, linear_internal_units, pcg64_state, posterior_inputs_many,
-                     star_arrays)
+from .device import (DeviceStar, _torch, linear_internal_units, pcg64_state,
+                     posterior_inputs_table, star_arrays)
 from .likelihood_helpers import validate_prepare_data
 from .thejoker import IterativeRejection, TheJoker
 
@@ -512,8 +512,8 @@ class JokerServer:
             self._mark("reject")
         ok = [r for r in retired if r.error is None]
         if ok:
-            posts = posterior_inputs_many([r.star for r in ok], self.cache,
-                                          [r.it.result()[0] for r in ok])
+            posts = posterior_inputs_table(self._table, [r.slot for r in ok], self.cache,
+                                           [r.it.result()[0] for r in ok])
             for r, post in zip(ok, posts):
                 r.post = post
         for r in retired:
