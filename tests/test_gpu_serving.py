@@ -147,3 +147,31 @@ def test_zero_uniform_takes_the_exact_path(cache):
                                                  growth_factor=8)
         assert np.array_equal(r.result["P"].value, alone["P"].value), i
         assert np.array_equal(r.result["K"].value, alone["K"].value), i
+
+
+@pytest.mark.parametrize("in_memory", [True, False])
+def test_nan_rows_fail_like_the_single_star_path(in_memory):
+    """A prior cache with an invalid row (e = 1.5 gives NaN lnL): in memory
+    the reference raises 'NaN or Inf likelihood values' (likelihood_helpers.py:
+    173-176), otherwise the NaN maximum accepts nothing and 'Failed to find
+    any good samples!' follows (:187-188).  The server fails the request with
+    the error the single-star call raises, and the other requests finish."""
+    cols = syn.prior_columns(1 << 14, seed=3, jitter=True)
+    cols["e"][7] = 1.5
+    bad = D.PriorCache(cols["P"], cols["e"], cols["omega"], cols["M0"], cols["s"])
+    prior = _prior()
+    datas = _stars(2, seed=31)
+    srv = S.JokerServer(prior, bad, n_requested=8, growth_factor=8, in_memory=in_memory,
+                        shape=B.LaunchShape(128, 0, 1024))
+    for i, d in enumerate(datas):
+        srv.submit(d, np.random.default_rng(60 + i), rid=i)
+    while srv.busy():
+        srv.step()
+    got = sorted(srv.finished, key=lambda r: r.id)
+    for i, r in enumerate(got):
+        joker = tj.TheJoker(prior, rng=np.random.default_rng(60 + i))
+        with pytest.raises(RuntimeError) as exc:
+            joker.iterative_rejection_sample(datas[i], bad, n_requested_samples=8,
+                                             growth_factor=8, in_memory=in_memory)
+        assert isinstance(r.error, RuntimeError)
+        assert str(r.error) == str(exc.value)
