@@ -101,6 +101,10 @@ def mixed_vs_f64(torch, np, D, dstar, cache, llf, llm, bnd, seed, max_keep=None)
     st = D.pcg64_state(np.random.default_rng(seed))
     TheJoker._settle(dstar, cache, llm, bnd, st)
     res["rows_settled_in_fp64"] = int((bnd == 0).sum())
+    # what the product path hands on: the settled buffer's live-row error
+    live = llf >= llf.max() - WINDOW
+    res["max_abs_dlnl_live_settled"] = float((llf - llm).abs()[live].max())
+    del live
     a_f, n_f, _ = D.reject(llf, st, max_keep)
     a_m, _, _ = D.reject(llm, st, max_keep)
     res["accepted"] = int(min(n_f, max_keep))
@@ -109,7 +113,8 @@ def mixed_vs_f64(torch, np, D, dstar, cache, llf, llm, bnd, seed, max_keep=None)
 
 
 def fmt_mixed(r):
-    return (f"max|dlnL| all {r['max_abs_dlnl_all']:.2e} live {r['max_abs_dlnl_live']:.2e}, "
+    return (f"max|dlnL| all {r['max_abs_dlnl_all']:.2e} live {r['max_abs_dlnl_live']:.2e} "
+            f"(settled {r['max_abs_dlnl_live_settled']:.2e}), "
             f"max |dlnL|/bound {r['max_dlnl_over_bound']:.3f}, settled "
             f"{r['rows_settled_in_fp64']} rows, flips {r['accept_flips_vs_fp64']}")
 
@@ -299,6 +304,13 @@ def leg_batched(args, torch, np, D, syn):
     torch.cuda.synchronize()
     e2e_f_ms = (time.perf_counter() - t0) / 2 * 1e3
     flips = sum(len(np.setxor1d(x, y)) for x, y in zip(good_m, good_f))
+    # the live-row error of the settled buffer the mixed step handed on
+    dmax_live_settled = 0.0
+    for s in range(n_stars):
+        a = llf[off[s]:off[s] + n]
+        live = a >= a.max() - WINDOW
+        dmax_live_settled = max(dmax_live_settled,
+                                float((a - llm[off[s]:off[s] + n]).abs()[live].max()))
     # the same mixed step captured once in a HIP graph and replayed
     t0 = time.perf_counter()
     graph = B.StepGraph(batch, cache, d_items, len(items), total, off, sizes, states(), 256,
@@ -333,6 +345,7 @@ def leg_batched(args, torch, np, D, syn):
            "f64_end_to_end_ms": e2e_f_ms,
            "f64_end_to_end_sample_epochs_per_s": work / (e2e_f_ms * 1e-3),
            "max_abs_dlnl_all": dmax_all, "max_abs_dlnl_live": dmax_live,
+           "max_abs_dlnl_live_settled": dmax_live_settled,
            "max_dlnl_over_bound": ratio,
            "rows_settled_in_fp64": settled[0], "rows_settled_fraction": settled[0] / total,
            "accept_flips_vs_fp64": int(flips),
@@ -340,7 +353,8 @@ def leg_batched(args, torch, np, D, syn):
     log(f"[bench] batched: mixed {m_ms:.2f} ms ({res['mixed_sample_epochs_per_s']:.3e}/s), f64 "
         f"{f_ms:.2f} ms; end to end mixed {e2e_ms:.2f} ms (graph replay {graph_ms:.2f} ms), "
         f"f64 {e2e_f_ms:.2f} ms; |dlnL| all "
-        f"{dmax_all:.2e} live {dmax_live:.2e}, max |dlnL|/bound {ratio:.3f}, settled "
+        f"{dmax_all:.2e} live {dmax_live:.2e} (settled {dmax_live_settled:.2e}), max "
+        f"|dlnL|/bound {ratio:.3f}, settled "
         f"{settled[0] / total:.2e} of rows, flips {flips}")
     del llm, llf, cache, batch
     torch.cuda.empty_cache()

@@ -1745,13 +1745,15 @@ hipError_t jk_launch_polish_batch(const jk_batch *b, int n_seg, const SegDev *d_
 // |lnL(mixed) - lnL(FP64)| (0 for rows already FP64):
 //   1. lo = max(ll - B) is a lower bound on the FP64 maximum; a row with
 //      ll + B >= lo could be the maximum: re-evaluate it in FP64 (B := 0).
+//      So is a row with ll + B >= lo - 40 and B > 1e-3 (k_seg_shell_flags).
 //      A NaN anywhere makes lo NaN and every unsettled row is re-evaluated.
 //   2. mx = max(ll) is then the FP64 maximum.
 //   3. u = the row's numpy uniform (the stream the rejection step draws); a
 //      row whose decision exp(x - mx) > u differs between x = ll - B and
 //      x = ll + B is re-evaluated in FP64 (B := 0).
 // Afterwards exp(ll - mx) > u holds for exactly the rows for which it holds
-// with every row's FP64 likelihood.
+// with every row's FP64 likelihood, and every row within 40 of mx is within
+// 1e-3 of its FP64 lnL.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_seg_lo_partial(const double *__restrict__ ll,
                                                         const float *__restrict__ bound,
@@ -1772,6 +1774,13 @@ __global__ __launch_bounds__(256) void k_seg_lo_partial(const double *__restrict
             jk_nanmax(jk_nanmax(red[0], red[1]), jk_nanmax(red[2], red[3]));
 }
 
+// Rows the settling re-evaluates besides the shell of the maximum: those that
+// may lie within JK_LIVE_WINDOW of it (rows with a non-negligible chance of
+// acceptance, exp(-40) < 4.3e-18) and whose bound exceeds JK_LIVE_TOL, so
+// that after settling every such row is within JK_LIVE_TOL of its FP64 lnL.
+#define JK_LIVE_WINDOW 40.0
+#define JK_LIVE_TOL 1e-3f
+
 __global__ __launch_bounds__(256) void k_seg_shell_flags(const double *__restrict__ ll,
                                                          const float *__restrict__ bound,
                                                          const SegDev *__restrict__ segs,
@@ -1789,7 +1798,10 @@ __global__ __launch_bounds__(256) void k_seg_shell_flags(const double *__restric
         bool keep = false;
         if (i < sg.n) {
             const float b = bound[sg.off + i];
-            keep = b > 0.0f && !(ll[sg.off + i] + (double)b < l0);
+            const double hi = ll[sg.off + i] + (double)b;
+            // could be the maximum; or within JK_LIVE_WINDOW of it with a
+            // bound above JK_LIVE_TOL
+            keep = b > 0.0f && (!(hi < l0) || (b > JK_LIVE_TOL && !(hi < l0 - JK_LIVE_WINDOW)));
         }
         const uint64_t word = __ballot(keep);
         if (lane == 0) mask[sg.mask_off + (base >> 6)] = word;
