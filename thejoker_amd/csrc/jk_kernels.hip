@@ -348,7 +348,15 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
 #pragma unroll 2
     for (int k = 0; k < N; ++k) {
         const double dtk = st.dt[k];
-        const double x = jk_reduce_2pi(fma(nP, dtk, -M0v));
+        double x;
+        if constexpr (MIXED) {
+            // the FP32 Kepler step reads x rounded to FP32: the low part of
+            // 2 pi (k * 2.4e-16 rad) is far below that rounding
+            const double ph = fma(nP, dtk, -M0v);
+            x = fma(-rint(ph * JK_INV_TWO_PI), JK_TWO_PI_HI, ph);
+        } else {
+            x = jk_reduce_2pi(fma(nP, dtk, -M0v));
+        }
         double m;
         float denb = 0.0f;
         if constexpr (MIXED) {
