@@ -152,3 +152,48 @@ def test_default_prior_doctest_star(gpu):
     assert 0 < len(samples) < 1000
     for col in ("P", "e", "omega", "M0", "s", "K", "v0"):
         assert np.all(np.isfinite(samples[col].value))
+
+
+def test_prior_cache_from_hdf5(gpu, tmp_path):
+    """A prior cache file streams into the device columns block by block,
+    each column converted to the internal units: equal to the in-memory
+    route, and to the values the HDF5 library wrote into the golden file."""
+    import os
+
+    import torch
+
+    from thejoker_amd.device import PriorCache
+    samples, _ = _samples(5000, seed=3, jitter=True)
+    path = str(tmp_path / "prior.hdf5")
+    samples.write(path)
+    a = PriorCache.from_hdf5(path, rv_unit=u.km / u.s, block_rows=777)
+    b = PriorCache.from_samples(samples, rv_unit=u.km / u.s)
+    for x, y in zip((a.P, a.e, a.omega, a.M0, a.s), (b.P, b.e, b.omega, b.M0, b.s)):
+        assert torch.equal(x, y)
+    assert np.array_equal(a.ln_prior, b.ln_prior)
+
+    golden = os.path.join(os.path.dirname(__file__), "golden")
+    with np.load(os.path.join(golden, "hdf5_expected.npz")) as f:
+        ref = f["prior_default"]
+    c = PriorCache.from_hdf5(os.path.join(golden, "hdf5_prior_default.hdf5"), rv_unit=u.m / u.s,
+                             block_rows=128)
+    for name, col in zip(("P", "e", "omega", "M0"), (c.P, c.e, c.omega, c.M0)):
+        assert np.array_equal(col.cpu().numpy(), ref[name])
+    assert np.array_equal(c.s.cpu().numpy(), ref["s"] * 1000.0)
+    assert np.array_equal(c.ln_prior, ref["ln_prior"])
+
+
+def test_rejection_sample_from_hdf5_file(gpu, tmp_path):
+    """TheJoker given a prior cache filename (thejoker.py:129-134, 215-218)
+    returns what it returns for the same samples in memory."""
+    star = syn.single_star()
+    samples, _ = _samples(1 << 16, seed=5)
+    path = str(tmp_path / "prior.hdf5")
+    samples.write(path)
+    outs = []
+    for src in (path, samples):
+        joker = tj.TheJoker(_prior(), rng=np.random.default_rng(42))
+        outs.append(joker.rejection_sample(star.to_rvdata(), src, max_posterior_samples=256))
+    assert len(outs[0]) == len(outs[1]) > 0
+    for k in ("P", "e", "omega", "M0", "K", "v0"):
+        assert np.array_equal(outs[0][k].value, outs[1][k].value), k

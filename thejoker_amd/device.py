@@ -3,8 +3,9 @@
 * :class:`DeviceStar` -- the per-star constants of CJokerHelper.__init__
   (fast_likelihood.pyx:125-253) held on the GPU (a ``jk_star``).
 * :class:`PriorCache` -- the prior samples as device-resident
-  structure-of-arrays columns: an FP64 master (P, e, omega, M0, s) and an
-  FP32 copy of e, omega, M0, s for the mixed-precision kernel.  Replaces the
+  structure-of-arrays columns: an FP64 master (P, e, omega, M0, s) read by
+  both precisions, plus an optional FP32 copy of e, omega, M0, s
+  (``fp32_copy=True``) that no kernel reads.  Replaces the
   reference's HDF5 temp-file transport (utils.py:248-302,
   multiproc_helpers.py:63-120).
 * :func:`lnl`, :func:`reject`, :func:`linear_posterior` -- thin wrappers of
@@ -282,6 +283,62 @@ class PriorCache:
         packed, _ = samples.pack(units=units, names=NONLINEAR_PACKED_ORDER)
         lnp = np.asarray(samples["ln_prior"].value) if "ln_prior" in samples else None
         return cls.from_packed(packed, ln_prior=lnp, device=device, fp32_copy=fp32_copy)
+
+    @classmethod
+    def from_hdf5(cls, filename, rv_unit=u.km / u.s, path="samples", device=0, fp32_copy=False,
+                  block_rows=1 << 22):
+        """Stream the table of an HDF5 samples file -- the reference's prior
+        cache files (samples.py:480-545), read there in row batches with a
+        unit conversion per column (utils.py:168-228) -- into the device
+        columns, `block_rows` rows at a time, converting each column to the
+        internal units (P day, e 1, omega and M0 rad, s `rv_unit`).  A missing
+        s column is zero; ln_prior, when present, is kept on the host.  A file
+        without its YAML header is taken to hold internal units."""
+        import warnings
+
+        from . import hdf5
+        from .samples import read_hdf5_header
+
+        torch = _torch()
+        _lib.ensure_device(device)
+        dev = torch.device("cuda", device)
+        want = dict(NONLINEAR_INTERNAL_UNITS)
+        want["s"] = rv_unit
+        with hdf5.File(filename) as f:
+            header, units = read_hdf5_header(f, path)
+            if not units:
+                warnings.warn(f"{filename}: no table header; columns taken to be in internal "
+                              "units (day, rad, the data's RV unit)", RuntimeWarning)
+                units = dict(want)
+            ds = f[path]
+            names = ds.dtype.names or ()
+            missing = [k for k in ("P", "e", "omega", "M0") if k not in names]
+            if missing:
+                raise ValueError(f"{filename}: table '{path}' lacks the column(s) {missing}")
+            n = len(ds)
+            self = cls.__new__(cls)
+            self.device = device
+            self.P, self.e, self.omega, self.M0, self.s = [
+                torch.zeros(n, dtype=torch.float64, device=dev) for _ in range(5)]
+            cols = dict(zip(NONLINEAR_PACKED_ORDER, (self.P, self.e, self.omega, self.M0, self.s)))
+            scale = {k: u._as_unit(units.get(k, u.one)).to(want[k])
+                     for k in NONLINEAR_PACKED_ORDER if k in names}
+            lnp = [] if "ln_prior" in names else None
+            for lo in range(0, n, int(block_rows)):
+                hi = min(n, lo + int(block_rows))
+                block = ds.read(lo, hi)
+                for k, fac in scale.items():
+                    v = np.ascontiguousarray(block[k], dtype=np.float64)
+                    if fac != 1.0:
+                        v = v * fac
+                    cols[k][lo:hi].copy_(torch.from_numpy(v))
+                if lnp is not None:
+                    lnp.append(np.array(block["ln_prior"], dtype=np.float64))
+        self.ln_prior = None if lnp is None else (np.concatenate(lnp) if lnp else np.zeros(0))
+        self.fp32 = None
+        if fp32_copy:
+            self.fp32 = tuple(c.to(torch.float32) for c in (self.e, self.omega, self.M0, self.s))
+        return self
 
     @classmethod
     def generate(cls, n, seed=0, P_min=2.0, P_max=2048.0, e_ab=(0.867, 3.03), ln_s=None,

@@ -204,32 +204,79 @@ class JokerSamples:
 
     # -- I/O ---------------------------------------------------------------
     def write(self, output, overwrite=False, append=False):
-        """Write to a ``.npz`` file (columns + units + metadata).  HDF5 and
-        FITS (samples.py:480-545) need h5py/astropy, which are not
-        dependencies."""
+        """Write to an HDF5 (``.hdf5``/``.h5``) or ``.npz`` file
+        (samples.py:480-545).
+
+        HDF5 files have the reference's layout (samples_helpers.py:35-272):
+        the compound dataset ``samples`` plus ``samples.__table_column_meta__``
+        holding the astropy YAML header, so the reference's readers open
+        them.  As there, an existing file raises OSError unless `overwrite`
+        (replace it) or `append` (extend its table; the meta and the column
+        names and units must match, else ValueError).  FITS needs astropy
+        and is not supported."""
         ext = os.path.splitext(output)[1]
-        if ext != ".npz":
-            raise NotImplementedError("JokerSamples.write supports .npz files only")
-        if os.path.exists(output) and not overwrite and not append:
-            raise OSError(f"{output} exists (pass overwrite=True)")
+        if ext == ".fits":
+            raise NotImplementedError("FITS output needs astropy; write .hdf5, .h5 or .npz")
+        if ext not in (".hdf5", ".h5", ".npz"):
+            raise NotImplementedError("We currently only support writing to HDF5 files, with "
+                                      "extension .hdf5 or .h5, or .npz files.")
+        if os.path.exists(output) and not append:
+            if not overwrite:
+                raise OSError(f"File exists: {output}")
+            os.remove(output)
+        src = self
         if append and os.path.exists(output):
-            old = JokerSamples.read(output)
-            for k in self.par_names:
-                old.tbl[k] = u.Quantity(np.concatenate([old[k].value, self[k].to_value(old[k].unit)]),
-                                        old[k].unit)
-            src = old
-        else:
-            src = self
-        arrays = {f"col__{k}": np.asarray(v.value) for k, v in src.tbl.items()}
+            src = JokerSamples.read(output)._appended(self)
+        if ext == ".npz":
+            src._write_npz(output)
+            return
+        from . import hdf5, table_meta
+        cols = [(k, v.unit) for k, v in src.tbl.items()]
+        header = table_meta.make_header(cols, src.meta)
+        hdf5.write_table(output, [(k, np.asarray(v.value, dtype=np.float64).ravel())
+                                  for k, v in src.tbl.items()], header, name=self._hdf5_path)
+
+    def _appended(self, other):
+        """self followed by other's rows (samples_helpers.py:159-243 checks)."""
+        for key in set(self.meta) | set(other.meta):
+            a, b = self.meta.get(key), other.meta.get(key)
+            same = (a is None and b is None) or (a is not None and b is not None and (
+                a == b or (isinstance(a, float) and isinstance(b, float) and np.isnan(a)
+                           and np.isnan(b))))
+            if not same:
+                raise ValueError("Cannot append table to existing file because the existing file "
+                                 f"table metadata and this object's metadata do not match ({key}: "
+                                 f"{a!r} vs. {b!r})")
+        if list(self.tbl) != list(other.tbl) or any(
+                not self.tbl[k].unit.is_equivalent(other.tbl[k].unit) for k in self.tbl):
+            raise ValueError("Cannot append table to existing file because the existing file "
+                             "table datatype and this object's table datatype do not match")
+        out = self.copy()
+        for k in self.tbl:
+            out.tbl[k] = u.Quantity(np.concatenate([self.tbl[k].value,
+                                                    other.tbl[k].to_value(self.tbl[k].unit)]),
+                                    self.tbl[k].unit)
+        return out
+
+    def _write_npz(self, output):
+        arrays = {f"col__{k}": np.asarray(v.value) for k, v in self.tbl.items()}
         unit_rec = np.array([[k, repr(v.unit.scale), ",".join(map(str, v.unit.dims)),
-                              v.unit.name or ""] for k, v in src.tbl.items()], dtype=object)
-        meta = {k: (np.nan if v is None else v) for k, v in src.meta.items()}
+                              v.unit.name or ""] for k, v in self.tbl.items()], dtype=object)
+        meta = {k: (np.nan if v is None else v) for k, v in self.meta.items()}
         np.savez(output, **arrays, __units__=unit_rec.astype(str),
                  __meta_keys__=np.array(list(meta.keys()), dtype=str),
                  __meta_vals__=np.array([float(v) for v in meta.values()]))
 
     @classmethod
     def read(cls, filename, path=None):
+        """Read samples written by :meth:`write` or by the reference
+        (samples.py:564-601): HDF5 (``.hdf5``/``.h5``, the table at `path`,
+        default ``samples``) or ``.npz``."""
+        ext = os.path.splitext(filename)[1]
+        if ext in (".hdf5", ".h5"):
+            return cls._read_hdf5(filename, path or cls._hdf5_path)
+        if ext != ".npz":
+            raise NotImplementedError(f"cannot read samples from '{ext}' files")
         with np.load(filename, allow_pickle=False) as f:
             units = {row[0]: u.Unit(float(row[1]), [int(x) for x in row[2].split(",")],
                                     row[3] or None) for row in f["__units__"]}
@@ -241,3 +288,37 @@ class JokerSamples:
             for k in units:
                 out.tbl[k] = u.Quantity(f[f"col__{k}"], units[k])
         return out
+
+    @classmethod
+    def _read_hdf5(cls, filename, path):
+        from . import hdf5
+        with hdf5.File(filename) as f:
+            header, units = read_hdf5_header(f, path)
+            data = f[path].read()
+        meta = table_meta_of(header)
+        out = cls(**meta)
+        for name in data.dtype.names:
+            out[name] = u.Quantity(np.ascontiguousarray(data[name], dtype=np.float64),
+                                   units.get(name, u.one))
+        return out
+
+
+def read_hdf5_header(f, path="samples"):
+    """(header dict, {column: Unit}) of the table at `path` of an open
+    :class:`thejoker_amd.hdf5.File` (utils.py:75-87).  A table written
+    without its header has dimensionless columns and an empty meta."""
+    from . import table_meta
+    mpath = path + ".__table_column_meta__"
+    if mpath not in f:
+        return {"meta": {}}, {}
+    header = table_meta.read_header(f[mpath].read())
+    return header, table_meta.column_units(header)
+
+
+def table_meta_of(header):
+    from . import table_meta
+    meta = table_meta.table_meta(header)
+    for key in ("poly_trend", "n_offsets"):
+        if meta.get(key) is not None:
+            meta[key] = int(meta[key])
+    return meta

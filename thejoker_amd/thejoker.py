@@ -117,11 +117,15 @@ class TheJoker:
         return star
 
     def _cache(self, star, prior_samples, return_logprobs=False):
-        """Prior samples (JokerSamples, PriorCache or .npz filename) as a
-        device-resident PriorCache."""
+        """Prior samples (JokerSamples, PriorCache, or an HDF5 or .npz
+        filename) as a device-resident PriorCache.  An HDF5 file streams into
+        the device columns without a host copy of the whole table."""
         if isinstance(prior_samples, PriorCache):
             return prior_samples
         if isinstance(prior_samples, str):
+            if os.path.splitext(prior_samples)[1] in (".hdf5", ".h5"):
+                return PriorCache.from_hdf5(prior_samples, rv_unit=star.data.rv.unit,
+                                            device=self.device)
             prior_samples = JokerSamples.read(prior_samples)
         if isinstance(prior_samples, JokerSamples):
             return PriorCache.from_samples(prior_samples, rv_unit=star.data.rv.unit,

@@ -13,11 +13,13 @@ If astropy is importable, astropy Quantities are accepted wherever a
 Quantity is (they are converted through their SI value).
 """
 import numbers
+import re
 
 import numpy as np
 
 __all__ = ["Unit", "Quantity", "as_quantity", "to_value", "day", "d", "yr", "year", "s", "m",
-           "km", "rad", "radian", "deg", "one", "dimensionless_unscaled", "au", "hour", "minute"]
+           "km", "rad", "radian", "deg", "one", "dimensionless_unscaled", "au", "hour", "minute",
+           "parse_unit", "UnitsError"]
 
 
 class Unit:
@@ -97,7 +99,7 @@ class Unit:
         return f"Unit({self})"
 
     def __str__(self):
-        return self.name if self.name else f"{self.scale:g} m^{self.dims[0]} s^{self.dims[1]} rad^{self.dims[2]}"
+        return self.name if self.name is not None else f"{self.scale:g} m^{self.dims[0]} s^{self.dims[1]} rad^{self.dims[2]}"
 
 
 class UnitsError(ValueError):
@@ -203,7 +205,7 @@ def _as_unit(u):
         key = u.strip()
         if key in _NAMED:
             return _NAMED[key]
-        raise UnitsError(f"unknown unit '{u}'")
+        return parse_unit(key)
     # astropy unit (duck-typed): go through SI.
     si = getattr(u, "si", None)
     if si is not None and hasattr(si, "scale") and hasattr(si, "bases"):
@@ -253,6 +255,102 @@ deg = Unit(np.pi / 180.0, (0, 0, 1), "deg")
 
 _NAMED = {
     "": one, "m": m, "km": km, "AU": au, "s": s, "min": minute, "h": hour, "d": day,
-    "day": day, "yr": yr, "year": yr, "rad": rad, "deg": deg,
+    "day": day, "yr": yr, "year": yr, "rad": rad, "deg": deg, "au": au, "dimensionless": one,
     "km / s": km / s, "m / s": m / s,
 }
+
+_UNIT_TOKEN = re.compile(r"\s*(?:(\*\*|\^|\*|\.|/|\(|\))|([A-Za-z_]+)([-+]?\d+)?|"
+                         r"([-+]?\d+(?:\.\d*)?(?:[eE][-+]?\d+)?))")
+
+
+def parse_unit(text):
+    """Unit from a string in astropy's generic format, as astropy writes them
+    into table headers: ``'km / s'``, ``'km / (d s)'``, ``'m / (d2 s)'``,
+    ``'s-1'``, ``'m s**-2'``.  Products are juxtaposition, ``*`` or ``.``;
+    ``/`` divides by the next factor; a power follows a name directly
+    (``d2``) or after ``**`` / ``^``."""
+    text = str(text).strip()
+    if text in _NAMED:
+        return _NAMED[text]
+    toks, pos = [], 0
+    while pos < len(text):
+        mt = _UNIT_TOKEN.match(text, pos)
+        if mt is None or mt.end() == pos:
+            if text[pos:].strip() == "":
+                break
+            raise UnitsError(f"cannot parse unit '{text}'")
+        toks.append(mt.groups())
+        pos = mt.end()
+    i = 0
+
+    def peek():
+        return toks[i] if i < len(toks) else (None, None, None, None)
+
+    def number():
+        nonlocal i
+        op, name, pw, num = peek()
+        if op == "(":
+            i += 1
+            val = number()
+            if peek()[0] != ")":
+                raise UnitsError(f"cannot parse unit '{text}'")
+            i += 1
+            return val
+        if num is None:
+            raise UnitsError(f"cannot parse unit '{text}'")
+        i += 1
+        return float(num)
+
+    def factor():
+        nonlocal i
+        op, name, pw, num = peek()
+        if op == "(":
+            i += 1
+            val = product()
+            if peek()[0] != ")":
+                raise UnitsError(f"cannot parse unit '{text}'")
+            i += 1
+        elif name is not None:
+            if name not in _NAMED:
+                raise UnitsError(f"unknown unit '{name}' in '{text}'")
+            i += 1
+            val = _NAMED[name]
+            if pw is not None:
+                val = val ** _int_power(float(pw))
+        elif num is not None:
+            i += 1
+            val = Unit(float(num), (0, 0, 0), num)
+        else:
+            raise UnitsError(f"cannot parse unit '{text}'")
+        if peek()[0] in ("**", "^"):
+            i += 1
+            val = val ** _int_power(number())
+        return val
+
+    def product():
+        nonlocal i
+        val = factor()
+        while i < len(toks):
+            op = peek()[0]
+            if op == ")":
+                break
+            if op == "/":
+                i += 1
+                val = val / factor()
+            elif op in ("*", "."):
+                i += 1
+                val = val * factor()
+            else:
+                val = val * factor()
+        return val
+
+    val = product()
+    if i != len(toks):
+        raise UnitsError(f"cannot parse unit '{text}'")
+    return Unit(val.scale, val.dims, text)
+
+
+def _int_power(p):
+    if p != int(p):
+        raise UnitsError(f"fractional unit power {p} is not supported")
+    return int(p)
