@@ -274,6 +274,33 @@ int jk_lnl_batch(const jk_batch *batch, int precision, int64_t n_items,
                  const void *M0, const void *s, const int64_t *rows, double *ll_out,
                  float *bound_out, int block, int grid, void *workspace, void *stream);
 
+/* jk_lnl_batch with pruning by an upper bound.  No reference counterpart:
+ * it returns the same rejection decisions as evaluating every row
+ * (likelihood_helpers.py:107-109) with less arithmetic.
+ * `probes` is a table with the same L and jitter mode whose entry k holds
+ * star k's data on a subset S of its epochs (jk_star_create on those
+ * epochs).  Because the rest's conditional density is at most that of its
+ * noise alone,
+ *   lnL(row) <= lnL_S(row) - 1/2 sum_{n not in S} ln(2 pi sigma_n^2).
+ * prune holds two doubles per table entry k: (thr_k, add_k).  For a row of
+ * star k whose bound lnL_S + add_k (plus, for precision 1, lnL_S's error
+ * bound) is finite and below thr_k, ll_out receives the bound (bound_out 0)
+ * instead of the row's lnL.  Every other row gets exactly what jk_lnl_batch
+ * writes.  thr_k = -inf evaluates all of star k's rows.
+ * A caller that sets thr_k = (a value known not to exceed the final maximum)
+ * minus a margin of 746 or more leaves every decision
+ * exp(ll - max) > u unchanged for all u in [0, 1).
+ * With a smaller margin (at least 37), only u == 0 can differ (see
+ * jk_reject_batch_zero_check).
+ * n_full_out (optional, device uint64) accumulates the number of rows
+ * evaluated in full.  Rows are read from the cache directly: work items
+ * address cache rows row_begin .. row_begin + n_rows. */
+int jk_lnl_prune_batch(const jk_batch *batch, const jk_batch *probes, const double *prune,
+                       int precision, int64_t n_items, const jk_work_item *items, const double *P,
+                       const void *e, const void *omega, const void *M0, const void *s,
+                       double *ll_out, float *bound_out, uint64_t *n_full_out, int block,
+                       int grid, void *workspace, void *stream);
+
 /* Segmented rejection step (jk_reject per segment, one launch sequence for
  * all): segs is a HOST array of n_seg descriptors; n_pass_out[i] and
  * llmax_out[i] (device) receive segment i's pass count and maximum. */

@@ -1,0 +1,185 @@
+"""Pruning by an upper bound (probe_star, jk_lnl_prune_batch): the probe's
+bound lies above the star's lnL on every row; rows that are not pruned get
+exactly jk_lnl_batch's values; pruned rows hold a bound below their
+threshold; the server's results do not change."""
+import numpy as np
+import pytest
+import torch
+
+import thejoker_amd as tj
+from thejoker_amd import _lib
+from thejoker_amd import batch as B
+from thejoker_amd import device as D
+from thejoker_amd import serving as S
+from thejoker_amd import synthetic as syn
+from thejoker_amd import units as u
+
+pytestmark = pytest.mark.gpu
+
+
+def _arrays(st, L, fcm=False):
+    lam = np.array([900.0, 1e4, 0.25, 1e-4][:L])
+    return dict(t=st.t, rv=st.rv, ivar=st.ivar(), trend=st.trend(poly_trend=L - 1),
+                mu=np.zeros(L), lam=lam, t_ref=st.t_ref, cov=None,
+                k_prior_mode=_lib.JK_KPRIOR_FIXED_COMPANION_MASS if fcm else _lib.JK_KPRIOR_NORMAL,
+                sigma_K0=30.0, P0=365.25, max_K=300.0)
+
+
+def _star(a, mode):
+    return D.DeviceStar(a["t"], a["rv"], a["ivar"], a["trend"], a["mu"], a["lam"], a["t_ref"],
+                        a["k_prior_mode"], a["sigma_K0"], a["P0"], a["max_K"], jitter_mode=mode)
+
+
+@pytest.fixture(scope="module")
+def setup(gpu):
+    stars = syn.many_stars(12, seed=31, M_range=(12, 64))
+    cols = syn.prior_columns(1 << 15, seed=33, jitter=True)
+    cache = D.PriorCache(cols["P"], cols["e"], cols["omega"], cols["M0"], cols["s"])
+    return stars, cache
+
+
+@pytest.mark.parametrize("mode,L,fcm", [("physical", 3, False), ("reference", 3, False),
+                                        ("physical", 2, False), ("physical", 2, True)])
+def test_probe_bounds_the_star(setup, mode, L, fcm):
+    stars, cache = setup
+    for st in stars[:6]:
+        a = _arrays(st, L, fcm)
+        full = D.lnl(_star(a, mode), cache, precision="f64").cpu().numpy()
+        probe, add = D.probe_star(a, mode)
+        assert probe.n == len(D.probe_epochs(st.M)) < st.M
+        ub = D.lnl(probe, cache, precision="f64").cpu().numpy() + add
+        ok = np.isfinite(full)
+        assert ok.mean() > 0.99
+        assert (ub[ok] >= full[ok]).all(), (ub[ok] - full[ok]).min()
+
+
+def _tables(stars, mode="physical", L=3):
+    arrs = [_arrays(st, L) for st in stars]
+    full = [_star(a, mode) for a in arrs]
+    probes = [D.probe_star(a, mode) for a in arrs]
+    return B.StarBatch(full), B.StarBatch([p for p, _ in probes]), [add for _, add in probes]
+
+
+@pytest.mark.parametrize("precision", ["f64", "mixed"])
+def test_prune_launch(setup, precision):
+    stars, cache = setup
+    table, probes, adds = _tables(stars)
+    n = len(cache)
+    items, off, total = B.work_items([n] * len(stars), 1000)
+    items = items[np.random.default_rng(1).permutation(len(items))]
+    d_items = B.upload_items(items)
+    mixed = precision == "mixed"
+    b_ref = torch.empty(total, dtype=torch.float32, device="cuda") if mixed else None
+    ref = B.lnl_batch(table, cache, d_items, len(items), total, precision,
+                      bound=b_ref).cpu().numpy()
+    ref64 = B.lnl_batch(table, cache, d_items, len(items), total, "f64").cpu().numpy()
+    thr = np.zeros((len(stars), 2))
+    for k in range(len(stars)):
+        seg = ref64[off[k]:off[k] + n]
+        thr[k] = (-np.inf if k % 3 == 2 else np.nanmax(seg) - 40.0, adds[k])
+    prune = torch.as_tensor(thr.reshape(-1), device="cuda")
+    out = torch.full((total,), np.nan, dtype=torch.float64, device="cuda")
+    bnd = torch.full((total,), -1.0, dtype=torch.float32, device="cuda") if mixed else None
+    n_full = torch.zeros(1, dtype=torch.int64, device="cuda")
+    B.lnl_prune_batch(table, probes, prune, cache, d_items, len(items), precision, out=out,
+                      bound=bnd, n_full=n_full)
+    out = out.cpu().numpy()
+    same = (out == ref) | (np.isnan(out) & np.isnan(ref))
+    pruned = ~same
+    if mixed:
+        bnd = bnd.cpu().numpy()
+        assert np.array_equal(bnd[same], b_ref.cpu().numpy()[same])
+        assert (bnd[pruned] == 0.0).all()
+    assert int(n_full.item()) == int(same.sum())
+    n_pruned = 0
+    for k in range(len(stars)):
+        sl = slice(off[k], off[k] + n)
+        p = pruned[sl]
+        if k % 3 == 2:
+            assert not p.any()
+            continue
+        n_pruned += int(p.sum())
+        assert (out[sl][p] < thr[k, 0]).all()
+        assert (out[sl][p] >= ref64[sl][p]).all()
+        # every row within the margin of the maximum is evaluated in full
+        assert not p[ref64[sl] >= thr[k, 0]].any()
+    assert n_pruned > len(cache)
+
+
+def test_prune_argument_checks(setup):
+    stars, cache = setup
+    table, probes, adds = _tables(stars[:3])
+    other, _, _ = _tables(stars[:3], L=2)
+    items, off, total = B.work_items([64] * 3, 64)
+    d_items = B.upload_items(items)
+    prune = torch.zeros(6, dtype=torch.float64, device="cuda")
+    out = torch.empty(total, dtype=torch.float64, device="cuda")
+    with pytest.raises(_lib.JokerLibError):
+        B.lnl_prune_batch(table, other, prune, cache, d_items, len(items), "f64", out=out)
+    with pytest.raises(ValueError):
+        B.lnl_prune_batch(table, probes, prune, cache, d_items, len(items), "mixed", out=out)
+    with pytest.raises(ValueError):
+        B.lnl_prune_batch(table, probes, prune[:4], cache, d_items, len(items), "f64", out=out)
+
+
+def _prior():
+    return tj.JokerPrior.default(
+        P_min=2 * u.day, P_max=2048 * u.day,
+        sigma_v=[100 * u.km / u.s, 0.5 * u.km / u.s / u.day], poly_trend=2,
+        pars={"K": tj.distributions.Normal(0.0, 30.0, unit=u.km / u.s),
+              "s": tj.distributions.LogNormal(-3.0, 1.0, unit=u.km / u.s)})
+
+
+@pytest.mark.parametrize("precision", ["mixed", "f64"])
+def test_server_prune_changes_no_result(setup, precision):
+    """The same requests with and without pruning: identical results, and
+    the pruned run evaluates fewer rows in full."""
+    _, cache = setup
+    prior = _prior()
+    _, stars = syn.arrivals(10, seed=37, M_range=(12, 120))
+    datas = [s.to_rvdata() for s in stars]
+    res = {}
+    for prune in (False, True):
+        srv = S.JokerServer(prior, cache, n_requested=16, precision=precision, max_active=4,
+                            growth_factor=8, shape=B.LaunchShape(128, 0, 1024), prune=prune)
+        for i, d in enumerate(datas):
+            srv.submit(d, np.random.default_rng(500 + i), rid=i)
+        while srv.busy():
+            srv.step()
+        res[prune] = sorted(srv.finished, key=lambda r: r.id)
+        if prune:
+            probed, full = srv.prune_stats()
+            assert probed > 0 and full < probed
+    for a, b in zip(res[False], res[True]):
+        assert a.error is None and b.error is None
+        for k in ("P", "e", "K", "v0", "v1", "ln_likelihood"):
+            if k in a.result:
+                assert np.array_equal(a.result[k].value, b.result[k].value), (a.id, k)
+
+
+def test_server_prune_zero_uniform_in_a_later_round(setup):
+    """A uniform of exactly 0 in the second round, at a row that the probe
+    pruned: the zero-uniform path evaluates it in full, and the result equals
+    the FP64 run of the star alone."""
+    from tests.test_serving_host import zero_uniform_rng
+    _, cache = setup
+    prior = _prior()
+    _, stars = syn.arrivals(1, seed=41, M_range=(60, 60))
+    data = stars[0].to_rvdata()
+    # round 1 draws 128 uniforms; round 2 redraws one per row evaluated so far
+    k = 128 + 128 + 700
+    srv = S.JokerServer(prior, cache, n_requested=16, precision="mixed", max_active=1,
+                        growth_factor=8, shape=B.LaunchShape(128, 0, 1024))
+    srv.submit(data, zero_uniform_rng(k), rid=0)
+    while srv.busy():
+        srv.step()
+    r = srv.finished[0]
+    assert r.error is None, r.error
+    assert r.it.iteration >= 2 and r.rows_evaluated > 128 + 700
+    assert srv.zero_uniform_events >= 1
+    probed, full = srv.prune_stats()
+    assert full < probed
+    joker = tj.TheJoker(prior, rng=zero_uniform_rng(k), precision="f64")
+    alone = joker.iterative_rejection_sample(data, cache, n_requested_samples=16, growth_factor=8)
+    assert np.array_equal(r.result["P"].value, alone["P"].value)
+    assert np.array_equal(r.result["K"].value, alone["K"].value)

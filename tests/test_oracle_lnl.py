@@ -121,3 +121,51 @@ def test_threads_match_serial():
     a = oracle.batch_lnl(ost, rows, n_threads=1)
     b = oracle.batch_lnl(ost, rows, n_threads=5)
     assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("jitter_mode", [0, 1])
+def test_subset_upper_bound(jitter_mode):
+    """The bound the probe stars of the serving loop rest on: the marginal
+    likelihood of a star on a subset S of its epochs, plus
+    -1/2 sum over the other epochs of ln(2 pi sigma_n^2), is at least the
+    whole star's, for every row (the other epochs' conditional density
+    given y_S is at most that of their noise).  Checked with the oracle's
+    reference-form lnL on binaries and a constant star, jitter included."""
+    from thejoker_amd import synthetic as syn
+    from thejoker_amd.device import probe_epochs
+
+    cols = syn.prior_columns(1 << 12, seed=51, jitter=True)
+    rows = oracle.pack_rows(cols["P"], cols["e"], cols["omega"], cols["M0"], cols["s"])
+    mu, lam = np.zeros(3), np.array([900.0, 1e4, 0.25])
+    stars = syn.many_stars(6, seed=53, M_range=(12, 40))
+    fractions = []
+    for st in stars:
+        def ostar(idx):
+            dt = st.t[idx] - st.t_ref
+            return oracle.OracleStar(st.t[idx], st.rv[idx], st.ivar()[idx],
+                                     np.stack([np.ones_like(dt), dt], 1), mu, lam, st.t_ref,
+                                     jitter_mode=jitter_mode)
+        full, capped = oracle.batch_lnl(ostar(np.arange(st.M)), rows, n_threads=8,
+                                        return_capped=True)
+        idx = probe_epochs(st.M)
+        rest = np.setdiff1d(np.arange(st.M), idx)
+        ub = oracle.batch_lnl(ostar(idx), rows, n_threads=8)
+        ub = ub - 0.5 * np.sum(np.log(2 * np.pi / st.ivar()[rest]))
+        ok = np.isfinite(full) & ~capped
+        assert ok.mean() > 0.99
+        assert (ub[ok] >= full[ok] - 1e-9 * np.abs(full[ok])).all()
+        fractions.append(np.mean(ub[ok] < full[ok].max() - 40.0))
+    # the bound is not vacuous: on some binary most rows fall 40 below the
+    # maximum by the bound alone (with jitter; without it, 2^12 rows leave
+    # these binaries' maxima far below most rows' bounds)
+    if jitter_mode == 1:
+        assert max(fractions[0::2]) > 0.5, fractions
+
+
+def test_probe_epochs_span_the_baseline():
+    from thejoker_amd.device import probe_epochs
+    for n in (12, 13, 40, 97, 200):
+        idx = probe_epochs(n)
+        assert idx[0] == 0 and idx[-1] == n - 1
+        assert len(idx) == min(n, max(6, round(n / 6))) and np.all(np.diff(idx) > 0)
+    assert np.array_equal(probe_epochs(5), np.arange(5))

@@ -10,7 +10,7 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libjoker.so")
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 JK_OK = 0
 JK_JITTER_REFERENCE = 0
@@ -61,6 +61,9 @@ SIGNATURES = [
     ("jk_lnl_batch", ctypes.c_int,
      [_vp, ctypes.c_int, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int,
       ctypes.c_int, _vp, _vp]),
+    ("jk_lnl_prune_batch", ctypes.c_int,
+     [_vp, _vp, _vp, ctypes.c_int, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+      ctypes.c_int, ctypes.c_int, _vp, _vp]),
     ("jk_reject_batch", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]),
     ("jk_reject_batch_zero_check", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("jk_polish_batch", ctypes.c_int,

@@ -19,7 +19,8 @@ from .device import (DeviceStar, _dptr, _split, _stream, _torch, pcg64_advance_s
                      to_host)
 
 __all__ = ["WORK_ITEM_DTYPE", "SEGMENT_DTYPE", "POLISH_SEGMENT_DTYPE", "SETTLE_SEGMENT_DTYPE",
-           "LIVE_SEGMENT_DTYPE", "StarBatch", "work_items", "lnl_batch", "reject_batch",
+           "LIVE_SEGMENT_DTYPE", "StarBatch", "work_items", "lnl_batch", "lnl_prune_batch",
+           "reject_batch",
            "polish_batch", "settle_batch", "settle_reject_batch", "settle_live_batch",
            "LIVE_MARGIN",
            "LIVE_MARGIN_NONZERO_U",
@@ -174,6 +175,32 @@ def lnl_batch(batch, cache, d_items, n_items, n_out, precision="mixed", shape=No
                                 *[_dptr(c) for c in cols], _dptr(rows), _dptr(out), _dptr(bound),
                                 shape.block, shape.grid, _dptr(workspace),
                                 _stream(cache.device)), "jk_lnl_batch")
+    return out
+
+
+def lnl_prune_batch(batch, probes, prune, cache, d_items, n_items, precision="mixed", shape=None,
+                    out=None, bound=None, workspace=None, n_full=None):
+    """lnl_batch with pruning (jk_lnl_prune_batch): `probes` is a table whose
+    entry k is star k's probe star (probe_star), `prune` a device float64
+    tensor of (threshold, add) per table entry.  Rows whose probe bound lies
+    below their star's threshold get that bound instead of their lnL (and
+    bound 0).  n_full: optional device int64 counter of the rows evaluated
+    in full (accumulated)."""
+    lib = _lib.ensure_device(cache.device)
+    shape = shape or LaunchShape()
+    if precision not in ("f64", "mixed"):
+        raise ValueError("precision must be 'f64' or 'mixed'")
+    if (precision == "mixed") != (bound is not None):
+        raise ValueError("mixed precision writes a bound; f64 does not")
+    if out is None or prune.numel() < 2 * len(batch) or prune.dtype != _torch().float64:
+        raise ValueError("out is required and prune needs two float64 entries per table slot")
+    cols = (cache.e, cache.omega, cache.M0, cache.s)
+    _lib.check(lib.jk_lnl_prune_batch(batch.handle, probes.handle, _dptr(prune),
+                                      1 if precision == "mixed" else 0, int(n_items),
+                                      _dptr(d_items), _dptr(cache.P), *[_dptr(c) for c in cols],
+                                      _dptr(out), _dptr(bound), _dptr(n_full), shape.block,
+                                      shape.grid, _dptr(workspace), _stream(cache.device)),
+               "jk_lnl_prune_batch")
     return out
 
 

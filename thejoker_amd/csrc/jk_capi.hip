@@ -11,7 +11,7 @@
 
 #include "jk_internal.h"
 
-#define JK_ABI_VERSION 4
+#define JK_ABI_VERSION 5
 #define JK_POOL_KEEP (32ull << 30)
 
 namespace {
@@ -619,6 +619,42 @@ int jk_lnl_batch(const jk_batch *batch, int precision, int64_t n_items, const jk
     if (!ws && hipMallocAsync(&ws, 256, st) != hipSuccess) return JK_ENOMEM;
     hipError_t err = jk_launch_lnl_batch(batch, precision, n_items, items, P, e, omega, M0, s, rows,
                                          ll_out, bound_out, block, grid, (unsigned *)ws, st);
+    hipError_t err2 = workspace ? hipSuccess : hipFreeAsync(ws, st);
+    if (err != hipSuccess) return JK_EHIP;
+    return check_hip(err2);
+}
+
+int jk_lnl_prune_batch(const jk_batch *batch, const jk_batch *probes, const double *prune,
+                       int precision, int64_t n_items, const jk_work_item *items, const double *P,
+                       const void *e, const void *omega, const void *M0, const void *s,
+                       double *ll_out, float *bound_out, uint64_t *n_full_out, int block,
+                       int grid, void *workspace, void *stream)
+{
+    if (!batch || !probes || n_items < 0 || (precision != 0 && precision != 1)) return JK_EINVAL;
+    if ((precision == 1) != (bound_out != nullptr)) return JK_EINVAL;
+    if (probes->L != batch->L || probes->jitter_mode != batch->jitter_mode ||
+        probes->n_stars < batch->n_stars || probes->device != batch->device)
+        return JK_EINVAL;
+    if (n_items == 0) return JK_OK;
+    if (!prune || !items || !P || !e || !omega || !M0 || !ll_out) return JK_EINVAL;
+    if (batch->jitter_mode == JK_JITTER_PHYSICAL && !s) return JK_EINVAL;
+    if (block != 64 && block != 128 && block != 256) return JK_EINVAL;
+    if (n_items > (int64_t)0xFFFFFFF0u) return JK_ERANGE;
+    const int dev = current_device_inited();
+    if (dev < 0) return JK_ENODEV;
+    if (grid <= 0) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return JK_EHIP;
+        grid = 8 * cus;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    void *ws = workspace;
+    if (!ws && hipMallocAsync(&ws, 256, st) != hipSuccess) return JK_ENOMEM;
+    hipError_t err = jk_launch_lnl_prune_batch(batch, probes, prune, precision, n_items, items, P,
+                                               e, omega, M0, s, ll_out, bound_out,
+                                               (unsigned long long *)n_full_out, block, grid,
+                                               (unsigned *)ws, st);
     hipError_t err2 = workspace ? hipSuccess : hipFreeAsync(ws, st);
     if (err != hipSuccess) return JK_EHIP;
     return check_hip(err2);

@@ -90,6 +90,12 @@ hipError_t jk_launch_lnl_batch(const jk_batch *b, int precision, int64_t n_items
                                const void *om, const void *M0, const void *s,
                                const int64_t *rows, double *ll, float *bound, int block, int grid,
                                unsigned *counter, hipStream_t stream);
+hipError_t jk_launch_lnl_prune_batch(const jk_batch *b, const jk_batch *pb, const double *prune,
+                                     int precision, int64_t n_items, const jk_work_item *items,
+                                     const double *P, const void *e, const void *om,
+                                     const void *M0, const void *s, double *ll, float *bound,
+                                     unsigned long long *n_full, int block, int grid,
+                                     unsigned *counter, hipStream_t stream);
 hipError_t jk_launch_reject_batch(const double *ll, int n_seg, const SegDev *d_segs,
                                   int64_t max_n, int64_t max_nb, double *partials, int pb,
                                   uint64_t *mask, int64_t *bsum, int64_t *idx_out,

@@ -1403,6 +1403,169 @@ hipError_t jk_launch_lnl_batch(const jk_batch *b, int precision, int64_t n_items
 }
 
 // ---------------------------------------------------------------------------
+// Many stars per launch with pruning by an upper bound.  probes[k] holds star
+// k's data on a subset S of its epochs.  The marginal likelihood factors as
+// p(y) = p(y_S) p(y_rest | y_S), and the conditional covariance of y_rest is
+// at least the noise covariance C_rest, so
+//   lnL <= lnL_S - 1/2 sum_{n not in S} ln(2 pi sigma_n^2) = lnL_S + add,
+// for every prior row (with jitter the variances only grow).  prune[k] =
+// (thr, add): a row whose bound lnL_S (+ its mixed error bound) + add is
+// finite and below thr gets that bound as its value (bound 0) and is not
+// evaluated further; every other row is evaluated as in k_lnl_batch.  Rows
+// are probed a chunk of blockDim at a time and the chunk's survivors are
+// gathered in LDS, so the full evaluations run on dense waves.  thr = -inf
+// turns the probe off for the star.  n_full (optional) accumulates the rows
+// evaluated in full.
+// ---------------------------------------------------------------------------
+template <int L, bool PHYS, bool MIXED>
+__global__ __launch_bounds__(256) void k_lnl_prune_batch(const StarDev *__restrict__ stars,
+                                                         const StarDev *__restrict__ probes,
+                                                         const double2 *__restrict__ prune,
+                                                         const jk_work_item *__restrict__ items,
+                                                         int64_t n_items,
+                                                         unsigned *__restrict__ counter,
+                                                         const double *__restrict__ P,
+                                                         const void *__restrict__ e_,
+                                                         const void *__restrict__ om_,
+                                                         const void *__restrict__ M0_,
+                                                         const void *__restrict__ s_,
+                                                         double *__restrict__ ll,
+                                                         float *__restrict__ bound,
+                                                         unsigned long long *__restrict__ n_full)
+{
+    __shared__ unsigned s_item;
+    __shared__ int s_n;
+    __shared__ int s_list[256];
+    __shared__ double2 s_tab[JK_TAB_LDS];
+    jk_stage_tab(s_tab);
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
+        __syncthreads();
+        const int64_t it = (int64_t)__builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
+        if (it >= n_items) break;
+        const jk_work_item w = items[it];
+        const StarDev &st = stars[w.star];
+        const double2 pr = prune[w.star];
+        if (!(pr.x > -INFINITY)) {
+            for (int j = threadIdx.x; j < w.n_rows; j += blockDim.x) {
+                const int64_t r = w.row_begin + j;
+                if constexpr (MIXED) {
+                    float b;
+                    ll[w.out_offset + j] =
+                        jk_lnl_row<L, PHYS, true, false, true>(st, r, P, e_, om_, M0_, s_, s_tab, &b);
+                    bound[w.out_offset + j] = b;
+                } else {
+                    ll[w.out_offset + j] = jk_lnl_row<L, PHYS, false>(st, r, P, e_, om_, M0_, s_, s_tab);
+                }
+            }
+            if (n_full && threadIdx.x == 0) atomicAdd(n_full, (unsigned long long)w.n_rows);
+            continue;
+        }
+        const StarDev &pb = probes[w.star];
+        unsigned n_surv = 0;
+        for (int base = 0; base < w.n_rows; base += blockDim.x) {
+            if (threadIdx.x == 0) s_n = 0;
+            __syncthreads();
+            const int j = base + (int)threadIdx.x;
+            if (j < w.n_rows) {
+                const int64_t r = w.row_begin + j;
+                double v;
+                if constexpr (MIXED) {
+                    float b;
+                    v = jk_lnl_row<L, PHYS, true, false, true>(pb, r, P, e_, om_, M0_, s_, s_tab, &b);
+                    v += (double)b + pr.y;
+                } else {
+                    v = jk_lnl_row<L, PHYS, false>(pb, r, P, e_, om_, M0_, s_, s_tab);
+                    // FP64 rounding of both evaluations, far below this slack
+                    v += pr.y + 1e-9 * (1.0 + fabs(v));
+                }
+                if (isfinite(v) && v < pr.x) {
+                    ll[w.out_offset + j] = v;
+                    if constexpr (MIXED) bound[w.out_offset + j] = 0.0f;
+                } else {
+                    s_list[atomicAdd(&s_n, 1)] = j;
+                }
+            }
+            __syncthreads();
+            const int ns = s_n;
+            n_surv += (unsigned)ns;
+            for (int k = threadIdx.x; k < ns; k += blockDim.x) {
+                const int jj = s_list[k];
+                const int64_t r = w.row_begin + jj;
+                if constexpr (MIXED) {
+                    float b;
+                    ll[w.out_offset + jj] =
+                        jk_lnl_row<L, PHYS, true, false, true>(st, r, P, e_, om_, M0_, s_, s_tab, &b);
+                    bound[w.out_offset + jj] = b;
+                } else {
+                    ll[w.out_offset + jj] = jk_lnl_row<L, PHYS, false>(st, r, P, e_, om_, M0_, s_, s_tab);
+                }
+            }
+            __syncthreads();   // s_list is read before the next chunk resets s_n
+        }
+        if (n_full && threadIdx.x == 0 && n_surv) atomicAdd(n_full, (unsigned long long)n_surv);
+    }
+}
+
+template <int L, bool PHYS, bool MIXED>
+static void launch_prune_t(const jk_batch *b, const jk_batch *pb, const double2 *prune,
+                           int64_t n_items, const jk_work_item *items, const double *P,
+                           const void *e, const void *om, const void *M0, const void *s,
+                           double *ll, float *bound, unsigned long long *n_full, int block,
+                           int grid, unsigned *counter, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, stream, counter);
+    hipLaunchKernelGGL((k_lnl_prune_batch<L, PHYS, MIXED>), dim3(grid), dim3(block), 0, stream,
+                       b->d_stars, pb->d_stars, prune, items, n_items, counter, P, e, om, M0, s,
+                       ll, bound, n_full);
+}
+
+hipError_t jk_launch_lnl_prune_batch(const jk_batch *b, const jk_batch *pb, const double *prune,
+                                     int precision, int64_t n_items, const jk_work_item *items,
+                                     const double *P, const void *e, const void *om,
+                                     const void *M0, const void *s, double *ll, float *bound,
+                                     unsigned long long *n_full, int block, int grid,
+                                     unsigned *counter, hipStream_t stream)
+{
+    if (n_items <= 0) return hipSuccess;
+    const bool phys = b->jitter_mode == JK_JITTER_PHYSICAL;
+    const double2 *pr = reinterpret_cast<const double2 *>(prune);
+#define JK_PCASE(LL)                                                                              \
+    case LL:                                                                                      \
+        if (precision == 1) {                                                                     \
+            if (phys)                                                                             \
+                launch_prune_t<LL, true, true>(b, pb, pr, n_items, items, P, e, om, M0, s, ll,     \
+                                               bound, n_full, block, grid, counter, stream);      \
+            else                                                                                  \
+                launch_prune_t<LL, false, true>(b, pb, pr, n_items, items, P, e, om, M0, s, ll,    \
+                                                bound, n_full, block, grid, counter, stream);     \
+        } else {                                                                                  \
+            if (phys)                                                                             \
+                launch_prune_t<LL, true, false>(b, pb, pr, n_items, items, P, e, om, M0, s, ll,    \
+                                                nullptr, n_full, block, grid, counter, stream);   \
+            else                                                                                  \
+                launch_prune_t<LL, false, false>(b, pb, pr, n_items, items, P, e, om, M0, s, ll,   \
+                                                 nullptr, n_full, block, grid, counter, stream);  \
+        }                                                                                         \
+        break;
+    switch (b->L) {
+        JK_PCASE(1)
+        JK_PCASE(2)
+        JK_PCASE(3)
+        JK_PCASE(4)
+        JK_PCASE(5)
+        JK_PCASE(6)
+        JK_PCASE(7)
+        JK_PCASE(8)
+    default:
+        return hipErrorInvalidValue;
+    }
+#undef JK_PCASE
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Segmented rejection: the steps of jk_launch_reject with blockIdx.y picking
 // the segment.  Blocks past a segment's extent exit at once.
 // ---------------------------------------------------------------------------

@@ -22,7 +22,7 @@ from . import _lib
 from . import units as u
 from .samples import NONLINEAR_INTERNAL_UNITS, NONLINEAR_PACKED_ORDER, JokerSamples
 
-__all__ = ["DeviceStar", "PriorCache", "star_arrays", "lnl", "lnl_reject", "lnl_reject_async",
+__all__ = ["DeviceStar", "PriorCache", "star_arrays", "probe_star", "probe_epochs", "lnl", "lnl_reject", "lnl_reject_async",
            "PendingReject", "polish", "reject",
            "posterior_inputs_many", "posterior_inputs_table", "to_host",
            "linear_posterior", "uniforms",
@@ -183,6 +183,37 @@ class DeviceStar:
         return cls(a["t"], a["rv"], a["ivar"], a["trend"], a["mu"], a["lam"], a["t_ref"],
                    a["k_prior_mode"], a["sigma_K0"], a["P0"], a["max_K"],
                    jitter_mode=jitter_mode, device=device, cov=a["cov"])
+
+
+def probe_epochs(n, k=None):
+    """The epochs of a star's probe: k of its n epochs (default
+    max(6, n / 6)) spread evenly over the time-sorted list, so that the probe
+    spans the whole baseline."""
+    k = max(6, int(round(n / 6.0))) if k is None else int(k)
+    k = min(k, n)
+    return np.unique(np.round(np.linspace(0, n - 1, k)).astype(np.int64))
+
+
+def probe_star(a, jitter_mode="physical", device=0, k=None):
+    """(probe DeviceStar, add) for the star of star_arrays() output `a`: the
+    star on the epochs probe_epochs(n, k) with the same prior, and
+    add = -1/2 sum over the other epochs of ln(2 pi sigma_n^2).  For every
+    prior row, lnL_probe + add >= lnL of the whole star: p(y) = p(y_S)
+    p(y_rest | y_S), and the conditional covariance of y_rest is at least the
+    noise covariance of those epochs, so their density is at most
+    prod_n (2 pi sigma_n^2)^(-1/2) (with jitter, sigma_n^2 + s^2 only makes it
+    smaller).  Diagonal noise only."""
+    if a.get("cov") is not None:
+        raise ValueError("probe stars need a diagonal noise covariance")
+    n = a["t"].size
+    idx = probe_epochs(n, k)
+    rest = np.setdiff1d(np.arange(n), idx)
+    add = -0.5 * float(np.sum(np.log(2.0 * np.pi / a["ivar"][rest])))
+    trend = np.asarray(a["trend"], dtype=np.float64).reshape(n, -1)[idx]
+    star = DeviceStar(a["t"][idx], a["rv"][idx], a["ivar"][idx], trend, a["mu"], a["lam"],
+                      a["t_ref"], a["k_prior_mode"], a["sigma_K0"], a["P0"], a["max_K"],
+                      jitter_mode=jitter_mode, device=device)
+    return star, add
 
 
 def star_arrays(data, prior, trend_M):

@@ -31,8 +31,8 @@ import time
 import numpy as np
 
 from .batch import LIVE_MARGIN, LIVE_MARGIN_NONZERO_U, LaunchShape, StarBatch, heaviest_first, \
-    lnl_batch, reject_batch, settle_live_batch, upload_items, work_items
-from .device import (DeviceStar, _torch, linear_internal_units, pcg64_state,
+    lnl_batch, lnl_prune_batch, reject_batch, settle_live_batch, upload_items, work_items
+from .device import (DeviceStar, _torch, linear_internal_units, pcg64_state, probe_star,
                      posterior_inputs_table, star_arrays)
 from .likelihood_helpers import validate_prepare_data
 from .thejoker import IterativeRejection, TheJoker
@@ -135,6 +135,8 @@ class Request:
         self.preempted = 0      # times restarted to free arena memory
         self.need_hint = 0      # arena values held when last preempted
         self.round_done = 0     # rows of the current round already evaluated
+        self.probe = None       # probe star (DeviceStar on a subset of the epochs)
+        self.prune_add = None   # its bound's addend; None: the request is not probed
 
     @property
     def latency(self):
@@ -170,6 +172,17 @@ class JokerServer:
         behind long ones.
     min_chunk_rows : int
         Smallest part of a round a step evaluates under a budget.
+    prune : bool
+        After a request's first round, skip the full evaluation of rows whose
+        upper bound from a probe on a subset of the star's epochs
+        (probe_star) lies more than the settling margin below the maximum of
+        the earlier rounds: such a row cannot pass the rejection step for
+        any uniform but 0 (mixed precision, where the zero-uniform check
+        covers that case) or at all (f64, margin 750).  It keeps its bound
+        as its value.  Results are unchanged.
+    prune_min_epochs : int
+        Stars with fewer epochs are not probed (the probe would cost about
+        as much as the evaluation it saves).
     """
 
     PHASES = ("admit", "plan", "lnl", "polish", "reject", "update", "finish")
@@ -177,8 +190,11 @@ class JokerServer:
     def __init__(self, prior, cache, n_requested=256, precision="mixed", jitter_mode="physical",
                  max_active=64, hbm_budget_bytes=4 << 30, in_memory=False, growth_factor=128,
                  n_linear_samples=1, shape=None, device=0, clock=time.perf_counter,
-                 profile=False, max_step_work=None, min_chunk_rows=1 << 16):
+                 profile=False, max_step_work=None, min_chunk_rows=1 << 16, prune=True,
+                 prune_min_epochs=12):
         torch = _torch()
+        self.prune = bool(prune)
+        self.prune_min_epochs = int(prune_min_epochs)
         self.max_step_work = None if max_step_work is None else int(max_step_work)
         self.min_chunk_rows = int(min_chunk_rows)
         self.prior = prior
@@ -209,6 +225,15 @@ class JokerServer:
         self._table = StarBatch.slots(self.max_active, 1 + prior.poly_trend + prior.n_offsets,
                                       jitter_mode, device=device)
         self._free_slots = list(range(self.max_active - 1, -1, -1))
+        self._probes = None
+        if self.prune:
+            self._probes = StarBatch.slots(self.max_active, self._table.L, jitter_mode,
+                                           device=device)
+            self._prune_host = np.zeros((self.max_active, 2), dtype=np.float64)
+            self._prune_dev = torch.zeros(2 * self.max_active, dtype=torch.float64,
+                                          device=cache.P.device)
+        self._n_full = torch.zeros(1, dtype=torch.int64, device=cache.P.device)
+        self.rows_probed = 0      # rows of launches that probed their star
         self._ws = torch.zeros(64, dtype=torch.int32, device=cache.P.device)
         self.profile = profile
         self.phase_s = dict.fromkeys(self.PHASES, 0.0)
@@ -249,6 +274,31 @@ class JokerServer:
         self._t_mark = t
 
     # ------------------------------------------------------------------
+    def _prune_thresholds(self, plan):
+        """(threshold, add) per table slot for this step's launch, or None when
+        no request in `plan` is probed.  A request is probed once it has a
+        maximum from earlier rounds (its floor, which the final maximum can
+        only exceed); the threshold is that floor minus the settling margin
+        (40 in mixed precision, whose zero-uniform check covers u == 0; 750
+        in FP64)."""
+        if not self.prune:
+            return None
+        margin = LIVE_MARGIN_NONZERO_U if self.precision == "mixed" else LIVE_MARGIN
+        thr = self._prune_host
+        thr[:, 0] = -np.inf
+        thr[:, 1] = 0.0
+        any_on = False
+        for r, _, _ in plan:
+            if r.prune_add is not None and np.isfinite(r.it.floor):
+                thr[r.slot] = (r.it.floor - margin, r.prune_add)
+                any_on = True
+        return thr if any_on else None
+
+    def prune_stats(self):
+        """(rows of probed launches, rows evaluated in full by pruned
+        launches); synchronises with the device."""
+        return self.rows_probed, int(self._n_full.item())
+
     def _release(self, r):
         if r.region is not None:
             self.alloc.release(*r.region)
@@ -295,6 +345,10 @@ class JokerServer:
                                           device=self.cache.P.device)
                 r.slot = self._free_slots.pop()
                 self._table.set(r.slot, r.star)
+                r.prune_add = None
+                if self.prune and a["t"].size >= self.prune_min_epochs:
+                    r.probe, r.prune_add = probe_star(a, self.jitter_mode, self.device)
+                    self._probes.set(r.slot, r.probe)
             except Exception as exc:       # a bad request fails alone
                 self._fail(r, exc)
                 continue
@@ -443,8 +497,17 @@ class JokerServer:
         if self.profile:
             finished = self._build_results()
             self._mark("finish")
-        lnl_batch(self._table, self.cache, d_items, len(items), 0, self.precision, self.shape,
-                  out=self.arena, workspace=self._ws, bound=self.arena_b)
+        thr = self._prune_thresholds(plan)
+        if thr is None:
+            lnl_batch(self._table, self.cache, d_items, len(items), 0, self.precision, self.shape,
+                      out=self.arena, workspace=self._ws, bound=self.arena_b)
+        else:
+            self._prune_dev.copy_(_torch().from_numpy(thr.reshape(-1)))
+            lnl_prune_batch(self._table, self._probes, self._prune_dev, self.cache, d_items,
+                            len(items), self.precision, self.shape, out=self.arena,
+                            bound=self.arena_b, workspace=self._ws, n_full=self._n_full)
+            self.rows_probed += int(sum(sizes[k] for k, (r, _, _) in enumerate(plan)
+                                        if np.isfinite(thr[r.slot, 0])))
         lap("lnl_launch")
         if not self.profile:
             finished = self._build_results()      # overlaps the launch above
