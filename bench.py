@@ -362,6 +362,16 @@ def leg_batched(args, torch, np, D, syn):
 
 
 # ---------------------------------------------------------------------------
+def _prune_stats(srv, prune):
+    """Rows of the requests whose star was probed and, of those, the rows
+    that kept their upper bound instead of a full evaluation."""
+    if not prune:
+        return None
+    probed, pruned = srv.prune_stats()
+    return {"rows_probed": int(probed), "rows_pruned": int(pruned),
+            "pruned_fraction_of_probed": (pruned / probed) if probed else None}
+
+
 def leg_serving(args, torch, np, D, syn):
     import thejoker_amd as tj
     from thejoker_amd import serving as S
@@ -383,8 +393,9 @@ def leg_serving(args, torch, np, D, syn):
     del wsrv
     arena = int(args.serving_arena_gib * (1 << 30))
     step_work = int(args.serving_step_work) if args.serving_step_work > 0 else None
+    prune = bool(args.serving_prune)
     srv = S.JokerServer(prior, cache, precision="mixed", max_active=args.serving_active,
-                        hbm_budget_bytes=arena, max_step_work=step_work)
+                        hbm_budget_bytes=arena, max_step_work=step_work, prune=prune)
     rngs = [np.random.default_rng(10_000 + i) for i in range(n_req)]
     t0 = time.perf_counter()
     done = S.run_poisson(srv, times, datas, rngs)
@@ -405,11 +416,12 @@ def leg_serving(args, torch, np, D, syn):
     res["preemptions"] = srv.preemptions
     res["arena_gib"] = args.serving_arena_gib
     res["max_step_work"] = step_work
+    res["prune"] = _prune_stats(srv, prune)
     del srv
     torch.cuda.empty_cache()
     # capacity: the whole stream submitted at once
     srv2 = S.JokerServer(prior, cache, precision="mixed", max_active=args.serving_active,
-                         hbm_budget_bytes=arena, max_step_work=step_work)
+                         hbm_budget_bytes=arena, max_step_work=step_work, prune=prune)
     rngs2 = [np.random.default_rng(10_000 + i) for i in range(n_req)]
     step_log, step_info = [], []
     step0 = srv2.step
@@ -431,7 +443,8 @@ def leg_serving(args, torch, np, D, syn):
     same = sum(int(np.array_equal(a.result["P"].value, b.result["P"].value))
                for a, b in zip(done, done2) if a.error is None and b.error is None)
     sat = {"preemptions": srv2.preemptions, "sample_epochs_per_s": srv2.sample_epochs / wall2,
-           "steps": srv2.steps, "region_moves": dict(srv2.region_moves)}
+           "steps": srv2.steps, "region_moves": dict(srv2.region_moves),
+           "prune": _prune_stats(srv2, prune)}
     del srv2, logged_step, step0
     gc.collect()
     torch.cuda.empty_cache()
@@ -454,7 +467,28 @@ def leg_serving(args, torch, np, D, syn):
     log(f"[bench] serving: saturated {res['saturated']['stars_per_s']:.1f} stars/s "
         f"({sat['sample_epochs_per_s']:.3e} sample-epochs/s, {sat['steps']} steps, "
         f"{sat['preemptions']} restarts, {n_req - len(ok2)} failed, {same} results equal the "
-        f"latency run's); host time per call {res['saturated']['host_wait_s']}")
+        f"latency run's); host time per call {res['saturated']['host_wait_s']}; "
+        f"prune {sat['prune']}")
+    if prune:
+        # the same saturated stream without pruning: its rate and its results
+        srv3 = S.JokerServer(prior, cache, precision="mixed", max_active=args.serving_active,
+                             hbm_budget_bytes=arena, max_step_work=step_work, prune=False)
+        rngs3 = [np.random.default_rng(10_000 + i) for i in range(n_req)]
+        t0 = time.perf_counter()
+        done3 = S.run_poisson(srv3, np.zeros(n_req), datas, rngs3)
+        wall3 = time.perf_counter() - t0
+        ok3 = [r for r in done3 if r.error is None]
+        same3 = sum(int(np.array_equal(a.result["P"].value, b.result["P"].value) and
+                        np.array_equal(a.result["K"].value, b.result["K"].value))
+                    for a, b in zip(done2, done3) if a.error is None and b.error is None)
+        res["saturated_no_prune"] = {"wall_s": wall3, "stars_per_s": len(ok3) / wall3,
+                                     "sample_epochs_per_s": srv3.sample_epochs / wall3,
+                                     "results_equal_pruned_run": same3}
+        log(f"[bench] serving: saturated without pruning {len(ok3) / wall3:.1f} stars/s; "
+            f"{same3} of {len(ok3)} results equal the pruned run's")
+        del srv3
+        gc.collect()
+        torch.cuda.empty_cache()
     # The plain path: the same stars one at a time through
     # TheJoker(precision="f64").iterative_rejection_sample.  Same rows, so
     # the same sample-epochs as in the server; also checks for flips.
@@ -551,6 +585,9 @@ def main():
                     help="sample-epochs one serving step may evaluate (0: whole rounds)")
     ap.add_argument("--serving-arena-gib", type=float, default=64.0,
                     help="HBM for the server's lnL arena (12 B per evaluated row)")
+    ap.add_argument("--serving-prune", type=int, default=1,
+                    help="1: the server prunes rows by the probe bound (results unchanged); "
+                         "the saturated stream is also run without pruning for comparison")
     ap.add_argument("--serving-plain", type=int, default=64,
                     help="stars of the serving stream also run one at a time in plain FP64")
     ap.add_argument("--stress-log2-rows", type=int, default=29)

@@ -37,6 +37,71 @@ u128 make_u128(uint64_t hi, uint64_t lo)
     return r;
 }
 
+// Scratch kept per (device, stream) between calls.  The settling and
+// rejection calls need up to a few GB of temporary device memory each; taken
+// from the stream-ordered pool at every call, it costs a new mapping whenever
+// the pool holds no free block of the size asked for, and such calls were
+// measured at up to seconds with the device idle meanwhile.  The next call on
+// the same stream reuses the buffer, after this call's work in stream order.
+// A buffer grows (after synchronising its stream) and is released at process
+// exit.  Under stream capture the stream-ordered pool is used instead, so that
+// a captured graph never refers to memory a later call may replace.
+struct ScratchSlot {
+    int device;
+    hipStream_t stream;
+    void *ptr;
+    size_t bytes;
+};
+std::mutex g_scratch_mu;
+std::vector<ScratchSlot> g_scratch;
+
+hipError_t scratch_get(hipStream_t st, size_t bytes, void **out, bool *pooled)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    hipError_t e = hipStreamIsCapturing(st, &cs);
+    if (e != hipSuccess) return e;
+    if (cs != hipStreamCaptureStatusNone) {
+        *pooled = true;
+        return hipMallocAsync(out, bytes, st);
+    }
+    *pooled = false;
+    int dev = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    ScratchSlot *slot = nullptr;
+    for (auto &sl : g_scratch)
+        if (sl.device == dev && sl.stream == st) slot = &sl;
+    if (slot && slot->bytes >= bytes) {
+        *out = slot->ptr;
+        return hipSuccess;
+    }
+    size_t want = bytes < 256 ? 256 : bytes;
+    if (slot) {
+        want = std::max(want, slot->bytes + slot->bytes / 2);
+        if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+        if ((e = hipFree(slot->ptr)) != hipSuccess) return e;
+        slot->ptr = nullptr;
+        slot->bytes = 0;
+    }
+    const size_t align = (size_t)2 << 20;
+    want = (want + align - 1) / align * align;
+    void *ptr = nullptr;
+    if ((e = hipMalloc(&ptr, want)) != hipSuccess) return e;
+    if (slot) {
+        slot->ptr = ptr;
+        slot->bytes = want;
+    } else {
+        g_scratch.push_back({dev, st, ptr, want});
+    }
+    *out = ptr;
+    return hipSuccess;
+}
+
+hipError_t scratch_put(hipStream_t st, void *ptr, bool pooled)
+{
+    return pooled ? hipFreeAsync(ptr, st) : hipSuccess;
+}
+
 }  // namespace
 
 extern "C" {
@@ -69,10 +134,10 @@ int jk_init(int device)
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) return JK_ENODEV;
     if (jk_upload_pcg_table(device) != hipSuccess) return JK_EHIP;
     if (jk_upload_sincos_table(device) != hipSuccess) return JK_EHIP;
-    // The stream-ordered scratch of the settling and rejection calls (up to a
-    // few GB for 10^9-row steps) is freed on the stream after each call; keep
-    // up to JK_POOL_KEEP bytes of it in the device's default pool instead of
-    // returning it at every synchronisation, so the next call maps nothing.
+    // The device's default stream-ordered pool serves the small workspaces of
+    // calls made without one and the scratch of calls under graph capture;
+    // keep up to JK_POOL_KEEP bytes of it instead of returning them at every
+    // synchronisation, so that the next call maps nothing.
     hipMemPool_t pool;
     if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
         uint64_t keep = JK_POOL_KEEP;
@@ -332,9 +397,11 @@ int jk_max_f64(const double *ll, int64_t n, double *max_out, void *stream)
     hipStream_t st = (hipStream_t)stream;
     const int np = jk_max_partials_count(n);
     double *partials = nullptr;
-    if (hipMallocAsync((void **)&partials, sizeof(double) * np, st) != hipSuccess) return JK_ENOMEM;
+    bool pooled = false;
+    if (scratch_get(st, sizeof(double) * np, (void **)&partials, &pooled) != hipSuccess)
+        return JK_ENOMEM;
     hipError_t err = jk_launch_max(ll, n, partials, np, max_out, st);
-    hipError_t err2 = hipFreeAsync(partials, st);
+    hipError_t err2 = scratch_put(st, partials, pooled);
     if (err != hipSuccess) return JK_EHIP;
     return check_hip(err2);
 }
@@ -353,7 +420,8 @@ static int reject_impl(const double *ll, int64_t n, const double *llmax_in, uint
     const int64_t nb = jk_reject_bsum_count(n);
     const size_t bytes = sizeof(double) * np + sizeof(uint64_t) * nwords + sizeof(int64_t) * nb;
     char *scratch = nullptr;
-    if (hipMallocAsync((void **)&scratch, bytes, st) != hipSuccess) return JK_ENOMEM;
+    bool pooled = false;
+    if (scratch_get(st, bytes, (void **)&scratch, &pooled) != hipSuccess) return JK_ENOMEM;
     double *partials = (double *)scratch;
     uint64_t *mask = (uint64_t *)(scratch + sizeof(double) * np);
     int64_t *bsum = (int64_t *)(scratch + sizeof(double) * np + sizeof(uint64_t) * nwords);
@@ -366,7 +434,7 @@ static int reject_impl(const double *ll, int64_t n, const double *llmax_in, uint
     if (err == hipSuccess)
         err = jk_launch_reject(ll, n, mx, make_u128(state_hi, state_lo), make_u128(inc_hi, inc_lo),
                                offset, max_keep, idx_out, n_pass_out, mask, bsum, st);
-    hipError_t err2 = hipFreeAsync(scratch, st);
+    hipError_t err2 = scratch_put(st, scratch, pooled);
     if (err != hipSuccess) return JK_EHIP;
     return check_hip(err2);
 }
@@ -402,13 +470,15 @@ int jk_lnl_reject_f64(const jk_star *star, int64_t n, const double *P, const dou
     if (current_device_inited() < 0) return JK_ENODEV;
     hipStream_t st = (hipStream_t)stream;
     void *scratch = nullptr;
-    if (hipMallocAsync(&scratch, (size_t)jk_lnl_reject_scratch_bytes(n), st) != hipSuccess)
+    bool pooled = false;
+    if (scratch_get(st, (size_t)jk_lnl_reject_scratch_bytes(n), (void **)&scratch, &pooled) !=
+        hipSuccess)
         return JK_ENOMEM;
     hipError_t err = jk_launch_lnl_reject_f64(star, n, P, e, omega, M0, s, rows, ll_out,
                                               make_u128(state_hi, state_lo),
                                               make_u128(inc_hi, inc_lo), stream_offset, max_keep,
                                               idx_out, n_pass_out, llmax_out, scratch, st);
-    hipError_t err2 = hipFreeAsync(scratch, st);
+    hipError_t err2 = scratch_put(st, scratch, pooled);
     if (err != hipSuccess) return JK_EHIP;
     return check_hip(err2);
 }
@@ -465,11 +535,13 @@ int jk_polish_f64(const jk_star *star, int64_t n, const double *P, const double 
     if (current_device_inited() < 0) return JK_ENODEV;
     hipStream_t st = (hipStream_t)stream;
     void *scratch = nullptr;
-    if (hipMallocAsync(&scratch, (size_t)jk_polish_scratch_bytes(n), st) != hipSuccess)
+    bool pooled = false;
+    if (scratch_get(st, (size_t)jk_polish_scratch_bytes(n), (void **)&scratch, &pooled) !=
+        hipSuccess)
         return JK_ENOMEM;
     hipError_t err = jk_launch_polish(star, n, P, e, omega, M0, s, rows, ll_inout, window, first,
                                       scratch, n_polished_out, st);
-    hipError_t err2 = hipFreeAsync(scratch, st);
+    hipError_t err2 = scratch_put(st, scratch, pooled);
     if (err != hipSuccess) return JK_EHIP;
     return check_hip(err2);
 }
@@ -700,7 +772,8 @@ static int reject_batch_impl(const double *ll, int n_seg, const jk_segment *segs
     const size_t bytes = seg_pad + sizeof(double) * (size_t)pb * n_seg +
                          sizeof(uint64_t) * mask_words + sizeof(int64_t) * bsum_n;
     char *scratch = nullptr;
-    if (hipMallocAsync((void **)&scratch, bytes, st) != hipSuccess) return JK_ENOMEM;
+    bool pooled = false;
+    if (scratch_get(st, bytes, (void **)&scratch, &pooled) != hipSuccess) return JK_ENOMEM;
     SegDev *d_segs = (SegDev *)scratch;
     double *partials = (double *)(scratch + seg_pad);
     uint64_t *mask = (uint64_t *)(partials + (size_t)pb * n_seg);
@@ -711,7 +784,7 @@ static int reject_batch_impl(const double *ll, int n_seg, const jk_segment *segs
                                      idx_out, n_pass_out, llmax_out, st, zero_u_out);
     // The host descriptor array must outlive the queued copy.
     hipError_t err3 = hipStreamSynchronize(st);
-    hipError_t err2 = hipFreeAsync(scratch, st);
+    hipError_t err2 = scratch_put(st, scratch, pooled);
     if (err != hipSuccess || err3 != hipSuccess) return JK_EHIP;
     return check_hip(err2);
 }
@@ -775,7 +848,8 @@ int jk_polish_batch(const jk_batch *batch, int n_seg, const jk_polish_segment *s
                          sizeof(uint64_t) * mask_words +
                          sizeof(int64_t) * (bsum_n + idx_n + n_seg + 1);
     char *scratch = nullptr;
-    if (hipMallocAsync((void **)&scratch, bytes, st) != hipSuccess) return JK_ENOMEM;
+    bool pooled = false;
+    if (scratch_get(st, bytes, (void **)&scratch, &pooled) != hipSuccess) return JK_ENOMEM;
     SegDev *d_segs = (SegDev *)scratch;
     double *partials = (double *)(scratch + seg_pad);
     double *llmax = partials + (size_t)pb * n_seg;
@@ -790,7 +864,7 @@ int jk_polish_batch(const jk_batch *batch, int n_seg, const jk_polish_segment *s
                                      n_polished_out, st);
     // The host descriptor array must outlive the queued copy.
     hipError_t err3 = hipStreamSynchronize(st);
-    hipError_t err2 = hipFreeAsync(scratch, st);
+    hipError_t err2 = scratch_put(st, scratch, pooled);
     if (err != hipSuccess || err3 != hipSuccess) return JK_EHIP;
     return check_hip(err2);
 }
@@ -841,7 +915,8 @@ int jk_settle_batch(const jk_batch *batch, int n_seg, const jk_settle_segment *s
                          sizeof(uint64_t) * mask_words +
                          sizeof(int64_t) * (bsum_n + idx_n + 2 * (size_t)n_seg + 1);
     char *scratch = nullptr;
-    if (hipMallocAsync((void **)&scratch, bytes, st) != hipSuccess) return JK_ENOMEM;
+    bool pooled = false;
+    if (scratch_get(st, bytes, (void **)&scratch, &pooled) != hipSuccess) return JK_ENOMEM;
     SegDev *d_segs = (SegDev *)scratch;
     double *partials = (double *)(scratch + seg_pad);
     double *lo = partials + (size_t)pb * n_seg;
@@ -858,7 +933,7 @@ int jk_settle_batch(const jk_batch *batch, int n_seg, const jk_settle_segment *s
                                      ll_inout, bound_inout, n_settled_out, st);
     // The host descriptor array must outlive the queued copy.
     hipError_t err3 = hipStreamSynchronize(st);
-    hipError_t err2 = hipFreeAsync(scratch, st);
+    hipError_t err2 = scratch_put(st, scratch, pooled);
     if (err != hipSuccess || err3 != hipSuccess) return JK_EHIP;
     return check_hip(err2);
 }
@@ -918,7 +993,8 @@ int jk_settle_reject_batch(const jk_batch *batch, int n_seg, const jk_settle_seg
                          2 * sizeof(uint64_t) * mask_words +
                          sizeof(int64_t) * (bsum_n + idx_n + 2 * (size_t)n_seg + 1);
     char *scratch = nullptr;
-    if (hipMallocAsync((void **)&scratch, bytes, st) != hipSuccess) return JK_ENOMEM;
+    bool pooled = false;
+    if (scratch_get(st, bytes, (void **)&scratch, &pooled) != hipSuccess) return JK_ENOMEM;
     SegDev *d_settle = (SegDev *)scratch;
     SegDev *d_reject = (SegDev *)(scratch + seg_pad);
     double *partials = (double *)(scratch + 2 * seg_pad);
@@ -940,7 +1016,7 @@ int jk_settle_reject_batch(const jk_batch *batch, int n_seg, const jk_settle_seg
                                             n_settled_out, st);
     // The host descriptor arrays must outlive the queued copies.
     hipError_t err3 = hipStreamSynchronize(st);
-    hipError_t err2 = hipFreeAsync(scratch, st);
+    hipError_t err2 = scratch_put(st, scratch, pooled);
     if (err != hipSuccess || err3 != hipSuccess) return JK_EHIP;
     return check_hip(err2);
 }
@@ -993,7 +1069,8 @@ int jk_settle_live_batch(const jk_batch *batch, int n_seg, const jk_live_segment
                          sizeof(uint64_t) * mask_words +
                          sizeof(int64_t) * (bsum_n + idx_n + (size_t)n_seg + 1);
     char *scratch = nullptr;
-    if (hipMallocAsync((void **)&scratch, bytes, st) != hipSuccess) return JK_ENOMEM;
+    bool pooled = false;
+    if (scratch_get(st, bytes, (void **)&scratch, &pooled) != hipSuccess) return JK_ENOMEM;
     SegDev *d_segs = (SegDev *)scratch;
     double *partials = (double *)(scratch + seg_pad);
     double *lo = partials + (size_t)pb * n_seg;
@@ -1008,7 +1085,7 @@ int jk_settle_live_batch(const jk_batch *batch, int n_seg, const jk_live_segment
                                           bound_inout, margin, n_settled_out, st);
     // The host descriptor array must outlive the queued copy.
     hipError_t err3 = hipStreamSynchronize(st);
-    hipError_t err2 = hipFreeAsync(scratch, st);
+    hipError_t err2 = scratch_put(st, scratch, pooled);
     if (err != hipSuccess || err3 != hipSuccess) return JK_EHIP;
     return check_hip(err2);
 }

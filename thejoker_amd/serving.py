@@ -233,7 +233,8 @@ class JokerServer:
             self._prune_dev = torch.zeros(2 * self.max_active, dtype=torch.float64,
                                           device=cache.P.device)
         self._n_full = torch.zeros(1, dtype=torch.int64, device=cache.P.device)
-        self.rows_probed = 0      # rows of launches that probed their star
+        self.rows_probed = 0      # rows of requests whose star was probed
+        self._rows_prune_launches = 0
         self._ws = torch.zeros(64, dtype=torch.int32, device=cache.P.device)
         self.profile = profile
         self.phase_s = dict.fromkeys(self.PHASES, 0.0)
@@ -295,9 +296,11 @@ class JokerServer:
         return thr if any_on else None
 
     def prune_stats(self):
-        """(rows of probed launches, rows evaluated in full by pruned
-        launches); synchronises with the device."""
-        return self.rows_probed, int(self._n_full.item())
+        """(rows probed, rows pruned): the rows of requests whose star was
+        probed, and of those the rows that kept their bound instead of a full
+        evaluation.  Synchronises with the device."""
+        pruned = self._rows_prune_launches - int(self._n_full.item())
+        return self.rows_probed, pruned
 
     def _release(self, r):
         if r.region is not None:
@@ -508,6 +511,7 @@ class JokerServer:
                             bound=self.arena_b, workspace=self._ws, n_full=self._n_full)
             self.rows_probed += int(sum(sizes[k] for k, (r, _, _) in enumerate(plan)
                                         if np.isfinite(thr[r.slot, 0])))
+            self._rows_prune_launches += int(sizes.sum())
         lap("lnl_launch")
         if not self.profile:
             finished = self._build_results()      # overlaps the launch above
