@@ -1412,11 +1412,13 @@ hipError_t jk_launch_lnl_batch(const jk_batch *b, int precision, int64_t n_items
 // (thr, add): a row whose bound lnL_S (+ its mixed error bound) + add is
 // finite and below thr gets that bound as its value (bound 0) and is not
 // evaluated further; every other row is evaluated as in k_lnl_batch.  Rows
-// are probed a chunk of blockDim at a time and the chunk's survivors are
-// gathered in LDS, so the full evaluations run on dense waves.  thr = -inf
+// are probed a chunk of blockDim at a time and the survivors are gathered in
+// LDS over several chunks, so the full evaluations run on dense waves.  thr = -inf
 // turns the probe off for the star.  n_full (optional) accumulates the rows
 // evaluated in full.
 // ---------------------------------------------------------------------------
+#define JK_PRUNE_LIST 2048   // survivors held in LDS before their evaluation
+
 template <int L, bool PHYS, bool MIXED>
 __global__ __launch_bounds__(256) void k_lnl_prune_batch(const StarDev *__restrict__ stars,
                                                          const StarDev *__restrict__ probes,
@@ -1435,7 +1437,7 @@ __global__ __launch_bounds__(256) void k_lnl_prune_batch(const StarDev *__restri
 {
     __shared__ unsigned s_item;
     __shared__ int s_n;
-    __shared__ int s_list[256];
+    __shared__ int s_list[JK_PRUNE_LIST];
     __shared__ double2 s_tab[JK_TAB_LDS];
     jk_stage_tab(s_tab);
     for (;;) {
@@ -1464,9 +1466,9 @@ __global__ __launch_bounds__(256) void k_lnl_prune_batch(const StarDev *__restri
         }
         const StarDev &pb = probes[w.star];
         unsigned n_surv = 0;
+        if (threadIdx.x == 0) s_n = 0;
+        __syncthreads();
         for (int base = 0; base < w.n_rows; base += blockDim.x) {
-            if (threadIdx.x == 0) s_n = 0;
-            __syncthreads();
             const int j = base + (int)threadIdx.x;
             if (j < w.n_rows) {
                 const int64_t r = w.row_begin + j;
@@ -1488,21 +1490,30 @@ __global__ __launch_bounds__(256) void k_lnl_prune_batch(const StarDev *__restri
                 }
             }
             __syncthreads();
+            // Survivors gather over chunks and are evaluated once another
+            // chunk might overflow the list, or at the item's end: the full
+            // evaluations then run on whole waves.
             const int ns = s_n;
-            n_surv += (unsigned)ns;
-            for (int k = threadIdx.x; k < ns; k += blockDim.x) {
-                const int jj = s_list[k];
-                const int64_t r = w.row_begin + jj;
-                if constexpr (MIXED) {
-                    float b;
-                    ll[w.out_offset + jj] =
-                        jk_lnl_row<L, PHYS, true, false, true>(st, r, P, e_, om_, M0_, s_, s_tab, &b);
-                    bound[w.out_offset + jj] = b;
-                } else {
-                    ll[w.out_offset + jj] = jk_lnl_row<L, PHYS, false>(st, r, P, e_, om_, M0_, s_, s_tab);
+            __syncthreads();   // every thread holds ns before the next append
+            if (ns > JK_PRUNE_LIST - (int)blockDim.x || base + (int)blockDim.x >= w.n_rows) {
+                n_surv += (unsigned)ns;
+                for (int k = threadIdx.x; k < ns; k += blockDim.x) {
+                    const int jj = s_list[k];
+                    const int64_t r = w.row_begin + jj;
+                    if constexpr (MIXED) {
+                        float b;
+                        ll[w.out_offset + jj] = jk_lnl_row<L, PHYS, true, false, true>(
+                            st, r, P, e_, om_, M0_, s_, s_tab, &b);
+                        bound[w.out_offset + jj] = b;
+                    } else {
+                        ll[w.out_offset + jj] =
+                            jk_lnl_row<L, PHYS, false>(st, r, P, e_, om_, M0_, s_, s_tab);
+                    }
                 }
+                __syncthreads();   // every read of s_list before it is reset
+                if (threadIdx.x == 0) s_n = 0;
+                __syncthreads();
             }
-            __syncthreads();   // s_list is read before the next chunk resets s_n
         }
         if (n_full && threadIdx.x == 0 && n_surv) atomicAdd(n_full, (unsigned long long)n_surv);
     }
