@@ -186,9 +186,11 @@ class DeviceStar:
 
 
 def probe_epochs(n, k=None):
-    """The epochs of a star's probe: k of its n epochs (default
-    max(6, n / 6)) spread evenly over the time-sorted list, so that the probe
-    spans the whole baseline."""
+    """The epochs of a star's probe: k of its n epochs (default n / 6, at
+    least 6) spread evenly over the time-sorted list, so that the probe
+    spans the whole baseline.  (Measured on the serving stream: n / 4 prunes
+    90% of the probed rows instead of 86% but costs more than it saves; a
+    cap of 12 epochs prunes 82% and loses too.)"""
     k = max(6, int(round(n / 6.0))) if k is None else int(k)
     k = min(k, n)
     return np.unique(np.round(np.linspace(0, n - 1, k)).astype(np.int64))
