@@ -197,3 +197,26 @@ def test_rejection_sample_from_hdf5_file(gpu, tmp_path):
     assert len(outs[0]) == len(outs[1]) > 0
     for k in ("P", "e", "omega", "M0", "K", "v0"):
         assert np.array_equal(outs[0][k].value, outs[1][k].value), k
+
+
+@pytest.mark.parametrize("precision,in_memory", [("f64", False), ("f64", True),
+                                                 ("mixed", False)])
+def test_iterative_prune_changes_no_result(gpu, precision, in_memory):
+    """TheJoker(prune=True): rows that the probe bound puts more than 750
+    below the earlier maximum skip their full evaluation; the samples, their
+    ln_likelihood and the random stream afterwards are those of the unpruned
+    run."""
+    star = syn.arrivals(1, seed=41, M_range=(60, 60))[1][0]
+    samples, _ = _samples(1 << 16, seed=13)
+    outs = []
+    for prune in (False, True):
+        rng = np.random.default_rng(77)
+        joker = tj.TheJoker(_prior(), rng=rng, precision=precision, prune=prune)
+        out = joker.iterative_rejection_sample(star.to_rvdata(), samples, n_requested_samples=32,
+                                               growth_factor=8, in_memory=in_memory,
+                                               return_logprobs=True)
+        outs.append((out, rng.uniform()))
+    (a, ua), (b, ub) = outs
+    assert len(a) == len(b) > 0 and ua == ub
+    for k in ("P", "e", "K", "v0", "ln_likelihood"):
+        assert np.array_equal(a[k].value, b[k].value), k

@@ -160,6 +160,15 @@ class DeviceStar:
     def L(self):
         return self.mu.size
 
+    def arrays(self):
+        """The star's host arrays and constants in the form of star_arrays()."""
+        return dict(t=self.t, rv=self.rv, ivar=self.ivar,
+                    trend=self.trend.reshape(self.n, -1) if self.L > 1
+                    else np.zeros((self.n, 0)),
+                    mu=self.mu, lam=self.lam, t_ref=self.t_ref, cov=self.cov,
+                    k_prior_mode=self.k_prior_mode, sigma_K0=self.sigma_K0, P0=self.P0,
+                    max_K=self.max_K)
+
     @property
     def handle(self):
         return self._h

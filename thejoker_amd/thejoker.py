@@ -66,10 +66,21 @@ class TheJoker:
         ln_likelihood values of accepted samples are FP64.
     device : int
         HIP device ordinal.
+    prune : bool
+        iterative_rejection_sample only: from the second round on, skip the
+        full evaluation of rows whose upper bound from a probe on a subset of
+        the epochs (device.probe_star) lies more than 750 below the maximum of
+        the earlier rounds -- rows no uniform can accept.  They keep the bound
+        as their value; the samples returned do not change.  Stars with fewer
+        than 12 epochs, full covariances and randomize_prior_order are not
+        probed.  Off by default, so the default path evaluates every row as
+        the reference does.
     """
 
+    PRUNE_MIN_EPOCHS = 12
+
     def __init__(self, prior, pool=None, rng=None, tempfile_path=None, jitter_mode="physical",
-                 precision="f64", device=0, random_state=None):
+                 precision="f64", device=0, random_state=None, prune=False):
         if random_state is not None and rng is None:
             rng = random_state
         if pool is not None and (not hasattr(pool, "map") or not hasattr(pool, "close")):
@@ -96,6 +107,7 @@ class TheJoker:
         self.jitter_mode = jitter_mode
         self.precision = precision
         self.device = device
+        self.prune = bool(prune)
 
     @property
     def tempfile_path(self):
@@ -189,6 +201,14 @@ class TheJoker:
             return np.zeros(0)
         r = torch.as_tensor(np.asarray(cache_rows, dtype=np.int64), device=cache.P.device)
         return to_host(lnl(star, cache, rows=r, precision="f64"))
+
+    def _pruner(self, star, it):
+        """The probe launcher of iterative_rejection_sample, or None when the
+        star is not probed."""
+        if (not self.prune or star.cov is not None or star.n < self.PRUNE_MIN_EPOCHS
+                or it._d_all_idx is not None):
+            return None
+        return _probe_pruner(star, self.precision, self.jitter_mode, self.device)
 
     def _n_batches(self, n_batches):
         if n_batches is not None:
@@ -340,11 +360,14 @@ class TheJoker:
         all_ll = torch.empty(it.max_prior, dtype=torch.float64, device=dev)
         mixed = self.precision == "mixed"
         all_b = torch.empty(it.max_prior, dtype=torch.float32, device=dev) if mixed else None
+        pruner = self._pruner(star, it)
         while not it.done:
             rows, start, n_now = it.next_batch()
             a, b = it.n_evals, it.n_evals + n_now
             part_b = all_b[a:b] if mixed else None
-            if rows is not None:
+            if pruner is not None and np.isfinite(it.floor):
+                pruner(cache, start, n_now, it.floor, all_ll[a:b], part_b)
+            elif rows is not None:
                 lnl(star, cache, rows=rows, precision=self.precision, out=all_ll[a:b],
                     bound=part_b)
             else:
@@ -368,6 +391,32 @@ class TheJoker:
                 samples["ln_likelihood"] = ll_good
                 samples["ln_prior"] = cache.ln_prior[full_idx]
         return samples
+
+
+def _probe_pruner(star, precision, jitter_mode, device):
+    """f(cache, start, n, floor, out, bound) evaluating cache rows
+    start .. start + n of `star` into out (and bound) with
+    jk_lnl_prune_batch: rows whose probe bound lies more than 750 below
+    `floor` keep that bound."""
+    import torch
+
+    from .batch import LIVE_MARGIN, LaunchShape, StarBatch, lnl_prune_batch, upload_items, \
+        work_items
+    from .device import probe_star
+
+    probe, add = probe_star(star.arrays(), jitter_mode, device)
+    # the tables hold the stars for as long as the launcher lives
+    table, probes = StarBatch([star], device=device), StarBatch([probe], device=device)
+    prune = torch.zeros(2, dtype=torch.float64, device=torch.device("cuda", device))
+    shape = LaunchShape()
+
+    def run(cache, start, n, floor, out, bound):
+        items, _, _ = work_items([n], shape.tile, row_begin=[start], stars=[0])
+        prune.copy_(torch.tensor([floor - LIVE_MARGIN, add], dtype=torch.float64))
+        lnl_prune_batch(table, probes, prune, cache, upload_items(items, device), len(items),
+                        precision, shape, out=out, bound=bound)
+
+    return run
 
 
 class IterativeRejection:
