@@ -185,7 +185,12 @@ def leg_single(args, torch, np, D, syn):
             tr = json.load(f)
         if tr.get("rows") == n:
             traffic = tr["hbm_bytes_per_launch"]
-    roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+    # The kernel is compute-bound, on the FP64 vector ALU (the per-row 2x2 / 3x3
+    # systems are not matrix products, so no MFMA).  The schema's compute bound
+    # is named "mfma"; MI355X's FP64 vector and FP64 dense-matrix peaks are the
+    # same 78.6 TFLOP/s, which is the peak used here.
+    roofline = {"bound": "mfma", "compute_unit": "FP64 vector ALU (VALU); no MFMA instructions",
+                "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                 "kernel": "k_lnl<2,true,false,false> (FP64)", "kernel_ms": k_ms,
                 "flops_per_launch": flops,
