@@ -407,7 +407,9 @@ def leg_serving(args, torch, np, D, syn):
     wall = time.perf_counter() - t0
     lat = np.array([r.latency for r in done if r.error is None])
     errs = [r for r in done if r.error is not None]
-    res = {"requests": n_req, "rate_per_s": 200.0, "completed": int(len(lat)),
+    res = {"sample_epochs_note": "rows x epochs of every row the iterative rule decides (the "
+                                 "reference's work), pruned rows included",
+           "requests": n_req, "rate_per_s": 200.0, "completed": int(len(lat)),
            "failed": len(errs), "p50_latency_s": float(np.percentile(lat, 50)),
            "p99_latency_s": float(np.percentile(lat, 99)), "mean_latency_s": float(lat.mean()),
            "wall_s": wall, "stars_per_s": len(lat) / wall,
