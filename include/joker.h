@@ -292,8 +292,8 @@ int jk_lnl_batch(const jk_batch *batch, int precision, int64_t n_items,
  * exp(ll - max) > u unchanged for all u in [0, 1).
  * With a smaller margin (at least 37), only u == 0 can differ (see
  * jk_reject_batch_zero_check).
- * n_full_out (optional, device uint64) accumulates the number of rows
- * evaluated in full.  Rows are read from the cache directly: work items
+ * n_full_out (optional, device uint64, one entry per table entry): entry k
+ * accumulates the number of star k's rows evaluated in full.  Rows are read from the cache directly: work items
  * address cache rows row_begin .. row_begin + n_rows. */
 int jk_lnl_prune_batch(const jk_batch *batch, const jk_batch *probes, const double *prune,
                        int precision, int64_t n_items, const jk_work_item *items, const double *P,

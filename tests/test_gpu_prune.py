@@ -80,7 +80,7 @@ def test_prune_launch(setup, precision):
     prune = torch.as_tensor(thr.reshape(-1), device="cuda")
     out = torch.full((total,), np.nan, dtype=torch.float64, device="cuda")
     bnd = torch.full((total,), -1.0, dtype=torch.float32, device="cuda") if mixed else None
-    n_full = torch.zeros(1, dtype=torch.int64, device="cuda")
+    n_full = torch.zeros(len(stars), dtype=torch.int64, device="cuda")
     B.lnl_prune_batch(table, probes, prune, cache, d_items, len(items), precision, out=out,
                       bound=bnd, n_full=n_full)
     out = out.cpu().numpy()
@@ -90,7 +90,9 @@ def test_prune_launch(setup, precision):
         bnd = bnd.cpu().numpy()
         assert np.array_equal(bnd[same], b_ref.cpu().numpy()[same])
         assert (bnd[pruned] == 0.0).all()
-    assert int(n_full.item()) == int(same.sum())
+    n_full = n_full.cpu().numpy()
+    for k in range(len(stars)):
+        assert n_full[k] == int(same[off[k]:off[k] + n].sum()), k
     n_pruned = 0
     for k in range(len(stars)):
         sl = slice(off[k], off[k] + n)

@@ -184,8 +184,8 @@ def lnl_prune_batch(batch, probes, prune, cache, d_items, n_items, precision="mi
     entry k is star k's probe star (probe_star), `prune` a device float64
     tensor of (threshold, add) per table entry.  Rows whose probe bound lies
     below their star's threshold get that bound instead of their lnL (and
-    bound 0).  n_full: optional device int64 counter of the rows evaluated
-    in full (accumulated)."""
+    bound 0).  n_full: optional device int64 tensor with an entry per table
+    slot, accumulating each star's rows evaluated in full."""
     lib = _lib.ensure_device(cache.device)
     shape = shape or LaunchShape()
     if precision not in ("f64", "mixed"):
@@ -194,6 +194,8 @@ def lnl_prune_batch(batch, probes, prune, cache, d_items, n_items, precision="mi
         raise ValueError("mixed precision writes a bound; f64 does not")
     if out is None or prune.numel() < 2 * len(batch) or prune.dtype != _torch().float64:
         raise ValueError("out is required and prune needs two float64 entries per table slot")
+    if n_full is not None and (n_full.numel() < len(batch) or n_full.dtype != _torch().int64):
+        raise ValueError("n_full needs an int64 entry per table slot")
     cols = (cache.e, cache.omega, cache.M0, cache.s)
     _lib.check(lib.jk_lnl_prune_batch(batch.handle, probes.handle, _dptr(prune),
                                       1 if precision == "mixed" else 0, int(n_items),

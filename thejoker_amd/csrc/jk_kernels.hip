@@ -1414,8 +1414,8 @@ hipError_t jk_launch_lnl_batch(const jk_batch *b, int precision, int64_t n_items
 // evaluated further; every other row is evaluated as in k_lnl_batch.  Rows
 // are probed a chunk of blockDim at a time and the survivors are gathered in
 // LDS over several chunks, so the full evaluations run on dense waves.  thr = -inf
-// turns the probe off for the star.  n_full (optional) accumulates the rows
-// evaluated in full.
+// turns the probe off for the star.  n_full (optional, one entry per star)
+// accumulates each star's rows evaluated in full.
 // ---------------------------------------------------------------------------
 #define JK_PRUNE_LIST 2048   // survivors held in LDS before their evaluation
 
@@ -1461,7 +1461,8 @@ __global__ __launch_bounds__(256) void k_lnl_prune_batch(const StarDev *__restri
                     ll[w.out_offset + j] = jk_lnl_row<L, PHYS, false>(st, r, P, e_, om_, M0_, s_, s_tab);
                 }
             }
-            if (n_full && threadIdx.x == 0) atomicAdd(n_full, (unsigned long long)w.n_rows);
+            if (n_full && threadIdx.x == 0)
+                atomicAdd(n_full + w.star, (unsigned long long)w.n_rows);
             continue;
         }
         const StarDev &pb = probes[w.star];
@@ -1515,7 +1516,8 @@ __global__ __launch_bounds__(256) void k_lnl_prune_batch(const StarDev *__restri
                 __syncthreads();
             }
         }
-        if (n_full && threadIdx.x == 0 && n_surv) atomicAdd(n_full, (unsigned long long)n_surv);
+        if (n_full && threadIdx.x == 0 && n_surv)
+            atomicAdd(n_full + w.star, (unsigned long long)n_surv);
     }
 }
 

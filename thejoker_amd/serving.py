@@ -32,8 +32,8 @@ import numpy as np
 
 from .batch import LIVE_MARGIN, LIVE_MARGIN_NONZERO_U, LaunchShape, StarBatch, heaviest_first, \
     lnl_batch, lnl_prune_batch, reject_batch, settle_live_batch, upload_items, work_items
-from .device import (DeviceStar, _torch, linear_internal_units, pcg64_state, probe_star,
-                     posterior_inputs_table, star_arrays)
+from .device import (DeviceStar, _torch, linear_internal_units, pcg64_state,
+                     posterior_inputs_table, probe_star, star_arrays)
 from .likelihood_helpers import validate_prepare_data
 from .thejoker import IterativeRejection, TheJoker
 
@@ -137,6 +137,7 @@ class Request:
         self.round_done = 0     # rows of the current round already evaluated
         self.probe = None       # probe star (DeviceStar on a subset of the epochs)
         self.prune_add = None   # its bound's addend; None: the request is not probed
+        self.full_seen = 0      # its slot's full-evaluation counter, last read
 
     @property
     def latency(self):
@@ -232,9 +233,9 @@ class JokerServer:
             self._prune_host = np.zeros((self.max_active, 2), dtype=np.float64)
             self._prune_dev = torch.zeros(2 * self.max_active, dtype=torch.float64,
                                           device=cache.P.device)
-        self._n_full = torch.zeros(1, dtype=torch.int64, device=cache.P.device)
+        self._n_full = torch.zeros(self.max_active, dtype=torch.int64, device=cache.P.device)
         self.rows_probed = 0      # rows of requests whose star was probed
-        self._rows_prune_launches = 0
+        self.rows_pruned = 0      # of those, the rows that kept their bound
         self._ws = torch.zeros(64, dtype=torch.int32, device=cache.P.device)
         self.profile = profile
         self.phase_s = dict.fromkeys(self.PHASES, 0.0)
@@ -298,9 +299,20 @@ class JokerServer:
     def prune_stats(self):
         """(rows probed, rows pruned): the rows of requests whose star was
         probed, and of those the rows that kept their bound instead of a full
-        evaluation.  Synchronises with the device."""
-        pruned = self._rows_prune_launches - int(self._n_full.item())
-        return self.rows_probed, pruned
+        evaluation."""
+        return self.rows_probed, self.rows_pruned
+
+    def _probe_feedback(self, probed):
+        """After a step: count each probed request's pruned rows from the
+        per-slot counters of full evaluations.  (Widening the probe of
+        requests that prune little, n/6 -> n/3 epochs, was measured and
+        did not pay: 89% instead of 86% of the rows pruned, same capacity.)"""
+        full = self._n_full.cpu().numpy()
+        for r, slot, size in probed:
+            d = int(full[slot]) - r.full_seen
+            r.full_seen = int(full[slot])
+            self.rows_probed += int(size)
+            self.rows_pruned += int(size) - d
 
     def _release(self, r):
         if r.region is not None:
@@ -352,6 +364,8 @@ class JokerServer:
                 if self.prune and a["t"].size >= self.prune_min_epochs:
                     r.probe, r.prune_add = probe_star(a, self.jitter_mode, self.device)
                     self._probes.set(r.slot, r.probe)
+                    self._n_full[r.slot] = 0
+                    r.full_seen = 0
             except Exception as exc:       # a bad request fails alone
                 self._fail(r, exc)
                 continue
@@ -501,6 +515,7 @@ class JokerServer:
             finished = self._build_results()
             self._mark("finish")
         thr = self._prune_thresholds(plan)
+        probed = []
         if thr is None:
             lnl_batch(self._table, self.cache, d_items, len(items), 0, self.precision, self.shape,
                       out=self.arena, workspace=self._ws, bound=self.arena_b)
@@ -509,9 +524,8 @@ class JokerServer:
             lnl_prune_batch(self._table, self._probes, self._prune_dev, self.cache, d_items,
                             len(items), self.precision, self.shape, out=self.arena,
                             bound=self.arena_b, workspace=self._ws, n_full=self._n_full)
-            self.rows_probed += int(sum(sizes[k] for k, (r, _, _) in enumerate(plan)
-                                        if np.isfinite(thr[r.slot, 0])))
-            self._rows_prune_launches += int(sizes.sum())
+            probed = [(r, r.slot, int(sizes[k])) for k, (r, _, _) in enumerate(plan)
+                      if np.isfinite(thr[r.slot, 0])]
         lap("lnl_launch")
         if not self.profile:
             finished = self._build_results()      # overlaps the launch above
@@ -590,6 +604,8 @@ class JokerServer:
                 self._release(r)
             else:
                 self._fail(r, r.error)
+        if probed:
+            self._probe_feedback(probed)
         self.steps += 1
         lap("update")
         self._mark("update")
