@@ -78,10 +78,31 @@ def cpu_baseline(star, rows, mu, lam, n_sample):
     t0 = time.perf_counter()
     oracle.batch_lnl(ost, sub, method="reference", n_threads=cores)
     dt = time.perf_counter() - t0
+    # beside it: one thread of the same, and the O(N L^2) Cholesky restatement
+    # on all cores (what the algorithm, not the hardware, is worth)
+    n1 = max(1, n_sample // 8)
+    t0 = time.perf_counter()
+    oracle.batch_lnl(ost, rows[:n1], method="reference", n_threads=1)
+    dt1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.batch_lnl(ost, sub, method="chol", n_threads=cores)
+    dtc = time.perf_counter() - t0
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")),
+                         "")
+    except OSError:
+        pass
     return {"value": n_sample * star.M / dt, "unit": "sample-epochs/s", "cores": cores,
             "kind": "port",
             "sample": f"first {n_sample} of the same prior rows, reference LU+Woodbury algorithm "
-                      f"restated in C (-O3), {cores} threads, {dt:.2f} s wall"}
+                      f"restated in C (-O3), {cores} threads, {dt:.2f} s wall",
+            "single_thread": {"value": n1 * star.M / dt1, "sample": f"first {n1} rows, 1 thread"},
+            "cholesky_all_cores": {"value": n_sample * star.M / dtc,
+                                   "sample": f"first {n_sample} rows, O(N L^2) form, "
+                                             f"{cores} threads"},
+            "host": {"cpus_visible": os.cpu_count(), "cpus_used": cores, "model": model}}
 
 
 def mixed_vs_f64(torch, np, D, dstar, cache, llf, llm, bnd, seed, max_keep=None):
