@@ -932,6 +932,18 @@ __global__ __launch_bounds__(256) void k_uniform(int64_t n, u128 s_base, u128 in
     }
 }
 
+// The rejection test exp(x) > u of likelihood_helpers.py:108 for a uniform u
+// on numpy's grid {k 2^-53} in [0, 1), evaluated exactly as written, but
+// without the exp where it cannot decide anything: for x < -40 and u > 0,
+// exp(x) <= exp(-40) < 4.3e-18 < 2^-53 <= u.  Most rows of a peaked
+// likelihood lie that far below the maximum, so most waves skip the exp.
+__device__ __forceinline__ bool jk_accept(double x, double u)
+{
+    bool keep = false;
+    if (x >= -40.0 || u == 0.0) keep = exp(x) > u;
+    return keep;
+}
+
 __global__ __launch_bounds__(256) void k_reject_flags(const double *__restrict__ ll, int64_t n,
                                                       const double *__restrict__ llmax,
                                                       u128 s_base, u128 inc, u128 AT, u128 DT,
@@ -947,7 +959,7 @@ __global__ __launch_bounds__(256) void k_reject_flags(const double *__restrict__
         bool keep = false;
         if (i < n) {
             const double uu = pcg_to_double(pcg_output(s));
-            keep = exp(ll[i] - mx) > uu;
+            keep = jk_accept(ll[i] - mx, uu);
         }
         const uint64_t word = __ballot(keep);
         if (lane == 0) mask[base >> 6] = word;
@@ -1632,7 +1644,7 @@ __global__ __launch_bounds__(256) void k_seg_flags(const double *__restrict__ ll
         bool keep = false;
         if (i < sg.n) {
             const double uu = pcg_to_double(pcg_output(s));
-            keep = exp(ll[sg.off + i] - mx) > uu;
+            keep = jk_accept(ll[sg.off + i] - mx, uu);
             if (zero_u != nullptr && uu == 0.0) zero_u[blockIdx.y] = 1;
         }
         const uint64_t word = __ballot(keep);
@@ -2016,7 +2028,7 @@ __global__ __launch_bounds__(256) void k_seg_uncertain_flags(const double *__res
             if (live && bf > 0.0f) {
                 const double x = ll[sg.off + i];
                 const double b = fma((double)bf, 1e-6, (double)bf) + 1e-12;
-                keep = (exp(x - b - mx) > uu) != (exp(x + b - mx) > uu);
+                keep = jk_accept(x - b - mx, uu) != jk_accept(x + b - mx, uu);
             }
         }
         const uint64_t word = __ballot(keep);
@@ -2147,11 +2159,11 @@ __global__ __launch_bounds__(256) void k_seg_decide(const double *__restrict__ l
             if (bf > 0.0f) {
                 // a NaN maximum makes both sides false: nothing accepted, nothing open
                 const double b = fma((double)bf, 1e-6, (double)bf) + 1e-12;
-                const bool lo = exp(x - b - mx) > uu, hi = exp(x + b - mx) > uu;
+                const bool lo = jk_accept(x - b - mx, uu), hi = jk_accept(x + b - mx, uu);
                 open = lo != hi;
                 acc = lo && hi;
             } else {
-                acc = exp(x - mx) > uu;
+                acc = jk_accept(x - mx, uu);
             }
         }
         const uint64_t wa = __ballot(acc), wo = __ballot(open);
@@ -2180,7 +2192,7 @@ __global__ __launch_bounds__(256) void k_seg_patch(const SegDev *__restrict__ se
         const int64_t i = idx[sg.idx_off + j];
         const u128 st = pcg_jump_table(sg.s_base, (uint64_t)i, sg.inc);
         const double uu = pcg_to_double(pcg_output(st));
-        if (exp(ll[sg.off + i] - mx) > uu)
+        if (jk_accept(ll[sg.off + i] - mx, uu))
             atomicOr((unsigned long long *)&mask[sg.mask_off + (i >> 6)], 1ull << (i & 63));
     }
 }
