@@ -648,9 +648,10 @@ __device__ __forceinline__ int64_t wave_incl_scan_i64(int64_t v, int lane)
 }
 
 // Dense launch: ll[i] = lnL(row rows[i] or i), one lane per row.  With
-// blockmax != nullptr each workgroup also writes the NaN-propagating maximum
-// of its rows to blockmax[blockIdx.x], so the rejection step need not read
-// the lnL array a second time to find the maximum.  With BOUND (mixed
+// blockmax != nullptr each wave also writes the NaN-propagating maximum of
+// its 64 rows to blockmax[i / 64] ((n + 63) / 64 entries, rounded up to whole
+// workgroups), so the rejection step need not read the lnL array a second
+// time to find the maximum.  With BOUND (mixed
 // precision), bound[i] receives the row's error bound (jk_lnl_row).
 template <int L, bool PHYS, bool MIXED, bool BOUND>
 __global__ __launch_bounds__(256) void k_lnl(const StarDev st, int64_t n,
@@ -665,7 +666,6 @@ __global__ __launch_bounds__(256) void k_lnl(const StarDev st, int64_t n,
                                              float *__restrict__ bound)
 {
     __shared__ double2 s_tab[JK_TAB_LDS];
-    __shared__ double red[4];
     jk_stage_tab(s_tab);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double v = -INFINITY;
@@ -681,11 +681,9 @@ __global__ __launch_bounds__(256) void k_lnl(const StarDev st, int64_t n,
         ll[i] = v;
     }
     if (blockmax) {
+        // one partial per wave: no barrier, so a finished wave retires at once
         v = wave_nanmax(v);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-        __syncthreads();
-        if (threadIdx.x == 0)
-            blockmax[blockIdx.x] = jk_nanmax(jk_nanmax(red[0], red[1]), jk_nanmax(red[2], red[3]));
+        if ((threadIdx.x & 63) == 0) blockmax[i >> 6] = v;
     }
 }
 
@@ -1193,11 +1191,11 @@ hipError_t jk_launch_reject(const double *ll, int64_t n, const double *llmax, u1
 }
 
 // FP64 lnL plus the rejection step of the same rows, with the maximum reduced
-// inside the likelihood launch (one partial per workgroup).
+// inside the likelihood launch (one partial per wave).
 int64_t jk_lnl_reject_scratch_bytes(int64_t n)
 {
-    const int64_t blocks = (n + 255) / 256;
-    return (int64_t)sizeof(double) * (blocks + 1024) +
+    const int64_t waves = 4 * ((n + 255) / 256);
+    return (int64_t)sizeof(double) * (waves + 1024) +
            (int64_t)sizeof(uint64_t) * jk_reject_mask_words(n) +
            (int64_t)sizeof(int64_t) * jk_reject_bsum_count(n);
 }
@@ -1209,9 +1207,9 @@ hipError_t jk_launch_lnl_reject_f64(const jk_star *st, int64_t n, const double *
                                     int64_t *n_pass_out, double *llmax_out, void *scratch,
                                     hipStream_t stream)
 {
-    const int64_t blocks = (n + 255) / 256;
-    double *partials = (double *)scratch;       // one per likelihood workgroup
-    double *partials2 = partials + blocks;      // second level, at most 1024
+    const int64_t waves = 4 * ((n + 255) / 256);
+    double *partials = (double *)scratch;       // one per likelihood wave
+    double *partials2 = partials + waves;       // second level, at most 1024
     uint64_t *mask = (uint64_t *)(partials2 + 1024);
     int64_t *bsum = (int64_t *)(mask + jk_reject_mask_words(n));
     hipError_t err;
@@ -1224,8 +1222,8 @@ hipError_t jk_launch_lnl_reject_f64(const jk_star *st, int64_t n, const double *
         err = launch_lnl_dispatch<false>(st, n, P, e, om, M0, s, rows, ll, partials, nullptr,
                                          stream);
         if (err != hipSuccess) return err;
-        // the per-workgroup maxima reduced like any array (two parallel levels)
-        err = jk_launch_max(partials, blocks, partials2, jk_max_partials_count(blocks), llmax_out,
+        // the per-wave maxima reduced like any array (two parallel levels)
+        err = jk_launch_max(partials, waves, partials2, jk_max_partials_count(waves), llmax_out,
                             stream);
     }
     if (err != hipSuccess) return err;
