@@ -647,11 +647,12 @@ __device__ __forceinline__ int64_t wave_incl_scan_i64(int64_t v, int lane)
     return v;
 }
 
-// Dense launch: ll[i] = lnL(row rows[i] or i), one lane per row.  With
-// blockmax != nullptr each wave also writes the NaN-propagating maximum of
-// its 64 rows to blockmax[i / 64] ((n + 63) / 64 entries, rounded up to whole
-// workgroups), so the rejection step need not read the lnL array a second
-// time to find the maximum.  With BOUND (mixed
+// Dense launch: ll[i] = lnL(row rows[i] or i), one lane per row, the grid
+// (jk_dense_blocks) striding over the rows so that each workgroup stages the
+// sin/cos table once for several rows.  With blockmax != nullptr each wave
+// also writes the NaN-propagating maximum of its rows to blockmax[wave]
+// (4 jk_dense_blocks(n) entries), so the rejection step need not read the
+// lnL array a second time to find the maximum.  With BOUND (mixed
 // precision), bound[i] receives the row's error bound (jk_lnl_row).
 template <int L, bool PHYS, bool MIXED, bool BOUND>
 __global__ __launch_bounds__(256) void k_lnl(const StarDev st, int64_t n,
@@ -667,10 +668,12 @@ __global__ __launch_bounds__(256) void k_lnl(const StarDev st, int64_t n,
 {
     __shared__ double2 s_tab[JK_TAB_LDS];
     jk_stage_tab(s_tab);
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    double v = -INFINITY;
-    if (i < n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    double vmax = -INFINITY;
+    for (int64_t i = t; i < n; i += stride) {
         const int64_t r = rows ? rows[i] : i;
+        double v;
         if constexpr (BOUND) {
             float b;
             v = jk_lnl_row<L, PHYS, MIXED, false, true>(st, r, P, e_, om_, M0_, s_, s_tab, &b);
@@ -679,11 +682,12 @@ __global__ __launch_bounds__(256) void k_lnl(const StarDev st, int64_t n,
             v = jk_lnl_row<L, PHYS, MIXED>(st, r, P, e_, om_, M0_, s_, s_tab);
         }
         ll[i] = v;
+        vmax = jk_nanmax(vmax, v);
     }
     if (blockmax) {
         // one partial per wave: no barrier, so a finished wave retires at once
-        v = wave_nanmax(v);
-        if ((threadIdx.x & 63) == 0) blockmax[i >> 6] = v;
+        vmax = wave_nanmax(vmax);
+        if ((threadIdx.x & 63) == 0) blockmax[t >> 6] = vmax;
     }
 }
 
@@ -773,13 +777,21 @@ __global__ __launch_bounds__(256) void k_lnl_scatter_f64(const StarDev st,
     }
 }
 
+// Workgroups of a dense launch over n rows: one row per lane up to 16384
+// workgroups (2^22 rows), then a stride.
+static int64_t jk_dense_blocks(int64_t n)
+{
+    const int64_t b = (n + 255) / 256;
+    return b < 16384 ? (b < 1 ? 1 : b) : 16384;
+}
+
 template <int L, bool PHYS, bool MIXED>
 static hipError_t launch_lnl_t(const jk_star *st, int64_t n, const double *P, const void *e,
                                const void *om, const void *M0, const void *s,
                                const int64_t *rows, double *ll, double *blockmax, float *bound,
                                hipStream_t stream)
 {
-    const int64_t blocks = (n + 255) / 256;
+    const int64_t blocks = jk_dense_blocks(n);
     if constexpr (MIXED) {
         if (bound) {
             hipLaunchKernelGGL((k_lnl<L, PHYS, true, true>), dim3((unsigned)blocks), dim3(256), 0,
@@ -1194,7 +1206,7 @@ hipError_t jk_launch_reject(const double *ll, int64_t n, const double *llmax, u1
 // inside the likelihood launch (one partial per wave).
 int64_t jk_lnl_reject_scratch_bytes(int64_t n)
 {
-    const int64_t waves = 4 * ((n + 255) / 256);
+    const int64_t waves = 4 * jk_dense_blocks(n);
     return (int64_t)sizeof(double) * (waves + 1024) +
            (int64_t)sizeof(uint64_t) * jk_reject_mask_words(n) +
            (int64_t)sizeof(int64_t) * jk_reject_bsum_count(n);
@@ -1207,7 +1219,7 @@ hipError_t jk_launch_lnl_reject_f64(const jk_star *st, int64_t n, const double *
                                     int64_t *n_pass_out, double *llmax_out, void *scratch,
                                     hipStream_t stream)
 {
-    const int64_t waves = 4 * ((n + 255) / 256);
+    const int64_t waves = 4 * jk_dense_blocks(n);
     double *partials = (double *)scratch;       // one per likelihood wave
     double *partials2 = partials + waves;       // second level, at most 1024
     uint64_t *mask = (uint64_t *)(partials2 + 1024);
