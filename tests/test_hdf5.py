@@ -240,3 +240,12 @@ print('ok')
     assert len(back) == 8000
     np.testing.assert_array_equal(back["P"].value[:5000], js["P"].value)
     np.testing.assert_array_equal(back["P"].value[5000:], np.arange(3000.0))
+
+
+def test_header_numpy_scalars():
+    lines = table_meta.make_header([("P", u.day)], {"t_ref": np.float64(58000.5),
+                                                    "poly_trend": np.int64(2),
+                                                    "n_offsets": np.int32(0),
+                                                    "flag": np.bool_(True)})
+    meta = table_meta.table_meta(table_meta.read_header(lines))
+    assert meta == {"t_ref": 58000.5, "poly_trend": 2, "n_offsets": 0, "flag": True}

@@ -117,6 +117,8 @@ def table_meta(header):
 
 
 def _scalar(v):
+    if hasattr(v, "item") and getattr(v, "ndim", 1) == 0:
+        v = v.item()                      # numpy scalars
     if v is None:
         return "null"
     if isinstance(v, bool):
