@@ -154,3 +154,24 @@ def test_lnl_reject_async_pipelined_equals_sequential(gpu):
     got.append(pending.pop(0).result()[1:])
     for (a, b, c), (x, y, z) in zip(want, got):
         assert np.array_equal(a, x) and b == y and c == z
+
+
+def test_scratch_reuse_across_sizes_and_streams(gpu):
+    """The library keeps each stream's scratch between calls and grows it on
+    demand: rejection steps of growing, then shrinking, sizes on the default
+    stream and on a second stream, interleaved, all equal numpy's decisions."""
+    rng = np.random.default_rng(5)
+    sizes = [1000, 1 << 20, 333, 1 << 22, 64, 1 << 21]
+    side = torch.cuda.Stream()
+    for k, n in enumerate(sizes):
+        ll = rng.normal(0.0, 3.0, n)
+        seed = 100 + k
+        want_u = np.random.default_rng(seed).uniform(size=n)
+        want = np.where(np.exp(ll - ll.max()) > want_u)[0][:500]
+        for stream in (torch.cuda.current_stream(), side):
+            with torch.cuda.stream(stream):
+                d_ll = torch.as_tensor(ll, device="cuda")
+                idx, n_pass, mx = D.reject(d_ll, D.pcg64_state(np.random.default_rng(seed)), 500)
+            stream.synchronize()
+            assert np.array_equal(idx, want), (n, stream)
+            assert mx == ll.max()
