@@ -20,7 +20,7 @@ import numpy as np
 
 from . import _lib
 from . import units as u
-from .samples import NONLINEAR_INTERNAL_UNITS, NONLINEAR_PACKED_ORDER, JokerSamples
+from .samples import NONLINEAR_INTERNAL_UNITS, NONLINEAR_PACKED_ORDER
 
 __all__ = ["DeviceStar", "PriorCache", "star_arrays", "probe_star", "probe_epochs", "lnl", "lnl_reject", "lnl_reject_async",
            "PendingReject", "polish", "reject",

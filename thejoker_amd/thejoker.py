@@ -29,12 +29,11 @@ import os
 import numpy as np
 
 from . import units as u
-from .data import RVData
 from .device import (DeviceStar, PriorCache, linear_internal_units, lnl, lnl_reject,
                      pcg64_advance_state, pcg64_state, reject, to_host)
 from .likelihood_helpers import batch_tasks, next_batch_size, validate_prepare_data
 from .prior import JokerPrior
-from .samples import NONLINEAR_PACKED_ORDER, JokerSamples
+from .samples import JokerSamples
 
 __all__ = ["TheJoker"]
 
