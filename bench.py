@@ -551,6 +551,48 @@ def leg_serving(args, torch, np, D, syn):
 
 
 # ---------------------------------------------------------------------------
+def leg_prior_io(args, torch, np, D, syn):
+    """A prior cache file in the reference's HDF5 layout: 2^24 rows written by
+    JokerSamples.write, then streamed into the device columns by
+    PriorCache.from_hdf5 (the file is in the page cache: this times the
+    format decoding, unit conversion and host-to-device copies)."""
+    import tempfile
+
+    import thejoker_amd as tj
+    from thejoker_amd import units as u
+
+    n = 1 << 24
+    cols = syn.prior_columns(n, seed=6, jitter=True)
+    js = tj.JokerSamples(poly_trend=1)
+    js["P"] = cols["P"] * u.day
+    js["e"] = cols["e"]
+    js["omega"] = cols["omega"] * u.rad
+    js["M0"] = cols["M0"] * u.rad
+    js["s"] = cols["s"] * 1000.0 * u.m / u.s
+    js["ln_prior"] = np.zeros(n)
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "prior.hdf5")
+        t0 = time.perf_counter()
+        js.write(path)
+        t_write = time.perf_counter() - t0
+        size = os.path.getsize(path)
+        D.PriorCache.from_hdf5(path, rv_unit=u.km / u.s, block_rows=1 << 20)   # warm the cache
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        c = D.PriorCache.from_hdf5(path, rv_unit=u.km / u.s)
+        torch.cuda.synchronize()
+        t_read = time.perf_counter() - t0
+    same = bool(np.array_equal(c.P.cpu().numpy(), cols["P"]) and
+                np.allclose(c.s.cpu().numpy(), cols["s"], rtol=1e-15, atol=0))
+    res = {"rows": n, "file_bytes": size, "write_s": t_write, "read_to_device_s": t_read,
+           "read_gb_per_s": size / t_read / 1e9, "columns_equal": same}
+    log(f"[bench] prior_io: {size / 1e9:.2f} GB HDF5 written in {t_write:.2f} s, streamed into "
+        f"HBM in {t_read:.2f} s ({res['read_gb_per_s']:.2f} GB/s), columns equal: {same}")
+    del c
+    torch.cuda.empty_cache()
+    return res
+
+
 def leg_stress(args, torch, np, D, syn):
     log2 = args.stress_log2_rows
     n = 1 << log2
@@ -608,7 +650,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--log2-rows", type=int, default=24)
     ap.add_argument("--workload", default="all",
-                    choices=["all", "single", "batched", "serving", "stress"])
+                    choices=["all", "single", "batched", "serving", "stress", "prior_io"])
     ap.add_argument("--serving-requests", type=int, default=1000)
     ap.add_argument("--serving-active", type=int, default=64)
     ap.add_argument("--serving-step-work", type=float, default=0.0,
@@ -650,7 +692,8 @@ def main():
 
     out = leg_single(args, torch, np, D, syn) if args.workload in ("all", "single") else {
         "metric": METRIC, "value": None, "unit": "sample-epochs/s", "n_gpus": 1}
-    legs = {"batched": leg_batched, "serving": leg_serving, "stress": leg_stress}
+    legs = {"batched": leg_batched, "serving": leg_serving, "stress": leg_stress,
+            "prior_io": leg_prior_io}
     for name, fn in legs.items():
         if args.workload in ("all", name):
             out[name] = fn(args, torch, np, D, syn)

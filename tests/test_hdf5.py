@@ -42,9 +42,11 @@ def test_reader_matches_hdf5_library(expected, fname, key, nchunks):
         got = ds.read()
         for name in ref.dtype.names:
             np.testing.assert_array_equal(got[name], ref[name])
-        # row slices across chunk boundaries, one field
+        # row slices across chunk boundaries, one field; and into a reused buffer
+        buf = np.full(len(ref) * ref.dtype.itemsize, 255, dtype=np.uint8)
         for lo, hi in [(0, 1), (7, 9), (123, 391), (len(ref) - 3, len(ref) + 10), (5, 5)]:
             np.testing.assert_array_equal(ds.read(lo, hi, field="omega"), ref["omega"][lo:hi])
+            np.testing.assert_array_equal(ds.read(lo, hi, buf=buf)["e"], ref["e"][lo:hi])
 
 
 def test_header_strings_fixed_and_variable_length():

@@ -366,9 +366,10 @@ class PriorCache:
             scale = {k: u._as_unit(units.get(k, u.one)).to(want[k])
                      for k in NONLINEAR_PACKED_ORDER if k in names}
             lnp = [] if "ln_prior" in names else None
+            buf = np.empty(min(n, int(block_rows)) * ds.dtype.itemsize, dtype=np.uint8)
             for lo in range(0, n, int(block_rows)):
                 hi = min(n, lo + int(block_rows))
-                block = ds.read(lo, hi)
+                block = ds.read(lo, hi, buf=buf)
                 for k, fac in scale.items():
                     v = np.ascontiguousarray(block[k], dtype=np.float64)
                     if fac != 1.0:

@@ -165,20 +165,29 @@ class Dataset:
     def size(self):
         return int(np.prod(self.shape, dtype=np.int64)) if self.shape else 1
 
-    def _raw(self, start, stop):
+    def _raw(self, start, stop, buf=None):
         """Elements [start, stop) of the flattened dataset in the stored dtype
-        (copied out of the file)."""
+        (copied out of the file; into the uint8 array `buf` when given, so
+        that a caller reading block after block reuses one host buffer)."""
         f, kind = self.file, self._layout[0]
         n = stop - start
         dt = self.dtype
+        if buf is not None and kind in ("contiguous", "chunked") and \
+                (kind == "contiguous" or len(self._layout[1]) == 1):
+            if buf.dtype != np.uint8 or buf.size < n * dt.itemsize:
+                raise ValueError("buf must be a uint8 array of at least the block's bytes")
+            buf = buf[:n * dt.itemsize]
+        else:
+            buf = None
         if kind == "compact":
             return np.frombuffer(self._layout[1], dtype=dt, count=self.size)[start:stop].copy()
         if kind == "contiguous":
             addr = self._layout[1]
             if addr == UNDEF:
                 return np.zeros(n, dtype=dt)
-            return np.frombuffer(f._mm, dtype=dt, count=n,
-                                 offset=f._abs(addr) + start * dt.itemsize).copy()
+            out = np.empty(n * dt.itemsize, dtype=np.uint8) if buf is None else buf
+            f._read_into(f._abs(addr) + start * dt.itemsize, out)
+            return out.view(dt)
         if kind == "implicit":
             if len(self._layout[1]) != 1:
                 raise NotImplementedError("implicit chunk index of rank > 1")
@@ -189,17 +198,28 @@ class Dataset:
             if start != 0 or stop != self.size:
                 raise NotImplementedError("row slices of chunked datasets of rank > 1")
             return self._read_nd_chunked(chunk_shape, chunks)
-        out = np.zeros(n, dtype=dt)
+        # read as bytes, chunk by chunk; rows of chunks never written are 0
+        isz = dt.itemsize
+        out = np.empty(n * isz, dtype=np.uint8) if buf is None else buf
         rows = chunk_shape[0]
+        covered = 0
         for off, addr, nbytes in chunks:
             lo, hi = max(off[0], start), min(off[0] + rows, stop, self.size)
             if lo >= hi:
                 continue
-            if nbytes < rows * dt.itemsize:
+            if nbytes < rows * isz:
                 raise H5FormatError(f"chunk of {nbytes} bytes at {addr} is too small")
-            src = np.frombuffer(f._mm, dtype=dt, count=rows, offset=f._abs(addr))
-            out[lo - start:hi - start] = src[lo - off[0]:hi - off[0]]
-        return out
+            f._read_into(f._abs(addr) + (lo - off[0]) * isz,
+                         out[(lo - start) * isz:(hi - start) * isz])
+            covered += hi - lo
+        if covered < n:
+            hit = np.zeros(n, dtype=bool)
+            for off, _, _ in chunks:
+                lo, hi = max(off[0], start), min(off[0] + rows, stop)
+                if lo < hi:
+                    hit[lo - start:hi - start] = True
+            out.reshape(n, isz)[~hit] = 0
+        return out.view(dt)
 
     def _read_nd_chunked(self, chunk_shape, chunks):
         out = np.zeros(self.shape, dtype=self.dtype)
@@ -210,19 +230,21 @@ class Dataset:
             out[dst] = src[tuple(slice(0, d.stop - d.start) for d in dst)]
         return out.reshape(-1)
 
-    def read(self, start=0, stop=None, field=None):
+    def read(self, start=0, stop=None, field=None, buf=None):
         """Rows [start, stop) (all by default) as a numpy array in native
         byte order; `field` picks one field of a compound dataset.
-        Variable-length strings come back as an object array of bytes."""
+        Variable-length strings come back as an object array of bytes.
+        buf (optional): a uint8 host array the rows are read into (and that
+        the result may be a view of) for 1-D contiguous or chunked data."""
         size = self.size
         stop = size if stop is None else min(int(stop), size)
         start = max(0, min(int(start), stop))
-        raw = self._raw(start, stop)
+        raw = self._raw(start, stop, buf)
         if self._vlen is not None:
             return self.file._vlen_decode(raw, self._vlen)
         if field is not None:
             raw = raw[field]
-        out = raw.astype(raw.dtype.newbyteorder("="), copy=False)
+        out = raw if raw.dtype.isnative else raw.astype(raw.dtype.newbyteorder("="))
         if not self.shape:
             return out.reshape(())
         if len(self.shape) > 1 and start == 0 and stop == size:
@@ -282,6 +304,21 @@ class File:
 
     def _addr(self, pos):
         return _uint(self._mm, pos, self._so)
+
+    def _read_into(self, pos, buf):
+        """Fill the uint8 array buf from absolute file position pos.  Bulk
+        data go through read() rather than the memory map, which would fault
+        in every page separately."""
+        if pos + buf.size > len(self._mm):
+            raise H5FormatError(f"data at {pos} run past the end of the file")
+        self._fh.seek(pos)
+        view = memoryview(buf)
+        got = 0
+        while got < buf.size:
+            k = self._fh.readinto(view[got:])
+            if not k:
+                raise H5FormatError("unexpected end of file")
+            got += k
 
     def _len(self, pos):
         return _uint(self._mm, pos, self._sl)
