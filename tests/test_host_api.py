@@ -62,6 +62,9 @@ def test_trend_columns():
     dt = d.t - d.t_ref
     assert M.shape == (10, 3)
     assert np.allclose(M[:, 0], 1.0) and np.allclose(M[:, 1], dt) and np.allclose(M[:, 2], dt ** 2)
+    # the powers round as np.vander's running products do
+    M5 = get_trend_design_matrix(d, None, 5)
+    assert np.array_equal(M5[:, 1:], np.vander(dt, N=5, increasing=True)[:, 1:])
 
 
 # -- batch_tasks (tests/test_utils.py:21-33) ---------------------------------

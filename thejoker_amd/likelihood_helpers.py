@@ -11,77 +11,85 @@ __all__ = ["get_constant_term_design_matrix", "get_trend_design_matrix", "valida
 
 
 def get_constant_term_design_matrix(data, ids=None):
-    """likelihood_helpers.py:8-25: a column of ones plus one indicator column
-    per extra data source."""
-    if ids is None:
-        ids = np.zeros(len(data), dtype=int)
-    ids = np.array(ids)
-    unq_ids = np.unique(ids)
-    constant_part = np.zeros((len(data), len(unq_ids)))
-    constant_part[:, 0] = 1.0
-    for j, id_ in enumerate(unq_ids[1:]):
-        constant_part[ids == id_, j + 1] = 1.0
-    return constant_part
+    """The constant-velocity columns of the trend design matrix
+    (likelihood_helpers.py:8-25), one per survey: column 0 is 1 on every epoch
+    (v0); the survey labels after the first, in sorted order, each get a 0/1
+    column marking their epochs (that survey's offset from v0)."""
+    n = len(data)
+    labels = np.zeros(n, dtype=int) if ids is None else np.asarray(ids)
+    surveys = np.unique(labels)
+    cols = np.zeros((n, surveys.size))
+    cols[:, 0] = 1.0
+    cols[:, 1:] = labels[:, None] == surveys[None, 1:]
+    return cols
 
 
 def get_trend_design_matrix(data, ids, poly_trend):
-    """likelihood_helpers.py:28-37: [1, offsets..., dt, dt^2, ...] with
-    dt = t - t_ref."""
-    const_M = get_constant_term_design_matrix(data, ids)
-    dt = data._t_bmjd - data._t_ref_bmjd
-    trend_M = np.vander(dt, N=poly_trend, increasing=True)[:, 1:]
-    return np.hstack((const_M, trend_M))
+    """Every linear column but the Keplerian one (likelihood_helpers.py:28-37):
+    the constant-velocity columns above, then dt, dt^2, ... up to
+    dt^(poly_trend - 1) with dt = t - t_ref.  The powers are running products
+    (dt, dt*dt, (dt*dt)*dt, ...), the rounding of np.vander's."""
+    const = get_constant_term_design_matrix(data, ids)
+    dt = np.asarray(data._t_bmjd - data._t_ref_bmjd, dtype=np.float64)
+    k = max(int(poly_trend) - 1, 0)
+    powers = np.cumprod(np.repeat(dt[:, None], k, axis=1), axis=1)
+    return np.concatenate([const, powers], axis=1)
+
+
+def _as_survey_dict(data):
+    """A mapping label -> RVData from a dict-like, or from an iterable
+    (labels 0, 1, ... in order)."""
+    if hasattr(data, "keys"):
+        return data
+    try:
+        return dict(enumerate(data))
+    except Exception as exc:
+        raise TypeError("data must be one RVData, an iterable of RVData objects or a mapping "
+                        f"whose values are RVData objects; got {type(data)}") from exc
 
 
 def validate_prepare_data(data, poly_trend, n_offsets):
-    """data_helpers.py:56-133: one RVData, or a list/dict of them (one per
-    survey), to (RVData, ids, trend_M)."""
+    """(RVData, survey labels, trend design matrix) of the input
+    (data_helpers.py:56-133).  One RVData stands alone (and then takes no v0
+    offsets); several surveys -- a list or a dict of RVData -- are merged into
+    one RVData in the first survey's RV unit, each epoch labelled by its
+    survey's key, and need exactly one v0 offset per survey beyond the first."""
     from .data import RVData
 
     if isinstance(data, RVData):
         if n_offsets != 0:
-            raise ValueError("If sampling over velocity offsets between data sources, you must "
-                             "pass in multiple data sources. To do this, pass in a list of RVData "
-                             "instances or a dictionary with RVData instances as values.")
+            raise ValueError(f"the prior has {n_offsets} v0 offset(s), which need one RVData "
+                             "per survey (a list or dict of RVData), not a single RVData")
         return data, np.zeros(len(data), dtype=int), get_trend_design_matrix(data, None, poly_trend)
 
-    if not hasattr(data, "keys"):
-        try:
-            data = {i: d for i, d in enumerate(data)}
-        except Exception as e:
-            raise TypeError("Failed to parse input data: data must either be an RVData instance, "
-                            "an iterable of RVData instances, or a dictionary with RVData "
-                            f"instances as values. Received: {type(data)}") from e
-
-    rv_unit = None
-    t, rv, err, ids = [], [], [], []
-    for k in data.keys():
-        d = data[k]
+    surveys = _as_survey_dict(data)
+    unit = None
+    parts = {"t": [], "rv": [], "err": [], "ids": []}
+    for key in surveys.keys():
+        d = surveys[key]
         if not isinstance(d, RVData):
-            raise TypeError("All data must be specified as RVData instances: Object at key "
-                            f"'{k}' is a '{type(d)}' instead.")
+            raise TypeError(f"survey '{key}' is a {type(d)}; every survey must be an RVData")
         if d._has_cov:
-            raise NotImplementedError("We currently don't support multi-survey data when a full "
-                                      "covariance matrix is specified.")
-        if rv_unit is None:
-            rv_unit = d.rv.unit
-        t.append(d._t_bmjd)
-        rv.append(d.rv.to_value(rv_unit))
-        err.append(d.rv_err.to_value(rv_unit))
-        ids.append([k] * len(d))
-    t = np.concatenate(t)
-    rv = np.concatenate(rv)
-    err = np.concatenate(err)
-    ids = np.concatenate(ids)
-    if (len(np.unique(ids)) - 1) != n_offsets:
-        raise ValueError("Number of data IDs + 1 must equal the number of priors on constant "
-                         "offsets specified (i.e. v0_offsets)")
-    # As at data_helpers.py:633-636: RVData sorts the concatenated epochs by
-    # time while `ids` keeps the concatenation order, and the offset columns
-    # are built from `ids` as given.  Kept as is for parity; it only matters
-    # when surveys interleave in time.
-    all_data = RVData(t=t, rv=u.Quantity(rv, rv_unit), rv_err=u.Quantity(err, rv_unit))
-    return all_data, ids, get_trend_design_matrix(all_data, ids, poly_trend)
+            raise NotImplementedError("several surveys cannot be combined when one has a full "
+                                      "covariance matrix")
+        unit = d.rv.unit if unit is None else unit
+        parts["t"].append(d._t_bmjd)
+        parts["rv"].append(d.rv.to_value(unit))
+        parts["err"].append(d.rv_err.to_value(unit))
+        parts["ids"].append(np.array([key] * len(d)))
+    ids = np.concatenate(parts["ids"])
+    n_surveys = np.unique(ids).size
+    if n_surveys - 1 != n_offsets:
+        raise ValueError(f"{n_surveys} surveys need {n_surveys - 1} v0 offset(s) in the prior; "
+                         f"it has {n_offsets}")
+    # RVData orders the merged epochs by time while the labels stay in
+    # concatenation order, and the offset columns follow the labels as
+    # given -- as in data_helpers.py:633-636; it matters only when surveys
+    # interleave in time.
+    merged = RVData(t=np.concatenate(parts["t"]),
+                    rv=u.Quantity(np.concatenate(parts["rv"]), unit),
+                    rv_err=u.Quantity(np.concatenate(parts["err"]), unit))
+    return merged, ids, get_trend_design_matrix(merged, ids, poly_trend)
 
 
 def batch_tasks(n_tasks, n_batches, arr=None, args=None, start_idx=0):
