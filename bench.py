@@ -62,9 +62,12 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(star, rows, mu, lam, n_sample):
+def cpu_baseline(star, rows, mu, lam, n_sample, gpu_ll=None):
     """Oracle (reference algorithm) on the host cores over the first n_sample
-    rows; sample-epochs/s."""
+    rows; sample-epochs/s.  gpu_ll: the GPU's FP64 lnL of the same rows (host
+    array), compared with the oracle's values (relative error with |lnL|
+    floored at 1, rows where the oracle's anomaly solve hit its iteration cap
+    excluded)."""
     from oracle import oracle
 
     try:
@@ -76,8 +79,17 @@ def cpu_baseline(star, rows, mu, lam, n_sample):
                             jitter_mode=1)
     sub = rows[:n_sample]
     t0 = time.perf_counter()
-    oracle.batch_lnl(ost, sub, method="reference", n_threads=cores)
+    want, capped = oracle.batch_lnl(ost, sub, method="reference", n_threads=cores,
+                                    return_capped=True)
     dt = time.perf_counter() - t0
+    parity = None
+    if gpu_ll is not None:
+        import numpy as np
+        ok = ~capped
+        rel = np.abs(gpu_ll[:n_sample][ok] - want[ok]) / np.maximum(np.abs(want[ok]), 1.0)
+        parity = {"max_rel_err_vs_gpu": float(rel.max()), "rows_compared": int(ok.sum()),
+                  "rows_capped_excluded": int(capped.sum()),
+                  "tolerance": 1e-10, "within_tolerance": bool(rel.max() <= 1e-10)}
     # beside it: one thread of the same, and the O(N L^2) Cholesky restatement
     # on all cores (what the algorithm, not the hardware, is worth)
     n1 = max(1, n_sample // 8)
@@ -102,7 +114,8 @@ def cpu_baseline(star, rows, mu, lam, n_sample):
             "cholesky_all_cores": {"value": n_sample * star.M / dtc,
                                    "sample": f"first {n_sample} rows, O(N L^2) form, "
                                              f"{cores} threads"},
-            "host": {"cpus_visible": os.cpu_count(), "cpus_used": cores, "model": model}}
+            "host": {"cpus_visible": os.cpu_count(), "cpus_used": cores, "model": model},
+            **(parity or {})}
 
 
 def mixed_vs_f64(torch, np, D, dstar, cache, llf, llm, bnd, seed, max_keep=None):
@@ -246,8 +259,10 @@ def leg_single(args, torch, np, D, syn):
         rows = np.ascontiguousarray(np.stack([cols[k] for k in ("P", "e", "omega", "M0", "s")],
                                              axis=1))
         n_cpu = min(n, 1 << 21)
-        out["cpu_baseline"] = cpu_baseline(star, rows, mu, lam, n_cpu)
-        log(f"[bench] cpu {out['cpu_baseline']['value']:.3e} sample-epochs/s")
+        gpu_ll = ll[:n_cpu].cpu().numpy()
+        out["cpu_baseline"] = cpu_baseline(star, rows, mu, lam, n_cpu, gpu_ll=gpu_ll)
+        log(f"[bench] cpu {out['cpu_baseline']['value']:.3e} sample-epochs/s, GPU lnL vs "
+            f"oracle max rel err {out['cpu_baseline']['max_rel_err_vs_gpu']:.2e}")
     del cache, ll, ll32
     torch.cuda.empty_cache()
     return out

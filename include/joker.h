@@ -64,11 +64,26 @@ const char *jk_status_string(int status);
 int jk_abi_version(void);
 
 /* Select and initialise the device; idempotent.  Replaces nothing in the
- * reference (CPU only); required before any other call on `device`.  Sets the
- * release threshold of the device's default stream-ordered memory pool to
- * 32 GiB, so that the scratch the settling and rejection calls take with
- * hipMallocAsync is reused from call to call. */
+ * reference (CPU only); required before any other call on `device`.
+ *
+ * Scratch.  The rejection, maximum, polishing and settling calls keep their
+ * temporary device memory (ballot words, block counts, partial maxima, FP64
+ * lists) in one buffer per (device, stream), allocated with hipMalloc on
+ * first use and reused by later calls on that stream in stream order.  It only
+ * grows (x1.5, in 2 MiB steps, after synchronising the stream) and is kept
+ * until jk_release_scratch or the end of the process: a server keeps the peak
+ * scratch of every stream it has used.  A call holds its stream's buffer from
+ * its first to its last launch, so host threads calling on one stream queue
+ * their launch sequences one after the other.  Under stream capture the call
+ * takes stream-ordered pool memory instead (hipMallocAsync / hipFreeAsync);
+ * jk_init sets the release threshold of the device's default pool to 32 GiB
+ * so that those blocks are reused. */
 int jk_init(int device);
+
+/* Free every scratch buffer kept for `device` (see jk_init), after waiting
+ * for the work queued on its stream.  Later calls allocate afresh.  No
+ * reference counterpart. */
+int jk_release_scratch(int device);
 
 /* Build the per-star constants.  Replaces CJokerHelper.__init__
  * (fast_likelihood.pyx:125-253) as constructed by TheJoker._make_joker_helper
@@ -184,6 +199,35 @@ int jk_lnl_reject_f64(const jk_star *star, int64_t n, const double *P, const dou
                       uint64_t inc_hi, uint64_t inc_lo, uint64_t stream_offset, int64_t max_keep,
                       int64_t *idx_out, int64_t *n_pass_out, double *llmax_out, void *stream);
 
+/* One prior row in full detail, for tests.  Replaces
+ * CJokerHelper.test_likelihood_worker and the public members it fills
+ * (fast_likelihood.pyx:109-114, 547-576).  row: device array of 5 doubles
+ * (P, e, omega, M0, s).  Device outputs:
+ *   MT_out[L*n]     design matrix M_T, row 0 the unit-K Keplerian column
+ *   Ainv_out[L*L]   Lambda^-1 + M_T C^-1 M_T^T (the reference's Ainv)
+ *   A_out[L*L]      its inverse (A)
+ *   a_out[L]        Ainv^-1 (M_T C^-1 y + Lambda^-1 mu) (a)
+ *   b_out[n]        M mu (b)
+ *   B_out[n*n]      C + M Lambda M^T (B); optional (NULL)
+ *   Binv_out[n*n]   C^-1 - C^-1 M A M^T C^-1 (Binv); optional (NULL)
+ *   scalars_out[4]  lnL, chi2 = (b - y)^T Binv (b - y), ln det(2 pi B),
+ *                   Lambda[0]
+ * C = diag(1/ivar) in the reference jitter mode (as the reference), else
+ * diag(1/ivar + s^2).  Lambda[0] is not clipped to max_K^2 on this path, as in
+ * test_likelihood_worker (:567-569).  lnL is +inf when Ainv is not positive
+ * definite, NaN for an invalid row.  Diagonal-covariance stars only
+ * (JK_EINVAL otherwise).  One lane: not for bulk use. */
+int jk_debug_single(const jk_star *star, const double *row, double *MT_out, double *Ainv_out,
+                    double *A_out, double *a_out, double *b_out, double *B_out, double *Binv_out,
+                    double *scalars_out, void *stream);
+
+/* Per-sample numeric failures of an lnL array ll[0..n) (device):
+ * counts_out[0] = entries at +inf (the L x L system was singular, where the
+ * reference writes INF: fast_likelihood.pyx:283-291, 347-348, 381-382),
+ * counts_out[1] = entries at -inf, counts_out[2] = NaN entries (invalid
+ * prior rows).  counts_out: device, 3 uint64, overwritten. */
+int jk_count_failures(const double *ll, int64_t n, uint64_t *counts_out, void *stream);
+
 /* numpy-identical uniforms u[i] = output offset+i of the PCG64 stream
  * (Generator.uniform, likelihood_helpers.py:107), for tests and callers that
  * want the draws themselves. */
@@ -284,8 +328,12 @@ int jk_lnl_batch(const jk_batch *batch, int precision, int64_t n_items,
  *   lnL(row) <= lnL_S(row) - 1/2 sum_{n not in S} ln(2 pi sigma_n^2).
  * prune holds two doubles per table entry k: (thr_k, add_k).  For a row of
  * star k whose bound lnL_S + add_k (plus, for precision 1, lnL_S's error
- * bound) is finite and below thr_k, ll_out receives the bound (bound_out 0)
- * instead of the row's lnL.  Every other row gets exactly what jk_lnl_batch
+ * bound) is finite and below thr_k, ll_out receives the bound instead of the row's
+ * lnL (bound_out -1: the entry is an upper bound on lnL, not an estimate;
+ * the settling calls re-evaluate such a row in FP64 when it comes within
+ * their margin).  bound_out is required for precision 1; for precision 0 it
+ * is optional and receives only these marks (0 for a row evaluated in FP64,
+ * -1 for an upper bound).  Every other row gets exactly what jk_lnl_batch
  * writes.  thr_k = -inf evaluates all of star k's rows.
  * A caller that sets thr_k = (a value known not to exceed the final maximum)
  * minus a margin of 746 or more leaves every decision
@@ -402,7 +450,9 @@ typedef struct {
  * draws new uniforms for all entries each round.  Of each segment's new
  * entries, the ones with lnL + bound >= lo - margin are re-evaluated in FP64
  * in place (bound := 0), where lo = max(floor, max over the new entries of
- * lnL - bound) is a lower bound on the FP64 maximum.  An entry left in mixed
+ * lnL - bound) is a lower bound on the FP64 maximum.  An entry whose bound is
+ * negative (pruned by jk_lnl_prune_batch: lnL is an upper bound) counts with
+ * bound 0 in the first test and not at all in lo.  An entry left in mixed
  * precision then lies more than margin below the maximum in both precisions,
  * so exp(lnL - max) is 0 and the entry is rejected whatever the uniform;
  * after a call on every new entry, jk_reject_batch over [0, n) accepts
@@ -411,12 +461,48 @@ typedef struct {
  * covers every uniform but exactly 0 (exp(-40) < 2^-53, the smallest
  * positive uniform): jk_reject_batch_zero_check reports that case, and the
  * caller then settles the segment with margin 750 from first = 0 and repeats
- * the rejection step.  Work is proportional to the new entries only.  n_settled_out[i] (device, int64) receives segment
- * i's count.  segs is a HOST array.  No reference counterpart. */
+ * the rejection step.  Work is proportional to the new entries only.
+ * n_settled_out[i] (device, int64) receives segment i's count.  segs is a HOST array.  No reference counterpart. */
 int jk_settle_live_batch(const jk_batch *batch, int n_seg, const jk_live_segment *segs,
                          const double *P, const double *e, const double *omega, const double *M0,
                          const double *s, const int64_t *rows, double *ll_inout,
                          float *bound_inout, double margin, int64_t *n_settled_out, void *stream);
+
+/* Settle a mixed-precision lnL buffer to a tolerance: of each segment's
+ * entries [first, n) (jk_live_segment; floor unused), every one whose bound
+ * exceeds tol, is NaN, or is negative (an upper bound left by
+ * jk_lnl_prune_batch) is re-evaluated in FP64 in place (bound := 0).
+ * Afterwards |lnL - lnL(FP64)| <= tol on every entry, whatever its distance
+ * from the maximum: the form for lnL arrays handed back to a caller (the
+ * reference returns FP64 values, thejoker.py:93-138, 252-257).  tol >= 0.
+ * n_settled_out[i] (device) receives segment i's count.  segs is a HOST
+ * array.  No reference counterpart (the reference has one precision). */
+int jk_settle_tol_batch(const jk_batch *batch, int n_seg, const jk_live_segment *segs,
+                        const double *P, const double *e, const double *omega, const double *M0,
+                        const double *s, const int64_t *rows, double *ll_inout,
+                        float *bound_inout, float tol, int64_t *n_settled_out, void *stream);
+
+/* Settle a buffer of values and upper bounds: a jk_lnl_prune_batch launch
+ * with threshold +inf leaves, for every probed row, an upper bound on its lnL
+ * (bound -1); this call evaluates in FP64 (in place, bound := 0) the rows that
+ * can matter, for each segment's entries [first, n) (jk_live_segment, floor
+ * unused):
+ *   1. with hi = max of ll + max(bound, 0), an upper bound on the FP64
+ *      maximum, every upper-bound row with ll >= hi - window;
+ *   2. with lo = max over the value rows (bound >= 0) of ll - bound, a lower
+ *      bound on the FP64 maximum, every upper-bound row with ll >= lo - margin.
+ * Afterwards every upper-bound row lies more than margin below the maximum,
+ * so (margin >= 40) a rejection step decides it like its FP64 value for
+ * every uniform but 0 -- which jk_settle_reject_batch handles by evaluating
+ * such a row, as any row its bound leaves open.  n_settled_out[i] (device)
+ * counts segment i's evaluated rows.  segs is a HOST array.  No reference
+ * counterpart: the same rejection decisions as fast_likelihood.pyx:428-469
+ * plus likelihood_helpers.py:107-109 with most rows never evaluated in full. */
+int jk_settle_pruned_batch(const jk_batch *batch, int n_seg, const jk_live_segment *segs,
+                           const double *P, const double *e, const double *omega,
+                           const double *M0, const double *s, const int64_t *rows,
+                           double *ll_inout, float *bound_inout, double window, double margin,
+                           int64_t *n_settled_out, void *stream);
 
 /* A segment plan: the descriptors and scratch of jk_settle_batch and
  * jk_reject_batch for a fixed layout of segments (jk_settle_segment: offsets,

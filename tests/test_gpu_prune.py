@@ -79,7 +79,7 @@ def test_prune_launch(setup, precision):
         thr[k] = (-np.inf if k % 3 == 2 else np.nanmax(seg) - 40.0, adds[k])
     prune = torch.as_tensor(thr.reshape(-1), device="cuda")
     out = torch.full((total,), np.nan, dtype=torch.float64, device="cuda")
-    bnd = torch.full((total,), -1.0, dtype=torch.float32, device="cuda") if mixed else None
+    bnd = torch.full((total,), 7.0, dtype=torch.float32, device="cuda") if mixed else None
     n_full = torch.zeros(len(stars), dtype=torch.int64, device="cuda")
     B.lnl_prune_batch(table, probes, prune, cache, d_items, len(items), precision, out=out,
                       bound=bnd, n_full=n_full)
@@ -89,7 +89,8 @@ def test_prune_launch(setup, precision):
     if mixed:
         bnd = bnd.cpu().numpy()
         assert np.array_equal(bnd[same], b_ref.cpu().numpy()[same])
-        assert (bnd[pruned] == 0.0).all()
+        # a pruned row's value is an upper bound on its lnL, so marked
+        assert (bnd[pruned] == -1.0).all()
     n_full = n_full.cpu().numpy()
     for k in range(len(stars)):
         assert n_full[k] == int(same[off[k]:off[k] + n].sum()), k
@@ -106,6 +107,61 @@ def test_prune_launch(setup, precision):
         # every row within the margin of the maximum is evaluated in full
         assert not p[ref64[sl] >= thr[k, 0]].any()
     assert n_pruned > len(cache)
+
+
+def test_zero_uniform_on_a_pruned_row(setup):
+    """A pruned row whose upper bound lies within 745 of the maximum but whose
+    FP64 lnL does not: with a uniform of exactly 0 at that row, the FP64 path
+    rejects it (exp(lnL - max) underflows to 0).  The 40-margin settling
+    leaves pruned rows alone; the 750-margin settling that follows a zero
+    uniform re-evaluates every pruned row within reach, so the rejection step
+    then decides like FP64 (likelihood_helpers.py:107-109)."""
+    from tests.test_serving_host import zero_uniform_rng
+    stars, cache = setup
+    n = len(cache)
+    for st in stars:
+        table, probes, adds = _tables([st])
+        items, off, total = B.work_items([n], 1000)
+        d_items = B.upload_items(items)
+        ref64 = B.lnl_batch(table, cache, d_items, len(items), total, "f64").cpu().numpy()
+        mx = np.nanmax(ref64)
+        prune = torch.as_tensor([mx - 40.0, adds[0]], device="cuda")
+        out = torch.full((total,), np.nan, dtype=torch.float64, device="cuda")
+        bnd = torch.full((total,), 7.0, dtype=torch.float32, device="cuda")
+        B.lnl_prune_batch(table, probes, prune, cache, d_items, len(items), "mixed", out=out,
+                          bound=bnd)
+        o, b = out.cpu().numpy(), bnd.cpu().numpy()
+        cand = np.flatnonzero((b == -1.0) & (o > mx - 700.0) & (ref64 < mx - 800.0))
+        if len(cand):
+            break
+    assert len(cand), "no pruned row with a loose enough bound in the test stars"
+    j = int(cand[0])
+    # settling with the 40 margin, as a round after the first does: the
+    # pruned rows keep their marks
+    cnt = B.settle_live_batch(table, cache, out, bnd, [0], [n], [0], [mx],
+                              margin=B.LIVE_MARGIN_NONZERO_U)
+    b40 = bnd.cpu().numpy()
+    assert (b40[b == -1.0] == -1.0).all()
+    # a uniform of exactly 0 at row j; before settling for it, the stored
+    # bound would accept the row
+    rng = zero_uniform_rng(j)
+    st = D.pcg64_state(rng)
+    u_host = zero_uniform_rng(j).uniform(size=n)
+    assert u_host[j] == 0.0
+    want = np.flatnonzero(np.exp(ref64 - mx) > u_host)
+    got, _, _, zero = B.reject_batch(out, [0], [n], [st], n, zero_check=True)
+    assert zero[0] == 1
+    assert j in got[0] and j not in want
+    # the zero-uniform path: settle every row for every uniform, decide again
+    cnt = B.settle_live_batch(table, cache, out, bnd, [0], [n], [0], [-np.inf], margin=750.0)
+    o2, b2 = out.cpu().numpy(), bnd.cpu().numpy()
+    assert b2[j] == 0.0 and o2[j] == ref64[j]
+    left = b2 == -1.0
+    assert (o2[left] < mx - 750.0).all() and (ref64[left] <= o2[left]).all()
+    assert int(cnt.cpu()[0]) >= int(((b == -1.0) & (o > mx - 750.0)).sum())
+    got, n_pass, llmax = B.reject_batch(out, [0], [n], [st], n)
+    assert llmax[0] == mx
+    assert np.array_equal(got[0], want) and n_pass[0] == len(want)
 
 
 def test_prune_argument_checks(setup):
@@ -185,3 +241,53 @@ def test_server_prune_zero_uniform_in_a_later_round(setup):
     alone = joker.iterative_rejection_sample(data, cache, n_requested_samples=16, growth_factor=8)
     assert np.array_equal(r.result["P"].value, alone["P"].value)
     assert np.array_equal(r.result["K"].value, alone["K"].value)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "f64"])
+@pytest.mark.parametrize("M", [16, 50])
+def test_pruned_rejection_sample_equals_plain(gpu, precision, M):
+    """TheJoker(prune=True).rejection_sample: every row bounded by the probe,
+    the rows within reach of the maximum evaluated in FP64 -- the samples
+    equal the unpruned run's, also when a uniform of exactly 0 falls on a row
+    the probe pruned (that row is then evaluated)."""
+    from tests.test_serving_host import zero_uniform_rng
+    star = syn.single_star(seed=5, M=M, baseline=1000.0 if M == 16 else 3650.0)
+    cols = syn.prior_columns(1 << 20, seed=M, jitter=True)
+    cache = D.PriorCache(cols["P"], cols["e"], cols["omega"], cols["M0"], cols["s"])
+    prior = tj.JokerPrior.default(P_min=2 * u.day, P_max=2048 * u.day, sigma_v=100 * u.km / u.s,
+                                  pars={"K": tj.distributions.Normal(0.0, 30.0, unit=u.km / u.s),
+                                        "s": tj.distributions.LogNormal(-3.0, 1.0,
+                                                                        unit=u.km / u.s)})
+    data = star.to_rvdata()
+    # a row far below the maximum whose probe bound is below it too
+    ll64 = tj.TheJoker(prior, precision="f64").marginal_ln_likelihood(data, cache)
+    far = int(np.flatnonzero(ll64 < ll64.max() - 2000.0)[7])
+    for rng_of in (lambda: np.random.default_rng(42), lambda: zero_uniform_rng(far)):
+        ref, ll_ref = tj.TheJoker(prior, rng=rng_of(), precision="f64").rejection_sample(
+            data, cache, max_posterior_samples=64, in_memory=True, return_all_logprobs=True)
+        joker = tj.TheJoker(prior, rng=rng_of(), precision=precision, prune=True)
+        got, ll_got = joker.rejection_sample(data, cache, max_posterior_samples=64,
+                                             in_memory=True, return_all_logprobs=True)
+        assert len(got) == len(ref) > 0
+        for k in ("P", "e", "K", "v0"):
+            assert np.array_equal(got[k].value, ref[k].value), k
+        tol = 0.0 if precision == "f64" else 1e-3
+        assert np.abs(ll_got - ll_ref).max() <= tol
+
+
+def test_pruned_rejection_sample_many_equals_plain(gpu):
+    stars = syn.many_stars(24, seed=3, M_range=(8, 64))
+    cols = syn.prior_columns(1 << 18, seed=44, jitter=True)
+    cache = D.PriorCache(cols["P"], cols["e"], cols["omega"], cols["M0"], cols["s"])
+    prior = _prior()
+    datas = [s.to_rvdata() for s in stars]
+    for precision in ("mixed", "f64"):
+        res = {}
+        for prune in (False, True):
+            rngs = [np.random.default_rng(300 + i) for i in range(len(datas))]
+            res[prune] = B.rejection_sample_many(datas, prior, cache, rngs,
+                                                 max_posterior_samples=32, precision=precision,
+                                                 prune=prune)
+        for a, b in zip(res[False], res[True]):
+            for k in ("P", "K", "v1"):
+                assert np.array_equal(a[k].value, b[k].value), k

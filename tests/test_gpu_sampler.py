@@ -220,3 +220,39 @@ def test_iterative_prune_changes_no_result(gpu, precision, in_memory):
     assert len(a) == len(b) > 0 and ua == ub
     for k in ("P", "e", "K", "v0", "ln_likelihood"):
         assert np.array_equal(a[k].value, b[k].value), k
+
+
+def _prior_l3():
+    return tj.JokerPrior.default(
+        P_min=2 * u.day, P_max=2048 * u.day,
+        sigma_v=[100 * u.km / u.s, 0.5 * u.km / u.s / u.day], poly_trend=2,
+        pars={"K": tj.distributions.Normal(0.0, 30.0, unit=u.km / u.s),
+              "s": tj.distributions.LogNormal(-3.0, 1.0, unit=u.km / u.s)})
+
+
+def test_mixed_api_arrays_meet_the_tolerance(gpu):
+    """Mixed precision: every value of the lnL arrays the API returns
+    (marginal_ln_likelihood, rejection_sample's return_all_logprobs) is
+    within 1e-3 of FP64 -- also on badly fitting rows far below the maximum,
+    where the raw mixed kernel's error reaches several units."""
+    from thejoker_amd import device as D
+    stars = syn.many_stars(12, seed=3)
+    samples, rows = _samples(1 << 17, seed=21, jitter=True)
+    raw_worst = 0.0
+    for st in stars:
+        data = st.to_rvdata()
+        ll64 = tj.TheJoker(_prior_l3(), precision="f64").marginal_ln_likelihood(data, samples)
+        llm = tj.TheJoker(_prior_l3(), precision="mixed").marginal_ln_likelihood(data, samples)
+        ok = np.isfinite(ll64)
+        assert np.array_equal(ok, np.isfinite(llm))
+        assert np.abs(llm[ok] - ll64[ok]).max() <= 1e-3
+        joker = tj.TheJoker(_prior_l3(), precision="mixed", rng=np.random.default_rng(1))
+        _, lls = joker.rejection_sample(data, samples, max_posterior_samples=64,
+                                        return_all_logprobs=True, in_memory=True)
+        assert np.abs(lls[ok] - ll64[ok]).max() <= 1e-3
+        # the raw kernel, for comparison: the gate had work to do
+        dstar = joker._make_joker_helper(data)
+        cache = D.PriorCache.from_packed(rows)
+        raw = D.lnl(dstar, cache, precision="mixed").cpu().numpy()
+        raw_worst = max(raw_worst, float(np.abs(raw[ok] - ll64[ok]).max()))
+    assert raw_worst > 1e-3

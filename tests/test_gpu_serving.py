@@ -2,6 +2,7 @@
 is the same alone (TheJoker on that star) and in the mix."""
 import numpy as np
 import pytest
+import torch
 
 import thejoker_amd as tj
 from thejoker_amd import batch as B
@@ -175,3 +176,87 @@ def test_nan_rows_fail_like_the_single_star_path(in_memory):
                                              growth_factor=8, in_memory=in_memory)
         assert isinstance(r.error, RuntimeError)
         assert str(r.error) == str(exc.value)
+
+
+def _reference_iterative(ll_all, rng, n_requested, safety, n_total, growth_factor=128):
+    """The loop of multiproc_helpers.py:356-403 (safety_factor 4) over lnL
+    computed beforehand (row order = evaluation order)."""
+    n_process = growth_factor * n_requested
+    got = np.zeros(0)
+    start = 0
+    for _ in range(128):
+        stop = min(start + n_process, n_total)
+        got = np.concatenate([got, ll_all[start:stop]])
+        uu = rng.uniform(size=len(got))
+        good = np.where(np.exp(got - got.max()) > uu)[0]
+        if len(good) == 0:
+            raise RuntimeError("no sample accepted")
+        if len(good) >= n_requested:
+            break
+        start += n_process
+        n_process = int(safety * (n_requested - len(good)) / len(good) * len(got))
+        if start + n_process > n_total:
+            n_process = n_total - start
+        if n_process <= 0:
+            break
+    return good[:n_requested], len(got)
+
+
+def test_serving_workload_against_fp64_alone_and_oracle(gpu):
+    """The serving workload: Poisson arrivals at 200 stars/s, M log-uniform
+    on [3, 200] (a star with M >= 190 included), a 2^26-row
+    device cache, iterative_rejection_sample to 256 samples per star, mixed
+    precision with pruning.  Every served result equals the same star's plain
+    FP64 run alone (0 flips), and for three stars -- the largest M among them
+    -- the accepted rows equal the reference's iterative loop over the
+    oracle's lnL (likelihood_helpers.py:130-229 / multiproc_helpers.py:289-427)."""
+    from oracle import oracle
+    from thejoker_amd.likelihood_helpers import validate_prepare_data
+
+    cache = D.PriorCache.generate(1 << 26, seed=0, ln_s=(-3.0, 1.0))
+    times, stars = syn.arrivals(200, rate=200.0, seed=4)
+    # the largest constant-velocity star of the stream (odd index): its
+    # broad likelihood keeps its oracle check to a few rounds of rows
+    big = max(range(1, len(stars), 2), key=lambda i: stars[i].M)
+    assert stars[big].M >= 190
+    pick = list(range(16)) + ([big] if big >= 16 else [])
+    datas = [stars[i].to_rvdata() for i in pick]
+    t = np.asarray(times)[pick]
+    t = t - t[0]
+    prior = _prior()
+    seeds = [9000 + i for i in pick]
+    srv = S.JokerServer(prior, cache, n_requested=256, precision="mixed", max_active=16,
+                        hbm_budget_bytes=16 << 30)
+    done = S.run_poisson(srv, t, datas, [np.random.default_rng(s) for s in seeds])
+    assert len(done) == len(datas) and srv.steps > 1
+    evals = {}
+    for k, r in enumerate(done):
+        assert r.error is None, (k, r.error)
+        alone = tj.TheJoker(prior, rng=np.random.default_rng(seeds[k]), precision="f64")
+        a = alone.iterative_rejection_sample(datas[k], cache, n_requested_samples=256)
+        for name in ("P", "e", "K", "v0", "v1"):
+            assert np.array_equal(r.result[name].value, a[name].value), (k, name)
+        evals[k] = r.it.n_evals
+    big_k = pick.index(big)
+    small = sorted((k for k in evals if k != big_k), key=lambda k: evals[k])[:2]
+    for k in [big_k] + small:
+        data, _, trend = validate_prepare_data(datas[k], prior.poly_trend, prior.n_offsets)
+        ost = oracle.OracleStar(data._t_bmjd, data.rv.value, data.ivar_value(), trend,
+                                np.zeros(3), np.array([900.0, 1e4, 0.25]), data._t_ref_bmjd,
+                                jitter_mode=1)
+        n = evals[k]
+        assert n <= 1 << 22, (k, n)
+        rows = cache.host_rows(np.arange(n))
+        ll, capped = oracle.batch_lnl(ost, rows, method="chol", n_threads=16,
+                                      return_capped=True)
+        # rows where the reference's anomaly iteration hits its cap are
+        # parity-unpinned: the device's FP64 value stands there
+        if capped.any():
+            dstar = D.DeviceStar.from_data_prior(data, prior, trend)
+            r_cap = torch.as_tensor(np.flatnonzero(capped), device="cuda")
+            ll[capped] = D.lnl(dstar, cache, rows=r_cap, precision="f64").cpu().numpy()
+        good, n_used = _reference_iterative(ll, np.random.default_rng(seeds[k]), 256, 4,
+                                            len(cache))
+        assert n_used == n, (k, n_used, n)
+        P = cache.host_rows(good)[:, 0]
+        assert np.array_equal(done[k].result["P"].value, P), k

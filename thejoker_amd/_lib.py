@@ -33,6 +33,7 @@ SIGNATURES = [
     ("jk_status_string", ctypes.c_char_p, [ctypes.c_int]),
     ("jk_abi_version", ctypes.c_int, []),
     ("jk_init", ctypes.c_int, [ctypes.c_int]),
+    ("jk_release_scratch", ctypes.c_int, [ctypes.c_int]),
     ("jk_star_create", ctypes.c_int,
      [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int,
       ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.POINTER(_vp)]),
@@ -50,6 +51,8 @@ SIGNATURES = [
     ("jk_lnl_reject_f64", ctypes.c_int,
      [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _u64, _u64, _u64, _u64, _i64, _vp, _vp,
       _vp, _vp]),
+    ("jk_debug_single", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("jk_count_failures", ctypes.c_int, [_vp, _i64, _vp, _vp]),
     ("jk_reject_with_max", ctypes.c_int,
      [_vp, _i64, _vp, _u64, _u64, _u64, _u64, _u64, _i64, _vp, _vp, _vp]),
     ("jk_pcg64_uniform", ctypes.c_int, [_u64, _u64, _u64, _u64, _u64, _i64, _vp, _vp]),
@@ -75,6 +78,11 @@ SIGNATURES = [
       _vp]),
     ("jk_settle_live_batch", ctypes.c_int,
      [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _vp, _vp]),
+    ("jk_settle_tol_batch", ctypes.c_int,
+     [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_float, _vp, _vp]),
+    ("jk_settle_pruned_batch", ctypes.c_int,
+     [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double,
+      ctypes.c_double, _vp, _vp]),
     ("jk_seg_plan_create", ctypes.c_int,
      [_vp, ctypes.c_int, _vp, _i64, ctypes.POINTER(_vp)]),
     ("jk_seg_plan_destroy", ctypes.c_int, [_vp]),

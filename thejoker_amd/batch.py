@@ -36,6 +36,11 @@ LIVE_SEGMENT_DTYPE = np.dtype([("offset", "<i8"), ("n", "<i8"), ("first", "<i8")
 # 0 below -745.14), and every uniform but exactly 0 (exp(-40) < 2^-53).
 LIVE_MARGIN = 750.0
 LIVE_MARGIN_NONZERO_U = 40.0
+# |lnL - lnL(FP64)| every mixed-precision lnL handed back to a caller meets
+API_TOL = 1e-3
+# Stars with fewer epochs are not probed: the probe (at least 6 epochs) would
+# cost about as much as the evaluation it saves.
+PRUNE_MIN_EPOCHS = 12
 SETTLE_SEGMENT_DTYPE = np.dtype([("offset", "<i8"), ("n", "<i8"), ("star", "<i4"),
                                  ("reserved", "<i4"), ("row_begin", "<i8"), ("state_hi", "<u8"),
                                  ("state_lo", "<u8"), ("inc_hi", "<u8"), ("inc_lo", "<u8"),
@@ -183,15 +188,17 @@ def lnl_prune_batch(batch, probes, prune, cache, d_items, n_items, precision="mi
     """lnl_batch with pruning (jk_lnl_prune_batch): `probes` is a table whose
     entry k is star k's probe star (probe_star), `prune` a device float64
     tensor of (threshold, add) per table entry.  Rows whose probe bound lies
-    below their star's threshold get that bound instead of their lnL (and
-    bound 0).  n_full: optional device int64 tensor with an entry per table
+    below their star's threshold get that bound instead of their lnL, marked
+    by bound -1.  `bound` is required in mixed precision; in f64 it is
+    optional and receives only the marks (0: an FP64 value, -1: an upper
+    bound).  n_full: optional device int64 tensor with an entry per table
     slot, accumulating each star's rows evaluated in full."""
     lib = _lib.ensure_device(cache.device)
     shape = shape or LaunchShape()
     if precision not in ("f64", "mixed"):
         raise ValueError("precision must be 'f64' or 'mixed'")
-    if (precision == "mixed") != (bound is not None):
-        raise ValueError("mixed precision writes a bound; f64 does not")
+    if precision == "mixed" and bound is None:
+        raise ValueError("mixed precision writes a bound: pass one")
     if out is None or prune.numel() < 2 * len(batch) or prune.dtype != _torch().float64:
         raise ValueError("out is required and prune needs two float64 entries per table slot")
     if n_full is not None and (n_full.numel() < len(batch) or n_full.dtype != _torch().int64):
@@ -370,6 +377,102 @@ def settle_live_batch(batch, cache, ll, bound, offsets, sizes, firsts, floors, s
     return count
 
 
+def settle_tol_batch(batch, cache, ll, bound, offsets, sizes, firsts=None, stars=None,
+                     row_begins=None, rows=None, device=0, tol=API_TOL):
+    """jk_settle_tol_batch: re-evaluate in FP64 every entry [firsts[i],
+    sizes[i]) of each segment whose error bound exceeds tol (or is an upper
+    bound), so that every entry is within tol of FP64.  Returns a device int64
+    tensor of per-segment counts."""
+    torch = _torch()
+    lib = _lib.ensure_device(device)
+    n_seg = len(sizes)
+    segs = np.zeros(n_seg, dtype=LIVE_SEGMENT_DTYPE)
+    segs["offset"] = offsets
+    segs["n"] = sizes
+    segs["first"] = 0 if firsts is None else firsts
+    segs["star"] = np.arange(n_seg) if stars is None else stars
+    segs["row_begin"] = 0 if row_begins is None else row_begins
+    count = torch.empty(n_seg, dtype=torch.int64, device=ll.device)
+    _lib.check(lib.jk_settle_tol_batch(batch.handle, n_seg, segs.ctypes.data_as(ctypes.c_void_p),
+                                       _dptr(cache.P), _dptr(cache.e), _dptr(cache.omega),
+                                       _dptr(cache.M0), _dptr(cache.s), _dptr(rows), _dptr(ll),
+                                       _dptr(bound), float(tol), _dptr(count), _stream(device)),
+               "jk_settle_tol_batch")
+    return count
+
+
+def settle_pruned_batch(batch, cache, ll, bound, offsets, sizes, stars=None, row_begins=None,
+                        device=0, window=LIVE_MARGIN_NONZERO_U, margin=LIVE_MARGIN_NONZERO_U):
+    """jk_settle_pruned_batch: in segments whose entries are values or upper
+    bounds (bound -1, from a probe launch), evaluate in FP64 the upper-bound
+    rows within `window` of the segment's upper bound on the maximum, then
+    those within `margin` of the lower bound that leaves.  Returns a device
+    int64 tensor of per-segment counts."""
+    torch = _torch()
+    lib = _lib.ensure_device(device)
+    n_seg = len(sizes)
+    segs = np.zeros(n_seg, dtype=LIVE_SEGMENT_DTYPE)
+    segs["offset"] = offsets
+    segs["n"] = sizes
+    segs["first"] = 0
+    segs["star"] = np.arange(n_seg) if stars is None else stars
+    segs["row_begin"] = 0 if row_begins is None else row_begins
+    count = torch.empty(n_seg, dtype=torch.int64, device=ll.device)
+    _lib.check(lib.jk_settle_pruned_batch(batch.handle, n_seg,
+                                          segs.ctypes.data_as(ctypes.c_void_p), _dptr(cache.P),
+                                          _dptr(cache.e), _dptr(cache.omega), _dptr(cache.M0),
+                                          _dptr(cache.s), _dptr(None), _dptr(ll), _dptr(bound),
+                                          float(window), float(margin), _dptr(count),
+                                          _stream(device)),
+               "jk_settle_pruned_batch")
+    return count
+
+
+def probe_table(stars, jitter_mode="physical", device=0, min_epochs=PRUNE_MIN_EPOCHS):
+    """(probe StarBatch, add per star, probed mask): entry k is star k's
+    probe (device.probe_star) when it has at least min_epochs epochs and a
+    diagonal noise covariance; other stars are not probed (their entry is the
+    star itself, never read as a probe)."""
+    from .device import probe_star
+    probes, adds, probed = [], [], []
+    for st in stars:
+        ok = st.cov is None and st.n >= min_epochs
+        if ok:
+            pr, add = probe_star(st.arrays(), jitter_mode, device)
+        else:
+            pr, add = st, 0.0
+        probes.append(pr)
+        adds.append(add)
+        probed.append(ok)
+    return StarBatch(probes, device=device), np.asarray(adds), np.asarray(probed)
+
+
+def prune_reject_batch(batch, probes, adds, probed, cache, d_items, n_items, offsets, sizes,
+                       rng_states, max_keep, precision="mixed", shape=None, device=0):
+    """Likelihood and rejection step of many stars (rows [0, sizes[k]) of the
+    cache for table entry k) with the rows bounded by a probe: a probe launch
+    leaves every probed star's rows as upper bounds, the rows that can matter
+    are evaluated in FP64 around the maximum (settle_pruned_batch), and the
+    settled rejection step decides every row like the FP64 path would, for
+    every uniform.  Returns (ll, bound, settle_reject_batch's result, rows
+    evaluated per star by the settling)."""
+    torch = _torch()
+    total = int(offsets[-1] + sizes[-1]) if len(sizes) else 0
+    dev = cache.P.device
+    prune = np.zeros((len(batch), 2))
+    prune[:, 0] = np.where(probed, np.inf, -np.inf)
+    prune[:, 1] = np.where(probed, adds, 0.0)
+    d_prune = torch.as_tensor(prune.reshape(-1), device=dev)
+    ll = torch.empty(total, dtype=torch.float64, device=dev)
+    bnd = torch.empty(total, dtype=torch.float32, device=dev)
+    lnl_prune_batch(batch, probes, d_prune, cache, d_items, n_items, precision, shape, out=ll,
+                    bound=bnd)
+    cnt = settle_pruned_batch(batch, cache, ll, bnd, offsets, sizes, device=device)
+    res = settle_reject_batch(batch, cache, ll, bnd, offsets, sizes, rng_states, max_keep,
+                              device=device)
+    return ll, bnd, res, cnt
+
+
 def autotune(batch, cache, n_rows, precision="mixed", candidates=None, sample_stars=64,
              reps=3, device=0):
     """Time candidate launch shapes on a subset of the workload (the first
@@ -406,14 +509,18 @@ def autotune(batch, cache, n_rows, precision="mixed", candidates=None, sample_st
 
 def rejection_sample_many(datas, prior, prior_samples, rngs, max_posterior_samples=256,
                           n_linear_samples=1, precision="mixed", jitter_mode="physical",
-                          shape=None, device=0, return_logprobs=False, return_all_logprobs=False):
+                          shape=None, device=0, return_logprobs=False, return_all_logprobs=False,
+                          prune=False):
     """TheJoker.rejection_sample(..., in_memory=True) for many stars at once.
 
     datas: RVData (or per-survey lists) per star; rngs: one numpy Generator
     per star (consumed exactly as rejection_sample_inmem consumes it,
     likelihood_helpers.py:91-127); prior_samples: a PriorCache (or
     JokerSamples) shared by all stars.  Returns one JokerSamples per star
-    (and the per-star lnL arrays with return_all_logprobs)."""
+    (and the per-star lnL arrays with return_all_logprobs).  prune: bound
+    the rows of stars with at least PRUNE_MIN_EPOCHS epochs by a probe and
+    evaluate in full only those that can matter (prune_reject_batch); the
+    samples do not change."""
     from .device import PriorCache, linear_internal_units, star_arrays
     from .likelihood_helpers import validate_prepare_data
     from .thejoker import TheJoker
@@ -447,19 +554,30 @@ def rejection_sample_many(datas, prior, prior_samples, rngs, max_posterior_sampl
     shape = shape or LaunchShape()
     items, off, total = work_items([n] * len(stars), shape.tile)
     items = items[heaviest_first(items, batch.M)]
-    bound = (torch.empty(total, dtype=torch.float32, device=cache.P.device)
-             if precision == "mixed" else None)
-    ll = lnl_batch(batch, cache, upload_items(items, device), len(items), total, precision, shape,
-                   bound=bound)
     states = [pcg64_state(g) for g in rngs]
-    if precision == "mixed":
+    sizes = np.full(len(stars), n)
+    d_items = upload_items(items, device)
+    if prune:
+        probes, adds, probed = probe_table(stars, jitter_mode, device)
+        ll, bound, res, _ = prune_reject_batch(batch, probes, adds, probed, cache, d_items,
+                                               len(items), off, sizes, states,
+                                               max_posterior_samples, precision, shape, device)
+        good = res[0]
+        if return_all_logprobs:
+            # the pruned rows hold upper bounds: evaluated before they are returned
+            settle_tol_batch(batch, cache, ll, bound, off, sizes, device=device)
+    elif precision == "mixed":
+        bound = torch.empty(total, dtype=torch.float32, device=cache.P.device)
+        ll = lnl_batch(batch, cache, d_items, len(items), total, precision, shape, bound=bound)
         # FP64 re-evaluation of exactly the rows whose decision the mixed
         # likelihood's error bound leaves open, and the rejection step
-        good = settle_reject_batch(batch, cache, ll, bound, off, np.full(len(stars), n), states,
+        good = settle_reject_batch(batch, cache, ll, bound, off, sizes, states,
                                    max_posterior_samples, device=device)[0]
+        if return_all_logprobs:
+            settle_tol_batch(batch, cache, ll, bound, off, sizes, device=device)
     else:
-        good = reject_batch(ll, off, np.full(len(stars), n), states, max_posterior_samples,
-                            device=device)[0]
+        ll = lnl_batch(batch, cache, d_items, len(items), total, precision, shape)
+        good = reject_batch(ll, off, sizes, states, max_posterior_samples, device=device)[0]
     for g in rngs:
         pcg64_advance_state(g, n)
     ll_h = ll.cpu().numpy() if return_all_logprobs else None

@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -43,63 +44,99 @@ u128 make_u128(uint64_t hi, uint64_t lo)
 // the pool holds no free block of the size asked for, and such calls were
 // measured at up to seconds with the device idle meanwhile.  The next call on
 // the same stream reuses the buffer, after this call's work in stream order.
-// A buffer grows (after synchronising its stream) and is released at process
-// exit.  Under stream capture the stream-ordered pool is used instead, so that
-// a captured graph never refers to memory a later call may replace.
+//
+// A call holds its slot (the slot's mutex) from scratch_get until
+// scratch_put, i.e. until its last launch is queued: two host threads that
+// call on the same stream therefore queue their launch sequences one after the
+// other instead of interleaving them over one buffer, and a slot grows (after
+// synchronising its stream) only while no other call holds its old pointer.
+// jk_release_scratch frees a device's slots; otherwise they live until the
+// process ends.  Under stream capture the stream-ordered pool is used instead,
+// so that a captured graph never refers to memory a later call may replace.
 struct ScratchSlot {
     int device;
     hipStream_t stream;
     void *ptr;
     size_t bytes;
+    bool dead;          // released by jk_release_scratch; look the slot up again
+    std::mutex mu;      // held by the call using the slot
 };
 std::mutex g_scratch_mu;
-std::vector<ScratchSlot> g_scratch;
+std::vector<std::shared_ptr<ScratchSlot>> g_scratch;
 
-hipError_t scratch_get(hipStream_t st, size_t bytes, void **out, bool *pooled)
+struct ScratchLease {
+    std::shared_ptr<ScratchSlot> slot;   // locked while held
+    bool pooled = false;                 // stream-ordered pool memory (capture)
+};
+
+hipError_t scratch_get(hipStream_t st, size_t bytes, void **out, ScratchLease *lease)
 {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     hipError_t e = hipStreamIsCapturing(st, &cs);
     if (e != hipSuccess) return e;
     if (cs != hipStreamCaptureStatusNone) {
-        *pooled = true;
+        lease->pooled = true;
         return hipMallocAsync(out, bytes, st);
     }
-    *pooled = false;
+    lease->pooled = false;
     int dev = 0;
     if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
-    std::lock_guard<std::mutex> lock(g_scratch_mu);
-    ScratchSlot *slot = nullptr;
-    for (auto &sl : g_scratch)
-        if (sl.device == dev && sl.stream == st) slot = &sl;
-    if (slot && slot->bytes >= bytes) {
-        *out = slot->ptr;
-        return hipSuccess;
+    std::shared_ptr<ScratchSlot> slot;
+    for (;;) {
+        {
+            std::lock_guard<std::mutex> lock(g_scratch_mu);
+            slot.reset();
+            for (auto &sl : g_scratch)
+                if (sl->device == dev && sl->stream == st) slot = sl;
+            if (!slot) {
+                slot = std::make_shared<ScratchSlot>();
+                slot->device = dev;
+                slot->stream = st;
+                slot->ptr = nullptr;
+                slot->bytes = 0;
+                slot->dead = false;
+                g_scratch.push_back(slot);
+            }
+        }
+        slot->mu.lock();
+        if (!slot->dead) break;
+        slot->mu.unlock();
     }
-    size_t want = bytes < 256 ? 256 : bytes;
-    if (slot) {
-        want = std::max(want, slot->bytes + slot->bytes / 2);
-        if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
-        if ((e = hipFree(slot->ptr)) != hipSuccess) return e;
-        slot->ptr = nullptr;
-        slot->bytes = 0;
-    }
-    const size_t align = (size_t)2 << 20;
-    want = (want + align - 1) / align * align;
-    void *ptr = nullptr;
-    if ((e = hipMalloc(&ptr, want)) != hipSuccess) return e;
-    if (slot) {
+    if (slot->bytes < bytes || !slot->ptr) {
+        size_t want = bytes < 256 ? 256 : bytes;
+        if (slot->ptr) {
+            want = std::max(want, slot->bytes + slot->bytes / 2);
+            if ((e = hipStreamSynchronize(st)) == hipSuccess) e = hipFree(slot->ptr);
+            if (e != hipSuccess) {
+                slot->mu.unlock();
+                return e;
+            }
+            slot->ptr = nullptr;
+            slot->bytes = 0;
+        }
+        const size_t align = (size_t)2 << 20;
+        want = (want + align - 1) / align * align;
+        void *ptr = nullptr;
+        if ((e = hipMalloc(&ptr, want)) != hipSuccess) {
+            slot->mu.unlock();
+            return e;
+        }
         slot->ptr = ptr;
         slot->bytes = want;
-    } else {
-        g_scratch.push_back({dev, st, ptr, want});
     }
-    *out = ptr;
+    *out = slot->ptr;
+    lease->slot = std::move(slot);
     return hipSuccess;
 }
 
-hipError_t scratch_put(hipStream_t st, void *ptr, bool pooled)
+hipError_t scratch_put(hipStream_t st, void *ptr, ScratchLease &lease)
 {
-    return pooled ? hipFreeAsync(ptr, st) : hipSuccess;
+    if (lease.pooled) return hipFreeAsync(ptr, st);
+    if (lease.slot) {
+        lease.slot->mu.unlock();
+        lease.slot.reset();
+    }
+    return hipSuccess;
 }
 
 }  // namespace
@@ -146,6 +183,38 @@ int jk_init(int device)
     if (hipDeviceSynchronize() != hipSuccess) return JK_EHIP;
     g_inited[device] = true;
     return JK_OK;
+}
+
+int jk_release_scratch(int device)
+{
+    if (device < 0 || device >= 64) return JK_EINVAL;
+    if (!g_inited[device]) return JK_OK;
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return JK_EHIP;
+    std::vector<std::shared_ptr<ScratchSlot>> mine;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        std::vector<std::shared_ptr<ScratchSlot>> keep;
+        for (auto &sl : g_scratch) (sl->device == device ? mine : keep).push_back(sl);
+        g_scratch.swap(keep);
+    }
+    if (mine.empty()) return JK_OK;
+    if (hipSetDevice(device) != hipSuccess) return JK_EHIP;
+    hipError_t e = hipSuccess;
+    for (auto &sl : mine) {
+        // waits for a call still queuing on the slot, then for its work
+        std::lock_guard<std::mutex> hold(sl->mu);
+        sl->dead = true;
+        if (sl->ptr) {
+            hipError_t e1 = hipStreamSynchronize(sl->stream);
+            if (e1 == hipSuccess) e1 = hipFree(sl->ptr);
+            if (e1 != hipSuccess && e == hipSuccess) e = e1;
+            sl->ptr = nullptr;
+            sl->bytes = 0;
+        }
+    }
+    (void)hipSetDevice(cur);
+    return check_hip(e);
 }
 
 // The star from epoch times t (epoch order) and rv, ivar, trend (N x (L-1)
@@ -397,11 +466,11 @@ int jk_max_f64(const double *ll, int64_t n, double *max_out, void *stream)
     hipStream_t st = (hipStream_t)stream;
     const int np = jk_max_partials_count(n);
     double *partials = nullptr;
-    bool pooled = false;
-    if (scratch_get(st, sizeof(double) * np, (void **)&partials, &pooled) != hipSuccess)
+    ScratchLease lease;
+    if (scratch_get(st, sizeof(double) * np, (void **)&partials, &lease) != hipSuccess)
         return JK_ENOMEM;
     hipError_t err = jk_launch_max(ll, n, partials, np, max_out, st);
-    hipError_t err2 = scratch_put(st, partials, pooled);
+    hipError_t err2 = scratch_put(st, partials, lease);
     if (err != hipSuccess) return JK_EHIP;
     return check_hip(err2);
 }
@@ -420,8 +489,8 @@ static int reject_impl(const double *ll, int64_t n, const double *llmax_in, uint
     const int64_t nb = jk_reject_bsum_count(n);
     const size_t bytes = sizeof(double) * np + sizeof(uint64_t) * nwords + sizeof(int64_t) * nb;
     char *scratch = nullptr;
-    bool pooled = false;
-    if (scratch_get(st, bytes, (void **)&scratch, &pooled) != hipSuccess) return JK_ENOMEM;
+    ScratchLease lease;
+    if (scratch_get(st, bytes, (void **)&scratch, &lease) != hipSuccess) return JK_ENOMEM;
     double *partials = (double *)scratch;
     uint64_t *mask = (uint64_t *)(scratch + sizeof(double) * np);
     int64_t *bsum = (int64_t *)(scratch + sizeof(double) * np + sizeof(uint64_t) * nwords);
@@ -434,7 +503,7 @@ static int reject_impl(const double *ll, int64_t n, const double *llmax_in, uint
     if (err == hipSuccess)
         err = jk_launch_reject(ll, n, mx, make_u128(state_hi, state_lo), make_u128(inc_hi, inc_lo),
                                offset, max_keep, idx_out, n_pass_out, mask, bsum, st);
-    hipError_t err2 = scratch_put(st, scratch, pooled);
+    hipError_t err2 = scratch_put(st, scratch, lease);
     if (err != hipSuccess) return JK_EHIP;
     return check_hip(err2);
 }
@@ -470,17 +539,37 @@ int jk_lnl_reject_f64(const jk_star *star, int64_t n, const double *P, const dou
     if (current_device_inited() < 0) return JK_ENODEV;
     hipStream_t st = (hipStream_t)stream;
     void *scratch = nullptr;
-    bool pooled = false;
-    if (scratch_get(st, (size_t)jk_lnl_reject_scratch_bytes(n), (void **)&scratch, &pooled) !=
+    ScratchLease lease;
+    if (scratch_get(st, (size_t)jk_lnl_reject_scratch_bytes(n), (void **)&scratch, &lease) !=
         hipSuccess)
         return JK_ENOMEM;
     hipError_t err = jk_launch_lnl_reject_f64(star, n, P, e, omega, M0, s, rows, ll_out,
                                               make_u128(state_hi, state_lo),
                                               make_u128(inc_hi, inc_lo), stream_offset, max_keep,
                                               idx_out, n_pass_out, llmax_out, scratch, st);
-    hipError_t err2 = scratch_put(st, scratch, pooled);
+    hipError_t err2 = scratch_put(st, scratch, lease);
     if (err != hipSuccess) return JK_EHIP;
     return check_hip(err2);
+}
+
+int jk_debug_single(const jk_star *star, const double *row, double *MT_out, double *Ainv_out,
+                    double *A_out, double *a_out, double *b_out, double *B_out, double *Binv_out,
+                    double *scalars_out, void *stream)
+{
+    if (!star || !row || !MT_out || !Ainv_out || !A_out || !a_out || !b_out || !scalars_out)
+        return JK_EINVAL;
+    if (star->dev.Qt) return JK_EINVAL;   // diagonal data covariance only
+    if (current_device_inited() < 0) return JK_ENODEV;
+    return check_hip(jk_launch_debug_row(star, row, MT_out, Ainv_out, A_out, a_out, b_out, B_out,
+                                         Binv_out, scalars_out, (hipStream_t)stream));
+}
+
+int jk_count_failures(const double *ll, int64_t n, uint64_t *counts_out, void *stream)
+{
+    if (n < 0 || !counts_out || (n > 0 && !ll)) return JK_EINVAL;
+    if (current_device_inited() < 0) return JK_ENODEV;
+    return check_hip(jk_launch_count_failures(ll, n, (unsigned long long *)counts_out,
+                                              (hipStream_t)stream));
 }
 
 int jk_pcg64_uniform(uint64_t state_hi, uint64_t state_lo, uint64_t inc_hi, uint64_t inc_lo,
@@ -535,13 +624,13 @@ int jk_polish_f64(const jk_star *star, int64_t n, const double *P, const double 
     if (current_device_inited() < 0) return JK_ENODEV;
     hipStream_t st = (hipStream_t)stream;
     void *scratch = nullptr;
-    bool pooled = false;
-    if (scratch_get(st, (size_t)jk_polish_scratch_bytes(n), (void **)&scratch, &pooled) !=
+    ScratchLease lease;
+    if (scratch_get(st, (size_t)jk_polish_scratch_bytes(n), (void **)&scratch, &lease) !=
         hipSuccess)
         return JK_ENOMEM;
     hipError_t err = jk_launch_polish(star, n, P, e, omega, M0, s, rows, ll_inout, window, first,
                                       scratch, n_polished_out, st);
-    hipError_t err2 = scratch_put(st, scratch, pooled);
+    hipError_t err2 = scratch_put(st, scratch, lease);
     if (err != hipSuccess) return JK_EHIP;
     return check_hip(err2);
 }
@@ -703,7 +792,7 @@ int jk_lnl_prune_batch(const jk_batch *batch, const jk_batch *probes, const doub
                        int grid, void *workspace, void *stream)
 {
     if (!batch || !probes || n_items < 0 || (precision != 0 && precision != 1)) return JK_EINVAL;
-    if ((precision == 1) != (bound_out != nullptr)) return JK_EINVAL;
+    if (precision == 1 && !bound_out) return JK_EINVAL;
     if (probes->L != batch->L || probes->jitter_mode != batch->jitter_mode ||
         probes->n_stars < batch->n_stars || probes->device != batch->device)
         return JK_EINVAL;
@@ -772,8 +861,8 @@ static int reject_batch_impl(const double *ll, int n_seg, const jk_segment *segs
     const size_t bytes = seg_pad + sizeof(double) * (size_t)pb * n_seg +
                          sizeof(uint64_t) * mask_words + sizeof(int64_t) * bsum_n;
     char *scratch = nullptr;
-    bool pooled = false;
-    if (scratch_get(st, bytes, (void **)&scratch, &pooled) != hipSuccess) return JK_ENOMEM;
+    ScratchLease lease;
+    if (scratch_get(st, bytes, (void **)&scratch, &lease) != hipSuccess) return JK_ENOMEM;
     SegDev *d_segs = (SegDev *)scratch;
     double *partials = (double *)(scratch + seg_pad);
     uint64_t *mask = (uint64_t *)(partials + (size_t)pb * n_seg);
@@ -784,7 +873,7 @@ static int reject_batch_impl(const double *ll, int n_seg, const jk_segment *segs
                                      idx_out, n_pass_out, llmax_out, st, zero_u_out);
     // The host descriptor array must outlive the queued copy.
     hipError_t err3 = hipStreamSynchronize(st);
-    hipError_t err2 = scratch_put(st, scratch, pooled);
+    hipError_t err2 = scratch_put(st, scratch, lease);
     if (err != hipSuccess || err3 != hipSuccess) return JK_EHIP;
     return check_hip(err2);
 }
@@ -848,8 +937,8 @@ int jk_polish_batch(const jk_batch *batch, int n_seg, const jk_polish_segment *s
                          sizeof(uint64_t) * mask_words +
                          sizeof(int64_t) * (bsum_n + idx_n + n_seg + 1);
     char *scratch = nullptr;
-    bool pooled = false;
-    if (scratch_get(st, bytes, (void **)&scratch, &pooled) != hipSuccess) return JK_ENOMEM;
+    ScratchLease lease;
+    if (scratch_get(st, bytes, (void **)&scratch, &lease) != hipSuccess) return JK_ENOMEM;
     SegDev *d_segs = (SegDev *)scratch;
     double *partials = (double *)(scratch + seg_pad);
     double *llmax = partials + (size_t)pb * n_seg;
@@ -864,7 +953,7 @@ int jk_polish_batch(const jk_batch *batch, int n_seg, const jk_polish_segment *s
                                      n_polished_out, st);
     // The host descriptor array must outlive the queued copy.
     hipError_t err3 = hipStreamSynchronize(st);
-    hipError_t err2 = scratch_put(st, scratch, pooled);
+    hipError_t err2 = scratch_put(st, scratch, lease);
     if (err != hipSuccess || err3 != hipSuccess) return JK_EHIP;
     return check_hip(err2);
 }
@@ -915,8 +1004,8 @@ int jk_settle_batch(const jk_batch *batch, int n_seg, const jk_settle_segment *s
                          sizeof(uint64_t) * mask_words +
                          sizeof(int64_t) * (bsum_n + idx_n + 2 * (size_t)n_seg + 1);
     char *scratch = nullptr;
-    bool pooled = false;
-    if (scratch_get(st, bytes, (void **)&scratch, &pooled) != hipSuccess) return JK_ENOMEM;
+    ScratchLease lease;
+    if (scratch_get(st, bytes, (void **)&scratch, &lease) != hipSuccess) return JK_ENOMEM;
     SegDev *d_segs = (SegDev *)scratch;
     double *partials = (double *)(scratch + seg_pad);
     double *lo = partials + (size_t)pb * n_seg;
@@ -933,7 +1022,7 @@ int jk_settle_batch(const jk_batch *batch, int n_seg, const jk_settle_segment *s
                                      ll_inout, bound_inout, n_settled_out, st);
     // The host descriptor array must outlive the queued copy.
     hipError_t err3 = hipStreamSynchronize(st);
-    hipError_t err2 = scratch_put(st, scratch, pooled);
+    hipError_t err2 = scratch_put(st, scratch, lease);
     if (err != hipSuccess || err3 != hipSuccess) return JK_EHIP;
     return check_hip(err2);
 }
@@ -993,8 +1082,8 @@ int jk_settle_reject_batch(const jk_batch *batch, int n_seg, const jk_settle_seg
                          2 * sizeof(uint64_t) * mask_words +
                          sizeof(int64_t) * (bsum_n + idx_n + 2 * (size_t)n_seg + 1);
     char *scratch = nullptr;
-    bool pooled = false;
-    if (scratch_get(st, bytes, (void **)&scratch, &pooled) != hipSuccess) return JK_ENOMEM;
+    ScratchLease lease;
+    if (scratch_get(st, bytes, (void **)&scratch, &lease) != hipSuccess) return JK_ENOMEM;
     SegDev *d_settle = (SegDev *)scratch;
     SegDev *d_reject = (SegDev *)(scratch + seg_pad);
     double *partials = (double *)(scratch + 2 * seg_pad);
@@ -1016,21 +1105,26 @@ int jk_settle_reject_batch(const jk_batch *batch, int n_seg, const jk_settle_seg
                                             n_settled_out, st);
     // The host descriptor arrays must outlive the queued copies.
     hipError_t err3 = hipStreamSynchronize(st);
-    hipError_t err2 = scratch_put(st, scratch, pooled);
+    hipError_t err2 = scratch_put(st, scratch, lease);
     if (err != hipSuccess || err3 != hipSuccess) return JK_EHIP;
     return check_hip(err2);
 }
 
-int jk_settle_live_batch(const jk_batch *batch, int n_seg, const jk_live_segment *segs,
-                         const double *P, const double *e, const double *omega, const double *M0,
-                         const double *s, const int64_t *rows, double *ll_inout,
-                         float *bound_inout, double margin, int64_t *n_settled_out, void *stream)
+// jk_settle_live_batch, jk_settle_tol_batch and jk_settle_pruned_batch over
+// the entries [first, n) of each segment.
+enum { SETTLE_LIVE = 0, SETTLE_TOL = 1, SETTLE_PRUNED = 2 };
+
+static int settle_new_rows(int mode, const jk_batch *batch, int n_seg,
+                           const jk_live_segment *segs, const double *P, const double *e,
+                           const double *omega, const double *M0, const double *s,
+                           const int64_t *rows, double *ll_inout, float *bound_inout,
+                           double margin, float tol, double window, int64_t *n_settled_out,
+                           void *stream)
 {
     if (!batch || n_seg <= 0 || !segs || !ll_inout || !bound_inout || !n_settled_out)
         return JK_EINVAL;
     if (!P || !e || !omega || !M0) return JK_EINVAL;
     if (batch->jitter_mode == JK_JITTER_PHYSICAL && !s) return JK_EINVAL;
-    if (!(margin >= 40.0)) return JK_EINVAL;
     if (current_device_inited() < 0) return JK_ENODEV;
     std::vector<SegDev> h(n_seg);
     int64_t max_n = 1, max_nb = 1, mask_words = 0, bsum_n = 0, idx_n = 0;
@@ -1067,10 +1161,10 @@ int jk_settle_live_batch(const jk_batch *batch, int n_seg, const jk_live_segment
     const size_t seg_pad = ((seg_bytes + 255) / 256) * 256;
     const size_t bytes = seg_pad + sizeof(double) * ((size_t)pb * n_seg + (size_t)n_seg) +
                          sizeof(uint64_t) * mask_words +
-                         sizeof(int64_t) * (bsum_n + idx_n + (size_t)n_seg + 1);
+                         sizeof(int64_t) * (bsum_n + idx_n + 2 * (size_t)n_seg + 1);
     char *scratch = nullptr;
-    bool pooled = false;
-    if (scratch_get(st, bytes, (void **)&scratch, &pooled) != hipSuccess) return JK_ENOMEM;
+    ScratchLease lease;
+    if (scratch_get(st, bytes, (void **)&scratch, &lease) != hipSuccess) return JK_ENOMEM;
     SegDev *d_segs = (SegDev *)scratch;
     double *partials = (double *)(scratch + seg_pad);
     double *lo = partials + (size_t)pb * n_seg;
@@ -1078,16 +1172,57 @@ int jk_settle_live_batch(const jk_batch *batch, int n_seg, const jk_live_segment
     int64_t *bsum = (int64_t *)(mask + mask_words);
     int64_t *idx = bsum + bsum_n;
     int64_t *cum = idx + idx_n;
+    int64_t *count1 = cum + n_seg + 1;
     hipError_t err = hipMemcpyAsync(d_segs, h.data(), seg_bytes, hipMemcpyHostToDevice, st);
-    if (err == hipSuccess)
+    if (err == hipSuccess && mode == SETTLE_LIVE)
         err = jk_launch_settle_live_batch(batch, n_seg, d_segs, max_n, max_nb, partials, pb, mask,
                                           bsum, idx, lo, cum, P, e, omega, M0, s, rows, ll_inout,
                                           bound_inout, margin, n_settled_out, st);
+    else if (err == hipSuccess && mode == SETTLE_TOL)
+        err = jk_launch_settle_tol_batch(batch, n_seg, d_segs, max_n, max_nb, mask, bsum, idx, cum,
+                                         P, e, omega, M0, s, rows, ll_inout, bound_inout, tol,
+                                         n_settled_out, st);
+    else if (err == hipSuccess)
+        err = jk_launch_settle_pruned_batch(batch, n_seg, d_segs, max_n, max_nb, partials, pb,
+                                            mask, bsum, idx, lo, cum, count1, P, e, omega, M0, s,
+                                            rows, ll_inout, bound_inout, window, margin,
+                                            n_settled_out, st);
     // The host descriptor array must outlive the queued copy.
     hipError_t err3 = hipStreamSynchronize(st);
-    hipError_t err2 = scratch_put(st, scratch, pooled);
+    hipError_t err2 = scratch_put(st, scratch, lease);
     if (err != hipSuccess || err3 != hipSuccess) return JK_EHIP;
     return check_hip(err2);
+}
+
+int jk_settle_live_batch(const jk_batch *batch, int n_seg, const jk_live_segment *segs,
+                         const double *P, const double *e, const double *omega, const double *M0,
+                         const double *s, const int64_t *rows, double *ll_inout,
+                         float *bound_inout, double margin, int64_t *n_settled_out, void *stream)
+{
+    if (!(margin >= 40.0)) return JK_EINVAL;
+    return settle_new_rows(SETTLE_LIVE, batch, n_seg, segs, P, e, omega, M0, s, rows, ll_inout,
+                           bound_inout, margin, 0.0f, 0.0, n_settled_out, stream);
+}
+
+int jk_settle_tol_batch(const jk_batch *batch, int n_seg, const jk_live_segment *segs,
+                        const double *P, const double *e, const double *omega, const double *M0,
+                        const double *s, const int64_t *rows, double *ll_inout,
+                        float *bound_inout, float tol, int64_t *n_settled_out, void *stream)
+{
+    if (!(tol >= 0.0f)) return JK_EINVAL;
+    return settle_new_rows(SETTLE_TOL, batch, n_seg, segs, P, e, omega, M0, s, rows, ll_inout,
+                           bound_inout, 750.0, tol, 0.0, n_settled_out, stream);
+}
+
+int jk_settle_pruned_batch(const jk_batch *batch, int n_seg, const jk_live_segment *segs,
+                           const double *P, const double *e, const double *omega,
+                           const double *M0, const double *s, const int64_t *rows,
+                           double *ll_inout, float *bound_inout, double window, double margin,
+                           int64_t *n_settled_out, void *stream)
+{
+    if (!(margin >= 40.0) || !(window >= 0.0)) return JK_EINVAL;
+    return settle_new_rows(SETTLE_PRUNED, batch, n_seg, segs, P, e, omega, M0, s, rows, ll_inout,
+                           bound_inout, margin, 0.0f, window, n_settled_out, stream);
 }
 
 // ---------------------------------------------------------------------------

@@ -70,6 +70,26 @@ hipError_t jk_launch_linear_posterior(const jk_star *st, int64_t n, const double
                                       const double *e, const double *om, const double *M0,
                                       const double *s, const int64_t *rows, double *a_out,
                                       double *cov_out, double *ll_out, hipStream_t stream);
+hipError_t jk_launch_settle_tol_batch(const jk_batch *b, int n_seg, const SegDev *d_segs,
+                                      int64_t max_n, int64_t max_nb, uint64_t *mask,
+                                      int64_t *bsum, int64_t *idx, int64_t *cum, const double *P,
+                                      const double *e, const double *om, const double *M0,
+                                      const double *s, const int64_t *rows, double *ll,
+                                      float *bound, float tol, int64_t *n_settled,
+                                      hipStream_t stream);
+hipError_t jk_launch_settle_pruned_batch(const jk_batch *b, int n_seg, const SegDev *d_segs,
+                                         int64_t max_n, int64_t max_nb, double *partials, int pb,
+                                         uint64_t *mask, int64_t *bsum, int64_t *idx,
+                                         double *level, int64_t *cum, int64_t *count1,
+                                         const double *P, const double *e, const double *om,
+                                         const double *M0, const double *s, const int64_t *rows,
+                                         double *ll, float *bound, double window, double margin,
+                                         int64_t *n_settled, hipStream_t stream);
+hipError_t jk_launch_debug_row(const jk_star *st, const double *row, double *MT, double *Ainv,
+                               double *A, double *a, double *b, double *B, double *Binv,
+                               double *sc, hipStream_t stream);
+hipError_t jk_launch_count_failures(const double *ll, int64_t n, unsigned long long *counts,
+                                    hipStream_t stream);
 hipError_t jk_launch_linear_posterior_batch(const jk_batch *b, int64_t n, const int32_t *star_idx,
                                             const double *P, const double *e, const double *om,
                                             const double *M0, const double *s,

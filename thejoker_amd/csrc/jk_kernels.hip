@@ -270,6 +270,10 @@ __device__ __forceinline__ void jk_kepler_markley_f32(float x, float e, float om
 // doubled again for the terms of second order and given a relative floor of
 // 1e-9.
 #define JK_BOUND_EPS 2.384185791015625e-07   // 2^-22
+// Error-bound value of a row whose lnL entry is only an upper bound on its
+// lnL (a row pruned by k_lnl_prune_batch in mixed precision); 0 marks an FP64
+// value, a positive bound a mixed-precision value.
+#define JK_BOUND_UPPER (-1.0f)
 
 template <int L, bool PHYS, bool MIXED, bool SAFE = false, bool BOUND = false>
 __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
@@ -1432,8 +1436,9 @@ hipError_t jk_launch_lnl_batch(const jk_batch *b, int precision, int64_t n_items
 //   lnL <= lnL_S - 1/2 sum_{n not in S} ln(2 pi sigma_n^2) = lnL_S + add,
 // for every prior row (with jitter the variances only grow).  prune[k] =
 // (thr, add): a row whose bound lnL_S (+ its mixed error bound) + add is
-// finite and below thr gets that bound as its value (bound 0) and is not
-// evaluated further; every other row is evaluated as in k_lnl_batch.  Rows
+// finite and below thr gets that bound as its value and is not evaluated
+// further (bound JK_BOUND_UPPER in mixed precision: the value is an upper
+// bound on the row's lnL, nothing more, and the settling kernels treat it so); every other row is evaluated as in k_lnl_batch.  Rows
 // are probed a chunk of blockDim at a time and the survivors are gathered in
 // LDS over several chunks, so the full evaluations run on dense waves.  thr = -inf
 // turns the probe off for the star.  n_full (optional, one entry per star)
@@ -1481,6 +1486,7 @@ __global__ __launch_bounds__(256) void k_lnl_prune_batch(const StarDev *__restri
                     bound[w.out_offset + j] = b;
                 } else {
                     ll[w.out_offset + j] = jk_lnl_row<L, PHYS, false>(st, r, P, e_, om_, M0_, s_, s_tab);
+                    if (bound) bound[w.out_offset + j] = 0.0f;
                 }
             }
             if (n_full && threadIdx.x == 0)
@@ -1507,7 +1513,7 @@ __global__ __launch_bounds__(256) void k_lnl_prune_batch(const StarDev *__restri
                 }
                 if (isfinite(v) && v < pr.x) {
                     ll[w.out_offset + j] = v;
-                    if constexpr (MIXED) bound[w.out_offset + j] = 0.0f;
+                    if (bound) bound[w.out_offset + j] = JK_BOUND_UPPER;
                 } else {
                     s_list[atomicAdd(&s_n, 1)] = j;
                 }
@@ -1531,6 +1537,7 @@ __global__ __launch_bounds__(256) void k_lnl_prune_batch(const StarDev *__restri
                     } else {
                         ll[w.out_offset + jj] =
                             jk_lnl_row<L, PHYS, false>(st, r, P, e_, om_, M0_, s_, s_tab);
+                        if (bound) bound[w.out_offset + jj] = 0.0f;
                     }
                 }
                 __syncthreads();   // every read of s_list before it is reset
@@ -1578,10 +1585,10 @@ hipError_t jk_launch_lnl_prune_batch(const jk_batch *b, const jk_batch *pb, cons
         } else {                                                                                  \
             if (phys)                                                                             \
                 launch_prune_t<LL, true, false>(b, pb, pr, n_items, items, P, e, om, M0, s, ll,    \
-                                                nullptr, n_full, block, grid, counter, stream);   \
+                                                bound, n_full, block, grid, counter, stream);     \
             else                                                                                  \
                 launch_prune_t<LL, false, false>(b, pb, pr, n_items, items, P, e, om, M0, s, ll,   \
-                                                 nullptr, n_full, block, grid, counter, stream);  \
+                                                 bound, n_full, block, grid, counter, stream);    \
         }                                                                                         \
         break;
     switch (b->L) {
@@ -1970,8 +1977,11 @@ __global__ __launch_bounds__(256) void k_seg_lo_partial(const double *__restrict
     const SegDev sg = segs[blockIdx.y];
     double m = -INFINITY;
     const int64_t stride = (int64_t)pb * 256;
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < sg.n; i += stride)
-        m = jk_nanmax(m, ll[sg.off + i] - (double)bound[sg.off + i]);
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < sg.n; i += stride) {
+        const float b = bound[sg.off + i];
+        // an upper-bound row gives no lower bound on the maximum
+        if (!(b < 0.0f)) m = jk_nanmax(m, ll[sg.off + i] - (double)b);
+    }
     m = wave_nanmax(m);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
@@ -2004,10 +2014,13 @@ __global__ __launch_bounds__(256) void k_seg_shell_flags(const double *__restric
         bool keep = false;
         if (i < sg.n) {
             const float b = bound[sg.off + i];
-            const double hi = ll[sg.off + i] + (double)b;
+            const double x = ll[sg.off + i];
+            const double hi = x + (double)b;
             // could be the maximum; or within JK_LIVE_WINDOW of it with a
-            // bound above JK_LIVE_TOL
-            keep = b > 0.0f && (!(hi < l0) || (b > JK_LIVE_TOL && !(hi < l0 - JK_LIVE_WINDOW)));
+            // bound above JK_LIVE_TOL (an upper-bound row: x bounds it from
+            // above only)
+            keep = b > 0.0f ? (!(hi < l0) || (b > JK_LIVE_TOL && !(hi < l0 - JK_LIVE_WINDOW)))
+                            : (b < 0.0f && !(x < l0 - JK_LIVE_WINDOW));
         }
         const uint64_t word = __ballot(keep);
         if (lane == 0) mask[sg.mask_off + (base >> 6)] = word;
@@ -2039,6 +2052,9 @@ __global__ __launch_bounds__(256) void k_seg_uncertain_flags(const double *__res
                 const double x = ll[sg.off + i];
                 const double b = fma((double)bf, 1e-6, (double)bf) + 1e-12;
                 keep = jk_accept(x - b - mx, uu) != jk_accept(x + b - mx, uu);
+            } else if (live && bf < 0.0f) {
+                // an upper bound decides a rejection only
+                keep = jk_accept(ll[sg.off + i] - mx, uu);
             }
         }
         const uint64_t word = __ballot(keep);
@@ -2127,7 +2143,8 @@ __global__ __launch_bounds__(256) void k_seg_live_flags(const double *__restrict
         bool keep = false;
         if (i < sg.n) {
             const float b = bound[sg.off + i];
-            keep = b > 0.0f && !(ll[sg.off + i] + (double)b < thr);
+            // an upper-bound row (b < 0): its value bounds lnL from above
+            keep = b != 0.0f && !(ll[sg.off + i] + (double)fmaxf(b, 0.0f) < thr);
         }
         const uint64_t word = __ballot(keep);
         if (lane == 0) mask[sg.mask_off + (base >> 6)] = word;
@@ -2172,6 +2189,9 @@ __global__ __launch_bounds__(256) void k_seg_decide(const double *__restrict__ l
                 const bool lo = jk_accept(x - b - mx, uu), hi = jk_accept(x + b - mx, uu);
                 open = lo != hi;
                 acc = lo && hi;
+            } else if (bf < 0.0f) {
+                // an upper bound decides a rejection only
+                open = jk_accept(x - mx, uu);
             } else {
                 acc = jk_accept(x - mx, uu);
             }
@@ -2284,6 +2304,150 @@ hipError_t jk_launch_settle_live_batch(const jk_batch *b, int n_seg, const SegDe
                        mask, bsum, idx, n_settled);
     return launch_seg_scatter(b, n_seg, d_segs, n_settled, cum, idx, P, e, om, M0, s, rows, ll,
                               bound, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Settling to a tolerance (jk_settle_tol_batch): every entry of a segment
+// whose error bound exceeds tol, or whose value is only an upper bound
+// (negative bound), is re-evaluated in FP64.  Afterwards every entry is
+// within tol of its FP64 lnL, whatever its distance from the maximum: the
+// form for lnL arrays handed to a caller.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_seg_tol_flags(const float *__restrict__ bound,
+                                                       const SegDev *__restrict__ segs, float tol,
+                                                       uint64_t *__restrict__ mask)
+{
+    const SegDev sg = segs[blockIdx.y];
+    const int lane = threadIdx.x & 63;
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t T = (int64_t)gridDim.x * 256;
+    if (g - lane >= sg.n) return;
+    for (int64_t base = g - lane; base < sg.n; base += T) {
+        const int64_t i = base + lane;
+        bool keep = false;
+        if (i < sg.n) {
+            const float b = bound[sg.off + i];
+            keep = b < 0.0f || b > tol || b != b;
+        }
+        const uint64_t word = __ballot(keep);
+        if (lane == 0) mask[sg.mask_off + (base >> 6)] = word;
+    }
+}
+
+hipError_t jk_launch_settle_tol_batch(const jk_batch *b, int n_seg, const SegDev *d_segs,
+                                      int64_t max_n, int64_t max_nb, uint64_t *mask,
+                                      int64_t *bsum, int64_t *idx, int64_t *cum, const double *P,
+                                      const double *e, const double *om, const double *M0,
+                                      const double *s, const int64_t *rows, double *ll,
+                                      float *bound, float tol, int64_t *n_settled,
+                                      hipStream_t stream)
+{
+    if (n_seg <= 0) return hipSuccess;
+    const int fb = jk_seg_flag_blocks(max_n);
+    hipLaunchKernelGGL(k_seg_tol_flags, dim3(fb, n_seg), dim3(256), 0, stream, bound, d_segs, tol,
+                       mask);
+    hipLaunchKernelGGL(k_seg_bsum, dim3((unsigned)max_nb, n_seg), dim3(256), 0, stream, d_segs,
+                       mask, bsum);
+    hipLaunchKernelGGL(k_seg_compact, dim3((unsigned)max_nb, n_seg), dim3(256), 0, stream, d_segs,
+                       mask, bsum, idx, n_settled);
+    return launch_seg_scatter(b, n_seg, d_segs, n_settled, cum, idx, P, e, om, M0, s, rows, ll,
+                              bound, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Settling a buffer whose entries are values or upper bounds
+// (jk_settle_pruned_batch): a probe launch leaves every row's upper bound
+// (bound < 0); the rows that can matter are then evaluated in FP64 around the
+// maximum, found in two steps.
+//   1. hi = max over the segment of (ll + max(b, 0)) bounds the FP64 maximum
+//      from above; the upper-bound rows with ll >= hi - window are evaluated.
+//   2. lo = max over the value rows (b >= 0) of ll - b bounds it from below;
+//      the upper-bound rows with ll >= lo - margin are evaluated.
+// Afterwards every upper-bound row lies more than margin below the maximum.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_seg_hi_partial(const double *__restrict__ ll,
+                                                        const float *__restrict__ bound,
+                                                        const SegDev *__restrict__ segs,
+                                                        double *__restrict__ partials, int pb)
+{
+    __shared__ double red[4];
+    const SegDev sg = segs[blockIdx.y];
+    double m = -INFINITY;
+    const int64_t stride = (int64_t)pb * 256;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < sg.n; i += stride)
+        m = jk_nanmax(m, ll[sg.off + i] + (double)fmaxf(bound[sg.off + i], 0.0f));
+    m = wave_nanmax(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        partials[(int64_t)blockIdx.y * pb + blockIdx.x] =
+            jk_nanmax(jk_nanmax(red[0], red[1]), jk_nanmax(red[2], red[3]));
+}
+
+// Upper-bound rows (b < 0) not below level[segment] - margin; every
+// upper-bound row when the level is NaN.
+__global__ __launch_bounds__(256) void k_seg_upper_flags(const double *__restrict__ ll,
+                                                         const float *__restrict__ bound,
+                                                         const SegDev *__restrict__ segs,
+                                                         const double *__restrict__ level,
+                                                         double margin,
+                                                         uint64_t *__restrict__ mask)
+{
+    const SegDev sg = segs[blockIdx.y];
+    const int lane = threadIdx.x & 63;
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t T = (int64_t)gridDim.x * 256;
+    if (g - lane >= sg.n) return;
+    const double thr = level[blockIdx.y] - margin;
+    for (int64_t base = g - lane; base < sg.n; base += T) {
+        const int64_t i = base + lane;
+        bool keep = false;
+        if (i < sg.n) keep = bound[sg.off + i] < 0.0f && !(ll[sg.off + i] < thr);
+        const uint64_t word = __ballot(keep);
+        if (lane == 0) mask[sg.mask_off + (base >> 6)] = word;
+    }
+}
+
+hipError_t jk_launch_settle_pruned_batch(const jk_batch *b, int n_seg, const SegDev *d_segs,
+                                         int64_t max_n, int64_t max_nb, double *partials, int pb,
+                                         uint64_t *mask, int64_t *bsum, int64_t *idx,
+                                         double *level, int64_t *cum, int64_t *count1,
+                                         const double *P, const double *e, const double *om,
+                                         const double *M0, const double *s, const int64_t *rows,
+                                         double *ll, float *bound, double window, double margin,
+                                         int64_t *n_settled, hipStream_t stream)
+{
+    if (n_seg <= 0) return hipSuccess;
+    const int fb = jk_seg_flag_blocks(max_n);
+    // 1. the rows whose bound reaches the window below the upper bound on the maximum
+    hipLaunchKernelGGL(k_seg_hi_partial, dim3(pb, n_seg), dim3(256), 0, stream, ll, bound, d_segs,
+                       partials, pb);
+    hipLaunchKernelGGL(k_seg_max_final, dim3(n_seg), dim3(64), 0, stream, partials, pb, level);
+    hipLaunchKernelGGL(k_seg_upper_flags, dim3(fb, n_seg), dim3(256), 0, stream, ll, bound, d_segs,
+                       level, window, mask);
+    hipLaunchKernelGGL(k_seg_bsum, dim3((unsigned)max_nb, n_seg), dim3(256), 0, stream, d_segs,
+                       mask, bsum);
+    hipLaunchKernelGGL(k_seg_compact, dim3((unsigned)max_nb, n_seg), dim3(256), 0, stream, d_segs,
+                       mask, bsum, idx, count1);
+    hipError_t err = launch_seg_scatter(b, n_seg, d_segs, count1, cum, idx, P, e, om, M0, s, rows,
+                                        ll, bound, stream);
+    if (err != hipSuccess) return err;
+    // 2. the rows whose bound reaches the margin below the lower bound on the maximum
+    hipLaunchKernelGGL(k_seg_lo_partial, dim3(pb, n_seg), dim3(256), 0, stream, ll, bound, d_segs,
+                       partials, pb);
+    hipLaunchKernelGGL(k_seg_max_final, dim3(n_seg), dim3(64), 0, stream, partials, pb, level);
+    hipLaunchKernelGGL(k_seg_upper_flags, dim3(fb, n_seg), dim3(256), 0, stream, ll, bound, d_segs,
+                       level, margin, mask);
+    hipLaunchKernelGGL(k_seg_bsum, dim3((unsigned)max_nb, n_seg), dim3(256), 0, stream, d_segs,
+                       mask, bsum);
+    hipLaunchKernelGGL(k_seg_compact, dim3((unsigned)max_nb, n_seg), dim3(256), 0, stream, d_segs,
+                       mask, bsum, idx, n_settled);
+    err = launch_seg_scatter(b, n_seg, d_segs, n_settled, cum, idx, P, e, om, M0, s, rows, ll,
+                             bound, stream);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(k_seg_add_counts, dim3((n_seg + 255) / 256), dim3(256), 0, stream, count1,
+                       n_settled, n_seg);
+    return hipGetLastError();
 }
 
 void jk_host_pcg_jump(u128 state, u128 inc, uint64_t k, u128 *out) { *out = h_jump(state, inc, k); }
@@ -2589,5 +2753,226 @@ hipError_t jk_launch_linear_posterior(const jk_star *st, int64_t n, const double
     const size_t lds = st->dev.Qt ? sizeof(double) * (size_t)st->dev.n * JK_COV_LANES : 0;
     hipLaunchKernelGGL(k_linear_posterior, dim3((unsigned)blocks), dim3(64), lds, stream, st->dev,
                        n, P, e, om, M0, s, rows, a_out, cov_out, ll_out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// One prior row in full detail (jk_debug_single): the quantities
+// CJokerHelper.test_likelihood_worker leaves in its public members
+// (fast_likelihood.pyx:109-114, 255-425, 547-576), for tests.  One lane, FP64,
+// diagonal data covariance C = diag(1/w) (w = ivar, or ivar / (1 + s^2 ivar)
+// in the physical jitter mode).  Lambda[0] is not clipped to max_K^2 here, as
+// in test_likelihood_worker (:567-569).
+//   M_T   (L x N)  design matrix, row 0 the unit-K Keplerian column
+//   Ainv  (L x L)  Lambda^-1 + M_T C^-1 M_T^T                  (:262-276)
+//   A     (L x L)  inverse of Ainv (here from its Cholesky factor)
+//   a     (L)      Ainv^-1 (M_T C^-1 y + Lambda^-1 mu)         (:394-420)
+//   b     (N)      M mu                                        (:305-309)
+//   B     (N x N)  C + M Lambda M^T (optional)                 (:315-327)
+//   Binv  (N x N)  C^-1 - C^-1 M A M^T C^-1 (optional)         (:329-337)
+//   sc    lnL, chi2 = (b - y)^T Binv (b - y), ln det(2 pi B), Lambda[0]
+// ln det(2 pi B) comes from the determinant lemma,
+// N ln 2 pi - sum ln w + sum ln Lambda + ln det Ainv, and chi2 from the same
+// algebra, sum w r^2 - v^T A v with v = M_T C^-1 r; the reference takes the
+// LU of B (:342-353) and sums over Binv (:387-392), the same quantities.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_debug_row(const StarDev st, const double *__restrict__ row,
+                                                 double *__restrict__ MT,
+                                                 double *__restrict__ Ainv,
+                                                 double *__restrict__ A,
+                                                 double *__restrict__ a,
+                                                 double *__restrict__ b,
+                                                 double *__restrict__ B,
+                                                 double *__restrict__ Binv,
+                                                 double *__restrict__ sc)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int L = st.L, N = st.n;
+    const bool phys = st.jitter_mode == JK_JITTER_PHYSICAL;
+    const double Pv = row[0], ev = row[1], omv = row[2], M0v = row[3];
+    const double sv = phys ? row[4] : 0.0;
+    const double nan = __builtin_nan("");
+    const bool valid = isfinite(Pv) && Pv != 0.0 && ev >= 0.0 && ev < 1.0 && isfinite(omv) &&
+                       isfinite(M0v) && isfinite(sv);
+    const double lam0 = st.k_prior_mode == JK_KPRIOR_FIXED_COMPANION_MASS
+                            ? (valid ? jk_k_variance(&st, Pv, ev, false) : nan)
+                            : st.lam0;
+    double lam[JK_LMAX];
+    lam[0] = lam0;
+    for (int i = 1; i < L; ++i) lam[i] = 1.0 / st.lam_inv[i];
+    double sw = 0.0, cw = 1.0;
+    if (valid) sincos(omv, &sw, &cw);
+    const double ecw = ev * cw, sq = valid ? sqrt((1.0 - ev) * (1.0 + ev)) : nan;
+    const double nP = JK_TWO_PI / Pv;
+    double G[JK_LMAX * JK_LMAX], rhs[JK_LMAX], v[JK_LMAX];
+    for (int q = 0; q < JK_LMAX * JK_LMAX; ++q) G[q] = 0.0;
+    for (int q = 0; q < JK_LMAX; ++q) rhs[q] = v[q] = 0.0;
+    double wrr = 0.0, sum_ln_w = 0.0;
+    for (int k = 0; k < N; ++k) {
+        double x0 = nan;
+        if (valid) {
+            double sE, cE;
+            jk_kepler_f64(jk_reduce_2pi(fma(nP, st.dt[k], -M0v)), ev, sE, cE);
+            x0 = jk_design_f64(sE, cE, ev, sq, cw, sw, ecw);
+        }
+        MT[k] = x0;
+        for (int j = 1; j < L; ++j) MT[(size_t)j * N + k] = st.T[(size_t)(j - 1) * N + k];
+        const double iv = st.ivar[k];
+        const double w = phys ? iv / fma(sv * sv, iv, 1.0) : iv;
+        double bk = 0.0;
+        for (int i = 0; i < L; ++i) bk = fma(MT[(size_t)i * N + k], st.mu[i], bk);
+        b[k] = bk;
+        const double r = bk - st.y[k];
+        wrr = fma(w * r, r, wrr);
+        sum_ln_w += log(w);
+        for (int i = 0; i < L; ++i) {
+            const double wx = w * MT[(size_t)i * N + k];
+            rhs[i] = fma(wx, st.y[k], rhs[i]);
+            v[i] = fma(wx, r, v[i]);
+            for (int j = 0; j < L; ++j) G[i * JK_LMAX + j] = fma(wx, MT[(size_t)j * N + k], G[i * JK_LMAX + j]);
+        }
+    }
+    double lnlam = 0.0;
+    for (int i = 0; i < L; ++i) {
+        G[i * JK_LMAX + i] += 1.0 / lam[i];
+        rhs[i] += st.mu[i] / lam[i];
+        lnlam += log(lam[i]);
+    }
+    for (int i = 0; i < L; ++i)
+        for (int j = 0; j < L; ++j) Ainv[i * L + j] = G[i * JK_LMAX + j];
+    // Cholesky Ainv = R R^T (R lower, kept in Rl)
+    double Rl[JK_LMAX * JK_LMAX];
+    for (int q = 0; q < JK_LMAX * JK_LMAX; ++q) Rl[q] = 0.0;
+    double lndet = 0.0;
+    bool ok = valid;
+    for (int j = 0; j < L && ok; ++j) {
+        double d = G[j * JK_LMAX + j];
+        for (int k = 0; k < j; ++k) d -= Rl[j * JK_LMAX + k] * Rl[j * JK_LMAX + k];
+        if (!(d > 0.0)) { ok = false; break; }
+        d = sqrt(d);
+        Rl[j * JK_LMAX + j] = d;
+        lndet += 2.0 * log(d);
+        for (int q = j + 1; q < L; ++q) {
+            double t = G[q * JK_LMAX + j];
+            for (int k = 0; k < j; ++k) t -= Rl[q * JK_LMAX + k] * Rl[j * JK_LMAX + k];
+            Rl[q * JK_LMAX + j] = t / d;
+        }
+    }
+    double Am[JK_LMAX * JK_LMAX];
+    if (ok) {
+        // A = R^-T R^-1 from the lower inverse Ri
+        double Ri[JK_LMAX * JK_LMAX];
+        for (int q = 0; q < JK_LMAX * JK_LMAX; ++q) Ri[q] = 0.0;
+        for (int c = 0; c < L; ++c)
+            for (int r = c; r < L; ++r) {
+                double t = r == c ? 1.0 : 0.0;
+                for (int k = c; k < r; ++k) t -= Rl[r * JK_LMAX + k] * Ri[k * JK_LMAX + c];
+                Ri[r * JK_LMAX + c] = t / Rl[r * JK_LMAX + r];
+            }
+        for (int i = 0; i < L; ++i)
+            for (int j = 0; j < L; ++j) {
+                double t = 0.0;
+                for (int k = (i > j ? i : j); k < L; ++k) t += Ri[k * JK_LMAX + i] * Ri[k * JK_LMAX + j];
+                Am[i * JK_LMAX + j] = t;
+            }
+    } else {
+        for (int q = 0; q < JK_LMAX * JK_LMAX; ++q) Am[q] = nan;
+    }
+    double chi2 = wrr;
+    for (int i = 0; i < L; ++i) {
+        double ai = 0.0;
+        for (int j = 0; j < L; ++j) {
+            A[i * L + j] = Am[i * JK_LMAX + j];
+            ai = fma(Am[i * JK_LMAX + j], rhs[j], ai);
+            chi2 -= v[i] * Am[i * JK_LMAX + j] * v[j];
+        }
+        a[i] = ai;
+    }
+    if (B) {
+        for (int n = 0; n < N; ++n)
+            for (int m = 0; m < N; ++m) {
+                double t = 0.0;
+                for (int i = 0; i < L; ++i)
+                    t = fma(MT[(size_t)i * N + n] * lam[i], MT[(size_t)i * N + m], t);
+                if (n == m) {
+                    const double iv = st.ivar[n];
+                    t += phys ? fma(sv, sv, 1.0 / iv) : 1.0 / iv;
+                }
+                B[(size_t)n * N + m] = t;
+            }
+    }
+    if (Binv) {
+        for (int n = 0; n < N; ++n) {
+            const double ivn = st.ivar[n];
+            const double wn = phys ? ivn / fma(sv * sv, ivn, 1.0) : ivn;
+            for (int m = 0; m < N; ++m) {
+                const double ivm = st.ivar[m];
+                const double wm = phys ? ivm / fma(sv * sv, ivm, 1.0) : ivm;
+                double t = 0.0;
+                for (int i = 0; i < L; ++i)
+                    for (int j = 0; j < L; ++j)
+                        t = fma(MT[(size_t)i * N + n] * Am[i * JK_LMAX + j], MT[(size_t)j * N + m], t);
+                Binv[(size_t)n * N + m] = (n == m ? wn : 0.0) - wn * t * wm;
+            }
+        }
+    }
+    const double logdet = (double)N * JK_LN_TWO_PI - sum_ln_w + lnlam + lndet;
+    double ll = -0.5 * (chi2 + logdet);
+    if (!valid) ll = nan;
+    else if (!ok) ll = INFINITY;
+    sc[0] = ll;
+    sc[1] = ok ? chi2 : nan;
+    sc[2] = ok ? logdet : nan;
+    sc[3] = lam0;
+}
+
+hipError_t jk_launch_debug_row(const jk_star *st, const double *row, double *MT, double *Ainv,
+                               double *A, double *a, double *b, double *B, double *Binv,
+                               double *sc, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_debug_row, dim3(1), dim3(64), 0, stream, st->dev, row, MT, Ainv, A, a,
+                       b, B, Binv, sc);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Per-sample failure counts of an lnL array (the reference writes +INF when
+// its L x L or N x N factorisation fails, fast_likelihood.pyx:283-291,
+// 347-348, 381-382, 422-423; invalid rows are NaN here): counts[0] += rows
+// at +inf, counts[1] += rows at -inf, counts[2] += NaN rows.  counts is
+// cleared by the launcher's first kernel.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_zero_counts3(unsigned long long *counts)
+{
+    if (threadIdx.x < 3) counts[threadIdx.x] = 0ull;
+}
+
+__global__ __launch_bounds__(256) void k_count_failures(const double *__restrict__ ll, int64_t n,
+                                                        unsigned long long *__restrict__ counts)
+{
+    unsigned long long c[3] = {0ull, 0ull, 0ull};
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const double x = ll[i];
+        c[0] += x == INFINITY;
+        c[1] += x == -INFINITY;
+        c[2] += x != x;
+    }
+    for (int q = 0; q < 3; ++q) {
+        unsigned long long t = c[q];
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+        if ((threadIdx.x & 63) == 0 && t) atomicAdd(counts + q, t);
+    }
+}
+
+hipError_t jk_launch_count_failures(const double *ll, int64_t n, unsigned long long *counts,
+                                    hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_zero_counts3, dim3(1), dim3(256), 0, stream, counts);
+    if (n > 0) {
+        int64_t blocks = (n + 255) / 256;
+        if (blocks > 2048) blocks = 2048;
+        hipLaunchKernelGGL(k_count_failures, dim3((unsigned)blocks), dim3(256), 0, stream, ll, n,
+                           counts);
+    }
     return hipGetLastError();
 }

@@ -25,7 +25,7 @@ from .samples import NONLINEAR_INTERNAL_UNITS, NONLINEAR_PACKED_ORDER
 __all__ = ["DeviceStar", "PriorCache", "star_arrays", "probe_star", "probe_epochs", "lnl", "lnl_reject", "lnl_reject_async",
            "PendingReject", "polish", "reject",
            "posterior_inputs_many", "posterior_inputs_table", "to_host",
-           "linear_posterior", "uniforms",
+           "linear_posterior", "uniforms", "debug_single", "count_failures",
            "pcg64_state", "pcg64_advance_state", "JITTER_MODES"]
 
 JITTER_MODES = {"reference": _lib.JK_JITTER_REFERENCE, "physical": _lib.JK_JITTER_PHYSICAL}
@@ -703,3 +703,48 @@ def linear_posterior(star, cache, rows):
     h = to_host(res)
     return (h[:k * L].reshape(k, L).copy(), h[k * L:k * (L + L * L)].reshape(k, L, L).copy(),
             h[k * (L + L * L):].copy())
+
+
+def debug_single(star, row):
+    """jk_debug_single: every intermediate of one prior row (P, e, omega, M0,
+    s), as CJokerHelper.test_likelihood_worker leaves them
+    (fast_likelihood.pyx:547-576).  Returns a dict of host arrays M_T (L, n),
+    Ainv, A (L, L), a (L,), b (n,), B, Binv (n, n) and floats ll, chi2,
+    logdet, Lambda0."""
+    torch = _torch()
+    lib = _lib.ensure_device(star.device)
+    n, L = star.n, star.L
+    dev = torch.device("cuda", star.device)
+    sizes = [("M_T", L * n), ("Ainv", L * L), ("A", L * L), ("a", L), ("b", n), ("B", n * n),
+             ("Binv", n * n), ("sc", 4)]
+    total = sum(k for _, k in sizes)
+    res = torch.empty(total + 5, dtype=torch.float64, device=dev)
+    res[total:].copy_(torch.as_tensor(np.asarray(row, dtype=np.float64).reshape(5)))
+    ptrs, off = {}, 0
+    for name, k in sizes:
+        ptrs[name] = ctypes.c_void_p(res.data_ptr() + 8 * off)
+        off += k
+    _lib.check(lib.jk_debug_single(star.handle, ctypes.c_void_p(res.data_ptr() + 8 * total),
+                                   *[ptrs[k] for k, _ in sizes], _stream(star.device)),
+               "jk_debug_single")
+    h = to_host(res)
+    out, off = {}, 0
+    shapes = {"M_T": (L, n), "Ainv": (L, L), "A": (L, L), "a": (L,), "b": (n,), "B": (n, n),
+              "Binv": (n, n), "sc": (4,)}
+    for name, k in sizes:
+        out[name] = h[off:off + k].reshape(shapes[name]).copy()
+        off += k
+    sc = out.pop("sc")
+    out.update(ll=float(sc[0]), chi2=float(sc[1]), logdet=float(sc[2]), Lambda0=float(sc[3]))
+    return out
+
+
+def count_failures(ll):
+    """jk_count_failures on a device lnL tensor: (n_posinf, n_neginf, n_nan)."""
+    torch = _torch()
+    lib = _lib.ensure_device(ll.device.index or 0)
+    counts = torch.empty(3, dtype=torch.int64, device=ll.device)
+    _lib.check(lib.jk_count_failures(_dptr(ll), int(ll.shape[0]), _dptr(counts),
+                                     _stream(ll.device.index or 0)), "jk_count_failures")
+    h = to_host(counts)
+    return int(h[0]), int(h[1]), int(h[2])

@@ -62,18 +62,24 @@ class TheJoker:
         difference to FP64; the rows whose rejection decision (or claim to the
         maximum) the bound leaves open are re-evaluated in FP64 before the
         rejection step, so the accepted set equals the FP64 one.  Reported
-        ln_likelihood values of accepted samples are FP64.
+        ln_likelihood values of accepted samples are FP64, and every value of
+        an lnL array returned (marginal_ln_likelihood, return_all_logprobs)
+        is within 1e-3 of FP64: rows whose bound exceeds that are
+        re-evaluated in FP64 first.
     device : int
         HIP device ordinal.
     prune : bool
-        iterative_rejection_sample only: from the second round on, skip the
-        full evaluation of rows whose upper bound from a probe on a subset of
-        the epochs (device.probe_star) lies more than 750 below the maximum of
-        the earlier rounds -- rows no uniform can accept.  They keep the bound
-        as their value; the samples returned do not change.  Stars with fewer
-        than 12 epochs, full covariances and randomize_prior_order are not
-        probed.  Off by default, so the default path evaluates every row as
-        the reference does.
+        Bound every row by a probe on a subset of the epochs
+        (device.probe_star) and skip the full evaluation of the rows no
+        uniform can accept.  rejection_sample: the probe bounds every row,
+        then the rows within reach of the maximum are evaluated in FP64
+        (batch.prune_reject_batch); iterative_rejection_sample: from the
+        second round on, rows whose bound lies more than 750 below the
+        maximum of the earlier rounds keep the bound as their value.  The
+        samples returned do not change.  Stars with fewer than 12 epochs,
+        full covariances and randomize_prior_order are not probed.  Off by
+        default, so the default path evaluates every row as the reference
+        does.
     """
 
     PRUNE_MIN_EPOCHS = 12
@@ -192,6 +198,19 @@ class TheJoker:
                           [float(floor)], rows=rows, device=device)
 
     @staticmethod
+    def _settle_tol(star, cache, ll, bound, rows=None, device=0):
+        """Mixed precision, an lnL array handed back to the caller: every row
+        whose error bound exceeds 1e-3 re-evaluated in FP64, so that every
+        value is within 1e-3 of FP64 (jk_settle_tol_batch on a one-star
+        table)."""
+        from .batch import StarBatch, settle_tol_batch
+        table = getattr(star, "_settle_table", None)
+        if table is None:
+            table = star._settle_table = StarBatch([star], device=device)
+        return settle_tol_batch(table, cache, ll, bound, [0], [int(ll.shape[0])], rows=rows,
+                                device=device)
+
+    @staticmethod
     def _fp64_at(star, cache, cache_rows):
         """FP64 lnL of the given cache rows (the values reported for accepted
         samples, whatever the precision of the search)."""
@@ -200,6 +219,33 @@ class TheJoker:
             return np.zeros(0)
         r = torch.as_tensor(np.asarray(cache_rows, dtype=np.int64), device=cache.P.device)
         return to_host(lnl(star, cache, rows=r, precision="f64"))
+
+    def _prunable(self, star):
+        return self.prune and star.cov is None and star.n >= self.PRUNE_MIN_EPOCHS
+
+    def _pruned_reject(self, star, cache, n, rng_state, max_keep):
+        """rejection_sample's likelihood and rejection step over cache rows
+        [0, n) with the rows bounded by a probe (batch.prune_reject_batch on a
+        one-star table): (lnL buffer, its bounds, accepted indices)."""
+        from .batch import StarBatch, prune_reject_batch, upload_items, work_items
+        from .batch import LaunchShape
+        from .device import probe_star
+        table = getattr(star, "_settle_table", None)
+        if table is None:
+            table = star._settle_table = StarBatch([star], device=self.device)
+        probes = getattr(star, "_probe_table", None)
+        if probes is None:
+            probe, add = probe_star(star.arrays(), self.jitter_mode, self.device)
+            probes = star._probe_table = StarBatch([probe], device=self.device)
+            star._probe_add = add
+        shape = LaunchShape()
+        items, off, _ = work_items([n], shape.tile)
+        ll, bnd, res, _ = prune_reject_batch(table, probes, np.array([star._probe_add]),
+                                             np.array([True]), cache,
+                                             upload_items(items, self.device), len(items), off,
+                                             np.array([n]), [rng_state], max_keep,
+                                             self.precision, shape, self.device)
+        return ll, bnd, res[0][0]
 
     def _pruner(self, star, it):
         """The probe launcher of iterative_rejection_sample, or None when the
@@ -259,7 +305,13 @@ class TheJoker:
         """thejoker.py:93-138: lnL at every prior sample (np.ndarray)."""
         star = self._make_joker_helper(data)
         cache = self._cache(star, prior_samples)
-        return self._lnl_rows(star, cache).cpu().numpy()
+        if self.precision == "f64":
+            return to_host(self._lnl_rows(star, cache)).copy()
+        import torch
+        bound = torch.empty(len(cache), dtype=torch.float32, device=cache.P.device)
+        ll = self._lnl_rows(star, cache, bound=bound)
+        self._settle_tol(star, cache, ll, bound, device=self.device)
+        return to_host(ll).copy()
 
     def rejection_sample(self, data, prior_samples, n_prior_samples=None,
                          max_posterior_samples=None, n_linear_samples=1, return_logprobs=False,
@@ -304,7 +356,11 @@ class TheJoker:
         d_rows = None
         if rows is not None:
             d_rows = torch.as_tensor(np.asarray(rows, dtype=np.int64), device=cache.P.device)
-        if self.precision == "f64":
+        bound = None
+        if self._prunable(star) and rows is None:
+            lls, bound, good = self._pruned_reject(star, cache, n_use, pcg64_state(rng),
+                                                   max_posterior_samples)
+        elif self.precision == "f64":
             # likelihood and rejection step in one call
             lls, good, _, _ = lnl_reject(star, cache, pcg64_state(rng), max_posterior_samples,
                                          rows=d_rows, start=0, n=n_use)
@@ -332,7 +388,10 @@ class TheJoker:
             samples["ln_likelihood"] = ll_good
             samples["ln_prior"] = cache.ln_prior[full_idx]
         if return_all_logprobs:
-            return samples, to_host(lls)
+            if bound is not None:
+                # mixed values to 1e-3, and the probe's upper bounds evaluated
+                self._settle_tol(star, cache, lls, bound, rows=d_rows, device=self.device)
+            return samples, to_host(lls).copy()
         return samples
 
     def iterative_rejection_sample(self, data, prior_samples, n_requested_samples,
