@@ -199,6 +199,25 @@ int jk_lnl_reject_f64(const jk_star *star, int64_t n, const double *P, const dou
                       uint64_t inc_hi, uint64_t inc_lo, uint64_t stream_offset, int64_t max_keep,
                       int64_t *idx_out, int64_t *n_pass_out, double *llmax_out, void *stream);
 
+/* jk_lnl_reject_f64 without any allocation: the same arguments, kernels and
+ * outputs, with the scratch (per-wave maxima, ballot words, block counts) in a
+ * file-scope device array of 24 MiB that each GPU holds from jk_init to the
+ * end of the process.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  n <= JK_NOALLOC_MAX_ROWS, else
+ * JK_ERANGE.  All calls on one GPU share its array, whatever the stream or
+ * thread: calls that overlap in time on one GPU race and corrupt each other's
+ * results, so the caller serialises them (one stream, one thread).  Nothing
+ * in the array needs clearing between calls. */
+#define JK_NOALLOC_MAX_ROWS ((int64_t)1 << 27)
+int jk_lnl_reject_f64_noalloc(const jk_star *star, int64_t n, const double *P,
+                              const double *e, const double *omega, const double *M0,
+                              const double *s, const int64_t *rows, double *ll_out,
+                              uint64_t state_hi, uint64_t state_lo, uint64_t inc_hi,
+                              uint64_t inc_lo, uint64_t stream_offset, int64_t max_keep,
+                              int64_t *idx_out, int64_t *n_pass_out, double *llmax_out,
+                              void *stream);
+
 /* One prior row in full detail, for tests.  Replaces
  * CJokerHelper.test_likelihood_worker and the public members it fills
  * (fast_likelihood.pyx:109-114, 547-576).  row: device array of 5 doubles

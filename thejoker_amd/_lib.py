@@ -51,6 +51,9 @@ SIGNATURES = [
     ("jk_lnl_reject_f64", ctypes.c_int,
      [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _u64, _u64, _u64, _u64, _i64, _vp, _vp,
       _vp, _vp]),
+    ("jk_lnl_reject_f64_noalloc", ctypes.c_int,
+     [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u64, _u64, _u64, _u64, _u64, _i64, _vp, _vp,
+      _vp, _vp]),
     ("jk_debug_single", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("jk_count_failures", ctypes.c_int, [_vp, _i64, _vp, _vp]),
     ("jk_reject_with_max", ctypes.c_int,

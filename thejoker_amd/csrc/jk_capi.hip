@@ -15,10 +15,22 @@
 #define JK_ABI_VERSION 5
 #define JK_POOL_KEEP (32ull << 30)
 
+// Scratch of jk_lnl_reject_f64_noalloc: a file-scope device array, so that
+// the entry point allocates nothing.  Each GPU the library runs on holds its
+// own copy, at its own address, until the process ends.  The array's name is
+// a host-side stand-in, not a device address: jk_init looks the address up
+// once per device (hipGetSymbolAddress, with that device current) and every
+// launch takes that pointer.  No kernel needs any of it cleared: every byte
+// a launch reads (per-wave maxima, ballot words, block counts) is written
+// earlier in the same launch sequence.
+#define JK_NOALLOC_SCRATCH_BYTES ((size_t)24 << 20)
+static __device__ __align__(256) unsigned char scratch[JK_NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 std::mutex g_init_mu;
 bool g_inited[64] = {false};
+void *g_noalloc_scratch[64] = {nullptr};   // device address of `scratch` per device
 
 int check_hip(hipError_t e) { return e == hipSuccess ? JK_OK : JK_EHIP; }
 
@@ -171,6 +183,9 @@ int jk_init(int device)
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) return JK_ENODEV;
     if (jk_upload_pcg_table(device) != hipSuccess) return JK_EHIP;
     if (jk_upload_sincos_table(device) != hipSuccess) return JK_EHIP;
+    void *sp = nullptr;
+    if (hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)) != hipSuccess || !sp) return JK_EHIP;
+    g_noalloc_scratch[device] = sp;
     // The device's default stream-ordered pool serves the small workspaces of
     // calls made without one and the scratch of calls under graph capture;
     // keep up to JK_POOL_KEEP bytes of it instead of returning them at every
@@ -550,6 +565,30 @@ int jk_lnl_reject_f64(const jk_star *star, int64_t n, const double *P, const dou
     hipError_t err2 = scratch_put(st, scratch, lease);
     if (err != hipSuccess) return JK_EHIP;
     return check_hip(err2);
+}
+
+int jk_lnl_reject_f64_noalloc(const jk_star *star, int64_t n, const double *P,
+                              const double *e, const double *omega, const double *M0,
+                              const double *s, const int64_t *rows, double *ll_out,
+                              uint64_t state_hi, uint64_t state_lo, uint64_t inc_hi,
+                              uint64_t inc_lo, uint64_t stream_offset, int64_t max_keep,
+                              int64_t *idx_out, int64_t *n_pass_out, double *llmax_out,
+                              void *stream)
+{
+    if (!star || n <= 0 || !P || !e || !omega || !M0 || !ll_out) return JK_EINVAL;
+    if (star->dev.jitter_mode == JK_JITTER_PHYSICAL && !s) return JK_EINVAL;
+    if (max_keep < 0 || !n_pass_out || !llmax_out || (max_keep > 0 && !idx_out)) return JK_EINVAL;
+    if (n > JK_NOALLOC_MAX_ROWS ||
+        (size_t)jk_lnl_reject_scratch_bytes(n) > JK_NOALLOC_SCRATCH_BYTES)
+        return JK_ERANGE;
+    const int dev = current_device_inited();
+    if (dev < 0 || !g_noalloc_scratch[dev]) return JK_ENODEV;
+    if (star->device != dev) return JK_EINVAL;
+    return check_hip(jk_launch_lnl_reject_f64(star, n, P, e, omega, M0, s, rows, ll_out,
+                                              make_u128(state_hi, state_lo),
+                                              make_u128(inc_hi, inc_lo), stream_offset, max_keep,
+                                              idx_out, n_pass_out, llmax_out,
+                                              g_noalloc_scratch[dev], (hipStream_t)stream));
 }
 
 int jk_debug_single(const jk_star *star, const double *row, double *MT_out, double *Ainv_out,
