@@ -55,6 +55,9 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 BYTES_PER_SAMPLE_F64 = 56   # 5 x f64 read + lnL write + lnL read by the rejection step
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_lnl_f64_single_star.json")
+VALU_FILE = os.path.join(ROOT, "profiles", "valu_counters.json")
+PEAK_CLOCK_HZ = 2.4e9      # the clock MI355X's published peaks assume
+N_SIMD = 1024              # 256 CUs x 4 SIMDs
 WINDOW = 40.0
 
 
@@ -153,6 +156,32 @@ def fmt_mixed(r):
             f"{r['rows_settled_in_fp64']} rows, flips {r['accept_flips_vs_fp64']}")
 
 
+def valu_from_counters(key, kernel_ms):
+    """The committed rocprofv3 VALU counters of one likelihood kernel
+    (profiles/valu_counters.json, same workload), with the rates taken over
+    this run's own kernel time: hardware FP64 / FP32 FLOP/s from the
+    FLOPS_* counters, and the issue-cycle model's share of the SIMDs' cycles
+    at the published peak clock."""
+    if not os.path.exists(VALU_FILE):
+        return None
+    with open(VALU_FILE) as f:
+        k = json.load(f)["kernels"].get(key)
+    if k is None:
+        return None
+    t = kernel_ms * 1e-3
+    return {"source": "profiles/valu_counters.json", "counters_kernel_ms": k["dispatch_s"] * 1e3,
+            "valu_insts_per_launch": k["valu_insts"],
+            "valu_insts_per_wave_epoch": k["valu_insts_per_wave_epoch"],
+            "fp64_insts_per_wave_epoch": k["f64_insts_per_wave_epoch"],
+            "fp32_insts_per_wave_epoch": k["f32_insts_per_wave_epoch"],
+            "hw_fp64_tflops": k["hw_fp64_flops"] / t / 1e12,
+            "hw_fp64_frac_of_peak": k["hw_fp64_flops"] / t / 1e12 / PEAK_FP64_TFLOPS,
+            "hw_fp32_tflops": k["hw_fp32_flops"] / t / 1e12,
+            "hw_fp32_frac_of_peak": k["hw_fp32_flops"] / t / 1e12 / 157.3,
+            "issue_frac_at_peak_clock": k["issue_cycles_model"] / (N_SIMD * PEAK_CLOCK_HZ * t),
+            "issue_frac_at_profiled_clock": k["issue_frac"]}
+
+
 def timed(fn, reps, torch):
     """Mean ms of fn() over reps launches, HIP events on the current stream."""
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -228,7 +257,8 @@ def leg_single(args, torch, np, D, syn):
                 "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                 "kernel": "k_lnl<2,true,false,false> (FP64)", "kernel_ms": k_ms,
                 "flops_per_launch": flops,
-                "hbm_gbs_prior_stream": n * BYTES_PER_SAMPLE_F64 / (k_ms * 1e-3) / 1e9}
+                "hbm_gbs_prior_stream": n * BYTES_PER_SAMPLE_F64 / (k_ms * 1e-3) / 1e9,
+                "valu": valu_from_counters("single_f64", k_ms) if n == 1 << 24 else None}
     log(f"[bench] single: lnL kernel {k_ms:.3f} ms -> {achieved:.2f} TFLOP/s algorithmic "
         f"({100 * achieved / PEAK_FP64_TFLOPS:.1f}% of FP64 peak)")
 
@@ -252,6 +282,7 @@ def leg_single(args, torch, np, D, syn):
     D.lnl(dstar, cache, precision="f64", out=ll)
     out["mixed"] = {"kernel_ms": m_ms, "sample_epochs_per_s": n * star.M / (m_ms * 1e-3),
                     "speedup_vs_f64_kernel": k_ms / m_ms,
+                    "valu": valu_from_counters("single_mixed", m_ms) if n == 1 << 24 else None,
                     **mixed_vs_f64(torch, np, D, dstar, cache, ll, ll32, bnd, seed=7)}
     log(f"[bench] mixed: {m_ms:.3f} ms ({k_ms / m_ms:.2f}x), {fmt_mixed(out['mixed'])}")
 
@@ -372,10 +403,52 @@ def leg_batched(args, torch, np, D, syn):
     graph_flips = sum(len(np.setxor1d(x, y)) for x, y in zip(good_g, good_f))
     graph.plan.close()
     del graph
+    # the lnL arrays the API hands back in mixed precision: every row whose
+    # bound exceeds 1e-3 evaluated in FP64 first (jk_settle_tol_batch)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    n_tol = int(B.settle_tol_batch(batch, cache, llm, bnd, off, sizes).sum())
+    torch.cuda.synchronize()
+    tol_ms = (time.perf_counter() - t0) * 1e3
+    dmax_all_api = max(float((llf[off[s]:off[s] + n] - llm[off[s]:off[s] + n]).abs().max())
+                       for s in range(n_stars))
+    api = {"tolerance": 1e-3, "ms": tol_ms, "rows_evaluated_in_fp64": n_tol,
+           "rows_fraction": n_tol / total, "max_abs_dlnl_all_after": dmax_all_api}
+    log(f"[bench] batched: API arrays gated at 1e-3 in {tol_ms:.1f} ms ({n_tol} rows), max "
+        f"|dlnL| all rows {dmax_all_api:.2e}")
+    # the mixed step with the rows bounded by a probe (stars with >= 12 epochs)
+    del llm, bnd
+    torch.cuda.empty_cache()
+    probes, adds, probed = B.probe_table(dstars)
+
+    def step_p():
+        ll_p, b_p, res_p, cnt_p = B.prune_reject_batch(batch, probes, adds, probed, cache,
+                                                       d_items, len(items), off, sizes,
+                                                       states(), 256, "mixed", shape)
+        return res_p, int(cnt_p.sum()), int((b_p < 0).sum())
+
+    step_p()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(2):
+        res_p, n_eval_p, n_left_p = step_p()
+    torch.cuda.synchronize()
+    e2e_p_ms = (time.perf_counter() - t0) / 2 * 1e3
+    flips_p = sum(len(np.setxor1d(x, y)) for x, y in zip(res_p[0], good_f))
+    pruned = {"end_to_end_ms": e2e_p_ms,
+              "end_to_end_sample_epochs_per_s": work / (e2e_p_ms * 1e-3),
+              "stars_probed": int(probed.sum()),
+              "rows_left_as_upper_bounds": n_left_p,
+              "rows_evaluated_after_probe": n_eval_p,
+              "accept_flips_vs_fp64": int(flips_p)}
+    log(f"[bench] batched: pruned mixed step {e2e_p_ms:.2f} ms, {n_left_p / total:.3f} of rows "
+        f"left as bounds, flips {flips_p}")
     res = {"stars": n_stars, "prior_rows": n, "epochs_total": int(M.sum()), "linear_params": 3,
            "launch_shape": shape.as_dict(), "autotune_s": tune_s,
            "autotune_ms": {k: round(v, 3) for k, v in table.items()},
            "mixed_lnl_ms": m_ms, "f64_lnl_ms": f_ms,
+           "valu": {"mixed": valu_from_counters("batched_mixed", m_ms),
+                    "f64": valu_from_counters("batched_f64", f_ms)},
            "mixed_sample_epochs_per_s": work / (m_ms * 1e-3),
            "f64_sample_epochs_per_s": work / (f_ms * 1e-3),
            "mixed_speedup_vs_f64": f_ms / m_ms,
@@ -390,14 +463,18 @@ def leg_batched(args, torch, np, D, syn):
            "max_dlnl_over_bound": ratio,
            "rows_settled_in_fp64": settled[0], "rows_settled_fraction": settled[0] / total,
            "accept_flips_vs_fp64": int(flips),
-           "accepted_total": int(np.minimum(np_f, 256).sum())}
+           "accepted_total": int(np.minimum(np_f, 256).sum()),
+           "api_gate": api, "pruned": pruned,
+           "speedup_vs_plain_f64": {"mixed": e2e_f_ms / e2e_ms,
+                                    "mixed_graph": e2e_f_ms / graph_ms,
+                                    "mixed_pruned": e2e_f_ms / e2e_p_ms}}
     log(f"[bench] batched: mixed {m_ms:.2f} ms ({res['mixed_sample_epochs_per_s']:.3e}/s), f64 "
         f"{f_ms:.2f} ms; end to end mixed {e2e_ms:.2f} ms (graph replay {graph_ms:.2f} ms), "
         f"f64 {e2e_f_ms:.2f} ms; |dlnL| all "
         f"{dmax_all:.2e} live {dmax_live:.2e} (settled {dmax_live_settled:.2e}), max "
         f"|dlnL|/bound {ratio:.3f}, settled "
         f"{settled[0] / total:.2e} of rows, flips {flips}")
-    del llm, llf, cache, batch
+    del llf, cache, batch, probes
     torch.cuda.empty_cache()
     return res
 
@@ -456,6 +533,14 @@ def leg_serving(args, torch, np, D, syn):
         f"{res['p99_latency_s'] * 1e3:.1f} ms, {res['stars_per_s']:.1f} stars/s, "
         f"{res['sample_epochs_per_s']:.3e} sample-epochs/s, {srv.steps} steps, "
         f"{srv.preemptions} restarts")
+    # where the slowest requests' time goes: waiting for admission, then
+    # their own rounds
+    ok = [r for r in done if r.error is None]
+    slow = sorted(ok, key=lambda r: -r.latency)[:max(1, len(ok) // 100)]
+    res["slowest_1pct"] = [{"M": int(len(r.data)), "latency_s": round(r.latency, 4),
+                            "queued_s": round(r.t_admit - r.arrival, 4),
+                            "rounds": int(r.it.iteration), "rows": int(r.rows_evaluated)}
+                           for r in slow]
     res["preemptions"] = srv.preemptions
     res["arena_gib"] = args.serving_arena_gib
     res["max_step_work"] = step_work
@@ -556,6 +641,34 @@ def leg_serving(args, torch, np, D, syn):
         se / t_alone)
     res["fp64_alone_checked"] = checked
     res["accept_flips_vs_fp64_alone"] = flips
+    # The mixed-precision lnL the served stars' decisions rest on, against
+    # FP64 on the same rows: the same iterative path run alone (its samples
+    # equal the server's) keeps its final buffer; every row that holds a
+    # value (not a probe's upper bound) is compared, and the rows within 40
+    # of the maximum separately.
+    d_all, d_live, n_cmp, n_same = 0.0, 0.0, 0, 0
+    for i in range(0, n_req, max(1, n_req // 64)):
+        if done[i].error is not None:
+            continue
+        joker = tj.TheJoker(prior, rng=np.random.default_rng(10_000 + i), precision="mixed",
+                            prune=prune)
+        alone = joker.iterative_rejection_sample(datas[i], cache, n_requested_samples=256)
+        n_same += int(np.array_equal(alone["P"].value, done[i].result["P"].value))
+        ll_m, b_m = joker.last_lnl
+        dstar = joker.last_star
+        ll_f = D.lnl(dstar, cache, n=int(ll_m.shape[0]), precision="f64")
+        val = b_m >= 0
+        dd = (ll_f - ll_m).abs()
+        d_all = max(d_all, float(dd[val].max()))
+        live = val & (ll_f >= ll_f.max() - WINDOW)
+        d_live = max(d_live, float(dd[live].max()))
+        n_cmp += 1
+    res["mixed_vs_fp64"] = {"stars": n_cmp, "results_equal_server": n_same,
+                            "max_abs_dlnl_evaluated_rows": d_all,
+                            "max_abs_dlnl_live_rows": d_live,
+                            "returned_ln_likelihood": "FP64 values (max |dlnL| 0)"}
+    log(f"[bench] serving: mixed vs FP64 on {n_cmp} stars' buffers: max |dlnL| {d_all:.2e} "
+        f"(live {d_live:.2e}); {n_same} results equal the server's")
     log(f"[bench] serving: plain FP64 one star at a time {se / t_alone:.3e} sample-epochs/s "
         f"({checked / t_alone:.1f} stars/s) -> serving "
         f"{res['speedup_vs_plain_f64_one_at_a_time']:.1f}x; flips vs FP64 alone on {checked} "
@@ -675,7 +788,7 @@ def main():
     ap.add_argument("--serving-prune", type=int, default=1,
                     help="1: the server prunes rows by the probe bound (results unchanged); "
                          "the saturated stream is also run without pruning for comparison")
-    ap.add_argument("--serving-plain", type=int, default=64,
+    ap.add_argument("--serving-plain", type=int, default=1000,
                     help="stars of the serving stream also run one at a time in plain FP64")
     ap.add_argument("--stress-log2-rows", type=int, default=29)
     ap.add_argument("--no-cpu", action="store_true")

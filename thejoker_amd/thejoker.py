@@ -417,25 +417,37 @@ class TheJoker:
         dev = cache.P.device
         all_ll = torch.empty(it.max_prior, dtype=torch.float64, device=dev)
         mixed = self.precision == "mixed"
-        all_b = torch.empty(it.max_prior, dtype=torch.float32, device=dev) if mixed else None
         pruner = self._pruner(star, it)
+        # per-row bounds: the mixed error bound, or (f64 with pruning) the
+        # marks 0 (an FP64 value) and -1 (a probe's upper bound)
+        all_b = (torch.empty(it.max_prior, dtype=torch.float32, device=dev)
+                 if mixed or pruner is not None else None)
         while not it.done:
             rows, start, n_now = it.next_batch()
             a, b = it.n_evals, it.n_evals + n_now
-            part_b = all_b[a:b] if mixed else None
+            part_b = all_b[a:b] if all_b is not None else None
             if pruner is not None and np.isfinite(it.floor):
                 pruner(cache, start, n_now, it.floor, all_ll[a:b], part_b)
-            elif rows is not None:
-                lnl(star, cache, rows=rows, precision=self.precision, out=all_ll[a:b],
-                    bound=part_b)
             else:
-                lnl(star, cache, start=start, n=n_now, precision=self.precision,
-                    out=all_ll[a:b], bound=part_b)
+                lnl_b = part_b if mixed else None
+                if rows is not None:
+                    lnl(star, cache, rows=rows, precision=self.precision, out=all_ll[a:b],
+                        bound=lnl_b)
+                else:
+                    lnl(star, cache, start=start, n=n_now, precision=self.precision,
+                        out=all_ll[a:b], bound=lnl_b)
+                if part_b is not None and not mixed:
+                    part_b.zero_()
             if mixed:
                 # the new rows, for this and every later round's uniforms
                 self._settle_live(star, cache, all_ll[:b], all_b[:b], a, it.floor,
                                   rows=it._d_all_idx, device=self.device)
             it.absorb(all_ll[:b], self.device)
+        # the final lnL buffer (and bounds: 0 for FP64 values, -1 for a probe's
+        # upper bound), for callers that inspect the run
+        self.last_lnl = (all_ll[:it.n_evals], all_b[:it.n_evals] if all_b is not None else
+                         torch.zeros(it.n_evals, dtype=torch.float32, device=dev))
+        self.last_star = star
         full_idx, good = it.result()
         samples = self._full_samples(star, cache, full_idx, self.rng, n_linear_samples,
                                      spawn_batches=None if in_memory else self._n_batches(n_batches))
