@@ -515,9 +515,23 @@ def leg_serving(args, torch, np, D, syn):
     srv = S.JokerServer(prior, cache, precision="mixed", max_active=args.serving_active,
                         hbm_budget_bytes=arena, max_step_work=step_work, prune=prune)
     rngs = [np.random.default_rng(10_000 + i) for i in range(n_req)]
+    lat_steps = []
+    lat_step0 = srv.step
+
+    def lat_logged_step():
+        t = time.perf_counter()
+        n_act = len(srv.active) + len(srv.pending)
+        out = lat_step0()
+        lat_steps.append(dict(srv.last_step, step_s=round(time.perf_counter() - t, 4),
+                              active_or_pending=n_act))
+        return out
+
+    srv.step = lat_logged_step
     t0 = time.perf_counter()
     done = S.run_poisson(srv, times, datas, rngs)
     wall = time.perf_counter() - t0
+    srv.step = lat_step0
+    durs = np.array([d["step_s"] for d in lat_steps])
     lat = np.array([r.latency for r in done if r.error is None])
     errs = [r for r in done if r.error is not None]
     res = {"sample_epochs_note": "rows x epochs of every row the iterative rule decides (the "
@@ -537,6 +551,8 @@ def leg_serving(args, torch, np, D, syn):
     # their own rounds
     ok = [r for r in done if r.error is None]
     slow = sorted(ok, key=lambda r: -r.latency)[:max(1, len(ok) // 100)]
+    res["step_s_quantiles"] = [float(q) for q in np.percentile(durs, [50, 90, 99, 100])]
+    res["longest_steps"] = sorted(lat_steps, key=lambda d: -d["step_s"])[:5]
     res["slowest_1pct"] = [{"M": int(len(r.data)), "latency_s": round(r.latency, 4),
                             "queued_s": round(r.t_admit - r.arrival, 4),
                             "rounds": int(r.it.iteration), "rows": int(r.rows_evaluated)}
@@ -597,7 +613,7 @@ def leg_serving(args, torch, np, D, syn):
         f"{sat['preemptions']} restarts, {n_req - len(ok2)} failed, {same} results equal the "
         f"latency run's); host time per call {res['saturated']['host_wait_s']}; "
         f"prune {sat['prune']}")
-    if prune:
+    if prune and not args.serving_quick:
         # the same saturated stream without pruning: its rate and its results
         srv3 = S.JokerServer(prior, cache, precision="mixed", max_active=args.serving_active,
                              hbm_budget_bytes=arena, max_step_work=step_work, prune=False)
@@ -620,8 +636,12 @@ def leg_serving(args, torch, np, D, syn):
     # The plain path: the same stars one at a time through
     # TheJoker(precision="f64").iterative_rejection_sample.  Same rows, so
     # the same sample-epochs as in the server; also checks for flips.
+    if args.serving_quick:
+        del cache
+        torch.cuda.empty_cache()
+        return res
     flips, checked, se, t_alone = 0, 0, 0, 0.0
-    for i in range(0, n_req, max(1, n_req // args.serving_plain)):
+    for i in range(0, n_req, max(1, n_req // max(1, args.serving_plain))):
         if done[i].error is not None:
             continue
         joker = tj.TheJoker(prior, rng=np.random.default_rng(10_000 + i), precision="f64")
@@ -790,6 +810,8 @@ def main():
                          "the saturated stream is also run without pruning for comparison")
     ap.add_argument("--serving-plain", type=int, default=1000,
                     help="stars of the serving stream also run one at a time in plain FP64")
+    ap.add_argument("--serving-quick", action="store_true",
+                    help="serving: latency and saturated runs only (no comparison runs)")
     ap.add_argument("--stress-log2-rows", type=int, default=29)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",

@@ -149,10 +149,17 @@ def test_fp32_table_sincos_accuracy():
     assert np.abs((c - np.cos(El)).astype(float)).max() < 1.2e-7
 
 
+def _fmaf(a, b, c):
+    """FP32 fused multiply-add: the exact product and sum rounded once."""
+    return (a.astype(np.float64) * b.astype(np.float64) + c.astype(np.float64)).astype(f32)
+
+
 def _mixed_design(M, e, om, newton=True):
     """The mixed row's design value m: FP32 starter, FP32 table sin/cos, one
     first-order Newton step in FP32 (fused multiply-adds rounded once),
-    1 / (1 - e cos E) in FP32, the combination in FP64."""
+    1 / (1 - e cos E) in FP32, and the combination
+    m = fma(fma(cos w, cos E - e, -(sin w sqrt(1-e^2)) sin E), den, e cos w)
+    in FP32 from FP32 copies of the per-sample constants."""
     ef = np.minimum(e.astype(f32), f32(0.99999994))
     E1 = starter(M.astype(f32), ef)
     s1, c1 = _sincos_tab_f32(E1)
@@ -165,10 +172,13 @@ def _mixed_design(M, e, om, newton=True):
         cE = (-s1.astype(np.float64) * t + c1).astype(f32)
     else:
         sE, cE = s1, c1
-    den = (f32(1) / (f32(1) - ef * cE)).astype(f32)
-    m = (np.cos(om) * (cE.astype(np.float64) - e) - np.sin(om) * np.sqrt((1 - e) * (1 + e)) * sE) \
-        * den + e * np.cos(om)
-    return m, den.astype(np.float64)
+    den = (f32(1) / _fmaf(-ef, cE, f32(1))).astype(f32)
+    cwf = np.cos(om).astype(f32)
+    swsqf = (np.sin(om) * np.sqrt((1 - e) * (1 + e))).astype(f32)
+    ecwf = (e * np.cos(om)).astype(f32)
+    inner = _fmaf(cwf, (cE - ef).astype(f32), (-swsqf * sE).astype(f32))
+    m = _fmaf(inner, den, ecwf)
+    return m.astype(np.float64), den.astype(np.float64)
 
 
 @pytest.mark.parametrize("dist", ["beta", 0.9, 0.999, 0.99999, 1 - 1e-6])

@@ -41,6 +41,10 @@ API_TOL = 1e-3
 # Stars with fewer epochs are not probed: the probe (at least 6 epochs) would
 # cost about as much as the evaluation it saves.
 PRUNE_MIN_EPOCHS = 12
+# Nor are stars whose trend-only fit leaves less than this in lnL for an orbit
+# to gain (device.trend_signal): their rows all lie near the maximum, and a
+# probe would prune next to none of them.
+PRUNE_MIN_SIGNAL = 50.0
 SETTLE_SEGMENT_DTYPE = np.dtype([("offset", "<i8"), ("n", "<i8"), ("star", "<i4"),
                                  ("reserved", "<i4"), ("row_begin", "<i8"), ("state_hi", "<u8"),
                                  ("state_lo", "<u8"), ("inc_hi", "<u8"), ("inc_lo", "<u8"),
@@ -428,15 +432,18 @@ def settle_pruned_batch(batch, cache, ll, bound, offsets, sizes, stars=None, row
     return count
 
 
-def probe_table(stars, jitter_mode="physical", device=0, min_epochs=PRUNE_MIN_EPOCHS):
+def probe_table(stars, jitter_mode="physical", device=0, min_epochs=PRUNE_MIN_EPOCHS,
+                min_signal=PRUNE_MIN_SIGNAL):
     """(probe StarBatch, add per star, probed mask): entry k is star k's
-    probe (device.probe_star) when it has at least min_epochs epochs and a
-    diagonal noise covariance; other stars are not probed (their entry is the
-    star itself, never read as a probe)."""
-    from .device import probe_star
+    probe (device.probe_star) when it has at least min_epochs epochs, a
+    diagonal noise covariance and a trend_signal of at least min_signal;
+    other stars are not probed (their entry is the star itself, never read as
+    a probe)."""
+    from .device import probe_star, trend_signal
     probes, adds, probed = [], [], []
     for st in stars:
-        ok = st.cov is None and st.n >= min_epochs
+        ok = (st.cov is None and st.n >= min_epochs and
+              trend_signal(st.arrays()) >= min_signal)
         if ok:
             pr, add = probe_star(st.arrays(), jitter_mode, device)
         else:
@@ -467,7 +474,13 @@ def prune_reject_batch(batch, probes, adds, probed, cache, d_items, n_items, off
     bnd = torch.empty(total, dtype=torch.float32, device=dev)
     lnl_prune_batch(batch, probes, d_prune, cache, d_items, n_items, precision, shape, out=ll,
                     bound=bnd)
-    cnt = settle_pruned_batch(batch, cache, ll, bnd, offsets, sizes, device=device)
+    # only the probed stars' segments hold upper bounds
+    k = np.flatnonzero(probed)
+    cnt = torch.zeros(len(sizes), dtype=torch.int64, device=dev)
+    if len(k):
+        cnt[torch.as_tensor(k, device=dev)] = settle_pruned_batch(
+            batch, cache, ll, bnd, np.asarray(offsets)[k], np.asarray(sizes)[k], stars=k,
+            device=device)
     res = settle_reject_batch(batch, cache, ll, bnd, offsets, sizes, rng_states, max_keep,
                               device=device)
     return ll, bnd, res, cnt

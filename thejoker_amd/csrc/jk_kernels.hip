@@ -327,6 +327,8 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
     jk_sincos_tab_f64(omv, tab, sw, cw);
     ecw = ev * cw;
     swsq = sw * sqrt((1.0 - ev) * (1.0 + ev));
+    // FP32 copies for the mixed row's design value, which is combined in FP32
+    const float cwf = (float)cw, swsqf = (float)swsq, ecwf = (float)ecw;
     // The FP32 starter sees e rounded to FP32; within 1e-7 of 1 that can put
     // it outside the correction's reach, so such samples take the fallback.
     const double dmax = ev > 0.9999999 ? -1.0 : 1e-3;
@@ -367,10 +369,10 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
             float sE, cE;
             jk_kepler_markley_f32((float)x, ef, omef, a1f, tab, sE, cE);
             const float den = __builtin_amdgcn_rcpf(fmaf(-ef, cE, 1.0f));
-            // the FP32 (sin E, cos E) combined into the design value in FP64
-            // (same issue rate as FP32 here): no rounding beyond their own
-            const double cEd = (double)cE;
-            m = fma(fma(cw, cEd - ev, -swsq * (double)sE), (double)den, ecw);
+            // the design value combined in FP32 too (an FP32 instruction
+            // issues in half the cycles of an FP64 one): its rounding stays
+            // inside the per-epoch error model of the bound
+            m = (double)fmaf(fmaf(cwf, cE - ef, -swsqf * sE), den, ecwf);
             denb = den;
         } else {
             double sE, cE;
@@ -386,13 +388,25 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
         const double rp = st.rp[k];
         double w;
         if constexpr (PHYS) {
-            // w = 1 / (sigma^2 + s^2), relative error below 2.4e-15 (one Newton
-            // step); sum ln w = -ln of the running product
+            // w = 1 / (sigma^2 + s^2): FP64 to a relative 2.4e-15 (one Newton
+            // step on the hardware reciprocal), or in the mixed row the FP32
+            // reciprocal (relative error below 2^-22, which the bound
+            // covers); sum ln w = -ln of the running product of the FP64
+            // variances, renormalised every fourth epoch (frexp scales by a
+            // power of two, exactly, so the product's rounding is that of
+            // renormalising every epoch; four factors stay far inside the
+            // exponent range)
             const double d = st.sig2[k] + s2;
-            w = jk_rcp1(d);
-            int ex;
-            prod = frexp(prod * d, &ex);
-            pexp += ex;
+            if constexpr (MIXED)
+                w = (double)__builtin_amdgcn_rcpf((float)d);
+            else
+                w = jk_rcp1(d);
+            prod *= d;
+            if ((k & 3) == 3) {
+                int ex;
+                prod = frexp(prod, &ex);
+                pexp += ex;
+            }
         } else {
             w = st.ivar[k];
         }
@@ -467,10 +481,11 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
         const double ek = t1 * t1 * fma(1e-3, (double)dmx, 1.0);
         double b = 2.0 * JK_BOUND_EPS * ek * sqrt(Sw) *
                    fma(aK, sqrt(fmax(post[2], 0.0)), sqrt(fmax(post[0], 0.0)));
-        // the weights' relative error e_w < 2.4e-15 moves lnL by at most
+        // the weights' relative error e_w moves lnL by at most
         // e_w (chi2_min + L) / 2 (envelope theorem on chi2_min; the log-det
-        // term is tr(G^-1 X^T W X) e_w <= L e_w)
-        b += 2e-15 * (fmax(post[2], 0.0) + (double)L);
+        // term is tr(G^-1 X^T W X) e_w <= L e_w); e_w < 2^-22 for the FP32
+        // reciprocal of the physical jitter mode, 2.4e-15 otherwise
+        b += (PHYS ? JK_BOUND_EPS : 2e-15) * (fmax(post[2], 0.0) + (double)L);
         b = fma(1e-9, 1.0 + fabs(ll), b) * 1.000001;
         *bnd = (isfinite(b) && isfinite(ll)) ? (float)b : INFINITY;
         return ll;

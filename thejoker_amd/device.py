@@ -22,7 +22,7 @@ from . import _lib
 from . import units as u
 from .samples import NONLINEAR_INTERNAL_UNITS, NONLINEAR_PACKED_ORDER
 
-__all__ = ["DeviceStar", "PriorCache", "star_arrays", "probe_star", "probe_epochs", "lnl", "lnl_reject", "lnl_reject_async",
+__all__ = ["DeviceStar", "PriorCache", "star_arrays", "probe_star", "probe_epochs", "trend_signal", "lnl", "lnl_reject", "lnl_reject_async",
            "PendingReject", "polish", "reject",
            "posterior_inputs_many", "posterior_inputs_table", "to_host",
            "linear_posterior", "uniforms", "debug_single", "count_failures",
@@ -225,6 +225,26 @@ def probe_star(a, jitter_mode="physical", device=0, k=None):
                       a["t_ref"], a["k_prior_mode"], a["sigma_K0"], a["P0"], a["max_K"],
                       jitter_mode=jitter_mode, device=device)
     return star, add
+
+
+def trend_signal(a):
+    """How far a Keplerian orbit can lift a star's likelihood above its
+    trend-only fit: half the excess of the trend-only chi^2 over its degrees
+    of freedom, (chi2 - (n - n_trend)) / 2, from the weighted least-squares
+    fit of the RVs on the trend columns of star_arrays() output `a`.  Data a
+    constant (or trend) model fits leave every prior row within a few units
+    of the maximum, so a probe prunes nothing there."""
+    y = np.asarray(a["rv"], dtype=np.float64)
+    w = np.asarray(a["ivar"], dtype=np.float64)
+    T = np.asarray(a["trend"], dtype=np.float64).reshape(y.size, -1)
+    k = T.shape[1]
+    r = y
+    if k:
+        sw = np.sqrt(w)
+        beta, *_ = np.linalg.lstsq(T * sw[:, None], y * sw, rcond=None)
+        r = y - T @ beta
+    chi2 = float(np.sum(w * r * r))
+    return 0.5 * (chi2 - (y.size - k))
 
 
 def star_arrays(data, prior, trend_M):

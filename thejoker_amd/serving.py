@@ -30,10 +30,11 @@ import time
 
 import numpy as np
 
-from .batch import LIVE_MARGIN, LIVE_MARGIN_NONZERO_U, LaunchShape, StarBatch, heaviest_first, \
-    lnl_batch, lnl_prune_batch, reject_batch, settle_live_batch, upload_items, work_items
+from .batch import (LIVE_MARGIN, LIVE_MARGIN_NONZERO_U, PRUNE_MIN_SIGNAL, LaunchShape, StarBatch,
+                    heaviest_first, lnl_batch, lnl_prune_batch, reject_batch, settle_live_batch,
+                    upload_items, work_items)
 from .device import (DeviceStar, _torch, linear_internal_units, pcg64_state,
-                     posterior_inputs_table, probe_star, star_arrays)
+                     posterior_inputs_table, probe_star, star_arrays, trend_signal)
 from .likelihood_helpers import validate_prepare_data
 from .thejoker import IterativeRejection, TheJoker
 
@@ -184,6 +185,10 @@ class JokerServer:
     prune_min_epochs : int
         Stars with fewer epochs are not probed (the probe would cost about
         as much as the evaluation it saves).
+    prune_min_signal : float
+        Nor are stars whose trend-only fit leaves less than this for an orbit
+        to gain in lnL (device.trend_signal): a probe prunes next to none of
+        their rows.
     """
 
     PHASES = ("admit", "plan", "lnl", "polish", "reject", "update", "finish")
@@ -192,10 +197,11 @@ class JokerServer:
                  max_active=64, hbm_budget_bytes=4 << 30, in_memory=False, growth_factor=128,
                  n_linear_samples=1, shape=None, device=0, clock=time.perf_counter,
                  profile=False, max_step_work=None, min_chunk_rows=1 << 16, prune=True,
-                 prune_min_epochs=12):
+                 prune_min_epochs=12, prune_min_signal=PRUNE_MIN_SIGNAL):
         torch = _torch()
         self.prune = bool(prune)
         self.prune_min_epochs = int(prune_min_epochs)
+        self.prune_min_signal = float(prune_min_signal)
         self.max_step_work = None if max_step_work is None else int(max_step_work)
         self.min_chunk_rows = int(min_chunk_rows)
         self.prior = prior
@@ -361,7 +367,8 @@ class JokerServer:
                 r.slot = self._free_slots.pop()
                 self._table.set(r.slot, r.star)
                 r.prune_add = None
-                if self.prune and a["t"].size >= self.prune_min_epochs:
+                if (self.prune and a["t"].size >= self.prune_min_epochs and
+                        trend_signal(a) >= self.prune_min_signal):
                     r.probe, r.prune_add = probe_star(a, self.jitter_mode, self.device)
                     self._probes.set(r.slot, r.probe)
                     self._n_full[r.slot] = 0

@@ -221,7 +221,10 @@ class TheJoker:
         return to_host(lnl(star, cache, rows=r, precision="f64"))
 
     def _prunable(self, star):
-        return self.prune and star.cov is None and star.n >= self.PRUNE_MIN_EPOCHS
+        from .batch import PRUNE_MIN_SIGNAL
+        from .device import trend_signal
+        return (self.prune and star.cov is None and star.n >= self.PRUNE_MIN_EPOCHS and
+                trend_signal(star.arrays()) >= PRUNE_MIN_SIGNAL)
 
     def _pruned_reject(self, star, cache, n, rng_state, max_keep):
         """rejection_sample's likelihood and rejection step over cache rows
@@ -250,8 +253,7 @@ class TheJoker:
     def _pruner(self, star, it):
         """The probe launcher of iterative_rejection_sample, or None when the
         star is not probed."""
-        if (not self.prune or star.cov is not None or star.n < self.PRUNE_MIN_EPOCHS
-                or it._d_all_idx is not None):
+        if not self._prunable(star) or it._d_all_idx is not None:
             return None
         return _probe_pruner(star, self.precision, self.jitter_mode, self.device)
 
