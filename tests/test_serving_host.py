@@ -107,6 +107,38 @@ def test_share_budget():
         assert (out * M).sum() <= max(budget, 0) + forced.sum()
 
 
+def test_share_budget_srpt():
+    """Shortest remaining work first: the smallest rounds complete, the
+    budget left goes whole to the request nearest completion, the others
+    take the minimum chunk; costs per row may be fractional (a probed
+    round's row costs less than its epochs)."""
+    from thejoker_amd.serving import share_budget
+    out = share_budget([5, 1000, 1000], [10, 10, 20], 6050, 1, policy="srpt")
+    assert out[0] == 5 and out[1] == 600 and out[2] == 1
+    out = share_budget([5, 1000, 1000], [10, 10, 20], 6050, 64, policy="srpt")
+    assert out[0] == 5 and out[1] == 600 and out[2] == 64
+    out = share_budget([100, 1000], [2.5, 1.0], 10_000, 1, policy="srpt")
+    assert list(out) == [100, 1000]
+    rng = np.random.default_rng(2)
+    for _ in range(200):
+        n = int(rng.integers(1, 20))
+        rem = rng.integers(1, 10**6, n)
+        M = rng.uniform(3.0, 200.0, n)
+        budget = int(rng.integers(10**4, 10**9))
+        out = share_budget(rem, M, budget, 256, policy="srpt")
+        assert (out <= rem).all() and (out >= np.minimum(rem, 256)).all()
+        forced = np.minimum(rem, 256) * M
+        assert (out * M).sum() <= max(budget, 0) + forced.sum() + 1e-6
+        # past the first request (in order of remaining work) left incomplete,
+        # the others take only the minimum chunk (what that one leaves, under
+        # one of its rows' cost, buys no later request 256 rows at M >= 3)
+        order = np.argsort(rem * M, kind="stable")
+        short = np.flatnonzero(out[order] < rem[order])
+        if len(short):
+            rest = order[short[0] + 1:]
+            assert (out[rest] == np.minimum(rem[rest], 256)).all()
+
+
 def zero_uniform_rng(k, inc_seed=5):
     """A numpy Generator (PCG64) whose k-th uniform (0-based) is exactly 0:
     the state after k + 1 LCG steps is 0, whose XSL-RR output is 0."""

@@ -98,18 +98,22 @@ class RangeAllocator:
         return self.capacity - sum(b - a for a, b in self.free)
 
 
-def share_budget(rem, M, budget, min_rows):
-    """Rows for each request from a budget of sample-epochs: in increasing
-    order of remaining work rem[i] * M[i], each takes its remainder or an
-    equal share of the budget left (at least min_rows), whichever is less."""
+def share_budget(rem, M, budget, min_rows, policy="share"):
+    """Rows for each request from a budget of sample-epochs, M[i] being the
+    cost of one of request i's rows, in increasing order of remaining work
+    rem[i] * M[i].  policy "share": each takes its remainder or an equal
+    share of the budget left, whichever is less; "srpt" (shortest remaining
+    work first): each takes as much of its remainder as the budget left
+    covers.  Either way every request takes at least min_rows (or its
+    remainder), so each one advances in every step."""
     rem = np.asarray(rem, dtype=np.int64)
-    M = np.asarray(M, dtype=np.int64)
+    M = np.maximum(np.asarray(M, dtype=np.float64), 1e-9)
     left = float(budget)
     out = np.zeros(len(rem), dtype=np.int64)
     order = np.argsort(rem * M, kind="stable")
     for k, i in enumerate(order):
-        share = left / (len(order) - k)
-        rows = min(int(rem[i]), max(int(min_rows), int(share // M[i])))
+        share = left / (len(order) - k) if policy == "share" else left
+        rows = min(int(rem[i]), max(int(min_rows), int(max(share, 0.0) // M[i])))
         out[i] = rows
         left -= rows * M[i]
     return out
@@ -139,6 +143,7 @@ class Request:
         self.probe = None       # probe star (DeviceStar on a subset of the epochs)
         self.prune_add = None   # its bound's addend; None: the request is not probed
         self.full_seen = 0      # its slot's full-evaluation counter, last read
+        self.full_rate = 0.15   # share of its probed rows evaluated in full, last measured
 
     @property
     def latency(self):
@@ -167,11 +172,21 @@ class JokerServer:
         phase's wall time to ``self.phase_s`` (admit, plan, lnl, polish,
         reject, update, finish).
     max_step_work : int or None
-        Sample-epochs (rows x epochs) one step may evaluate; None evaluates
-        every active request's whole round in each step.  A round larger than
-        a request's share is spread over several steps (its rejection step
-        runs when the round is complete), so that short requests are not held
-        behind long ones.
+        Sample-epochs one step may evaluate (a probed request's row costs its
+        probe's epochs plus its measured share of rows evaluated in full
+        times its epochs); None evaluates every active request's whole round
+        in each step.  A round larger than a request's share is spread over
+        several steps (its rejection step runs when the round is complete),
+        so that short requests are not held behind long ones, and new
+        arrivals wait at most one short step for admission.
+    budget_policy : "srpt" or "share"
+        How a step budget is shared: shortest remaining work first (the
+        smallest rounds complete at once and the budget left goes to the
+        request nearest completion), or equal shares.
+    budget_max_pending : int or None
+        The budget applies only while fewer requests than this wait for
+        admission (default max_active); with a longer queue every round runs
+        whole, for throughput.
     min_chunk_rows : int
         Smallest part of a round a step evaluates under a budget.
     prune : bool
@@ -197,12 +212,18 @@ class JokerServer:
                  max_active=64, hbm_budget_bytes=4 << 30, in_memory=False, growth_factor=128,
                  n_linear_samples=1, shape=None, device=0, clock=time.perf_counter,
                  profile=False, max_step_work=None, min_chunk_rows=1 << 16, prune=True,
-                 prune_min_epochs=12, prune_min_signal=PRUNE_MIN_SIGNAL):
+                 prune_min_epochs=12, prune_min_signal=PRUNE_MIN_SIGNAL, budget_policy="srpt",
+                 budget_max_pending=None):
         torch = _torch()
         self.prune = bool(prune)
         self.prune_min_epochs = int(prune_min_epochs)
         self.prune_min_signal = float(prune_min_signal)
         self.max_step_work = None if max_step_work is None else int(max_step_work)
+        if budget_policy not in ("srpt", "share"):
+            raise ValueError("budget_policy must be 'srpt' or 'share'")
+        self.budget_policy = budget_policy
+        self.budget_max_pending = (int(max_active) if budget_max_pending is None
+                                   else int(budget_max_pending))
         self.min_chunk_rows = int(min_chunk_rows)
         self.prior = prior
         self.cache = cache
@@ -317,6 +338,8 @@ class JokerServer:
         for r, slot, size in probed:
             d = int(full[slot]) - r.full_seen
             r.full_seen = int(full[slot])
+            if size:
+                r.full_rate = d / float(size)
             self.rows_probed += int(size)
             self.rows_pruned += int(size) - d
 
@@ -451,10 +474,15 @@ class JokerServer:
         remaining work each take up to an equal share of what is left (at
         least min_chunk_rows), so short requests finish in one short step."""
         rem = np.array([n - r.round_done for r, _, n in plan], dtype=np.int64)
-        if self.max_step_work is None:
+        if self.max_step_work is None or len(self.pending) >= self.budget_max_pending:
             return rem
-        M = self._table.M[[r.slot for r, _, _ in plan]]
-        return share_budget(rem, M, self.max_step_work, self.min_chunk_rows)
+        M = self._table.M[[r.slot for r, _, _ in plan]].astype(np.float64)
+        # a probed round costs the probe's epochs plus the share of rows
+        # evaluated in full (measured on the request's earlier rounds)
+        for k, (r, _, _) in enumerate(plan):
+            if r.prune_add is not None and np.isfinite(r.it.floor):
+                M[k] = r.probe.n + r.full_rate * M[k]
+        return share_budget(rem, M, self.max_step_work, self.min_chunk_rows, self.budget_policy)
 
     def step(self):
         """Admit, evaluate the next rows of every active request (a whole
