@@ -558,6 +558,7 @@ def leg_serving(args, torch, np, D, syn):
                             "rounds": int(r.it.iteration), "rows": int(r.rows_evaluated)}
                            for r in slow]
     res["preemptions"] = srv.preemptions
+    res["exact_switches"] = srv.exact_switches
     res["arena_gib"] = args.serving_arena_gib
     res["max_step_work"] = step_work
     res["prune"] = _prune_stats(srv, prune)
@@ -588,6 +589,7 @@ def leg_serving(args, torch, np, D, syn):
                for a, b in zip(done, done2) if a.error is None and b.error is None)
     sat = {"preemptions": srv2.preemptions, "sample_epochs_per_s": srv2.sample_epochs / wall2,
            "steps": srv2.steps, "region_moves": dict(srv2.region_moves),
+           "exact_switches": srv2.exact_switches,
            "prune": _prune_stats(srv2, prune)}
     del srv2, logged_step, step0
     gc.collect()

@@ -325,6 +325,18 @@ int jk_batch_create_slots(int capacity, int L, int jitter_mode, jk_batch **out);
  * update is ordered on `stream` after the work already queued there. */
 int jk_batch_set_star(jk_batch *batch, int slot, const jk_star *star, void *stream);
 
+/* Mark slot `slot` of a star table exact (exact != 0) or not: mixed-precision
+ * launches over the table (jk_lnl_batch and jk_lnl_prune_batch with
+ * precision 1) evaluate an exact slot's rows in FP64 and write error bound 0
+ * for them, so the settling calls leave them alone.  For stars whose rows
+ * nearly all lie within the settling margin of the maximum (flat
+ * likelihoods), where every mixed value would be re-evaluated in FP64
+ * anyway.  jk_batch_set_star clears the mark.  Ordered on `stream`.  No
+ * counterpart in the reference, which evaluates everything in FP64
+ * (fast_likelihood.pyx:255-469); a request loop that serves
+ * iterative_rejection_sample (thejoker.py:259-370) uses it per request. */
+int jk_batch_set_exact(jk_batch *batch, int slot, int exact, void *stream);
+
 /* Persistent work-queue launch over device-resident work items.
  * precision 0: FP64; 1: mixed (as jk_lnl_mixed).  The columns are the FP64
  * ones in both cases.  bound_out (mixed only, optional): per-row error bounds as in

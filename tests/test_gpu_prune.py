@@ -46,7 +46,7 @@ def test_probe_bounds_the_star(setup, mode, L, fcm):
         a = _arrays(st, L, fcm)
         full = D.lnl(_star(a, mode), cache, precision="f64").cpu().numpy()
         probe, add = D.probe_star(a, mode)
-        assert probe.n == len(D.probe_epochs(st.M)) < st.M
+        assert probe.n == len(D.probe_epochs_residual(a)) < st.M
         ub = D.lnl(probe, cache, precision="f64").cpu().numpy() + add
         ok = np.isfinite(full)
         assert ok.mean() > 0.99
@@ -107,6 +107,73 @@ def test_prune_launch(setup, precision):
         # every row within the margin of the maximum is evaluated in full
         assert not p[ref64[sl] >= thr[k, 0]].any()
     assert n_pruned > len(cache)
+
+
+def test_exact_slots_evaluate_in_fp64(setup):
+    """jk_batch_set_exact: inside mixed launches (plain, with bounds, and with
+    pruning) a marked slot's rows are the FP64 launch's values bit for bit
+    with bound 0; the other slots keep their mixed values and bounds;
+    set_exact(slot, False) and set_star clear the mark."""
+    stars, cache = setup
+    table, probes, adds = _tables(stars)
+    n = len(cache)
+    items, off, total = B.work_items([n] * len(stars), 1000)
+    items = items[np.random.default_rng(2).permutation(len(items))]
+    d_items = B.upload_items(items)
+
+    def run(precision, bound=True):
+        b = torch.full((total,), 7.0, dtype=torch.float32, device="cuda") if bound else None
+        ll = B.lnl_batch(table, cache, d_items, len(items), total, precision, bound=b)
+        return ll.cpu().numpy(), None if b is None else b.cpu().numpy()
+
+    ref64, _ = run("f64", bound=False)
+    mix, bmix = run("mixed")
+    mix_nb, _ = run("mixed", bound=False)
+    exact = {1, 4, 7}
+    for k in exact:
+        table.set_exact(k)
+    got, bgot = run("mixed")
+    got_nb, _ = run("mixed", bound=False)
+    for k in range(len(stars)):
+        sl = slice(off[k], off[k] + n)
+        if k in exact:
+            assert np.array_equal(got[sl], ref64[sl], equal_nan=True), k
+            assert np.array_equal(got_nb[sl], ref64[sl], equal_nan=True), k
+            assert (bgot[sl] == 0.0).all(), k
+        else:
+            assert np.array_equal(got[sl], mix[sl], equal_nan=True), k
+            assert np.array_equal(bgot[sl], bmix[sl]), k
+            assert np.array_equal(got_nb[sl], mix_nb[sl], equal_nan=True), k
+    # the pruned launch: unpruned rows of marked slots are FP64 with bound 0
+    thr = np.zeros((len(stars), 2))
+    for k in range(len(stars)):
+        thr[k] = (-np.inf if k == 4 else np.nanmax(ref64[off[k]:off[k] + n]) - 40.0, adds[k])
+    prune = torch.as_tensor(thr.reshape(-1), device="cuda")
+    out = torch.full((total,), np.nan, dtype=torch.float64, device="cuda")
+    bnd = torch.full((total,), 7.0, dtype=torch.float32, device="cuda")
+    B.lnl_prune_batch(table, probes, prune, cache, d_items, len(items), "mixed", out=out,
+                      bound=bnd)
+    out, bnd = out.cpu().numpy(), bnd.cpu().numpy()
+    for k in range(len(stars)):
+        sl = slice(off[k], off[k] + n)
+        kept = bnd[sl] != -1.0
+        if k == 4:
+            assert kept.all()
+        want, wb = (ref64[sl], 0.0) if k in exact else (mix[sl], bmix[sl])
+        assert np.array_equal(out[sl][kept], want[kept], equal_nan=True), k
+        assert np.array_equal(bnd[sl][kept], (np.zeros(n, np.float32) + wb)[kept]), k
+    table.set_exact(1, False)
+    got, bgot = run("mixed")
+    assert np.array_equal(got[off[1]:off[1] + n], mix[off[1]:off[1] + n], equal_nan=True)
+    # a slot table: set_star clears the mark
+    slots = B.StarBatch.slots(2, 3)
+    slots.set(0, table.stars[0])
+    slots.set(1, table.stars[1])
+    slots.set_exact(0)
+    slots.set(0, table.stars[0])
+    it2, off2, tot2 = B.work_items([n, n], 1000)
+    ll2 = B.lnl_batch(slots, cache, B.upload_items(it2), len(it2), tot2, "mixed").cpu().numpy()
+    assert np.array_equal(ll2[:n], mix_nb[off[0]:off[0] + n], equal_nan=True)
 
 
 def test_zero_uniform_on_a_pruned_row(setup):

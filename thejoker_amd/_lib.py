@@ -64,6 +64,7 @@ SIGNATURES = [
     ("jk_batch_create_slots", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                               ctypes.POINTER(_vp)]),
     ("jk_batch_set_star", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
+    ("jk_batch_set_exact", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp]),
     ("jk_lnl_batch", ctypes.c_int,
      [_vp, ctypes.c_int, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int,
       ctypes.c_int, _vp, _vp]),

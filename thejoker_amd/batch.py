@@ -108,6 +108,12 @@ class StarBatch:
         self.stars[slot] = star
         self.M[slot] = star.n
 
+    def set_exact(self, slot, exact=True):
+        """Evaluate slot `slot` in FP64 inside mixed-precision launches, with
+        error bound 0 (jk_batch_set_exact); set_star clears it."""
+        _lib.check(self._lib.jk_batch_set_exact(self._h, int(slot), int(bool(exact)),
+                                                 _stream(self.device)), "jk_batch_set_exact")
+
     @property
     def handle(self):
         return self._h

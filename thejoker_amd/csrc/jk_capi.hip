@@ -781,6 +781,13 @@ int jk_batch_set_star(jk_batch *b, int slot, const jk_star *star, void *stream)
                                     hipMemcpyHostToDevice, (hipStream_t)stream));
 }
 
+int jk_batch_set_exact(jk_batch *b, int slot, int exact, void *stream)
+{
+    if (!b || slot < 0 || slot >= b->n_stars) return JK_EINVAL;
+    if (current_device_inited() < 0) return JK_ENODEV;
+    return check_hip(jk_launch_set_exact(b, slot, exact, (hipStream_t)stream));
+}
+
 int jk_batch_destroy(jk_batch *b)
 {
     if (!b) return JK_EINVAL;

@@ -35,6 +35,8 @@ struct StarDev {
     int L;             // linear parameters
     int jitter_mode;   // JK_JITTER_*
     int k_prior_mode;  // JK_KPRIOR_*
+    int exact;         // batched table slots: 1 evaluates the star's rows in FP64
+                       // inside mixed-precision launches (error bound 0)
     double sigK0sq;    // sigma_K0^2      (FixedCompanionMass)
     double P0;         // reference period (FixedCompanionMass)
     double maxKsq;     // max_K^2         (FixedCompanionMass)

@@ -110,6 +110,7 @@ hipError_t jk_launch_lnl_batch(const jk_batch *b, int precision, int64_t n_items
                                const void *om, const void *M0, const void *s,
                                const int64_t *rows, double *ll, float *bound, int block, int grid,
                                unsigned *counter, hipStream_t stream);
+hipError_t jk_launch_set_exact(const jk_batch *b, int slot, int exact, hipStream_t stream);
 hipError_t jk_launch_lnl_prune_batch(const jk_batch *b, const jk_batch *pb, const double *prune,
                                      int precision, int64_t n_items, const jk_work_item *items,
                                      const double *P, const void *e, const void *om,

@@ -1337,6 +1337,32 @@ hipError_t jk_launch_polish(const jk_star *st, int64_t n, const double *P, const
 // ---------------------------------------------------------------------------
 __global__ void k_zero_u32(unsigned *p) { *p = 0u; }
 
+// One row of a mixed-precision batched launch: in FP64 with error bound 0
+// for a star whose table slot is marked exact (jk_batch_set_exact; a star
+// whose rows nearly all land within the settling margin of its maximum, so
+// that mixed values would be re-evaluated anyway), else in mixed precision
+// with its bound.  The branch is uniform over a work item.
+template <int L, bool PHYS>
+__device__ __forceinline__ double jk_lnl_row_sel(const StarDev &st, int64_t r,
+                                                 const double *__restrict__ P,
+                                                 const void *__restrict__ e_,
+                                                 const void *__restrict__ om_,
+                                                 const void *__restrict__ M0_,
+                                                 const void *__restrict__ s_,
+                                                 const double2 *__restrict__ tab, float *bnd)
+{
+    if (st.exact) {
+        *bnd = 0.0f;
+        return jk_lnl_row<L, PHYS, false>(st, r, P, e_, om_, M0_, s_, tab);
+    }
+    return jk_lnl_row<L, PHYS, true, false, true>(st, r, P, e_, om_, M0_, s_, tab, bnd);
+}
+
+// Slot `slot` of a star table evaluated in FP64 (exact != 0) or as the launch
+// asks (jk_batch_set_exact); a kernel, so that it is ordered on the stream and
+// reads no host memory after it returns.
+__global__ void k_set_exact(StarDev *stars, int slot, int exact) { stars[slot].exact = exact; }
+
 template <int L, bool PHYS, bool MIXED, bool BOUND>
 __global__ __launch_bounds__(256) void k_lnl_batch(const StarDev *__restrict__ stars,
                                                    const jk_work_item *__restrict__ items,
@@ -1366,14 +1392,23 @@ __global__ __launch_bounds__(256) void k_lnl_batch(const StarDev *__restrict__ s
             const int64_t r = rows ? rows[q] : q;
             if constexpr (BOUND) {
                 float b;
-                ll[w.out_offset + j] =
-                    jk_lnl_row<L, PHYS, MIXED, false, true>(st, r, P, e_, om_, M0_, s_, s_tab, &b);
+                ll[w.out_offset + j] = jk_lnl_row_sel<L, PHYS>(st, r, P, e_, om_, M0_, s_, s_tab, &b);
                 bound[w.out_offset + j] = b;
+            } else if constexpr (MIXED) {
+                ll[w.out_offset + j] =
+                    st.exact ? jk_lnl_row<L, PHYS, false>(st, r, P, e_, om_, M0_, s_, s_tab)
+                             : jk_lnl_row<L, PHYS, true>(st, r, P, e_, om_, M0_, s_, s_tab);
             } else {
-                ll[w.out_offset + j] = jk_lnl_row<L, PHYS, MIXED>(st, r, P, e_, om_, M0_, s_, s_tab);
+                ll[w.out_offset + j] = jk_lnl_row<L, PHYS, false>(st, r, P, e_, om_, M0_, s_, s_tab);
             }
         }
     }
+}
+
+hipError_t jk_launch_set_exact(const jk_batch *b, int slot, int exact, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_set_exact, dim3(1), dim3(1), 0, stream, b->d_stars, slot, exact ? 1 : 0);
+    return hipGetLastError();
 }
 
 template <int L, bool PHYS, bool MIXED>
@@ -1497,7 +1532,7 @@ __global__ __launch_bounds__(256) void k_lnl_prune_batch(const StarDev *__restri
                 if constexpr (MIXED) {
                     float b;
                     ll[w.out_offset + j] =
-                        jk_lnl_row<L, PHYS, true, false, true>(st, r, P, e_, om_, M0_, s_, s_tab, &b);
+                        jk_lnl_row_sel<L, PHYS>(st, r, P, e_, om_, M0_, s_, s_tab, &b);
                     bound[w.out_offset + j] = b;
                 } else {
                     ll[w.out_offset + j] = jk_lnl_row<L, PHYS, false>(st, r, P, e_, om_, M0_, s_, s_tab);
@@ -1546,8 +1581,8 @@ __global__ __launch_bounds__(256) void k_lnl_prune_batch(const StarDev *__restri
                     const int64_t r = w.row_begin + jj;
                     if constexpr (MIXED) {
                         float b;
-                        ll[w.out_offset + jj] = jk_lnl_row<L, PHYS, true, false, true>(
-                            st, r, P, e_, om_, M0_, s_, s_tab, &b);
+                        ll[w.out_offset + jj] =
+                            jk_lnl_row_sel<L, PHYS>(st, r, P, e_, om_, M0_, s_, s_tab, &b);
                         bound[w.out_offset + jj] = b;
                     } else {
                         ll[w.out_offset + jj] =

@@ -26,6 +26,7 @@ again with its random stream as it was at admission, so it repeats the same
 rounds and its result is unchanged.  No request is admitted while another
 waits for memory.
 """
+import gc
 import time
 
 import numpy as np
@@ -39,6 +40,16 @@ from .likelihood_helpers import validate_prepare_data
 from .thejoker import IterativeRejection, TheJoker
 
 __all__ = ["Request", "RangeAllocator", "JokerServer", "run_poisson", "share_budget"]
+
+# A mixed-precision request whose round settled more than this share of its
+# rows in FP64 continues in FP64 (its slot marked exact).  A mixed row costs
+# about 0.65 of an FP64 one and a settled row about one more FP64 evaluation,
+# so past about a third settled, FP64 alone is cheaper; flat likelihoods
+# (near-constant stars, whose rows nearly all lie within the settling margin
+# of the maximum) settle almost every row.
+EXACT_SWITCH = 0.3
+# Step-budget cost of an exact row relative to a mixed one.
+EXACT_COST = 1.5
 
 
 class RangeAllocator:
@@ -144,6 +155,8 @@ class Request:
         self.prune_add = None   # its bound's addend; None: the request is not probed
         self.full_seen = 0      # its slot's full-evaluation counter, last read
         self.full_rate = 0.15   # share of its probed rows evaluated in full, last measured
+        self.exact = False      # its slot evaluates in FP64 (flat likelihood)
+        self.round_settled = 0  # rows of the current round settled in FP64
 
     @property
     def latency(self):
@@ -204,6 +217,16 @@ class JokerServer:
         Nor are stars whose trend-only fit leaves less than this for an orbit
         to gain in lnL (device.trend_signal): a probe prunes next to none of
         their rows.
+    exact_switch : float or None
+        Mixed precision: a request whose round settled more than this share
+        of its rows in FP64 evaluates its later rounds in FP64
+        (StarBatch.set_exact); None never switches.  Results are unchanged.
+    freeze_gc : bool
+        Call gc.freeze() at construction: the objects alive then (the
+        interpreter's modules, torch, the prior cache) move to the garbage
+        collector's permanent generation and later collections skip them.
+        Otherwise a full collection scans them all (30-80 ms with torch
+        loaded) and stalls whichever step it falls in.
     """
 
     PHASES = ("admit", "plan", "lnl", "polish", "reject", "update", "finish")
@@ -213,11 +236,14 @@ class JokerServer:
                  n_linear_samples=1, shape=None, device=0, clock=time.perf_counter,
                  profile=False, max_step_work=None, min_chunk_rows=1 << 16, prune=True,
                  prune_min_epochs=12, prune_min_signal=PRUNE_MIN_SIGNAL, budget_policy="srpt",
-                 budget_max_pending=None):
+                 budget_max_pending=None, exact_switch=EXACT_SWITCH, freeze_gc=True):
         torch = _torch()
         self.prune = bool(prune)
         self.prune_min_epochs = int(prune_min_epochs)
         self.prune_min_signal = float(prune_min_signal)
+        self.exact_switch = None if exact_switch is None else float(exact_switch)
+        if freeze_gc:
+            gc.freeze()
         self.max_step_work = None if max_step_work is None else int(max_step_work)
         if budget_policy not in ("srpt", "share"):
             raise ValueError("budget_policy must be 'srpt' or 'share'")
@@ -274,6 +300,7 @@ class JokerServer:
         self._waiting = 0         # active requests the last step could not grow
         self.preemptions = 0
         self.zero_uniform_events = 0
+        self.exact_switches = 0
         self.region_moves = {"extend": 0, "move": 0, "overlap": 0}
         self._helper = TheJoker(prior, rng=np.random.default_rng(0), jitter_mode=jitter_mode,
                                 precision=precision, device=device)
@@ -411,6 +438,8 @@ class JokerServer:
         self._release(r)
         r.it = None
         r.round_done = 0
+        r.round_settled = 0
+        r.exact = False
         r.rng.bit_generator.state = r.rng_state0
         r.preempted += 1
         self.preemptions += 1
@@ -482,6 +511,8 @@ class JokerServer:
         for k, (r, _, _) in enumerate(plan):
             if r.prune_add is not None and np.isfinite(r.it.floor):
                 M[k] = r.probe.n + r.full_rate * M[k]
+            if r.exact:
+                M[k] *= EXACT_COST
         return share_budget(rem, M, self.max_step_work, self.min_chunk_rows, self.budget_policy)
 
     def step(self):
@@ -528,6 +559,7 @@ class JokerServer:
                 self._waiting = 0
                 return [r]
             self._preempt(max(self.active, key=lambda q: q.t_admit))
+        lap("region")
         slots = np.array([r.slot for r, _, _ in plan], dtype=np.int64)
         n_round = np.array([n for _, _, n in plan], dtype=np.int64)
         done0 = np.array([r.round_done for r, _, _ in plan], dtype=np.int64)
@@ -543,8 +575,9 @@ class JokerServer:
         counts = (sizes + tile - 1) // tile
         items["out_offset"] += np.repeat(seg_off + before + done0 - off, counts)
         items = items[heaviest_first(items, self._table.M)]
-        d_items = upload_items(items, self.device)
         lap("plan")
+        d_items = upload_items(items, self.device)
+        lap("upload")
         self._mark("plan")
         if self.profile:
             finished = self._build_results()
@@ -576,6 +609,9 @@ class JokerServer:
                                     before + done0 + sizes, before + done0, floors, stars=slots,
                                     device=self.device, margin=LIVE_MARGIN_NONZERO_U)
             lap("settle")
+            cnt = cnt.cpu().numpy()        # the settling call has synchronised the stream
+            for k, (r, _, _) in enumerate(plan):
+                r.round_settled += int(cnt[k])
             if self.profile:
                 self._cur["settled"] = int(cnt.sum())
         self._mark("polish")
@@ -613,6 +649,12 @@ class JokerServer:
             for j, k in enumerate(ends):
                 r = plan[k][0]
                 r.round_done = 0
+                if (mixed and not r.exact and self.exact_switch is not None and
+                        r.round_settled > self.exact_switch * n_round[k]):
+                    self._table.set_exact(r.slot)
+                    r.exact = True
+                    self.exact_switches += 1
+                r.round_settled = 0
                 try:
                     if self.in_memory and not np.isfinite(llmax[j]):
                         raise RuntimeError("There are NaN or Inf likelihood values in iteration "
