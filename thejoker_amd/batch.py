@@ -38,13 +38,22 @@ LIVE_MARGIN = 750.0
 LIVE_MARGIN_NONZERO_U = 40.0
 # |lnL - lnL(FP64)| every mixed-precision lnL handed back to a caller meets
 API_TOL = 1e-3
-# Stars with fewer epochs are not probed: the probe (at least 6 epochs) would
+# Stars with fewer epochs are not probed: the probe (at least 5 epochs) would
 # cost about as much as the evaluation it saves.
 PRUNE_MIN_EPOCHS = 12
 # Nor are stars whose trend-only fit leaves less than this in lnL for an orbit
 # to gain (device.trend_signal): their rows all lie near the maximum, and a
 # probe would prune next to none of them.
 PRUNE_MIN_SIGNAL = 50.0
+# Iterative loops in mixed precision: after a round whose live settling
+# re-evaluated more than this share of its new rows in FP64, the star leaves
+# the live settling (settled rows stay FP64 for every later uniform) for
+# settling at each rejection step (the rows that step's uniforms leave open),
+# or for FP64 evaluation.  A mixed row costs about 0.65 of an FP64 one and a
+# settled row about one more FP64 evaluation, so past about a third settled
+# either is cheaper; flat likelihoods (near-constant stars, rows all within
+# the margin of the maximum) settle nearly every row.
+EXACT_SWITCH = 0.3
 SETTLE_SEGMENT_DTYPE = np.dtype([("offset", "<i8"), ("n", "<i8"), ("star", "<i4"),
                                  ("reserved", "<i4"), ("row_begin", "<i8"), ("state_hi", "<u8"),
                                  ("state_lo", "<u8"), ("inc_hi", "<u8"), ("inc_lo", "<u8"),

@@ -31,9 +31,10 @@ import time
 
 import numpy as np
 
-from .batch import (LIVE_MARGIN, LIVE_MARGIN_NONZERO_U, PRUNE_MIN_SIGNAL, LaunchShape, StarBatch,
-                    heaviest_first, lnl_batch, lnl_prune_batch, reject_batch, settle_live_batch,
-                    upload_items, work_items)
+from .batch import (EXACT_SWITCH, LIVE_MARGIN, LIVE_MARGIN_NONZERO_U, PRUNE_MIN_SIGNAL,
+                    LaunchShape, StarBatch, heaviest_first, lnl_batch, lnl_prune_batch,
+                    reject_batch, settle_live_batch, settle_reject_batch, upload_items,
+                    work_items)
 from .device import (DeviceStar, _torch, linear_internal_units, pcg64_state,
                      posterior_inputs_table, probe_star, star_arrays, trend_signal)
 from .likelihood_helpers import validate_prepare_data
@@ -41,13 +42,6 @@ from .thejoker import IterativeRejection, TheJoker
 
 __all__ = ["Request", "RangeAllocator", "JokerServer", "run_poisson", "share_budget"]
 
-# A mixed-precision request whose round settled more than this share of its
-# rows in FP64 continues in FP64 (its slot marked exact).  A mixed row costs
-# about 0.65 of an FP64 one and a settled row about one more FP64 evaluation,
-# so past about a third settled, FP64 alone is cheaper; flat likelihoods
-# (near-constant stars, whose rows nearly all lie within the settling margin
-# of the maximum) settle almost every row.
-EXACT_SWITCH = 0.3
 # Step-budget cost of an exact row relative to a mixed one.
 EXACT_COST = 1.5
 
@@ -156,6 +150,7 @@ class Request:
         self.full_seen = 0      # its slot's full-evaluation counter, last read
         self.full_rate = 0.15   # share of its probed rows evaluated in full, last measured
         self.exact = False      # its slot evaluates in FP64 (flat likelihood)
+        self.shell = False      # settles at each rejection step (flat likelihood)
         self.round_settled = 0  # rows of the current round settled in FP64
 
     @property
@@ -219,8 +214,16 @@ class JokerServer:
         their rows.
     exact_switch : float or None
         Mixed precision: a request whose round settled more than this share
-        of its rows in FP64 evaluates its later rounds in FP64
-        (StarBatch.set_exact); None never switches.  Results are unchanged.
+        of its rows in FP64 (a flat likelihood, nearly every row within the
+        margin of the maximum) leaves the live settling for `flat_mode`;
+        None never switches.  Results are unchanged.
+    flat_mode : "shell" or "exact"
+        "shell": the request's later rows stay in mixed precision, unsettled,
+        and each rejection step settles what that step's uniforms need
+        (settle_reject_batch: the rows that could hold the maximum and those
+        whose decision the bound leaves open, a few per million) -- one more
+        pass over its rows per round instead of an FP64 evaluation of each.
+        "exact": its later rows are evaluated in FP64 (StarBatch.set_exact).
     freeze_gc : bool
         Call gc.freeze() at construction: the objects alive then (the
         interpreter's modules, torch, the prior cache) move to the garbage
@@ -236,12 +239,16 @@ class JokerServer:
                  n_linear_samples=1, shape=None, device=0, clock=time.perf_counter,
                  profile=False, max_step_work=None, min_chunk_rows=1 << 16, prune=True,
                  prune_min_epochs=12, prune_min_signal=PRUNE_MIN_SIGNAL, budget_policy="srpt",
-                 budget_max_pending=None, exact_switch=EXACT_SWITCH, freeze_gc=True):
+                 budget_max_pending=None, exact_switch=EXACT_SWITCH, flat_mode="shell",
+                 freeze_gc=True):
         torch = _torch()
         self.prune = bool(prune)
         self.prune_min_epochs = int(prune_min_epochs)
         self.prune_min_signal = float(prune_min_signal)
         self.exact_switch = None if exact_switch is None else float(exact_switch)
+        if flat_mode not in ("shell", "exact"):
+            raise ValueError("flat_mode must be 'shell' or 'exact'")
+        self.flat_mode = flat_mode
         if freeze_gc:
             gc.freeze()
         self.max_step_work = None if max_step_work is None else int(max_step_work)
@@ -300,7 +307,7 @@ class JokerServer:
         self._waiting = 0         # active requests the last step could not grow
         self.preemptions = 0
         self.zero_uniform_events = 0
-        self.exact_switches = 0
+        self.flat_switches = 0
         self.region_moves = {"extend": 0, "move": 0, "overlap": 0}
         self._helper = TheJoker(prior, rng=np.random.default_rng(0), jitter_mode=jitter_mode,
                                 precision=precision, device=device)
@@ -440,6 +447,7 @@ class JokerServer:
         r.round_done = 0
         r.round_settled = 0
         r.exact = False
+        r.shell = False
         r.rng.bit_generator.state = r.rng_state0
         r.preempted += 1
         self.preemptions += 1
@@ -514,6 +522,53 @@ class JokerServer:
             if r.exact:
                 M[k] *= EXACT_COST
         return share_budget(rem, M, self.max_step_work, self.min_chunk_rows, self.budget_policy)
+
+    def _reject_ends(self, plan, ends, seg_off, slots, seg_n, states):
+        """The rejection step of the requests plan[k], k in `ends`, whose round
+        is complete: (accepted index arrays, pass counts, maxima).  Requests in
+        shell mode settle and decide in one sequence (settle_reject_batch:
+        FP64 for the rows that could hold the maximum or whose decision
+        under this round's uniforms the bound leaves open); the others, whose
+        rows within the margin of the maximum are FP64 already, take the plain
+        step with the zero-uniform check."""
+        mixed = self.precision == "mixed"
+        ends = np.asarray(ends)
+        n = len(ends)
+        good, n_pass, llmax = [None] * n, np.zeros(n, dtype=np.int64), np.zeros(n)
+        sh = np.array([mixed and plan[k][0].shell for k in ends], dtype=bool)
+        lv = np.flatnonzero(~sh)
+        if len(lv):
+            ks = ends[lv]
+            got = reject_batch(self.arena, seg_off[ks], seg_n[lv], [states[j] for j in lv],
+                               self.n_requested, device=self.device, zero_check=mixed)
+            for t, j in enumerate(lv):
+                good[j], n_pass[j], llmax[j] = got[0][t], got[1][t], got[2][t]
+            if mixed and got[3].any():
+                # A uniform of exactly 0 accepts any row within 745 of the
+                # maximum, which the 40 margin did not settle: settle those
+                # segments for every uniform and repeat their rejection step.
+                z = lv[np.flatnonzero(got[3])]
+                ks = ends[z]
+                settle_live_batch(self._table, self.cache, self.arena, self.arena_b, seg_off[ks],
+                                  seg_n[z], np.zeros(len(z), dtype=np.int64),
+                                  np.full(len(z), -np.inf), stars=slots[ks], device=self.device,
+                                  margin=LIVE_MARGIN)
+                g2, p2, m2 = reject_batch(self.arena, seg_off[ks], seg_n[z],
+                                          [states[j] for j in z], self.n_requested,
+                                          device=self.device)
+                for t, j in enumerate(z):
+                    good[j], n_pass[j], llmax[j] = g2[t], p2[t], m2[t]
+                self.zero_uniform_events += len(z)
+        fl = np.flatnonzero(sh)
+        if len(fl):
+            ks = ends[fl]
+            g3, p3, m3, _ = settle_reject_batch(self._table, self.cache, self.arena, self.arena_b,
+                                                seg_off[ks], seg_n[fl], [states[j] for j in fl],
+                                                self.n_requested, stars=slots[ks],
+                                                device=self.device)
+            for t, j in enumerate(fl):
+                good[j], n_pass[j], llmax[j] = g3[t], p3[t], m3[t]
+        return good, n_pass, llmax
 
     def step(self):
         """Admit, evaluate the next rows of every active request (a whole
@@ -604,12 +659,17 @@ class JokerServer:
         if self.precision == "mixed":
             # FP64 for the new rows within reach of the maximum: with them,
             # every row decides like FP64 in this and all later rounds
+            # (requests in shell mode settle at their rejection step instead)
             floors = np.array([r.it.floor for r, _, _ in plan], dtype=np.float64)
-            cnt = settle_live_batch(self._table, self.cache, self.arena, self.arena_b, seg_off,
-                                    before + done0 + sizes, before + done0, floors, stars=slots,
-                                    device=self.device, margin=LIVE_MARGIN_NONZERO_U)
+            live = np.array([not r.shell for r, _, _ in plan])
+            cnt = np.zeros(len(plan), dtype=np.int64)
+            if live.any():
+                c = settle_live_batch(self._table, self.cache, self.arena, self.arena_b,
+                                      seg_off[live], (before + done0 + sizes)[live],
+                                      (before + done0)[live], floors[live], stars=slots[live],
+                                      device=self.device, margin=LIVE_MARGIN_NONZERO_U)
+                cnt[live] = c.cpu().numpy()    # the settling call has synchronised the stream
             lap("settle")
-            cnt = cnt.cpu().numpy()        # the settling call has synchronised the stream
             for k, (r, _, _) in enumerate(plan):
                 r.round_settled += int(cnt[k])
             if self.profile:
@@ -625,35 +685,20 @@ class JokerServer:
             seg_n = before[ends] + n_round[ends]
             states = [pcg64_state(plan[k][0].rng) for k in ends]
             mixed = self.precision == "mixed"
-            got = reject_batch(self.arena, seg_off[ends], seg_n, states, self.n_requested,
-                               device=self.device, zero_check=mixed)
-            good, n_pass, llmax = got[:3]
-            if mixed and got[3].any():
-                # A uniform of exactly 0 accepts any row within 745 of the
-                # maximum, which the 40 margin did not settle: settle those
-                # segments for every uniform and repeat their rejection step.
-                z = np.flatnonzero(got[3])
-                ks = np.asarray(ends)[z]
-                settle_live_batch(self._table, self.cache, self.arena, self.arena_b, seg_off[ks],
-                                  seg_n[z], np.zeros(len(z), dtype=np.int64),
-                                  np.full(len(z), -np.inf), stars=slots[ks], device=self.device,
-                                  margin=LIVE_MARGIN)
-                g2, p2, m2 = reject_batch(self.arena, seg_off[ks], seg_n[z],
-                                          [states[j] for j in z], self.n_requested,
-                                          device=self.device)
-                for t, j in enumerate(z):
-                    good[j], n_pass[j], llmax[j] = g2[t], p2[t], m2[t]
-                self.zero_uniform_events += len(z)
+            good, n_pass, llmax = self._reject_ends(plan, ends, seg_off, slots, seg_n, states)
             lap("reject")
             self._mark("reject")
             for j, k in enumerate(ends):
                 r = plan[k][0]
                 r.round_done = 0
-                if (mixed and not r.exact and self.exact_switch is not None and
+                if (mixed and not (r.exact or r.shell) and self.exact_switch is not None and
                         r.round_settled > self.exact_switch * n_round[k]):
-                    self._table.set_exact(r.slot)
-                    r.exact = True
-                    self.exact_switches += 1
+                    if self.flat_mode == "exact":
+                        self._table.set_exact(r.slot)
+                        r.exact = True
+                    else:
+                        r.shell = True
+                    self.flat_switches += 1
                 r.round_settled = 0
                 try:
                     if self.in_memory and not np.isfinite(llmax[j]):

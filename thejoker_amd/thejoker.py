@@ -189,13 +189,14 @@ class TheJoker:
         ll[first:] that lie within a 750 margin of the maximum, so that the
         rejection step over ll decides like FP64 for every later uniform
         (jk_settle_live_batch on a one-star table; floor: the maximum of
-        ll[:first])."""
+        ll[:first]).  Returns the number of rows re-evaluated."""
         from .batch import StarBatch, settle_live_batch
         table = getattr(star, "_settle_table", None)
         if table is None:
             table = star._settle_table = StarBatch([star], device=device)
-        settle_live_batch(table, cache, ll, bound, [0], [int(ll.shape[0])], [int(first)],
-                          [float(floor)], rows=rows, device=device)
+        cnt = settle_live_batch(table, cache, ll, bound, [0], [int(ll.shape[0])], [int(first)],
+                                [float(floor)], rows=rows, device=device)
+        return int(cnt.cpu()[0])
 
     @staticmethod
     def _settle_tol(star, cache, ll, bound, rows=None, device=0):
@@ -424,6 +425,13 @@ class TheJoker:
         # marks 0 (an FP64 value) and -1 (a probe's upper bound)
         all_b = (torch.empty(it.max_prior, dtype=torch.float32, device=dev)
                  if mixed or pruner is not None else None)
+        # Mixed precision: after a round whose live settling re-evaluated more
+        # than EXACT_SWITCH of its rows in FP64 (a flat likelihood), the later
+        # rounds leave their rows unsettled and each rejection step settles
+        # what its own uniforms need (settle_reject_batch over the buffer).
+        from .batch import EXACT_SWITCH, settle_reject_batch
+        shell = False
+        self.last_shell_from = None     # rows evaluated before the switch, if any
         while not it.done:
             rows, start, n_now = it.next_batch()
             a, b = it.n_evals, it.n_evals + n_now
@@ -440,10 +448,21 @@ class TheJoker:
                         out=all_ll[a:b], bound=lnl_b)
                 if part_b is not None and not mixed:
                     part_b.zero_()
+            if shell:
+                it.check_finite(all_ll[:b])
+                idx, n_pass, mx, _ = settle_reject_batch(
+                    star._settle_table, cache, all_ll[:b], all_b[:b], [0], [b],
+                    [pcg64_state(it.rng)], it.n_requested, rows=it._d_all_idx, device=self.device)
+                it.floor = float(mx[0])
+                it.update(b, idx[0], int(n_pass[0]))
+                continue
             if mixed:
                 # the new rows, for this and every later round's uniforms
-                self._settle_live(star, cache, all_ll[:b], all_b[:b], a, it.floor,
-                                  rows=it._d_all_idx, device=self.device)
+                n_set = self._settle_live(star, cache, all_ll[:b], all_b[:b], a, it.floor,
+                                          rows=it._d_all_idx, device=self.device)
+                if n_set > EXACT_SWITCH * n_now:
+                    shell = True
+                    self.last_shell_from = b
             it.absorb(all_ll[:b], self.device)
         # the final lnL buffer (and bounds: 0 for FP64 values, -1 for a probe's
         # upper bound), for callers that inspect the run
@@ -554,13 +573,19 @@ class IterativeRejection:
         import torch
 
         n_evals = int(ll_so_far.shape[0])
-        if self.in_memory and not bool(torch.isfinite(ll_so_far).all()):
-            raise RuntimeError(f"There are NaN or Inf likelihood values in iteration step "
-                               f"{self.iteration}!")
+        self.check_finite(ll_so_far)
         good, n_good, mx = reject(ll_so_far, pcg64_state(self.rng), self.n_requested,
                                   device=device)
         self.floor = mx
         self.update(n_evals, good, n_good)
+
+    def check_finite(self, ll_so_far):
+        """The in-memory loop's check (likelihood_helpers.py:180-183)."""
+        import torch
+
+        if self.in_memory and not bool(torch.isfinite(ll_so_far).all()):
+            raise RuntimeError(f"There are NaN or Inf likelihood values in iteration step "
+                               f"{self.iteration}!")
 
     def update(self, n_evals, good, n_good):
         """Bookkeeping after the rejection step of a round whose lnL buffer now
