@@ -178,6 +178,39 @@ def upload_items(items, device=0):
     return torch.as_tensor(raw.copy(), device=torch.device("cuda", device))
 
 
+class ItemUploader:
+    """Work items to the device through one pinned host buffer and one
+    device buffer, both kept between calls and grown geometrically: the
+    upload is a host memcpy and an asynchronous DMA, instead of a pageable
+    copy that stages through the runtime on the host.  The returned device
+    view is valid until the next call; a call waits for the previous call's
+    copy before it rewrites the host buffer, and its own copy is ordered on
+    the stream before any later launch reads the device buffer."""
+
+    def __init__(self, device=0):
+        self.device = device
+        self._host = self._host_np = self._dev = self._done = None
+
+    def __call__(self, items):
+        torch = _torch()
+        nb = int(items.nbytes)
+        if self._host is None or self._host.numel() < nb:
+            cap = max(nb, 2 * (0 if self._host is None else self._host.numel()), 1 << 20)
+            if self._done is not None:
+                self._done.synchronize()
+            self._host = torch.empty(cap, dtype=torch.uint8, pin_memory=True)
+            self._host_np = self._host.numpy()
+            self._dev = torch.empty(cap, dtype=torch.uint8,
+                                    device=torch.device("cuda", self.device))
+        elif self._done is not None:
+            self._done.synchronize()
+        np.frombuffer(self._host_np, dtype=items.dtype, count=len(items))[:] = items
+        self._dev[:nb].copy_(self._host[:nb], non_blocking=True)
+        self._done = torch.cuda.Event()
+        self._done.record()
+        return self._dev[:nb]
+
+
 def lnl_batch(batch, cache, d_items, n_items, n_out, precision="mixed", shape=None, out=None,
               rows=None, workspace=None, bound=None):
     """lnL of every work item into a device float64 tensor of n_out entries;

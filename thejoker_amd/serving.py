@@ -32,8 +32,8 @@ import time
 import numpy as np
 
 from .batch import (EXACT_SWITCH, LIVE_MARGIN, LIVE_MARGIN_NONZERO_U, PRUNE_MIN_SIGNAL,
-                    LaunchShape, StarBatch, heaviest_first, lnl_batch, lnl_prune_batch,
-                    reject_batch, settle_live_batch, settle_reject_batch, upload_items,
+                    ItemUploader, LaunchShape, StarBatch, heaviest_first, lnl_batch,
+                    lnl_prune_batch, reject_batch, settle_live_batch, settle_reject_batch,
                     work_items)
 from .device import (DeviceStar, _torch, linear_internal_units, pcg64_state,
                      posterior_inputs_table, probe_star, star_arrays, trend_signal)
@@ -297,6 +297,7 @@ class JokerServer:
         self.rows_probed = 0      # rows of requests whose star was probed
         self.rows_pruned = 0      # of those, the rows that kept their bound
         self._ws = torch.zeros(64, dtype=torch.int32, device=cache.P.device)
+        self._uploader = ItemUploader(device)
         self.profile = profile
         self.phase_s = dict.fromkeys(self.PHASES, 0.0)
         self._t_mark = None
@@ -631,7 +632,7 @@ class JokerServer:
         items["out_offset"] += np.repeat(seg_off + before + done0 - off, counts)
         items = items[heaviest_first(items, self._table.M)]
         lap("plan")
-        d_items = upload_items(items, self.device)
+        d_items = self._uploader(items)
         lap("upload")
         self._mark("plan")
         if self.profile:
