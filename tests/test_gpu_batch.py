@@ -425,3 +425,24 @@ def test_posterior_table_equals_per_star(setup):
     got = D.posterior_inputs_table(batch, pick, cache, rows_list)
     for (pw, aw, cw), (pg, ag, cg) in zip(want, got):
         assert np.array_equal(pw, pg) and np.array_equal(aw, ag) and np.array_equal(cw, cg)
+
+
+def test_item_uploader_equals_pageable_upload(setup):
+    """ItemUploader (pinned host buffer + kept device buffer): the device bytes
+    equal the items on every call, through growth and shrinking, and a
+    launch on the uploaded items equals one on upload_items' copy."""
+    stars, cache, _ = setup
+    batch = B.StarBatch([d for _, d in stars])
+    up = B.ItemUploader()
+    n = len(cache)
+    rng = np.random.default_rng(4)
+    for k, n_rows in enumerate([1000, 40_000, n, 300, 7 * n // 8]):
+        sel = rng.choice(len(stars), size=1 + k % 3 + k, replace=True)
+        items, off, total = B.work_items([n_rows] * len(sel), 512, stars=sel)
+        items = items[rng.permutation(len(items))]
+        d = up(items)
+        assert np.array_equal(d.cpu().numpy(), np.frombuffer(items.tobytes(), np.uint8))
+        got = B.lnl_batch(batch, cache, d, len(items), total, "f64").cpu().numpy()
+        ref = B.lnl_batch(batch, cache, B.upload_items(items), len(items), total,
+                          "f64").cpu().numpy()
+        assert np.array_equal(got, ref, equal_nan=True), k

@@ -205,9 +205,11 @@ class ItemUploader:
         elif self._done is not None:
             self._done.synchronize()
         np.frombuffer(self._host_np, dtype=items.dtype, count=len(items))[:] = items
-        self._dev[:nb].copy_(self._host[:nb], non_blocking=True)
+        stream = torch.cuda.current_stream(self.device)
+        with torch.cuda.stream(stream):
+            self._dev[:nb].copy_(self._host[:nb], non_blocking=True)
         self._done = torch.cuda.Event()
-        self._done.record()
+        self._done.record(stream)
         return self._dev[:nb]
 
 
