@@ -179,17 +179,22 @@ def test_zero_uniform_takes_the_exact_path(cache):
 
 
 @pytest.mark.parametrize("in_memory", [True, False])
-def test_nan_rows_fail_like_the_single_star_path(in_memory):
+@pytest.mark.parametrize("bad_row", [7, 700, 6000])
+def test_nan_rows_fail_like_the_single_star_path(in_memory, bad_row):
     """A prior cache with an invalid row (e = 1.5 gives NaN lnL): in memory
     the reference raises 'NaN or Inf likelihood values' (likelihood_helpers.py:
     173-176), otherwise the NaN maximum accepts nothing and 'Failed to find
-    any good samples!' follows (:187-188).  The server fails the request with
-    the error the single-star call raises, and the other requests finish."""
+    any good samples!' follows (:187-188).  The server (mixed precision)
+    fails the request with the error the single-star FP64 call raises, or
+    returns its result when the run never reaches the row; the other
+    requests finish.  Rows past the first round reach requests that have
+    switched to step settling (flat likelihoods), whose NaN maximum must act
+    the same."""
     cols = syn.prior_columns(1 << 14, seed=3, jitter=True)
-    cols["e"][7] = 1.5
+    cols["e"][bad_row] = 1.5
     bad = D.PriorCache(cols["P"], cols["e"], cols["omega"], cols["M0"], cols["s"])
     prior = _prior()
-    datas = _stars(2, seed=31)
+    datas = _stars(4, seed=31)
     srv = S.JokerServer(prior, bad, n_requested=8, growth_factor=8, in_memory=in_memory,
                         shape=B.LaunchShape(128, 0, 1024))
     for i, d in enumerate(datas):
@@ -197,13 +202,29 @@ def test_nan_rows_fail_like_the_single_star_path(in_memory):
     while srv.busy():
         srv.step()
     got = sorted(srv.finished, key=lambda r: r.id)
+    n_err = 0
     for i, r in enumerate(got):
-        joker = tj.TheJoker(prior, rng=np.random.default_rng(60 + i))
-        with pytest.raises(RuntimeError) as exc:
-            joker.iterative_rejection_sample(datas[i], bad, n_requested_samples=8,
-                                             growth_factor=8, in_memory=in_memory)
-        assert isinstance(r.error, RuntimeError)
-        assert str(r.error) == str(exc.value)
+        outs = []
+        for precision in ("f64", "mixed"):
+            joker = tj.TheJoker(prior, rng=np.random.default_rng(60 + i), precision=precision)
+            try:
+                outs.append(joker.iterative_rejection_sample(
+                    datas[i], bad, n_requested_samples=8, growth_factor=8, in_memory=in_memory))
+            except RuntimeError as exc:
+                outs.append(exc)
+        alone, alone_mixed = outs
+        if isinstance(alone, RuntimeError):
+            n_err += 1
+            for e in (r.error, alone_mixed):
+                assert isinstance(e, RuntimeError), i
+                assert str(e) == str(alone), i
+            continue
+        assert r.error is None, (i, r.error)
+        for name in ("P", "e", "K", "v0", "v1"):
+            assert np.array_equal(r.result[name].value, alone[name].value), (i, name)
+            assert np.array_equal(alone_mixed[name].value, alone[name].value), (i, name)
+    if bad_row == 7:
+        assert n_err == len(datas)
 
 
 def _reference_iterative(ll_all, rng, n_requested, safety, n_total, growth_factor=128):
