@@ -97,7 +97,9 @@ int jk_release_scratch(int device);
  *              in M_T row order (:202-252; lambda_diag[0] is ignored for
  *              JK_KPRIOR_FIXED_COMPANION_MASS)
  *   t_ref      data._t_ref_bmjd (:162)
- * The star lives on the device selected by jk_init. */
+ * The star lives on the device selected by jk_init.  Its arrays are uploaded
+ * on a stream of the library's own, so the call does not wait for the
+ * launches queued on the caller's streams. */
 int jk_star_create(int n, const double *t, const double *rv, const double *ivar,
                    int L, const double *trend, const double *mu, const double *lambda_diag,
                    double t_ref, int k_prior_mode, double sigma_K0, double P0, double max_K,

@@ -31,6 +31,12 @@ namespace {
 std::mutex g_init_mu;
 bool g_inited[64] = {false};
 void *g_noalloc_scratch[64] = {nullptr};   // device address of `scratch` per device
+// Per device, a non-blocking stream for the upload of a new star's epoch
+// arrays: a copy on the default stream waits for every launch queued there.
+// On its own stream the upload waits for its copy only (though where the
+// runtime copies with a kernel, that kernel still needs a free compute unit,
+// which a persistent launch occupying the device does not leave).
+hipStream_t g_upload_stream[64] = {nullptr};
 
 int check_hip(hipError_t e) { return e == hipSuccess ? JK_OK : JK_EHIP; }
 
@@ -183,6 +189,9 @@ int jk_init(int device)
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) return JK_ENODEV;
     if (jk_upload_pcg_table(device) != hipSuccess) return JK_EHIP;
     if (jk_upload_sincos_table(device) != hipSuccess) return JK_EHIP;
+    if (!g_upload_stream[device] &&
+        hipStreamCreateWithFlags(&g_upload_stream[device], hipStreamNonBlocking) != hipSuccess)
+        return JK_EHIP;
     void *sp = nullptr;
     if (hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)) != hipSuccess || !sp) return JK_EHIP;
     g_noalloc_scratch[device] = sp;
@@ -324,7 +333,12 @@ static int star_build(int n, const double *t, const double *rv, const double *iv
         delete st;
         return JK_ENOMEM;
     }
-    if (hipMemcpy(dmem, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+    // on the device's upload stream, so that the copy does not wait for the
+    // launches queued on the default stream; h must outlive the copy
+    hipError_t ce = hipMemcpyAsync(dmem, h.data(), h.size() * sizeof(double),
+                                   hipMemcpyHostToDevice, g_upload_stream[dev]);
+    if (ce == hipSuccess) ce = hipStreamSynchronize(g_upload_stream[dev]);
+    if (ce != hipSuccess) {
         (void)hipFree(dmem);
         delete st;
         return JK_EHIP;

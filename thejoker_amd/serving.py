@@ -150,6 +150,9 @@ class Request:
         self.full_seen = 0      # its slot's full-evaluation counter, last read
         self.full_rate = 0.15   # share of its probed rows evaluated in full, last measured
         self.exact = False      # its slot evaluates in FP64 (flat likelihood)
+        self.prepared = False   # data laid out and iterative state made, not yet admitted
+        self.arrays = self.data_prepared = None
+        self.to_probe = False
         self.shell = False      # settles at each rejection step (flat likelihood)
         self.round_settled = 0  # rows of the current round settled in FP64
 
@@ -224,6 +227,10 @@ class JokerServer:
         whose decision the bound leaves open, a few per million) -- one more
         pass over its rows per round instead of an FP64 evaluation of each.
         "exact": its later rows are evaluated in FP64 (StarBatch.set_exact).
+    prepare_extra : int
+        While a step's likelihood launch runs, the server prepares (checks
+        and lays out the data, sets up the iterative state) the pending
+        requests that free slots will admit next, plus this many more.
     freeze_gc : bool
         Call gc.freeze() at construction: the objects alive then (the
         interpreter's modules, torch, the prior cache) move to the garbage
@@ -240,7 +247,7 @@ class JokerServer:
                  profile=False, max_step_work=None, min_chunk_rows=1 << 16, prune=True,
                  prune_min_epochs=12, prune_min_signal=PRUNE_MIN_SIGNAL, budget_policy="srpt",
                  budget_max_pending=None, exact_switch=EXACT_SWITCH, flat_mode="shell",
-                 freeze_gc=True):
+                 freeze_gc=True, prepare_extra=32):
         torch = _torch()
         self.prune = bool(prune)
         self.prune_min_epochs = int(prune_min_epochs)
@@ -249,6 +256,7 @@ class JokerServer:
         if flat_mode not in ("shell", "exact"):
             raise ValueError("flat_mode must be 'shell' or 'exact'")
         self.flat_mode = flat_mode
+        self.prepare_extra = int(prepare_extra)
         if freeze_gc:
             gc.freeze()
         self.max_step_work = None if max_step_work is None else int(max_step_work)
@@ -392,6 +400,55 @@ class JokerServer:
         self._release(r)
         self.finished.append(r)
 
+    def _prepare(self, r):
+        """The host half of admission, which needs neither the device nor a
+        slot or arena memory: the star's data checked and laid out, its
+        iterative state set up, the decision to probe it.  Raises what a bad
+        request raises."""
+        if r.rng_state0 is None:
+            r.rng_state0 = r.rng.bit_generator.state
+        dd, ids, trend = validate_prepare_data(r.data, self.prior.poly_trend,
+                                               self.prior.n_offsets)
+        a = star_arrays(dd, self.prior, trend)
+        if a["cov"] is not None:
+            raise NotImplementedError("data with a full covariance are evaluated one "
+                                      "star at a time (TheJoker, precision='f64')")
+        r.arrays, r.data_prepared = a, dd
+        r.it = IterativeRejection(len(self.cache), self.n_requested, r.rng,
+                                  in_memory=self.in_memory, growth_factor=self.growth_factor,
+                                  device=self.cache.P.device)
+        r.to_probe = (self.prune and a["t"].size >= self.prune_min_epochs and
+                      trend_signal(a) >= self.prune_min_signal)
+        r.prepared = True
+
+    def _create_device_stars(self, r):
+        """The device half of admission: the request's device star (and
+        probe).  At the start of a step, when the device is idle: an upload
+        behind a running persistent launch would wait for it to end."""
+        a, dd = r.arrays, r.data_prepared
+        r.star = DeviceStar(a["t"], a["rv"], a["ivar"], a["trend"], a["mu"], a["lam"],
+                            a["t_ref"], a["k_prior_mode"], a["sigma_K0"], a["P0"],
+                            a["max_K"], jitter_mode=self.jitter_mode, device=self.device)
+        r.star.data = dd
+        r.star.internal_units = linear_internal_units(self.prior, dd.rv.unit)
+        r.probe, r.prune_add = None, None
+        if r.to_probe:
+            r.probe, r.prune_add = probe_star(a, self.jitter_mode, self.device)
+        r.arrays = r.data_prepared = None
+
+    def _prepare_ahead(self):
+        """The host half of admission for the pending requests that the next
+        step will admit, while this step's likelihood launch runs."""
+        room = self.max_active - len(self.active)
+        for r in list(self.pending[:max(room, 0) + self.prepare_extra]):
+            if r.prepared or r.need_hint:
+                continue
+            try:
+                self._prepare(r)
+            except Exception as exc:       # a bad request fails alone
+                self.pending.remove(r)
+                self._fail(r, exc)
+
     def _admit(self):
         # no admissions while an active request waits for arena memory
         while self.pending and len(self.active) < self.max_active and not self._waiting:
@@ -405,29 +462,12 @@ class JokerServer:
             self.pending.pop(0)
             r.region = (start, first)
             try:
-                if r.rng_state0 is None:
-                    r.rng_state0 = r.rng.bit_generator.state
-                dd, ids, trend = validate_prepare_data(r.data, self.prior.poly_trend,
-                                                       self.prior.n_offsets)
-                a = star_arrays(dd, self.prior, trend)
-                if a["cov"] is not None:
-                    raise NotImplementedError("data with a full covariance are evaluated one "
-                                              "star at a time (TheJoker, precision='f64')")
-                r.star = DeviceStar(a["t"], a["rv"], a["ivar"], a["trend"], a["mu"], a["lam"],
-                                    a["t_ref"], a["k_prior_mode"], a["sigma_K0"], a["P0"],
-                                    a["max_K"], jitter_mode=self.jitter_mode, device=self.device)
-                r.star.data = dd
-                r.star.internal_units = linear_internal_units(self.prior, dd.rv.unit)
-                r.it = IterativeRejection(len(self.cache), self.n_requested, r.rng,
-                                          in_memory=self.in_memory,
-                                          growth_factor=self.growth_factor,
-                                          device=self.cache.P.device)
+                if not r.prepared:
+                    self._prepare(r)
+                self._create_device_stars(r)
                 r.slot = self._free_slots.pop()
                 self._table.set(r.slot, r.star)
-                r.prune_add = None
-                if (self.prune and a["t"].size >= self.prune_min_epochs and
-                        trend_signal(a) >= self.prune_min_signal):
-                    r.probe, r.prune_add = probe_star(a, self.jitter_mode, self.device)
+                if r.prune_add is not None:
                     self._probes.set(r.slot, r.probe)
                     self._n_full[r.slot] = 0
                     r.full_seen = 0
@@ -445,6 +485,7 @@ class JokerServer:
         r.need_hint = r.region[1]
         self._release(r)
         r.it = None
+        r.prepared = False
         r.round_done = 0
         r.round_settled = 0
         r.exact = False
@@ -654,6 +695,8 @@ class JokerServer:
         if not self.profile:
             finished = self._build_results()      # overlaps the launch above
             lap("finish")
+            self._prepare_ahead()                 # so does this
+            lap("prepare")
         M = self._table.M[slots]
         self.sample_epochs += int(np.sum(sizes * M))
         self._mark("lnl")
