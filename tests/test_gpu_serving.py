@@ -92,18 +92,27 @@ def test_flat_likelihoods_switch_and_change_no_result(cache, flat_mode):
         assert rngs[i].uniform() == rng.uniform(), i
 
 
-def test_bad_request_fails_alone(cache):
+@pytest.mark.parametrize("max_active", [4, 1])
+def test_bad_request_fails_alone(cache, max_active):
+    """A malformed request fails alone, whether it is prepared at its
+    admission or ahead of it while a step's launch runs (max_active 1: the
+    later requests wait in the queue and are prepared ahead)."""
     prior = _prior()
     datas = _stars(3, seed=8)
-    srv = S.JokerServer(prior, cache, n_requested=8, growth_factor=8)
+    srv = S.JokerServer(prior, cache, n_requested=8, growth_factor=8, max_active=max_active)
     srv.submit(datas[0], np.random.default_rng(1), rid=0)
     srv.submit("not data", np.random.default_rng(2), rid=1)
     srv.submit(datas[2], np.random.default_rng(3), rid=2)
     while srv.busy():
         srv.step()
     res = {r.id: r for r in srv.finished}
+    assert len(res) == 3
     assert res[1].error is not None
     assert res[0].error is None and res[2].error is None and len(res[0].result) > 0
+    for i in (0, 2):
+        alone = tj.TheJoker(prior, rng=np.random.default_rng(1 + i)).iterative_rejection_sample(
+            datas[i], cache, n_requested_samples=8, growth_factor=8)
+        assert np.array_equal(res[i].result["P"].value, alone["P"].value), i
 
 
 def test_rejection_sample_many_equals_single(cache):
