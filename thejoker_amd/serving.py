@@ -374,9 +374,8 @@ class JokerServer:
 
     def _probe_feedback(self, probed):
         """After a step: count each probed request's pruned rows from the
-        per-slot counters of full evaluations.  (Widening the probe of
-        requests that prune little, n/6 -> n/3 epochs, was measured and
-        did not pay: 89% instead of 86% of the rows pruned, same capacity.)"""
+        per-slot counters of full evaluations, and keep each request's share
+        of rows evaluated in full for the step budget's cost model."""
         full = self._n_full.cpu().numpy()
         for r, slot, size in probed:
             d = int(full[slot]) - r.full_seen
