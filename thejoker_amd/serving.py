@@ -500,7 +500,13 @@ class JokerServer:
         start, cap = r.region
         if need <= cap:
             return True
-        caps = [min(max(need, 2 * cap), r.it.max_prior)]
+        # A round that needs a quarter of the cache or more is followed, if at
+        # all, by one that needs the rest: take the whole cache's room at
+        # once (when the arena has it), so the values are not moved again.
+        want = max(need, 2 * cap)
+        if need >= r.it.max_prior // 4:
+            want = r.it.max_prior
+        caps = [min(want, r.it.max_prior)]
         if caps[0] > need:
             caps.append(need)
         # in place, when the range after the region is free
