@@ -9,7 +9,8 @@
 //                          exp(ll - max) > u, one 64-bit ballot word per wave.
 //  k_reject_bsum/compact   order-preserving compaction of the ballot words.
 //  k_uniform               the raw uniforms (tests / callers).
-//  k_linear_posterior      a = G^-1 rhs and G^-1 for accepted rows.
+//  k_linear_posterior      a = G^-1 rhs and G^-1 for accepted rows (one wave
+//                          per row for diagonal-covariance stars).
 #include <cmath>
 #include <type_traits>
 
@@ -2613,8 +2614,79 @@ hipError_t jk_launch_prior_fill(int64_t n, int64_t row0, uint64_t seed, double P
 // lane per row, runtime L <= JK_LMAX.  Lambda[0] is not clipped here
 // (fast_likelihood.pyx:519-521).
 // ---------------------------------------------------------------------------
-// One accepted row of star st: a, cov and lnL into slot i of the outputs.
-// m_col: this lane's LDS column for a full-covariance star (else unused).
+// The end of a row's linear posterior from its sums over the epochs (G lower
+// triangle, rhs, u, chi0, sum ln w): the prior added, Cholesky, cov = G^-1,
+// a = cov rhs, and the row's lnL.
+__device__ __forceinline__ void jk_posterior_finish(const StarDev &st, int64_t i, double lam0,
+                                                    double *G, double *rhs, const double *u,
+                                                    double chi0, double sum_ln_w,
+                                                    double *__restrict__ a_out,
+                                                    double *__restrict__ cov_out,
+                                                    double *__restrict__ ll_out)
+{
+    const int L = st.L, N = st.n;
+    double lnlam = log(lam0) + st.ln_lam_trend;
+    G[0] += 1.0 / lam0;
+    rhs[0] += st.mu0 / lam0;
+    for (int a = 1; a < L; ++a) {
+        G[a * JK_LMAX + a] += st.lam_inv[a];
+        rhs[a] += st.mu[a] * st.lam_inv[a];
+    }
+    // Cholesky G = R R^T (lower, in place).
+    double lndet = 0.0;
+    bool ok = true;
+    for (int j = 0; j < L && ok; ++j) {
+        double d = G[j * JK_LMAX + j];
+        for (int k = 0; k < j; ++k) d -= G[j * JK_LMAX + k] * G[j * JK_LMAX + k];
+        if (!(d > 0.0)) { ok = false; break; }
+        d = sqrt(d);
+        G[j * JK_LMAX + j] = d;
+        lndet += 2.0 * log(d);
+        for (int q = j + 1; q < L; ++q) {
+            double v = G[q * JK_LMAX + j];
+            for (int k = 0; k < j; ++k) v -= G[q * JK_LMAX + k] * G[j * JK_LMAX + k];
+            G[q * JK_LMAX + j] = v / d;
+        }
+    }
+    if (!ok) {
+        for (int q = 0; q < L; ++q) a_out[i * L + q] = __builtin_nan("");
+        for (int q = 0; q < L * L; ++q) cov_out[i * L * L + q] = __builtin_nan("");
+        ll_out[i] = INFINITY;
+        return;
+    }
+    // Rinv (lower) by forward substitution, then cov = Rinv^T Rinv.
+    double Ri[JK_LMAX * JK_LMAX];
+    for (int q = 0; q < JK_LMAX * JK_LMAX; ++q) Ri[q] = 0.0;
+    for (int c = 0; c < L; ++c) {
+        for (int a = c; a < L; ++a) {
+            double v = a == c ? 1.0 : 0.0;
+            for (int k = c; k < a; ++k) v -= G[a * JK_LMAX + k] * Ri[k * JK_LMAX + c];
+            Ri[a * JK_LMAX + c] = v / G[a * JK_LMAX + a];
+        }
+    }
+    for (int a = 0; a < L; ++a)
+        for (int b = 0; b < L; ++b) {
+            double v = 0.0;
+            for (int k = (a > b ? a : b); k < L; ++k) v += Ri[k * JK_LMAX + a] * Ri[k * JK_LMAX + b];
+            cov_out[i * L * L + a * L + b] = v;
+        }
+    // z = R^-1 u for chi2; a = cov rhs.
+    double zz = 0.0;
+    for (int a = 0; a < L; ++a) {
+        double v = 0.0;
+        for (int k = 0; k <= a; ++k) v += Ri[a * JK_LMAX + k] * u[k];
+        zz += v * v;
+    }
+    for (int a = 0; a < L; ++a) {
+        double v = 0.0;
+        for (int b = 0; b < L; ++b) v += cov_out[i * L * L + a * L + b] * rhs[b];
+        a_out[i * L + a] = v;
+    }
+    ll_out[i] = -0.5 * (chi0 - zz + (double)N * JK_LN_TWO_PI - sum_ln_w + lnlam + lndet);
+}
+
+// One accepted row of star st, one lane: a, cov and lnL into slot i of the
+// outputs.  m_col: this lane's LDS column for a full-covariance star.
 __device__ __forceinline__ void jk_posterior_row(const StarDev &st, int64_t i, int64_t r,
                                                  const double *__restrict__ P,
                                                  const double *__restrict__ E_,
@@ -2681,65 +2753,105 @@ __device__ __forceinline__ void jk_posterior_row(const StarDev &st, int64_t i, i
             for (int b = 0; b <= a; ++b) G[a * JK_LMAX + b] = fma(wx, x[b], G[a * JK_LMAX + b]);
         }
     }
-    double lnlam = log(lam0) + st.ln_lam_trend;
-    G[0] += 1.0 / lam0;
-    rhs[0] += st.mu0 / lam0;
-    for (int a = 1; a < L; ++a) {
-        G[a * JK_LMAX + a] += st.lam_inv[a];
-        rhs[a] += st.mu[a] * st.lam_inv[a];
-    }
-    // Cholesky G = R R^T (lower, in place).
-    double lndet = 0.0;
-    bool ok = true;
-    for (int j = 0; j < L && ok; ++j) {
-        double d = G[j * JK_LMAX + j];
-        for (int k = 0; k < j; ++k) d -= G[j * JK_LMAX + k] * G[j * JK_LMAX + k];
-        if (!(d > 0.0)) { ok = false; break; }
-        d = sqrt(d);
-        G[j * JK_LMAX + j] = d;
-        lndet += 2.0 * log(d);
-        for (int q = j + 1; q < L; ++q) {
-            double v = G[q * JK_LMAX + j];
-            for (int k = 0; k < j; ++k) v -= G[q * JK_LMAX + k] * G[j * JK_LMAX + k];
-            G[q * JK_LMAX + j] = v / d;
+    jk_posterior_finish(st, i, lam0, G, rhs, u, chi0, sum_ln_w, a_out, cov_out, ll_out);
+}
+
+// One accepted row of a diagonal-covariance star per wave: lane l takes the
+// epochs l, l + 64, ..., the 64 partial sums of each quantity are combined by
+// the same butterfly in every launch (so the one-star and the many-star
+// launches give the same bits), and lane 0 finishes the row.  A row's epochs
+// then cost one wave about N / 64 epoch steps, instead of one lane N.
+__device__ __forceinline__ double jk_wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ void jk_posterior_row_wave(const StarDev &st, int64_t i, int64_t r,
+                                                      const double *__restrict__ P,
+                                                      const double *__restrict__ E_,
+                                                      const double *__restrict__ OM,
+                                                      const double *__restrict__ MM,
+                                                      const double *__restrict__ S,
+                                                      double *__restrict__ a_out,
+                                                      double *__restrict__ cov_out,
+                                                      double *__restrict__ ll_out)
+{
+    const int lane = threadIdx.x & 63;
+    const int L = st.L, N = st.n;
+    const double Pv = P[r], ev = E_[r], omv = OM[r], M0v = MM[r];
+    const double sv = st.jitter_mode == JK_JITTER_PHYSICAL ? S[r] : 0.0;
+    const bool valid = isfinite(Pv) && Pv != 0.0 && ev >= 0.0 && ev < 1.0 && isfinite(omv) &&
+                       isfinite(M0v) && isfinite(sv);
+    if (!valid) {
+        if (lane == 0) {
+            for (int q = 0; q < L; ++q) a_out[i * L + q] = __builtin_nan("");
+            for (int q = 0; q < L * L; ++q) cov_out[i * L * L + q] = __builtin_nan("");
+            ll_out[i] = __builtin_nan("");
         }
-    }
-    if (!ok) {
-        for (int q = 0; q < L; ++q) a_out[i * L + q] = __builtin_nan("");
-        for (int q = 0; q < L * L; ++q) cov_out[i * L * L + q] = __builtin_nan("");
-        ll_out[i] = INFINITY;
         return;
     }
-    // Rinv (lower) by forward substitution, then cov = Rinv^T Rinv.
-    double Ri[JK_LMAX * JK_LMAX];
-    for (int q = 0; q < JK_LMAX * JK_LMAX; ++q) Ri[q] = 0.0;
-    for (int c = 0; c < L; ++c) {
-        for (int a = c; a < L; ++a) {
-            double v = a == c ? 1.0 : 0.0;
-            for (int k = c; k < a; ++k) v -= G[a * JK_LMAX + k] * Ri[k * JK_LMAX + c];
-            Ri[a * JK_LMAX + c] = v / G[a * JK_LMAX + a];
+    const double lam0 = st.k_prior_mode == JK_KPRIOR_FIXED_COMPANION_MASS
+                            ? jk_k_variance(&st, Pv, ev, false)
+                            : st.lam0;
+    double sw, cw;
+    sincos(omv, &sw, &cw);
+    const double ecw = ev * cw, sq = sqrt((1.0 - ev) * (1.0 + ev));
+    const double nP = JK_TWO_PI / Pv;
+    double G[JK_LMAX * JK_LMAX], rhs[JK_LMAX], u[JK_LMAX], x[JK_LMAX];
+    for (int q = 0; q < JK_LMAX * JK_LMAX; ++q) G[q] = 0.0;
+    for (int q = 0; q < JK_LMAX; ++q) { rhs[q] = 0.0; u[q] = 0.0; }
+    double chi0 = 0.0, sum_ln_w = 0.0;
+    for (int k = lane; k < N; k += 64) {
+        double sE, cE;
+        jk_kepler_f64(jk_reduce_2pi(fma(nP, st.dt[k], -M0v)), ev, sE, cE);
+        x[0] = jk_design_f64(sE, cE, ev, sq, cw, sw, ecw);
+        for (int j = 1; j < L; ++j) x[j] = st.T[(j - 1) * N + k];
+        const double iv = st.ivar[k];
+        const double w = st.jitter_mode == JK_JITTER_PHYSICAL ? iv / fma(sv * sv, iv, 1.0) : iv;
+        const double rr = st.rp[k] - st.mu0 * x[0];
+        chi0 = fma(w * rr, rr, chi0);
+        sum_ln_w += log(w);
+        for (int a = 0; a < L; ++a) {
+            const double wx = w * x[a];
+            rhs[a] = fma(wx, st.y[k], rhs[a]);
+            u[a] = fma(wx, rr, u[a]);
+            for (int b = 0; b <= a; ++b) G[a * JK_LMAX + b] = fma(wx, x[b], G[a * JK_LMAX + b]);
         }
     }
-    for (int a = 0; a < L; ++a)
-        for (int b = 0; b < L; ++b) {
-            double v = 0.0;
-            for (int k = (a > b ? a : b); k < L; ++k) v += Ri[k * JK_LMAX + a] * Ri[k * JK_LMAX + b];
-            cov_out[i * L * L + a * L + b] = v;
-        }
-    // z = R^-1 u for chi2; a = cov rhs.
-    double zz = 0.0;
+    chi0 = jk_wave_sum(chi0);
+    sum_ln_w = jk_wave_sum(sum_ln_w);
     for (int a = 0; a < L; ++a) {
-        double v = 0.0;
-        for (int k = 0; k <= a; ++k) v += Ri[a * JK_LMAX + k] * u[k];
-        zz += v * v;
+        rhs[a] = jk_wave_sum(rhs[a]);
+        u[a] = jk_wave_sum(u[a]);
+        for (int b = 0; b <= a; ++b) G[a * JK_LMAX + b] = jk_wave_sum(G[a * JK_LMAX + b]);
     }
-    for (int a = 0; a < L; ++a) {
-        double v = 0.0;
-        for (int b = 0; b < L; ++b) v += cov_out[i * L * L + a * L + b] * rhs[b];
-        a_out[i * L + a] = v;
-    }
-    ll_out[i] = -0.5 * (chi0 - zz + (double)N * JK_LN_TWO_PI - sum_ln_w + lnlam + lndet);
+    if (lane == 0)
+        jk_posterior_finish(st, i, lam0, G, rhs, u, chi0, sum_ln_w, a_out, cov_out, ll_out);
 }
+
+// Diagonal-covariance rows of one star, one wave (a 64-thread block) per row.
+__global__ __launch_bounds__(64) void k_linear_posterior_wave(const StarDev st, int64_t n,
+                                                              const double *__restrict__ P,
+                                                              const double *__restrict__ E_,
+                                                              const double *__restrict__ OM,
+                                                              const double *__restrict__ MM,
+                                                              const double *__restrict__ S,
+                                                              const int64_t *__restrict__ rows,
+                                                              double *__restrict__ a_out,
+                                                              double *__restrict__ cov_out,
+                                                              double *__restrict__ ll_out,
+                                                              int64_t i0)
+{
+    const int64_t i = i0 + blockIdx.x;
+    if (i >= n) return;
+    jk_posterior_row_wave(st, i, rows ? rows[i] : i, P, E_, OM, MM, S, a_out, cov_out, ll_out);
+}
+
+// Rows per launch of the one-wave-per-row posterior kernels (the grid's
+// blocks x 64 threads stay far inside 2^32).
+#define JK_POST_CHUNK ((int64_t)1 << 24)
 
 __global__ __launch_bounds__(64) void k_linear_posterior(const StarDev st, int64_t n,
                                                          const double *__restrict__ P,
@@ -2772,12 +2884,13 @@ __global__ __launch_bounds__(64) void k_linear_posterior_batch(const StarDev *__
                                                                const int64_t *__restrict__ rows,
                                                                double *__restrict__ a_out,
                                                                double *__restrict__ cov_out,
-                                                               double *__restrict__ ll_out)
+                                                               double *__restrict__ ll_out,
+                                                               int64_t i0)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = i0 + blockIdx.x;
     if (i >= n) return;
-    jk_posterior_row(stars[star_idx[i]], i, rows[i], P, E_, OM, MM, S, a_out, cov_out, ll_out,
-                     nullptr);
+    jk_posterior_row_wave(stars[star_idx[i]], i, rows[i], P, E_, OM, MM, S, a_out, cov_out,
+                          ll_out);
 }
 
 hipError_t jk_launch_linear_posterior_batch(const jk_batch *b, int64_t n, const int32_t *star_idx,
@@ -2787,9 +2900,12 @@ hipError_t jk_launch_linear_posterior_batch(const jk_batch *b, int64_t n, const 
                                             double *ll_out, hipStream_t stream)
 {
     if (n <= 0) return hipSuccess;
-    const int64_t blocks = (n + 63) / 64;
-    hipLaunchKernelGGL(k_linear_posterior_batch, dim3((unsigned)blocks), dim3(64), 0, stream,
-                       b->d_stars, star_idx, n, P, e, om, M0, s, rows, a_out, cov_out, ll_out);
+    for (int64_t i0 = 0; i0 < n; i0 += JK_POST_CHUNK) {
+        const int64_t nb = n - i0 < JK_POST_CHUNK ? n - i0 : JK_POST_CHUNK;
+        hipLaunchKernelGGL(k_linear_posterior_batch, dim3((unsigned)nb), dim3(64), 0, stream,
+                           b->d_stars, star_idx, n, P, e, om, M0, s, rows, a_out, cov_out,
+                           ll_out, i0);
+    }
     return hipGetLastError();
 }
 
@@ -2799,8 +2915,16 @@ hipError_t jk_launch_linear_posterior(const jk_star *st, int64_t n, const double
                                       double *cov_out, double *ll_out, hipStream_t stream)
 {
     if (n <= 0) return hipSuccess;
+    if (!st->dev.Qt) {
+        for (int64_t i0 = 0; i0 < n; i0 += JK_POST_CHUNK) {
+            const int64_t nb = n - i0 < JK_POST_CHUNK ? n - i0 : JK_POST_CHUNK;
+            hipLaunchKernelGGL(k_linear_posterior_wave, dim3((unsigned)nb), dim3(64), 0, stream,
+                               st->dev, n, P, e, om, M0, s, rows, a_out, cov_out, ll_out, i0);
+        }
+        return hipGetLastError();
+    }
     const int64_t blocks = (n + 63) / 64;
-    const size_t lds = st->dev.Qt ? sizeof(double) * (size_t)st->dev.n * JK_COV_LANES : 0;
+    const size_t lds = sizeof(double) * (size_t)st->dev.n * JK_COV_LANES;
     hipLaunchKernelGGL(k_linear_posterior, dim3((unsigned)blocks), dim3(64), lds, stream, st->dev,
                        n, P, e, om, M0, s, rows, a_out, cov_out, ll_out);
     return hipGetLastError();
