@@ -295,6 +295,31 @@ def leg_single(args, torch, np, D, syn):
                     **mixed_vs_f64(torch, np, D, dstar, cache, ll, ll32, bnd, seed=7)}
     log(f"[bench] mixed: {m_ms:.3f} ms ({k_ms / m_ms:.2f}x), {fmt_mixed(out['mixed'])}")
 
+    # The same rows through the probe bound: TheJoker(prune=True).rejection_
+    # sample's likelihood and rejection step (probe launch, FP64 evaluation of
+    # the rows near the maximum, settled rejection), against the plain path's
+    # accepted rows for one random stream.
+    import thejoker_amd as tj
+    from thejoker_amd import units as u
+    pr = tj.JokerPrior.default(P_min=2 * u.day, P_max=2048 * u.day, sigma_K0=30 * u.km / u.s,
+                               sigma_v=100 * u.km / u.s)
+    state = D.pcg64_state(np.random.default_rng(99))
+    _, ref_idx, ref_pass, _ = D.lnl_reject(dstar, cache, state, 256, out=ll)
+    out["pruned"] = {"plain_step_ms": ms_step}
+    for prec in ("f64", "mixed"):
+        jk = tj.TheJoker(pr, precision=prec, prune=True)
+        _, _, idx = jk._pruned_reject(dstar, cache, n, state, 256)
+        torch.cuda.synchronize()
+        t_p = time.perf_counter()
+        for _ in range(reps):
+            _, _, idx = jk._pruned_reject(dstar, cache, n, state, 256)
+        torch.cuda.synchronize()
+        p_ms = (time.perf_counter() - t_p) / reps * 1e3
+        out["pruned"][prec] = {"ms": p_ms, "speedup_vs_plain_step": ms_step / p_ms,
+                               "accepted_equal_plain": bool(np.array_equal(idx, ref_idx))}
+        log(f"[bench] single: pruned {prec} {p_ms:.3f} ms ({ms_step / p_ms:.2f}x the plain step), "
+            f"accepted rows equal the plain path's: {np.array_equal(idx, ref_idx)}")
+
     if not args.no_cpu:
         rows = np.ascontiguousarray(np.stack([cols[k] for k in ("P", "e", "omega", "M0", "s")],
                                              axis=1))
