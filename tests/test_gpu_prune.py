@@ -245,6 +245,10 @@ def test_prune_argument_checks(setup):
         B.lnl_prune_batch(table, probes, prune, cache, d_items, len(items), "mixed", out=out)
     with pytest.raises(ValueError):
         B.lnl_prune_batch(table, probes, prune[:4], cache, d_items, len(items), "f64", out=out)
+    # jk_batch_set_exact: a slot outside the table is refused
+    for slot in (-1, 3):
+        with pytest.raises(_lib.JokerLibError):
+            table.set_exact(slot)
 
 
 def _prior():
