@@ -236,7 +236,9 @@ class JokerServer:
         interpreter's modules, torch, the prior cache) move to the garbage
         collector's permanent generation and later collections skip them.
         Otherwise a full collection scans them all (30-80 ms with torch
-        loaded) and stalls whichever step it falls in.
+        loaded) and stalls whichever step it falls in.  Reference counting
+        still frees those objects; only a reference cycle among them would
+        stay uncollected (gc.unfreeze() undoes it).
     """
 
     PHASES = ("admit", "plan", "lnl", "polish", "reject", "update", "finish")
