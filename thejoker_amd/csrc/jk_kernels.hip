@@ -200,10 +200,18 @@ __device__ __forceinline__ double jk_rcp2(double x)
 // misleads the starter): the caller then re-evaluates the whole sample with
 // the bracketed Newton iteration, outside the epoch loop, so that rare path
 // adds no registers to the loop.
+// With WV the epoch's variance v (sigma^2 + s^2) comes in and its reciprocal
+// goes out in *w: both reciprocals come from one, R = 1 / ((1 - e cos E1) v),
+// as r = R v and w = R (1 - e cos E1) (two roundings more than separate
+// reciprocals, one hardware reciprocal fewer per epoch).
+// sin E takes the sign of x by copysign: sin E1 and its rotation are >= 0 on
+// [0, pi] up to rounding at the ends of the interval.
+template <bool WV = false>
 __device__ __forceinline__ bool jk_kepler_markley_f64(double x, double e, float ef, float omef,
                                                       float a1f, double dmax,
                                                       const double2 *__restrict__ tab,
-                                                      double &sinE, double &cosE)
+                                                      double &sinE, double &cosE,
+                                                      double v = 0.0, double *w = nullptr)
 {
     const double ax = fabs(x);
     const float E1f = jk_markley_starter((float)ax, ef, omef, a1f);
@@ -214,7 +222,15 @@ __device__ __forceinline__ bool jk_kepler_markley_f64(double x, double e, float 
     const double f0 = (E1 - ax) - es;
     // One Newton step makes r good to the square of the hardware estimate's
     // error, far below what d (<= 4.4e-4) needs.
-    const double r = jk_rcp1(1.0 - ec);
+    const double D1 = 1.0 - ec;
+    double r;
+    if constexpr (WV) {
+        const double R = jk_rcp1(D1 * v);
+        r = R * v;
+        *w = R * D1;
+    } else {
+        r = jk_rcp1(D1);
+    }
     const double t = -f0 * r;
     const double b2 = 0.5 * es * r;
     const double b3 = ec * r * (1.0 / 6.0);
@@ -227,7 +243,7 @@ __device__ __forceinline__ bool jk_kepler_markley_f64(double x, double e, float 
     const double sd = d * fma(d2, -1.0 / 6.0, 1.0);
     const double s = fma(s1, cd, c1 * sd);
     cosE = fma(c1, cd, -s1 * sd);
-    sinE = x < 0.0 ? -s : s;
+    sinE = copysign(s, x);
     return fabs(d) < dmax;
 }
 
@@ -314,7 +330,9 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
     else
         lam0 = st.lam0;
 
-    const double nP = JK_TWO_PI * jk_rcp2(Pv);
+    const double invP = jk_rcp2(Pv);
+    const double nP = JK_TWO_PI * invP;
+    const double M0t = M0v * JK_INV_TWO_PI;
     const double s2 = sv * sv;
 
     // Per-sample orbit constants.
@@ -362,8 +380,14 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
             const double ph = fma(nP, dtk, -M0v);
             x = fma(-rint(ph * JK_INV_TWO_PI), JK_TWO_PI_HI, ph);
         } else {
-            x = jk_reduce_2pi(fma(nP, dtk, -M0v));
+            // the phase in turns, dt / P - M0 / (2 pi), one rounding; its
+            // fractional part is exact (Sterbenz) and scales to radians with
+            // one more
+            const double pt = fma(invP, dtk, -M0t);
+            x = (pt - rint(pt)) * JK_TWO_PI;
         }
+        double w = 0.0;
+        const double vk = PHYS ? st.sig2[k] + s2 : 0.0;
         double m;
         float denb = 0.0f;
         if constexpr (MIXED) {
@@ -377,32 +401,32 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
             denb = den;
         } else {
             double sE, cE;
-            if constexpr (SAFE)
+            if constexpr (SAFE) {
                 jk_kepler_poly_f64(x, ev, tab, sE, cE);
-            else
+                if constexpr (PHYS) w = jk_rcp1(vk);
+            } else if constexpr (PHYS) {
+                bad |= !jk_kepler_markley_f64<true>(x, ev, ef, omef, a1f, dmax, tab, sE, cE,
+                                                    vk, &w);
+            } else {
                 bad |= !jk_kepler_markley_f64(x, ev, ef, omef, a1f, dmax, tab, sE, cE);
+            }
             // m = [cos w (cos E - e) - sin w sqrt(1-e^2) sin E] / (1 - e cos E) + e cos w
             // (one Newton step on the hardware reciprocal: relative 2e-15)
             const double den = jk_rcp1(fma(-ev, cE, 1.0));
             m = fma(fma(cw, cE - ev, -swsq * sE), den, ecw);
         }
         const double rp = st.rp[k];
-        double w;
         if constexpr (PHYS) {
-            // w = 1 / (sigma^2 + s^2): FP64 to a relative 2.4e-15 (one Newton
-            // step on the hardware reciprocal), or in the mixed row the FP32
-            // reciprocal (relative error below 2^-22, which the bound
-            // covers); sum ln w = -ln of the running product of the FP64
-            // variances, renormalised every fourth epoch (frexp scales by a
-            // power of two, exactly, so the product's rounding is that of
-            // renormalising every epoch; four factors stay far inside the
-            // exponent range)
-            const double d = st.sig2[k] + s2;
-            if constexpr (MIXED)
-                w = (double)__builtin_amdgcn_rcpf((float)d);
-            else
-                w = jk_rcp1(d);
-            prod *= d;
+            // w = 1 / (sigma^2 + s^2): FP64 to a relative 2.5e-15 (one Newton
+            // step on the hardware reciprocal, shared with the Kepler step's),
+            // or in the mixed row the FP32 reciprocal (relative error below
+            // 2^-22, which the bound covers); sum ln w = -ln of the running
+            // product of the FP64 variances, renormalised every fourth epoch
+            // (frexp scales by a power of two, exactly, so the product's
+            // rounding is that of renormalising every epoch; four factors
+            // stay far inside the exponent range)
+            if constexpr (MIXED) w = (double)__builtin_amdgcn_rcpf((float)vk);
+            prod *= vk;
             if ((k & 3) == 3) {
                 int ex;
                 prod = frexp(prod, &ex);
