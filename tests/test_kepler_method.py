@@ -12,9 +12,9 @@ LD = np.longdouble
 A0 = f32(7.6516382901912925)     # 3 pi^2 / (pi^2 - 6)
 A1N = f32(1.29898246041084)      # 1.6 pi / (pi^2 - 6)
 PIF = f32(np.pi)
-NT = 256
+NT = 1024
 _m, _ex = np.frexp(np.pi / NT)
-H = float(np.ldexp(np.round(_m * 2 ** 44), _ex - 44))
+H = float(np.ldexp(np.ceil(_m * 2 ** 42), _ex - 42))
 TS = np.sin(np.arange(NT + 1) * LD(H)).astype(np.float64)
 TC = np.cos(np.arange(NT + 1) * LD(H)).astype(np.float64)
 
@@ -41,7 +41,7 @@ def table_sincos(Ef):
     d = Ef.astype(np.float64) - idx * H
     d2 = d * d
     sd = d + d * d2 * (d2 / 120 - 1 / 6)
-    cd = 1 + d2 * (-0.5 + d2 * (1 / 24 - d2 / 720))
+    cd = 1 + d2 * (-0.5 + d2 / 24)
     return TS[idx] * cd + TC[idx] * sd, TC[idx] * cd - TS[idx] * sd
 
 
@@ -50,12 +50,13 @@ def solve(M, e):
     E1f = starter(M.astype(f32), np.minimum(e.astype(f32), f32(0.99999994)))
     E1 = E1f.astype(np.float64)
     s1, c1 = table_sincos(E1f)
-    es, ec = e * s1, e * c1
-    f0 = (E1 - M) - es
-    r = 1.0 / (1.0 - ec)
+    # fma(-e, s1, E1 - M) and fma(-e, c1, 1): the exact product and sum
+    # rounded once (long double holds the product exactly enough)
+    f0 = ((E1 - M).astype(LD) - e.astype(LD) * s1.astype(LD)).astype(np.float64)
+    r = 1.0 / (1.0 - e.astype(LD) * c1.astype(LD)).astype(np.float64)
     t = -f0 * r
-    b2 = 0.5 * es * r
-    b3 = ec * r / 6
+    b2 = (0.5 * e) * s1 * r
+    b3 = (r - 1.0) * (1 / 6)
     b22 = b2 * b2
     c3 = 2 * b22 - b3
     c4 = b2 * (5 * (b3 - b22) + 1 / 12)
@@ -91,6 +92,24 @@ def test_series_correction_reaches_conditioning(e0):
     assert np.abs((c - np.cos(T)).astype(float)).max() < 4 * eps + 2.5 * cond.max()
 
 
+def test_grid_constants_are_the_kernels():
+    """The emulation's grid is the one the kernel source defines: N
+    intervals, H = pi / N rounded up to 42 significant bits (i H exact in FP64
+    for i <= N, N H >= pi), the FP32 step and the FP32 index scale."""
+    import os
+    import re
+    src = open(os.path.join(os.path.dirname(__file__), "..", "thejoker_amd", "csrc",
+                            "jk_kernels.hip")).read()
+    define = lambda name: re.search(r"#define %s (\S+)" % name, src).group(1)
+    assert int(define("JK_TAB_N")) == NT
+    assert float.fromhex(define("JK_TAB_H")) == H
+    assert float.fromhex(define("JK_TAB_H32").rstrip("f")) == _H32
+    assert f32(define("JK_TAB_INV_H_F").rstrip("f")) == f32(NT / np.pi)
+    assert NT * H >= np.pi
+    from fractions import Fraction
+    assert all(Fraction(i) * Fraction(H) == Fraction(i * H) for i in range(NT + 1))
+
+
 def test_table_sincos_accuracy():
     Ef = np.random.default_rng(0).uniform(0, np.pi, 200_000).astype(f32)
     s, c = table_sincos(Ef)
@@ -119,7 +138,7 @@ def test_truncated_series_would_fail():
     assert (np.abs((E - T).astype(float)) / cond).max() > 10
 
 
-_H32 = np.float64(np.float32(np.pi / 256))
+_H32 = np.float64(np.float32(np.pi / NT))
 _G32 = np.arange(NT + 1, dtype=LD) * LD(_H32)
 _TS32 = np.sin(_G32).astype(f32)
 _TC32 = np.cos(_G32).astype(f32)
@@ -127,10 +146,10 @@ _TC32 = np.cos(_G32).astype(f32)
 
 def _sincos_tab_f32(x):
     """The mixed kernel's FP32 sin/cos on [0, pi]: the FP32 grid i * H32
-    (H32 = pi / 256 rounded to FP32, values rounded once from long double),
+    (H32 = pi / 1024 rounded to FP32, values rounded once from long double),
     d = x - i H32 rounded once (fma), sin d to d^3, cos d to d^2."""
     E = x.astype(f32)
-    idx = np.clip((E * f32(256 / np.pi)).astype(np.int32), 0, NT)
+    idx = np.clip((E * f32(NT / np.pi)).astype(np.int32), 0, NT)
     d = (E.astype(np.float64) - idx * _H32).astype(f32)
     d2 = d * d
     sd = (d.astype(np.float64) * d2 * (-1 / 6) + d).astype(f32)

@@ -30,16 +30,17 @@ __constant__ u128 c_tid_A[256];
 __constant__ u128 c_tid_D[256];
 
 // (sin g_i, cos g_i) on the grid g_i = i * JK_TAB_H, i = 0 .. JK_TAB_N, over
-// [0, pi].  JK_TAB_H is pi / 256 rounded to 44 significant bits, so i * H is
-// exact in FP64.  Uploaded once per device by jk_init; the kernels that
-// evaluate the FP64 likelihood stage it in LDS.
-#define JK_TAB_N 256
-#define JK_TAB_H 0x1.921fb54442e00p-7
-#define JK_TAB_INV_H_F 81.48733f
+// [0, pi].  JK_TAB_H is pi / 1024 rounded up to 42 significant bits, so i * H
+// (i < 2^11) is exact in FP64.  Uploaded once per device by jk_init; the
+// kernels that evaluate the likelihood stage it in LDS (24 KB with the FP32
+// grid).  At this spacing cos d needs only the terms to d^4.
+#define JK_TAB_N 1024
+#define JK_TAB_H 0x1.921fb54443000p-9
+#define JK_TAB_INV_H_F 325.9493f
 __device__ double2 g_sincos_tab[JK_TAB_N + 1];
 // The FP32 grid for the mixed-precision Kepler step: g_i = i * JK_TAB_H32
-// (pi / 256 rounded to FP32), (sin g_i, cos g_i) rounded once to FP32.
-#define JK_TAB_H32 0x1.921fb6p-7f
+// (pi / 1024 rounded to FP32), (sin g_i, cos g_i) rounded once to FP32.
+#define JK_TAB_H32 0x1.921fb6p-9f
 __device__ float2 g_sincos_tab32[JK_TAB_N + 1];
 // LDS image: the FP64 table, then the FP32 table packed two per double2 slot.
 #define JK_TAB_LDS (JK_TAB_N + 1 + (JK_TAB_N + 2) / 2)
@@ -68,7 +69,7 @@ __device__ __forceinline__ void jk_sincos_tab_f64(double x, const double2 *__res
     const double d = ay - (double)idx * JK_TAB_H;
     const double d2 = d * d;
     const double sd = fma(d * d2, fma(d2, 1.0 / 120.0, -1.0 / 6.0), d);
-    const double cd = fma(d2, fma(d2, fma(d2, -1.0 / 720.0, 1.0 / 24.0), -0.5), 1.0);
+    const double cd = fma(d2, fma(d2, 1.0 / 24.0, -0.5), 1.0);
     const double sa = fma(t.x, cd, t.y * sd);
     s = y < 0.0 ? -sa : sa;
     c = fma(t.y, cd, -(t.x * sd));
@@ -77,8 +78,9 @@ __device__ __forceinline__ void jk_sincos_tab_f64(double x, const double2 *__res
 // sin and cos in FP64 of an FP32 angle Ef in [0, pi] (a little outside is
 // fine): Ef = g_i + d with g_i the grid point below, |d| <= 1.01 H, then
 //   sin Ef = sin g_i cos d + cos g_i sin d,  cos Ef = cos g_i cos d - sin g_i sin d
-// with sin d to d^5 and cos d to d^6 (truncation below 1e-17).  d is exact:
-// Ef has 24 significant bits and g_i 53, and they agree in their leading bits.
+// with sin d to d^5 and cos d to d^4 (truncation below 2e-18).  d is exact:
+// Ef has 24 significant bits and g_i 42 on a grid of the same scale, so their
+// difference needs fewer than 53.
 // The index is clamped into the table whatever Ef is.
 __device__ __forceinline__ void jk_sincos_tab(float Ef, const double2 *__restrict__ tab,
                                               double &s, double &c)
@@ -89,7 +91,7 @@ __device__ __forceinline__ void jk_sincos_tab(float Ef, const double2 *__restric
     const double d = (double)Ef - (double)idx * JK_TAB_H;
     const double d2 = d * d;
     const double sd = fma(d * d2, fma(d2, 1.0 / 120.0, -1.0 / 6.0), d);
-    const double cd = fma(d2, fma(d2, fma(d2, -1.0 / 720.0, 1.0 / 24.0), -0.5), 1.0);
+    const double cd = fma(d2, fma(d2, 1.0 / 24.0, -0.5), 1.0);
     s = fma(t.x, cd, t.y * sd);
     c = fma(t.y, cd, -(t.x * sd));
 }
@@ -97,7 +99,7 @@ __device__ __forceinline__ void jk_sincos_tab(float Ef, const double2 *__restric
 // sin and cos in FP32 of E in [0, pi] (a little outside is fine) from the FP32
 // grid staged after the FP64 table: E = g_i + d, d = E - i H32 rounded once
 // (fma), |d| <= 1.01 H; sin d to d^3 and cos d to d^2 (truncation below
-// 1e-9, under FP32 rounding).  The index is clamped into the table.
+// 4e-12, under FP32 rounding).  The index is clamped into the table.
 __device__ __forceinline__ void jk_sincos_tab_f32(float E, const double2 *__restrict__ tab,
                                                   float &s, float &c)
 {
@@ -218,11 +220,10 @@ __device__ __forceinline__ bool jk_kepler_markley_f64(double x, double e, float 
     const double E1 = (double)E1f;
     double s1, c1;
     jk_sincos_tab(E1f, tab, s1, c1);
-    const double es = e * s1, ec = e * c1;
-    const double f0 = (E1 - ax) - es;
+    const double f0 = fma(-e, s1, E1 - ax);
     // One Newton step makes r good to the square of the hardware estimate's
     // error, far below what d (<= 4.4e-4) needs.
-    const double D1 = 1.0 - ec;
+    const double D1 = fma(-e, c1, 1.0);
     double r;
     if constexpr (WV) {
         const double R = jk_rcp1(D1 * v);
@@ -232,8 +233,10 @@ __device__ __forceinline__ bool jk_kepler_markley_f64(double x, double e, float 
         r = jk_rcp1(D1);
     }
     const double t = -f0 * r;
-    const double b2 = 0.5 * es * r;
-    const double b3 = ec * r * (1.0 / 6.0);
+    const double b2 = (0.5 * e) * s1 * r;
+    // e cos E1 r = r - 1 (r (1 - e cos E1) = 1 to a few ulp; the absolute
+    // error of r - 1 moves d by t^3 5e-16, far below ulp(E))
+    const double b3 = (r - 1.0) * (1.0 / 6.0);
     const double b22 = b2 * b2;
     const double c3 = fma(2.0, b22, -b3);
     const double c4 = b2 * fma(5.0, b3 - b22, 1.0 / 12.0);   // -b4 = b2 / 12
