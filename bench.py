@@ -320,6 +320,30 @@ def leg_single(args, torch, np, D, syn):
         log(f"[bench] single: pruned {prec} {p_ms:.3f} ms ({ms_step / p_ms:.2f}x the plain step), "
             f"accepted rows equal the plain path's: {np.array_equal(idx, ref_idx)}")
 
+    # Where the probe pays: the same rows against stars of more epochs, the
+    # plain pass (lnL + rejection, synchronous) against the pruned one.
+    out["pruned"]["by_epochs"] = []
+    ll_m = torch.empty_like(ll)     # ll keeps the M=16 star's values for the oracle check
+    for M in (24, 50, 100, 200):
+        s_m = syn.single_star(M=M)
+        d_m = D.DeviceStar(s_m.t, s_m.rv, s_m.ivar(), s_m.trend(), mu, lam, s_m.t_ref,
+                           jitter_mode="physical", device=0)
+        _, ref_m, _, _ = D.lnl_reject(d_m, cache, state, 256, out=ll_m)
+        rec = {"epochs": M, "plain_ms": timed(lambda: D.lnl_reject(d_m, cache, state, 256,
+                                                                   out=ll_m), reps, torch)}
+        for prec in ("f64", "mixed"):
+            jk = tj.TheJoker(pr, precision=prec, prune=True)
+            _, _, idx = jk._pruned_reject(d_m, cache, n, state, 256)
+            p_ms = timed(lambda: jk._pruned_reject(d_m, cache, n, state, 256), reps, torch)
+            rec[prec] = {"ms": p_ms, "speedup_vs_plain": rec["plain_ms"] / p_ms,
+                         "accepted_equal_plain": bool(np.array_equal(idx, ref_m))}
+        out["pruned"]["by_epochs"].append(rec)
+        log(f"[bench] single: M={M} plain {rec['plain_ms']:.3f} ms, pruned f64 "
+            f"{rec['f64']['ms']:.3f} ms ({rec['f64']['speedup_vs_plain']:.2f}x), mixed "
+            f"{rec['mixed']['ms']:.3f} ms ({rec['mixed']['speedup_vs_plain']:.2f}x), accepted "
+            f"equal {rec['f64']['accepted_equal_plain'] and rec['mixed']['accepted_equal_plain']}")
+    del ll_m
+
     if not args.no_cpu:
         rows = np.ascontiguousarray(np.stack([cols[k] for k in ("P", "e", "omega", "M0", "s")],
                                              axis=1))

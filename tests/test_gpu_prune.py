@@ -337,6 +337,9 @@ def test_pruned_rejection_sample_equals_plain(gpu, precision, M):
         ref, ll_ref = tj.TheJoker(prior, rng=rng_of(), precision="f64").rejection_sample(
             data, cache, max_posterior_samples=64, in_memory=True, return_all_logprobs=True)
         joker = tj.TheJoker(prior, rng=rng_of(), precision=precision, prune=True)
+        # below the single-pass threshold the plain pass runs; lowered here
+        # so that the probe path is what is compared at 16 epochs too
+        joker.PRUNE_MIN_EPOCHS_SINGLE = min(M, joker.PRUNE_MIN_EPOCHS_SINGLE)
         got, ll_got = joker.rejection_sample(data, cache, max_posterior_samples=64,
                                              in_memory=True, return_all_logprobs=True)
         assert len(got) == len(ref) > 0
