@@ -324,7 +324,7 @@ def leg_single(args, torch, np, D, syn):
     # plain pass (lnL + rejection, synchronous) against the pruned one.
     out["pruned"]["by_epochs"] = []
     ll_m = torch.empty_like(ll)     # ll keeps the M=16 star's values for the oracle check
-    for M in (24, 50, 100, 200):
+    for M in [int(x) for x in args.prune_epochs.split(",") if x.strip()]:
         s_m = syn.single_star(M=M)
         d_m = D.DeviceStar(s_m.t, s_m.rv, s_m.ivar(), s_m.trend(), mu, lam, s_m.t_ref,
                            jitter_mode="physical", device=0)
@@ -881,6 +881,9 @@ def main():
     ap.add_argument("--serving-quick", action="store_true",
                     help="serving: latency and saturated runs only (no comparison runs)")
     ap.add_argument("--stress-log2-rows", type=int, default=29)
+    ap.add_argument("--prune-epochs", default="24,50,100,200",
+                    help="epoch counts of the single-star pruned-vs-plain records (comma "
+                         "separated; empty for none, as in the kernel profiles of the M=16 star)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (the "
