@@ -994,6 +994,10 @@ __global__ __launch_bounds__(256) void k_uniform(int64_t n, u128 s_base, u128 in
 // without the exp where it cannot decide anything: for x < -40 and u > 0,
 // exp(x) <= exp(-40) < 4.3e-18 < 2^-53 <= u.  Most rows of a peaked
 // likelihood lie that far below the maximum, so most waves skip the exp.
+// Strides of the row loop whose loads the flag kernels issue together
+// (k_seg_decide on 2^24 rows of ll + bound: 66 -> 51 us).
+#define JK_FLAG_UNROLL 4
+
 __device__ __forceinline__ bool jk_accept(double x, double u)
 {
     bool keep = false;
@@ -1011,16 +1015,25 @@ __global__ __launch_bounds__(256) void k_reject_flags(const double *__restrict__
     const int64_t T = (int64_t)gridDim.x * 256;
     const double mx = *llmax;
     u128 s = pcg_thread_start(s_base, inc);
-    for (int64_t base = g - lane; base < n; base += T) {
-        const int64_t i = base + lane;
-        bool keep = false;
-        if (i < n) {
-            const double uu = pcg_to_double(pcg_output(s));
-            keep = jk_accept(ll[i] - mx, uu);
+    // JK_FLAG_UNROLL strides per pass, their loads issued first (more bytes in
+    // flight per wave; the pass stays bound by the LCG's integer multiplies)
+    for (int64_t base = g - lane; base < n; base += JK_FLAG_UNROLL * T) {
+        double x[JK_FLAG_UNROLL];
+#pragma unroll
+        for (int u = 0; u < JK_FLAG_UNROLL; ++u) {
+            const int64_t i = base + u * T + lane;
+            x[u] = i < n ? ll[i] : 0.0;
         }
-        const uint64_t word = __ballot(keep);
-        if (lane == 0) mask[base >> 6] = word;
-        s = pcg_affine(s, AT, DT, inc);
+#pragma unroll
+        for (int u = 0; u < JK_FLAG_UNROLL; ++u) {
+            const int64_t bu = base + u * T;    // the same for every lane of the wave
+            if (bu >= n) break;
+            bool keep = false;
+            if (bu + lane < n) keep = jk_accept(x[u] - mx, pcg_to_double(pcg_output(s)));
+            const uint64_t word = __ballot(keep);
+            if (lane == 0) mask[bu >> 6] = word;
+            s = pcg_affine(s, AT, DT, inc);
+        }
     }
 }
 
@@ -1734,17 +1747,28 @@ __global__ __launch_bounds__(256) void k_seg_flags(const double *__restrict__ ll
     if (g - lane >= sg.n) return;
     const double mx = llmax[blockIdx.y];
     u128 s = pcg_thread_start(sg.s_base, sg.inc);
-    for (int64_t base = g - lane; base < sg.n; base += T) {
-        const int64_t i = base + lane;
-        bool keep = false;
-        if (i < sg.n) {
-            const double uu = pcg_to_double(pcg_output(s));
-            keep = jk_accept(ll[sg.off + i] - mx, uu);
-            if (zero_u != nullptr && uu == 0.0) zero_u[blockIdx.y] = 1;
+    for (int64_t base0 = g - lane; base0 < sg.n; base0 += JK_FLAG_UNROLL * T) {
+        // the loads of JK_FLAG_UNROLL strides first (as in k_reject_flags)
+        double xs[JK_FLAG_UNROLL];
+#pragma unroll
+        for (int u = 0; u < JK_FLAG_UNROLL; ++u) {
+            const int64_t i = base0 + u * T + lane;
+            xs[u] = i < sg.n ? ll[sg.off + i] : 0.0;
         }
-        const uint64_t word = __ballot(keep);
-        if (lane == 0) mask[sg.mask_off + (base >> 6)] = word;
-        s = pcg_affine(s, AT, DT, sg.inc);
+#pragma unroll
+        for (int u = 0; u < JK_FLAG_UNROLL; ++u) {
+            const int64_t base = base0 + u * T;     // the same for every lane of the wave
+            if (base >= sg.n) break;
+            bool keep = false;
+            if (base + lane < sg.n) {
+                const double uu = pcg_to_double(pcg_output(s));
+                keep = jk_accept(xs[u] - mx, uu);
+                if (zero_u != nullptr && uu == 0.0) zero_u[blockIdx.y] = 1;
+            }
+            const uint64_t word = __ballot(keep);
+            if (lane == 0) mask[sg.mask_off + (base >> 6)] = word;
+            s = pcg_affine(s, AT, DT, sg.inc);
+        }
     }
 }
 
@@ -2254,32 +2278,46 @@ __global__ __launch_bounds__(256) void k_seg_decide(const double *__restrict__ l
     if (g - lane >= sg.n) return;
     const double mx = llmax[blockIdx.y];
     u128 s = pcg_thread_start(sg.s_base, sg.inc);
-    for (int64_t base = g - lane; base < sg.n; base += T) {
-        const int64_t i = base + lane;
-        bool acc = false, open = false;
-        if (i < sg.n) {
-            const double uu = pcg_to_double(pcg_output(s));
-            const double x = ll[sg.off + i];
-            const float bf = bound[sg.off + i];
-            if (bf > 0.0f) {
-                // a NaN maximum makes both sides false: nothing accepted, nothing open
-                const double b = fma((double)bf, 1e-6, (double)bf) + 1e-12;
-                const bool lo = jk_accept(x - b - mx, uu), hi = jk_accept(x + b - mx, uu);
-                open = lo != hi;
-                acc = lo && hi;
-            } else if (bf < 0.0f) {
-                // an upper bound decides a rejection only
-                open = jk_accept(x - mx, uu);
-            } else {
-                acc = jk_accept(x - mx, uu);
+    for (int64_t base0 = g - lane; base0 < sg.n; base0 += JK_FLAG_UNROLL * T) {
+        // the loads of JK_FLAG_UNROLL strides first (as in k_reject_flags)
+        double xs[JK_FLAG_UNROLL];
+        float bs[JK_FLAG_UNROLL];
+#pragma unroll
+        for (int u = 0; u < JK_FLAG_UNROLL; ++u) {
+            const int64_t i = base0 + u * T + lane;
+            xs[u] = i < sg.n ? ll[sg.off + i] : 0.0;
+            bs[u] = i < sg.n ? bound[sg.off + i] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < JK_FLAG_UNROLL; ++u) {
+            const int64_t base = base0 + u * T;     // the same for every lane of the wave
+            if (base >= sg.n) break;
+            const int64_t i = base + lane;
+            bool acc = false, open = false;
+            if (i < sg.n) {
+                const double uu = pcg_to_double(pcg_output(s));
+                const double x = xs[u];
+                const float bf = bs[u];
+                if (bf > 0.0f) {
+                    // a NaN maximum makes both sides false: nothing accepted, nothing open
+                    const double b = fma((double)bf, 1e-6, (double)bf) + 1e-12;
+                    const bool lo = jk_accept(x - b - mx, uu), hi = jk_accept(x + b - mx, uu);
+                    open = lo != hi;
+                    acc = lo && hi;
+                } else if (bf < 0.0f) {
+                    // an upper bound decides a rejection only
+                    open = jk_accept(x - mx, uu);
+                } else {
+                    acc = jk_accept(x - mx, uu);
+                }
             }
+            const uint64_t wa = __ballot(acc), wo = __ballot(open);
+            if (lane == 0) {
+                mask[sg.mask_off + (base >> 6)] = wa;
+                open_mask[sg.mask_off + (base >> 6)] = wo;
+            }
+            s = pcg_affine(s, AT, DT, sg.inc);
         }
-        const uint64_t wa = __ballot(acc), wo = __ballot(open);
-        if (lane == 0) {
-            mask[sg.mask_off + (base >> 6)] = wa;
-            open_mask[sg.mask_off + (base >> 6)] = wo;
-        }
-        s = pcg_affine(s, AT, DT, sg.inc);
     }
 }
 
