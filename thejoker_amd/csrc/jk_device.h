@@ -155,10 +155,42 @@ __device__ __forceinline__ double jk_k_variance(const StarDev *__restrict__ st, 
 // minimised chi^2) are also returned, for the mixed-precision error bound.
 // Returns +inf if the Cholesky factorisation breaks down, as the reference
 // returns INF when its LU of Ainv fails (fast_likelihood.pyx:283-291,381-382).
+// Natural logarithm in FP64 of a positive finite x (subnormals included:
+// frexp scales them exactly; +inf, 0, negatives and NaN give what log does),
+// from a table of JK_LOG_TAB_N entries: x = m 2^k
+// with m in [1, 2); for the interval of m's top seven fraction bits,
+// c_i = 1 + (i + 1/2) / 128 and ltab[i] = (R_i, -ln R_i), R_i = 1 / c_i
+// rounded to FP64 and its log taken in long double on the host.  Then
+// r = m R_i - 1 (one fma), |r| <= 1/256 + 2^-52, and
+//   ln x = k ln 2 - ln R_i + ln(1 + r),  ln(1 + r) = r - r^2/2 + ... - r^6/6
+// (truncation below 3e-18).  The absolute error is a few 1e-16 plus the
+// rounding of k ln 2 (below 6e-14 at the largest k): far inside the 1e-10
+// relative parity of the likelihood, in about 20 instructions where the
+// library's log, which carries a double-double intermediate, takes 85.
+#define JK_LOG_TAB_N 128
+#define JK_LN2 0.69314718055994530942
+
+__device__ __forceinline__ double jk_log_tab(double x, const double2 *__restrict__ ltab)
+{
+    if (!(x > 0.0 && x < INFINITY))
+        return x > 0.0 ? x : (x == 0.0 ? -INFINITY : __builtin_nan(""));
+    int k;
+    const double m = 2.0 * frexp(x, &k);
+    const double2 t = ltab[(__double2hiint(m) >> 13) & (JK_LOG_TAB_N - 1)];
+    const double r = fma(m, t.x, -1.0);
+    double q = fma(r, -1.0 / 6.0, 0.2);
+    q = fma(r, q, -0.25);
+    q = fma(r, q, 1.0 / 3.0);
+    q = fma(r, q, -0.5);
+    return fma((double)(k - 1), JK_LN2, t.y + fma(r * r, q, r));
+}
+
 template <int L, bool POST = false>
 __device__ __forceinline__ double jk_finish_lnl(const double *G /* LxL, lower used */,
                                                 const double *u, double chi0, double lnc,
-                                                double scale, double *post = nullptr)
+                                                double scale,
+                                                const double2 *__restrict__ ltab,
+                                                double *post = nullptr)
 {
     if constexpr (L == 1) {
         const double g = G[0];
@@ -169,7 +201,7 @@ __device__ __forceinline__ double jk_finish_lnl(const double *G /* LxL, lower us
             post[1] = u[0] / g;
             post[2] = chi2;
         }
-        return -0.5 * (chi2 + lnc + log(scale * g));
+        return -0.5 * (chi2 + lnc + jk_log_tab(scale * g, ltab));
     } else if constexpr (L == 2) {
         const double g00 = G[0], g10 = G[2], g11 = G[3];
         const double det = fma(g00, g11, -g10 * g10);
@@ -180,7 +212,7 @@ __device__ __forceinline__ double jk_finish_lnl(const double *G /* LxL, lower us
             post[1] = fma(g11, u[0], -g10 * u[1]) / det;
             post[2] = chi0 - q;
         }
-        return -0.5 * (chi0 - q + lnc + log(scale * det));
+        return -0.5 * (chi0 - q + lnc + jk_log_tab(scale * det, ltab));
     } else {
         // Cholesky G = R R^T without divisions: each pivot's 1/sqrt from the
         // hardware v_rsq_f64 refined by two Newton steps; R[j][j] itself is
@@ -238,7 +270,7 @@ __device__ __forceinline__ double jk_finish_lnl(const double *G /* LxL, lower us
             post[1] = a0;
             post[2] = chi0 - zz;
         }
-        return -0.5 * (chi0 - zz + lnc + (log(detm) + (double)dexp * 0.69314718055994530942));
+        return -0.5 * (chi0 - zz + lnc + (jk_log_tab(detm, ltab) + (double)dexp * 0.69314718055994530942));
     }
 }
 

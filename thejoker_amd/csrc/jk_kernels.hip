@@ -42,8 +42,12 @@ __device__ double2 g_sincos_tab[JK_TAB_N + 1];
 // (pi / 1024 rounded to FP32), (sin g_i, cos g_i) rounded once to FP32.
 #define JK_TAB_H32 0x1.921fb6p-9f
 __device__ float2 g_sincos_tab32[JK_TAB_N + 1];
-// LDS image: the FP64 table, then the FP32 table packed two per double2 slot.
-#define JK_TAB_LDS (JK_TAB_N + 1 + (JK_TAB_N + 2) / 2)
+// (R_i, -ln R_i) of jk_log_tab.
+__device__ double2 g_log_tab[JK_LOG_TAB_N];
+// LDS image: the FP64 table, then the FP32 table packed two per double2 slot,
+// then the log table.
+#define JK_TAB_LOG_OFF (JK_TAB_N + 1 + (JK_TAB_N + 2) / 2)
+#define JK_TAB_LDS (JK_TAB_LOG_OFF + JK_LOG_TAB_N)
 
 __device__ __forceinline__ void jk_stage_tab(double2 *s_tab)
 {
@@ -52,6 +56,8 @@ __device__ __forceinline__ void jk_stage_tab(double2 *s_tab)
         s_tab[i] = g_sincos_tab[i];
         t32[i] = g_sincos_tab32[i];
     }
+    for (int i = threadIdx.x; i < JK_LOG_TAB_N; i += blockDim.x)
+        s_tab[JK_TAB_LOG_OFF + i] = g_log_tab[i];
     __syncthreads();
 }
 
@@ -503,7 +509,7 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
     }
     if constexpr (BOUND) {
         double post[3];
-        const double ll = jk_finish_lnl<L, true>(G, u, chi0, lnc, scale, post);
+        const double ll = jk_finish_lnl<L, true>(G, u, chi0, lnc, scale, tab + JK_TAB_LOG_OFF, post);
         const double aK = fabs(mu0 + post[1]);
         const double t1 = 1.0 + (double)dmx;
         const double ek = t1 * t1 * fma(1e-3, (double)dmx, 1.0);
@@ -518,7 +524,7 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
         *bnd = (isfinite(b) && isfinite(ll)) ? (float)b : INFINITY;
         return ll;
     } else {
-        return jk_finish_lnl<L>(G, u, chi0, lnc, scale);
+        return jk_finish_lnl<L>(G, u, chi0, lnc, scale, tab + JK_TAB_LOG_OFF);
     }
 }
 
@@ -663,7 +669,7 @@ __device__ double jk_lnl_row_cov(const StarDev &st, int64_t r, const double *__r
             }
         }
     }
-    return jk_finish_lnl<L>(G, u, chi0, lnc, scale);
+    return jk_finish_lnl<L>(G, u, chi0, lnc, scale, tab + JK_TAB_LOG_OFF);
 }
 
 // ---------------------------------------------------------------------------
@@ -1212,8 +1218,9 @@ hipError_t jk_upload_pcg_table(int device)
     return hipMemcpyToSymbol(HIP_SYMBOL(c_tid_D), tD, sizeof(tD), 0, hipMemcpyHostToDevice);
 }
 
-// The sin / cos grids of jk_sincos_tab and jk_sincos_tab_f32, from 80-bit
-// long double on the host rounded once to FP64 / FP32.
+// The sin / cos grids of jk_sincos_tab and jk_sincos_tab_f32, and the log
+// table of jk_log_tab, from 80-bit long double on the host rounded once to
+// FP64 / FP32.
 hipError_t jk_upload_sincos_table(int device)
 {
     double2 tab[JK_TAB_N + 1];
@@ -1224,8 +1231,15 @@ hipError_t jk_upload_sincos_table(int device)
         const long double g32 = (long double)i * (long double)JK_TAB_H32;
         tab32[i] = make_float2((float)sinl(g32), (float)cosl(g32));
     }
+    double2 ltab[JK_LOG_TAB_N];
+    for (int i = 0; i < JK_LOG_TAB_N; ++i) {
+        const double R = (double)(1.0L / (1.0L + (i + 0.5L) / JK_LOG_TAB_N));
+        ltab[i] = make_double2(R, (double)(-logl((long double)R)));
+    }
     hipError_t err = hipMemcpyToSymbol(HIP_SYMBOL(g_sincos_tab), tab, sizeof(tab), 0,
                                        hipMemcpyHostToDevice);
+    if (err != hipSuccess) return err;
+    err = hipMemcpyToSymbol(HIP_SYMBOL(g_log_tab), ltab, sizeof(ltab), 0, hipMemcpyHostToDevice);
     if (err != hipSuccess) return err;
     return hipMemcpyToSymbol(HIP_SYMBOL(g_sincos_tab32), tab32, sizeof(tab32), 0,
                              hipMemcpyHostToDevice);
