@@ -49,6 +49,12 @@ __device__ double2 g_log_tab[JK_LOG_TAB_N];
 #define JK_TAB_LOG_OFF (JK_TAB_N + 1 + (JK_TAB_N + 2) / 2)
 #define JK_TAB_LDS (JK_TAB_LOG_OFF + JK_LOG_TAB_N)
 
+// Minimum waves per SIMD asked of the row kernels that evaluate many stars
+// or scattered rows: 4 (at most 128 VGPRs) for L <= 3, where the allocator
+// otherwise settled some variants at 129-132 VGPRs and 3 waves without
+// spilling at 128; none for larger L, which would spill.
+#define JK_ROW_WAVES(L) ((L) <= 3 ? 4 : 1)
+
 __device__ __forceinline__ void jk_stage_tab(double2 *s_tab)
 {
     float2 *t32 = reinterpret_cast<float2 *>(s_tab + JK_TAB_N + 1);
@@ -808,7 +814,7 @@ static hipError_t launch_lnl_cov(const jk_star *st, int64_t n, const double *P, 
 // i = idx[j], ll[i] = lnL(row rows[i] or i).  Grid-stride, so the host does
 // not need the count.
 template <int L, bool PHYS>
-__global__ __launch_bounds__(256) void k_lnl_scatter_f64(const StarDev st,
+__global__ __launch_bounds__(256, JK_ROW_WAVES(L)) void k_lnl_scatter_f64(const StarDev st,
                                                          const int64_t *__restrict__ count,
                                                          const int64_t *__restrict__ idx,
                                                          const double *__restrict__ P,
@@ -1419,7 +1425,7 @@ __device__ __forceinline__ double jk_lnl_row_sel(const StarDev &st, int64_t r,
 __global__ void k_set_exact(StarDev *stars, int slot, int exact) { stars[slot].exact = exact; }
 
 template <int L, bool PHYS, bool MIXED, bool BOUND>
-__global__ __launch_bounds__(256) void k_lnl_batch(const StarDev *__restrict__ stars,
+__global__ __launch_bounds__(256, JK_ROW_WAVES(L)) void k_lnl_batch(const StarDev *__restrict__ stars,
                                                    const jk_work_item *__restrict__ items,
                                                    int64_t n_items, unsigned *__restrict__ counter,
                                                    const double *__restrict__ P,
@@ -1552,7 +1558,7 @@ hipError_t jk_launch_lnl_batch(const jk_batch *b, int precision, int64_t n_items
 #define JK_PRUNE_LIST 2048   // survivors held in LDS before their evaluation
 
 template <int L, bool PHYS, bool MIXED>
-__global__ __launch_bounds__(256) void k_lnl_prune_batch(const StarDev *__restrict__ stars,
+__global__ __launch_bounds__(256, JK_ROW_WAVES(L)) void k_lnl_prune_batch(const StarDev *__restrict__ stars,
                                                          const StarDev *__restrict__ probes,
                                                          const double2 *__restrict__ prune,
                                                          const jk_work_item *__restrict__ items,
@@ -1957,7 +1963,7 @@ __global__ __launch_bounds__(256) void k_seg_count_scan(const int64_t *__restric
 // [64 w, 64 w + 64), all in one segment (found by binary search in cum and
 // made wave-uniform, so the star's constants stay in scalar loads).
 template <int L, bool PHYS>
-__global__ __launch_bounds__(256) void k_seg_scatter_f64(const StarDev *__restrict__ stars,
+__global__ __launch_bounds__(256, JK_ROW_WAVES(L)) void k_seg_scatter_f64(const StarDev *__restrict__ stars,
                                                          const SegDev *__restrict__ segs,
                                                          int n_seg,
                                                          const int64_t *__restrict__ count,
