@@ -32,8 +32,8 @@ __constant__ u128 c_tid_D[256];
 // (sin g_i, cos g_i) on the grid g_i = i * JK_TAB_H, i = 0 .. JK_TAB_N, over
 // [0, pi].  JK_TAB_H is pi / 1024 rounded up to 42 significant bits, so i * H
 // (i < 2^11) is exact in FP64.  Uploaded once per device by jk_init; the
-// kernels that evaluate the likelihood stage it in LDS (24 KB with the FP32
-// grid).  At this spacing cos d needs only the terms to d^4.
+// kernels that evaluate the likelihood stage it in LDS (26 KB with the FP32
+// grid and the log table).  At this spacing cos d needs only the terms to d^4.
 #define JK_TAB_N 1024
 #define JK_TAB_H 0x1.921fb54443000p-9
 #define JK_TAB_INV_H_F 325.9493f
