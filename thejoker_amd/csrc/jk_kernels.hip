@@ -5,9 +5,10 @@
 //                          lane per prior sample; the star's epoch arrays are
 //                          wave-uniform and read through scalar loads.
 //  k_max_partial / final   NaN-propagating max (np.max).
-//  k_reject_flags          numpy-PCG64 uniforms generated in registers,
-//                          exp(ll - max) > u, one 64-bit ballot word per wave.
-//  k_reject_bsum/compact   order-preserving compaction of the ballot words.
+//  k_reject_words          numpy-PCG64 uniforms generated in registers for the
+//                          rows that can pass, exp(ll - max) > u, one 64-bit
+//                          mask word per lane.
+//  k_reject_bsum/compact   order-preserving compaction of the mask words.
 //  k_uniform               the raw uniforms (tests / callers).
 //  k_linear_posterior      a = G^-1 rhs and G^-1 for accepted rows (one wave
 //                          per row for diagonal-covariance stars).
@@ -28,6 +29,9 @@ __constant__ PcgJump c_jump;
 // from its workgroup's first state.
 __constant__ u128 c_tid_A[256];
 __constant__ u128 c_tid_D[256];
+// (A^64t, D_64t) for t = 0 .. 255: the same for lanes that own 64 rows each.
+__constant__ u128 c_tid64_A[256];
+__constant__ u128 c_tid64_D[256];
 
 // (sin g_i, cos g_i) on the grid g_i = i * JK_TAB_H, i = 0 .. JK_TAB_N, over
 // [0, pi].  JK_TAB_H is pi / 1024 rounded up to 42 significant bits, so i * H
@@ -1001,15 +1005,15 @@ __global__ __launch_bounds__(256) void k_uniform(int64_t n, u128 s_base, u128 in
     }
 }
 
+// Strides of the row loop whose loads the flag kernels issue together
+// (k_seg_decide on 2^24 rows of ll + bound: 66 -> 51 us).
+#define JK_FLAG_UNROLL 4
+
 // The rejection test exp(x) > u of likelihood_helpers.py:108 for a uniform u
 // on numpy's grid {k 2^-53} in [0, 1), evaluated exactly as written, but
 // without the exp where it cannot decide anything: for x < -40 and u > 0,
 // exp(x) <= exp(-40) < 4.3e-18 < 2^-53 <= u.  Most rows of a peaked
 // likelihood lie that far below the maximum, so most waves skip the exp.
-// Strides of the row loop whose loads the flag kernels issue together
-// (k_seg_decide on 2^24 rows of ll + bound: 66 -> 51 us).
-#define JK_FLAG_UNROLL 4
-
 __device__ __forceinline__ bool jk_accept(double x, double u)
 {
     bool keep = false;
@@ -1017,36 +1021,73 @@ __device__ __forceinline__ bool jk_accept(double x, double u)
     return keep;
 }
 
-__global__ __launch_bounds__(256) void k_reject_flags(const double *__restrict__ ll, int64_t n,
+// Below this x = lnL - max no uniform, not even 0, passes the test: exp(x)
+// is 0 in FP64 from x < -745.14 on.
+#define JK_REJECT_FLOOR (-746.0)
+
+// The rejection flags with one lane per 64-row mask word: lane g owns rows
+// [64 g, 64 g + 64), so a wave owns 4096 contiguous rows.  First the wave
+// marks, with coalesced loads and one ballot per 64 rows, the rows with
+// x = lnL - max >= JK_REJECT_FLOOR, the only ones a uniform can accept:
+// iteration k reads rows 64 k + lane of the wave's range, and its ballot is
+// the candidate word of the wave's lane k.  Then each lane draws uniforms for
+// the marked rows of its own word alone: the state of its first row is the
+// workgroup's jump (the same for every lane, by the 2^i table from bit 14
+// on: 256 lanes x 64 rows per workgroup) followed by the lane's own
+// (A^64t, D_64t), and each marked row's state is reached from the previous
+// one by one affine step per set bit of the gap.  Every row gets the uniform
+// of its own index in the stream and the same jk_accept as a row-by-row pass,
+// so the words are the same; a lane whose rows all lie far below the maximum
+// (most, for a peaked likelihood) does no 128-bit arithmetic, where a
+// row-by-row pass steps the LCG once per row (2^24 rows of a peaked
+// likelihood: 46 us row by row, 33 us this way, 38 us with each lane reading
+// its own 64 rows).
+__global__ __launch_bounds__(256) void k_reject_words(const double *__restrict__ ll, int64_t n,
                                                       const double *__restrict__ llmax,
-                                                      u128 s_base, u128 inc, u128 AT, u128 DT,
+                                                      u128 s_base, u128 inc,
                                                       uint64_t *__restrict__ mask)
 {
-    const int lane = threadIdx.x & 63;
-    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t T = (int64_t)gridDim.x * 256;
+    const int t = threadIdx.x, lane = t & 63;
+    const int64_t g = (int64_t)blockIdx.x * 256 + t;
+    const int64_t w0 = (g - lane) * 64;          // the wave's first row
+    if (w0 >= n) return;                         // the whole wave
     const double mx = *llmax;
-    u128 s = pcg_thread_start(s_base, inc);
-    // JK_FLAG_UNROLL strides per pass, their loads issued first (more bytes in
-    // flight per wave; the pass stays bound by the LCG's integer multiplies)
-    for (int64_t base = g - lane; base < n; base += JK_FLAG_UNROLL * T) {
-        double x[JK_FLAG_UNROLL];
+    uint64_t cand = 0;
+    // sixteen loads in flight per lane before their ballots
+    for (int k0 = 0; k0 < 64; k0 += 16) {
+        double v[16];
 #pragma unroll
-        for (int u = 0; u < JK_FLAG_UNROLL; ++u) {
-            const int64_t i = base + u * T + lane;
-            x[u] = i < n ? ll[i] : 0.0;
+        for (int q = 0; q < 16; ++q) {
+            const int64_t r = w0 + 64 * (k0 + q) + lane;
+            v[q] = r < n ? ll[r] : -INFINITY;
         }
 #pragma unroll
-        for (int u = 0; u < JK_FLAG_UNROLL; ++u) {
-            const int64_t bu = base + u * T;    // the same for every lane of the wave
-            if (bu >= n) break;
-            bool keep = false;
-            if (bu + lane < n) keep = jk_accept(x[u] - mx, pcg_to_double(pcg_output(s)));
-            const uint64_t word = __ballot(keep);
-            if (lane == 0) mask[bu >> 6] = word;
-            s = pcg_affine(s, AT, DT, inc);
+        for (int q = 0; q < 16; ++q) {
+            const uint64_t bits = __ballot(v[q] - mx >= JK_REJECT_FLOOR);
+            cand = lane == k0 + q ? bits : cand;
         }
     }
+    const int64_t r0 = g * 64;
+    if (r0 >= n) return;
+    uint64_t word = 0;
+    if (cand != 0) {
+        u128 s = s_base;
+        uint64_t kb = blockIdx.x;
+#pragma unroll 1
+        for (int b = 14; kb != 0; ++b, kb >>= 1)
+            if (kb & 1) s = pcg_affine(s, c_jump.A[b], c_jump.D[b], inc);
+        s = pcg_affine(s, c_tid64_A[t], c_tid64_D[t], inc);
+        int p = 0;
+        while (cand != 0) {
+            const int j = __builtin_ctzll(cand);
+            cand &= cand - 1;
+            for (int d = j - p, b = 0; d != 0; ++b, d >>= 1)
+                if (d & 1) s = pcg_affine(s, c_jump.A[b], c_jump.D[b], inc);
+            p = j;
+            if (jk_accept(ll[r0 + j] - mx, pcg_to_double(pcg_output(s)))) word |= 1ull << j;
+        }
+    }
+    mask[g] = word;
 }
 
 // Rows to re-evaluate in FP64 after a mixed-precision pass: every row not
@@ -1221,7 +1262,12 @@ hipError_t jk_upload_pcg_table(int device)
     for (int t = 0; t < 256; ++t) jk_pcg_jump_coeffs((uint64_t)t, &tA[t], &tD[t]);
     err = hipMemcpyToSymbol(HIP_SYMBOL(c_tid_A), tA, sizeof(tA), 0, hipMemcpyHostToDevice);
     if (err != hipSuccess) return err;
-    return hipMemcpyToSymbol(HIP_SYMBOL(c_tid_D), tD, sizeof(tD), 0, hipMemcpyHostToDevice);
+    err = hipMemcpyToSymbol(HIP_SYMBOL(c_tid_D), tD, sizeof(tD), 0, hipMemcpyHostToDevice);
+    if (err != hipSuccess) return err;
+    for (int t = 0; t < 256; ++t) jk_pcg_jump_coeffs(64 * (uint64_t)t, &tA[t], &tD[t]);
+    err = hipMemcpyToSymbol(HIP_SYMBOL(c_tid64_A), tA, sizeof(tA), 0, hipMemcpyHostToDevice);
+    if (err != hipSuccess) return err;
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_tid64_D), tD, sizeof(tD), 0, hipMemcpyHostToDevice);
 }
 
 // The sin / cos grids of jk_sincos_tab and jk_sincos_tab_f32, and the log
@@ -1268,14 +1314,11 @@ hipError_t jk_launch_reject(const double *ll, int64_t n, const double *llmax, u1
                             int64_t *n_pass_out, uint64_t *mask, int64_t *bsum,
                             hipStream_t stream)
 {
-    const int grid = jk_rng_grid(n);
-    u128 AT, DT;
-    jk_pcg_jump_coeffs((uint64_t)grid * 256, &AT, &DT);
     const u128 s_base = h_jump(state, inc, offset + 1);
     const int64_t nwords = jk_reject_mask_words(n);
     const int64_t nb = jk_reject_bsum_count(n);
-    hipLaunchKernelGGL(k_reject_flags, dim3(grid), dim3(256), 0, stream, ll, n, llmax, s_base, inc,
-                       AT, DT, mask);
+    hipLaunchKernelGGL(k_reject_words, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, stream,
+                       ll, n, llmax, s_base, inc, mask);
     hipLaunchKernelGGL(k_reject_bsum, dim3((unsigned)nb), dim3(256), 0, stream, mask, nwords, bsum);
     hipLaunchKernelGGL(k_reject_compact, dim3((unsigned)nb), dim3(256), 0, stream, mask, nwords, bsum,
                        (int)nb, max_keep, idx_out, n_pass_out);
@@ -1768,7 +1811,7 @@ __global__ __launch_bounds__(256) void k_seg_flags(const double *__restrict__ ll
     const double mx = llmax[blockIdx.y];
     u128 s = pcg_thread_start(sg.s_base, sg.inc);
     for (int64_t base0 = g - lane; base0 < sg.n; base0 += JK_FLAG_UNROLL * T) {
-        // the loads of JK_FLAG_UNROLL strides first (as in k_reject_flags)
+        // the loads of JK_FLAG_UNROLL strides first
         double xs[JK_FLAG_UNROLL];
 #pragma unroll
         for (int u = 0; u < JK_FLAG_UNROLL; ++u) {
@@ -2299,7 +2342,7 @@ __global__ __launch_bounds__(256) void k_seg_decide(const double *__restrict__ l
     const double mx = llmax[blockIdx.y];
     u128 s = pcg_thread_start(sg.s_base, sg.inc);
     for (int64_t base0 = g - lane; base0 < sg.n; base0 += JK_FLAG_UNROLL * T) {
-        // the loads of JK_FLAG_UNROLL strides first (as in k_reject_flags)
+        // the loads of JK_FLAG_UNROLL strides first
         double xs[JK_FLAG_UNROLL];
         float bs[JK_FLAG_UNROLL];
 #pragma unroll
