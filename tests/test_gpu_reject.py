@@ -81,6 +81,34 @@ def test_reject_nan_and_inf(gpu):
     assert list(idx) == [1]
 
 
+@pytest.mark.parametrize("n", [(1 << 20) + 37, 70_000])
+def test_reject_sparse_rows_and_zero_uniforms(gpu, n):
+    """Most rows far below the maximum, where no uniform is drawn, and a few
+    within reach: the accepted set equals numpy's exp(ll - max) > u, also
+    with a uniform of exactly 0 on a row at -700 (accepted: exp is still
+    positive) and on a row at -745.5 (rejected: exp underflows to 0)."""
+    from tests.test_serving_host import zero_uniform_rng
+    g = np.random.default_rng(n)
+    ll_h = np.full(n, -5000.0)
+    near = g.choice(n, size=n // 200, replace=False)
+    ll_h[near] = -g.uniform(0.0, 50.0, size=len(near))
+    mid = g.choice(n, size=64, replace=False)
+    ll_h[mid] = -g.uniform(40.0, 745.0, size=len(mid))
+    ll_h[g.integers(n)] = 0.0
+    for x, accepted in ((-700.0, True), (-745.5, False)):
+        j = int(g.integers(n))
+        ll_j = ll_h.copy()
+        ll_j[j] = x
+        rng = zero_uniform_rng(j)
+        st = D.pcg64_state(rng)
+        u = zero_uniform_rng(j).uniform(size=n)
+        assert u[j] == 0.0
+        want = np.flatnonzero(np.exp(ll_j - ll_j.max()) > u)
+        assert (j in want) == accepted
+        idx, n_pass, _ = D.reject(torch.as_tensor(ll_j, device="cuda"), st, n)
+        assert n_pass == len(want) and np.array_equal(idx, want)
+
+
 def test_host_generator_stays_in_sync(gpu):
     """After the device draws n uniforms the host generator is advanced by n:
     the next host draw equals numpy's after uniform(size=n)."""
