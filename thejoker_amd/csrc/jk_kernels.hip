@@ -1024,6 +1024,9 @@ __device__ __forceinline__ bool jk_accept(double x, double u)
 // Below this x = lnL - max no uniform, not even 0, passes the test: exp(x)
 // is 0 in FP64 from x < -745.14 on.
 #define JK_REJECT_FLOOR (-746.0)
+// Marked rows among a wave's first 1024 above which k_reject_words walks the
+// wave's words row by row instead of lane by lane.
+#define JK_REJECT_DENSE 128
 
 // The rejection flags with one lane per 64-row mask word: lane g owns rows
 // [64 g, 64 g + 64), so a wave owns 4096 contiguous rows.  First the wave
@@ -1052,8 +1055,18 @@ __global__ __launch_bounds__(256) void k_reject_words(const double *__restrict__
     const int64_t w0 = (g - lane) * 64;          // the wave's first row
     if (w0 >= n) return;                         // the whole wave
     const double mx = *llmax;
+    const int64_t r0 = g * 64;
+    // the workgroup's jump, the same for every lane: 256 lanes x 64 rows
+    u128 s = s_base;
+    uint64_t kb = blockIdx.x;
+#pragma unroll 1
+    for (int b = 14; kb != 0; ++b, kb >>= 1)
+        if (kb & 1) s = pcg_affine(s, c_jump.A[b], c_jump.D[b], inc);
+    // marks of the wave's words, sixteen loads in flight per lane before their
+    // ballots; after the first sixteen words (1024 rows), a wave with many
+    // rows in reach takes the row-by-row walk instead
     uint64_t cand = 0;
-    // sixteen loads in flight per lane before their ballots
+    bool dense = false;
     for (int k0 = 0; k0 < 64; k0 += 16) {
         double v[16];
 #pragma unroll
@@ -1066,16 +1079,41 @@ __global__ __launch_bounds__(256) void k_reject_words(const double *__restrict__
             const uint64_t bits = __ballot(v[q] - mx >= JK_REJECT_FLOOR);
             cand = lane == k0 + q ? bits : cand;
         }
+        if (k0 == 0 && wave_sum_i64((int64_t)__popcll(cand)) > JK_REJECT_DENSE) {
+            dense = true;
+            break;
+        }
     }
-    const int64_t r0 = g * 64;
-    if (r0 >= n) return;
     uint64_t word = 0;
-    if (cand != 0) {
-        u128 s = s_base;
-        uint64_t kb = blockIdx.x;
-#pragma unroll 1
-        for (int b = 14; kb != 0; ++b, kb >>= 1)
-            if (kb & 1) s = pcg_affine(s, c_jump.A[b], c_jump.D[b], inc);
+    if (dense) {
+        // Many rows in reach (a flat likelihood): the wave walks its words in
+        // order, lane L at row 64 k + L of word k, as a row-by-row pass does
+        // with coalesced reads and one 64-row step of every state per word.
+        // State of lane L at row L: the wave's jump by 4096 w (bits 12-13)
+        // and the lane's (A^L, D_L).
+        for (int b = 12, kw = t >> 6; kw != 0; ++b, kw >>= 1)
+            if (kw & 1) s = pcg_affine(s, c_jump.A[b], c_jump.D[b], inc);
+        s = pcg_affine(s, c_tid_A[lane], c_tid_D[lane], inc);
+        const u128 A64 = c_jump.A[6], D64 = c_jump.D[6];
+        for (int k0 = 0; k0 < 64; k0 += JK_FLAG_UNROLL) {
+            double xv[JK_FLAG_UNROLL];
+#pragma unroll
+            for (int q = 0; q < JK_FLAG_UNROLL; ++q) {
+                const int64_t r = w0 + 64 * (k0 + q) + lane;
+                xv[q] = r < n ? ll[r] - mx : -INFINITY;
+            }
+#pragma unroll
+            for (int q = 0; q < JK_FLAG_UNROLL; ++q) {
+                bool keep = false;
+                if (xv[q] >= JK_REJECT_FLOOR)
+                    keep = jk_accept(xv[q], pcg_to_double(pcg_output(s)));
+                const uint64_t wk = __ballot(keep);
+                word = lane == k0 + q ? wk : word;
+                s = pcg_affine(s, A64, D64, inc);
+            }
+        }
+    } else if (cand != 0) {
+        // Few: each lane draws the uniforms of its own marked rows.
         s = pcg_affine(s, c_tid64_A[t], c_tid64_D[t], inc);
         int p = 0;
         while (cand != 0) {
@@ -1087,7 +1125,7 @@ __global__ __launch_bounds__(256) void k_reject_words(const double *__restrict__
             if (jk_accept(ll[r0 + j] - mx, pcg_to_double(pcg_output(s)))) word |= 1ull << j;
         }
     }
-    mask[g] = word;
+    if (r0 < n) mask[g] = word;
 }
 
 // Rows to re-evaluate in FP64 after a mixed-precision pass: every row not
