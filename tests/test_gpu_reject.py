@@ -81,12 +81,15 @@ def test_reject_nan_and_inf(gpu):
     assert list(idx) == [1]
 
 
-@pytest.mark.parametrize("n", [(1 << 20) + 37, 70_000])
-def test_reject_sparse_rows_and_zero_uniforms(gpu, n):
+@pytest.mark.parametrize("n,dense", [((1 << 20) + 37, False), (70_000, False),
+                                     ((1 << 20) + 37, True)])
+def test_reject_sparse_rows_and_zero_uniforms(gpu, n, dense):
     """Most rows far below the maximum, where no uniform is drawn, and a few
     within reach: the accepted set equals numpy's exp(ll - max) > u, also
     with a uniform of exactly 0 on a row at -700 (accepted: exp is still
-    positive) and on a row at -745.5 (rejected: exp underflows to 0)."""
+    positive) and on a row at -745.5 (rejected: exp underflows to 0).  With
+    dense, a block of 20000 rows all within reach puts its waves on the
+    row-by-row walk, and the zero uniforms fall inside it."""
     from tests.test_serving_host import zero_uniform_rng
     g = np.random.default_rng(n)
     ll_h = np.full(n, -5000.0)
@@ -94,9 +97,12 @@ def test_reject_sparse_rows_and_zero_uniforms(gpu, n):
     ll_h[near] = -g.uniform(0.0, 50.0, size=len(near))
     mid = g.choice(n, size=64, replace=False)
     ll_h[mid] = -g.uniform(40.0, 745.0, size=len(mid))
+    lo_blk = 300_001
+    if dense:
+        ll_h[lo_blk:lo_blk + 20_000] = -g.uniform(0.0, 30.0, size=20_000)
     ll_h[g.integers(n)] = 0.0
     for x, accepted in ((-700.0, True), (-745.5, False)):
-        j = int(g.integers(n))
+        j = int(g.integers(lo_blk, lo_blk + 20_000)) if dense else int(g.integers(n))
         ll_j = ll_h.copy()
         ll_j[j] = x
         rng = zero_uniform_rng(j)
