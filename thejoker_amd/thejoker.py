@@ -250,10 +250,15 @@ class TheJoker:
             probes = star._probe_table = StarBatch([probe], device=self.device)
             star._probe_add = add
         shape = LaunchShape()
-        items, off, _ = work_items([n], shape.tile)
+        # the work items depend only on n: planned and uploaded once per star
+        plan = getattr(star, "_prune_items", None)
+        if plan is None or plan[0] != (n, shape.tile):
+            items, off, _ = work_items([n], shape.tile)
+            plan = star._prune_items = ((n, shape.tile), upload_items(items, self.device),
+                                        len(items), off)
+        _, d_items, n_items, off = plan
         ll, bnd, res, _ = prune_reject_batch(table, probes, np.array([star._probe_add]),
-                                             np.array([True]), cache,
-                                             upload_items(items, self.device), len(items), off,
+                                             np.array([True]), cache, d_items, n_items, off,
                                              np.array([n]), [rng_state], max_keep,
                                              self.precision, shape, self.device)
         return ll, bnd, res[0][0]
