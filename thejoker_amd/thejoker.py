@@ -250,11 +250,12 @@ class TheJoker:
             probes = star._probe_table = StarBatch([probe], device=self.device)
             star._probe_add = add
         shape = LaunchShape()
-        # the work items depend only on n: planned and uploaded once per star
-        plan = getattr(star, "_prune_items", None)
+        # the work items of a one-star table depend only on n: planned and
+        # uploaded once per sampler and row count
+        plan = getattr(self, "_prune_items", None)
         if plan is None or plan[0] != (n, shape.tile):
             items, off, _ = work_items([n], shape.tile)
-            plan = star._prune_items = ((n, shape.tile), upload_items(items, self.device),
+            plan = self._prune_items = ((n, shape.tile), upload_items(items, self.device),
                                         len(items), off)
         _, d_items, n_items, off = plan
         ll, bnd, res, _ = prune_reject_batch(table, probes, np.array([star._probe_add]),
