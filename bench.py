@@ -14,6 +14,11 @@ step's indices.
 Further legs (each an object of the same line; --workload picks one):
   mixed     -- the mixed-precision kernel on the same rows, its error
                against FP64 (max |dlnL| all rows / live set);
+  pruned    -- TheJoker(prune=True)'s rejection pass (probe bound, FP64
+               rows near the maximum, settled rejection) on the same rows,
+               FP64 and mixed, and against the plain pass for stars of
+               --prune-epochs epochs on the same rows; accepted rows checked
+               against the plain path's;
   batched   -- 1024 stars, M ~ U{3..64}, baselines U[200, 3000] d, half
                binaries, poly_trend = 2, 2^20 shared rows: one autotuned
                persistent launch, mixed vs FP64 rates, errors, acceptance
