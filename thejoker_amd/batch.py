@@ -526,11 +526,16 @@ def prune_reject_batch(batch, probes, adds, probed, cache, d_items, n_items, off
                     bound=bnd)
     # only the probed stars' segments hold upper bounds
     k = np.flatnonzero(probed)
-    cnt = torch.zeros(len(sizes), dtype=torch.int64, device=dev)
-    if len(k):
-        cnt[torch.as_tensor(k, device=dev)] = settle_pruned_batch(
-            batch, cache, ll, bnd, np.asarray(offsets)[k], np.asarray(sizes)[k], stars=k,
-            device=device)
+    if len(k) == len(sizes):
+        # every star probed (one star, typically): the counts come back as they are
+        cnt = settle_pruned_batch(batch, cache, ll, bnd, np.asarray(offsets), np.asarray(sizes),
+                                  stars=k, device=device)
+    else:
+        cnt = torch.zeros(len(sizes), dtype=torch.int64, device=dev)
+        if len(k):
+            cnt[torch.as_tensor(k, device=dev)] = settle_pruned_batch(
+                batch, cache, ll, bnd, np.asarray(offsets)[k], np.asarray(sizes)[k], stars=k,
+                device=device)
     res = settle_reject_batch(batch, cache, ll, bnd, offsets, sizes, rng_states, max_keep,
                               device=device)
     return ll, bnd, res, cnt
