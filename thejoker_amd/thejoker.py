@@ -87,8 +87,8 @@ class TheJoker:
     # row near it: a cost nearly independent of the epoch count (1.1-1.7 ms for
     # 2^24 rows on one MI355X from 16 to 200 epochs) against about 55 us per
     # epoch for the full evaluation of the same rows, so the probe pays from
-    # about 20 epochs on (measured: 0.82x the plain pass at 16 epochs, 1.28x
-    # at 24, 2.2x at 50, 5.8x at 200)
+    # about 20 epochs on (measured: 0.82-0.89x the plain pass at 16 epochs,
+    # 1.3x at 24, 2.2x at 50, 5.8-6.0x at 200)
     PRUNE_MIN_EPOCHS_SINGLE = 20
 
     def __init__(self, prior, pool=None, rng=None, tempfile_path=None, jitter_mode="physical",
