@@ -49,6 +49,36 @@ def test_batch_equals_single_star_launches(setup, precision, shape):
         assert np.array_equal(ll[off[s]:off[s] + n], single), s
 
 
+@pytest.mark.parametrize("mode", ["physical", "reference"])
+@pytest.mark.parametrize("first", ["ones", "scaled"])
+def test_constant_first_column_bit_identical(gpu, mode, first):
+    """A single-star FP64 launch folds a first trend column of exact ones
+    (the constant offset) into its sums; the batched launch reads the column.
+    Both give the same lnL bit for bit, for L = 2 and 3, and so does a star
+    whose first column is not all ones (the single-star launch reads it)."""
+    cols = syn.prior_columns(1 << 13, seed=21, jitter=True)
+    cache = D.PriorCache(cols["P"], cols["e"], cols["omega"], cols["M0"], cols["s"])
+    for L in (2, 3):
+        devs = []
+        for st in syn.many_stars(4, seed=8):
+            trend = st.trend(poly_trend=L - 1)
+            assert np.all(trend[:, 0] == 1.0)
+            if first == "scaled":
+                trend = trend.copy()
+                trend[1:, 0] = 1.5
+            lam = np.array([900.0, 1e4, 0.25][:L])
+            devs.append(D.DeviceStar(st.t, st.rv, st.ivar(), trend, np.zeros(L), lam, st.t_ref,
+                                     jitter_mode=mode))
+        batch = B.StarBatch(devs)
+        n = len(cache)
+        items, off, total = B.work_items([n] * len(devs), 2048)
+        ll = B.lnl_batch(batch, cache, B.upload_items(items), len(items), total,
+                         "f64").cpu().numpy()
+        for s, dstar in enumerate(devs):
+            single = D.lnl(dstar, cache).cpu().numpy()
+            assert np.array_equal(ll[off[s]:off[s] + n], single, equal_nan=True), (L, s)
+
+
 def test_batch_vs_oracle(setup):
     stars, cache, cols = setup
     batch = B.StarBatch([d for _, d in stars])

@@ -321,6 +321,8 @@ static int star_build(int n, const double *t, const double *rv, const double *iv
             }
         }
     }
+    d.t0_one = L >= 2 && !Qt;
+    for (int k = 0; k < n && d.t0_one; ++k) d.t0_one = h_T[k] == 1.0;
     d.ln_c_phys = (double)n * JK_LN_TWO_PI + d.ln_lam_trend +
                   (k_prior_mode == JK_KPRIOR_NORMAL ? log(lambda_diag[0]) : 0.0);
     d.ln_c = d.ln_c_phys - d.sum_ln_ivar;

@@ -37,6 +37,8 @@ struct StarDev {
     int k_prior_mode;  // JK_KPRIOR_*
     int exact;         // batched table slots: 1 evaluates the star's rows in FP64
                        // inside mixed-precision launches (error bound 0)
+    int t0_one;        // 1 when the first trend column is 1.0 at every epoch (the
+                       // constant offset v0) and there is no full covariance
     double sigK0sq;    // sigma_K0^2      (FixedCompanionMass)
     double P0;         // reference period (FixedCompanionMass)
     double maxKsq;     // max_K^2         (FixedCompanionMass)

@@ -311,7 +311,9 @@ __device__ __forceinline__ void jk_kepler_markley_f32(float x, float e, float om
 // value, a positive bound a mixed-precision value.
 #define JK_BOUND_UPPER (-1.0f)
 
-template <int L, bool PHYS, bool MIXED, bool SAFE = false, bool BOUND = false>
+// With ONE the first trend column is the constant 1 (StarDev::t0_one): its
+// products fold away, with the same roundings (w * 1 = w exactly).
+template <int L, bool PHYS, bool MIXED, bool SAFE = false, bool BOUND = false, bool ONE = false>
 __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
                                              const double *__restrict__ P,
                                              const void *__restrict__ e_,
@@ -464,13 +466,15 @@ __device__ __forceinline__ double jk_lnl_row(const StarDev &st, int64_t r,
         if constexpr (L > 1) {
 #pragma unroll
             for (int j = 0; j < L - 1; ++j) {
-                const double Tj = st.T[j * N + k];
+                const double Tj = ONE && j == 0 ? 1.0 : st.T[j * N + k];
                 Smt[j] = fma(wm, Tj, Smt[j]);
                 if constexpr (PHYS) {
                     const double wT = w * Tj;
                     Str[j] = fma(wT, rp, Str[j]);
 #pragma unroll
-                    for (int q = 0; q <= j; ++q) Stt[j * LT + q] = fma(wT, st.T[q * N + k], Stt[j * LT + q]);
+                    for (int q = 0; q <= j; ++q)
+                        Stt[j * LT + q] =
+                            fma(wT, ONE && q == 0 ? 1.0 : st.T[q * N + k], Stt[j * LT + q]);
                 }
             }
         }
@@ -717,7 +721,7 @@ __device__ __forceinline__ int64_t wave_incl_scan_i64(int64_t v, int lane)
 // (4 jk_dense_blocks(n) entries), so the rejection step need not read the
 // lnL array a second time to find the maximum.  With BOUND (mixed
 // precision), bound[i] receives the row's error bound (jk_lnl_row).
-template <int L, bool PHYS, bool MIXED, bool BOUND>
+template <int L, bool PHYS, bool MIXED, bool BOUND, bool ONE = false>
 __global__ __launch_bounds__(256) void k_lnl(const StarDev st, int64_t n,
                                              const double *__restrict__ P,
                                              const void *__restrict__ e_,
@@ -742,7 +746,7 @@ __global__ __launch_bounds__(256) void k_lnl(const StarDev st, int64_t n,
             v = jk_lnl_row<L, PHYS, MIXED, false, true>(st, r, P, e_, om_, M0_, s_, s_tab, &b);
             bound[i] = b;
         } else {
-            v = jk_lnl_row<L, PHYS, MIXED>(st, r, P, e_, om_, M0_, s_, s_tab);
+            v = jk_lnl_row<L, PHYS, MIXED, false, false, ONE>(st, r, P, e_, om_, M0_, s_, s_tab);
         }
         ll[i] = v;
         vmax = jk_nanmax(vmax, v);
@@ -859,6 +863,15 @@ static hipError_t launch_lnl_t(const jk_star *st, int64_t n, const double *P, co
         if (bound) {
             hipLaunchKernelGGL((k_lnl<L, PHYS, true, true>), dim3((unsigned)blocks), dim3(256), 0,
                                stream, st->dev, n, P, e, om, M0, s, rows, ll, blockmax, bound);
+            return hipGetLastError();
+        }
+    }
+    if constexpr (!MIXED && (L == 2 || L == 3)) {
+        // the constant offset as the first trend column (the usual v0)
+        if (st->dev.t0_one) {
+            hipLaunchKernelGGL((k_lnl<L, PHYS, false, false, true>), dim3((unsigned)blocks),
+                               dim3(256), 0, stream, st->dev, n, P, e, om, M0, s, rows, ll,
+                               blockmax, nullptr);
             return hipGetLastError();
         }
     }
