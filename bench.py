@@ -268,7 +268,7 @@ def leg_single(args, torch, np, D, syn):
     roofline = {"bound": "mfma", "compute_unit": "FP64 vector ALU (VALU); no MFMA instructions",
                 "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
-                "kernel": "k_lnl<2,true,false,false> (FP64)", "kernel_ms": k_ms,
+                "kernel": "k_lnl<2,true,false,false,true> (FP64, constant-offset column folded)", "kernel_ms": k_ms,
                 "flops_per_launch": flops,
                 "hbm_gbs_prior_stream": n * BYTES_PER_SAMPLE_F64 / (k_ms * 1e-3) / 1e9,
                 "valu": valu_from_counters("single_f64", k_ms) if n == 1 << 24 else None}
